@@ -1,0 +1,251 @@
+"""ctypes front end of the oracle (test infrastructure only).
+
+* :class:`Oracle` wraps liboshun_oracle.so, the CPU restatement of the
+  reference's hot path (oshun_oracle.cpp cites each reference file:line).
+* :func:`run_ref` runs oracle/_ref/oshun_ref, a driver linked with the
+  unmodified reference translation units (built only where the reference
+  tree exists; see oracle/Makefile).
+
+Exchange layout for distribution functions: complex128[nh, ny, nx, np]
+(p fastest, harmonic order ind(l, m) of def-State.cpp:417-427); fields:
+complex128[6, ny, nx] in the order Ex, Ey, Ez, Bx, By, Bz.
+"""
+import ctypes
+import os
+import subprocess
+import tempfile
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboshun_oracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "oshun_ref")
+
+
+def num_harmonics(l0, m0):
+    """N_h of the (l, m) trapezoid, def-State.cpp:417."""
+    return ((m0 + 1) * (2 * l0 - m0 + 2)) // 2
+
+
+def hind(l, m, m0):
+    """Harmonic index ind(l, m), def-State.cpp:422-426."""
+    if l < m0 + 1:
+        return l * (l + 1) // 2 + m
+    return l * (m0 + 1) - m0 * (m0 + 1) // 2 + m
+
+
+def deck_dx(xmin, xmax, n, guards=4):
+    """Cell size exactly as the reference derives it from a deck.
+
+    xglob = Axis(n, xmin, xmax); x(0) = xglob(0) - RKLevel*xglob.dx();
+    x(1) = x(0) + xglob.dx(); dx = x(1) - x(0)   (def-Input.cpp:313-325,
+    matrices.h:860-880).
+    """
+    d = (xmax - xmin) / float(n - 1)
+    xg0 = 0.0 * d + xmin
+    xg1 = 1.0 * d + xmin
+    gdx = xg1 - xg0
+    x0 = xg0 - guards * gdx
+    x1 = x0 + gdx
+    return x1 - x0
+
+
+@dataclass
+class Config:
+    l0: int
+    m0: int
+    np: int
+    nx: int
+    ny: int = 1
+    pmax: float = 0.3
+    pmin: float = None  # defaults to pmax/(2 np) (def-Input.cpp:79)
+    dx: float = 1.0
+    dy: float = 1.0
+    density_np: float = 1.0e21
+    zeta: float = 1.0
+    small_dt: float = 1.0
+    nb: int = 4
+    coll_ee: int = 1
+    coll_flm: int = 1
+    ref_quirks: int = 1
+
+    def __post_init__(self):
+        if self.pmin is None:
+            self.pmin = self.pmax / (2.0 * self.np)
+
+    @property
+    def nh(self):
+        return num_harmonics(self.l0, self.m0)
+
+    def f_shape(self):
+        return (self.nh, self.ny, self.nx, self.np)
+
+    def fld_shape(self):
+        return (6, self.ny, self.nx)
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run make -C oracle)")
+        lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        D = ctypes.c_double
+        I = ctypes.c_int
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib.oo_create.restype = P
+        lib.oo_create.argtypes = [ip, dp]
+        lib.oo_destroy.argtypes = [P]
+        lib.oo_num_harmonics.argtypes = [P]
+        lib.oo_num_harmonics.restype = I
+        lib.oo_set_state.argtypes = [P, dp, dp]
+        lib.oo_get_state.argtypes = [P, dp, dp]
+        lib.oo_set_time.argtypes = [P, D]
+        lib.oo_time.argtypes = [P]
+        lib.oo_time.restype = D
+        lib.oo_rhs.argtypes = [P, dp, dp]
+        lib.oo_rk4.argtypes = [P, D]
+        lib.oo_collide_ee.argtypes = [P, D]
+        lib.oo_collide_flm.argtypes = [P, D, I]
+        lib.oo_step.argtypes = [P, D, I]
+        lib.oo_ee_getslope.argtypes = [P, dp, dp]
+        lib.oo_flm_solve.argtypes = [P, dp, D, I, dp]
+        lib.oo_flm_coeffs.argtypes = [P, dp, D, dp, dp, dp, dp]
+        lib.oo_tridiag.argtypes = [I, dp, dp, dp, dp, dp]
+        lib.oo_axes.argtypes = [P, dp, dp, dp]
+        _lib = lib
+    return _lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """CPU restatement of one OSHUN state plus its RK4/collision drivers."""
+
+    def __init__(self, cfg: Config):
+        lib = _load()
+        self.cfg = cfg
+        iv = (ctypes.c_int * 9)(cfg.l0, cfg.m0, cfg.np, cfg.nx, cfg.ny, cfg.ref_quirks, cfg.nb,
+                                cfg.coll_ee, cfg.coll_flm)
+        dv = (ctypes.c_double * 7)(cfg.pmin, cfg.pmax, cfg.dx, cfg.dy, cfg.density_np, cfg.zeta,
+                                   cfg.small_dt)
+        self._h = lib.oo_create(iv, dv)
+        if not self._h:
+            raise ValueError(f"oracle rejected config {cfg}")
+        self._lib = lib
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.oo_destroy(h)
+            self._h = None
+
+    def set_state(self, f, fld):
+        f = np.ascontiguousarray(f, dtype=np.complex128).reshape(self.cfg.f_shape())
+        fld = np.ascontiguousarray(fld, dtype=np.complex128).reshape(self.cfg.fld_shape())
+        self._lib.oo_set_state(self._h, _dptr(f.view(np.float64)), _dptr(fld.view(np.float64)))
+
+    def get_state(self):
+        f = np.zeros(self.cfg.f_shape(), np.complex128)
+        fld = np.zeros(self.cfg.fld_shape(), np.complex128)
+        self._lib.oo_get_state(self._h, _dptr(f.view(np.float64)), _dptr(fld.view(np.float64)))
+        return f, fld
+
+    def set_time(self, t):
+        self._lib.oo_set_time(self._h, float(t))
+
+    def time(self):
+        return self._lib.oo_time(self._h)
+
+    def rhs(self):
+        f = np.zeros(self.cfg.f_shape(), np.complex128)
+        fld = np.zeros(self.cfg.fld_shape(), np.complex128)
+        self._lib.oo_rhs(self._h, _dptr(f.view(np.float64)), _dptr(fld.view(np.float64)))
+        return f, fld
+
+    def rk4(self, h):
+        self._lib.oo_rk4(self._h, float(h))
+
+    def collide_ee(self, tnew):
+        self._lib.oo_collide_ee(self._h, float(tnew))
+
+    def collide_flm(self, tnew, which):
+        self._lib.oo_collide_flm(self._h, float(tnew), int(which))
+
+    def step(self, h, nsteps=1):
+        self._lib.oo_step(self._h, float(h), int(nsteps))
+
+    def ee_getslope(self, f00):
+        f00 = np.ascontiguousarray(f00, np.float64)
+        out = np.zeros_like(f00)
+        self._lib.oo_ee_getslope(self._h, _dptr(f00), _dptr(out))
+        return out
+
+    def flm_solve(self, f00, dt, el, col):
+        f00 = np.ascontiguousarray(f00, np.float64)
+        col = np.ascontiguousarray(col, np.complex128).copy()
+        self._lib.oo_flm_solve(self._h, _dptr(f00), float(dt), int(el), _dptr(col.view(np.float64)))
+        return col
+
+    def flm_coeffs(self, f00, dt):
+        f00 = np.ascontiguousarray(f00, np.float64)
+        n = self.cfg.np
+        out = [np.zeros(n) for _ in range(4)]
+        self._lib.oo_flm_coeffs(self._h, _dptr(f00), float(dt), *[_dptr(o) for o in out])
+        return out  # sub, diag, sup, scatt
+
+    def axes(self):
+        n = self.cfg.np
+        pd, pc, vr = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._lib.oo_axes(self._h, _dptr(pd), _dptr(pc), _dptr(vr))
+        return pd, pc, vr
+
+
+def tridiag(a, b, c, d):
+    """Thomas algorithm of def-Nmethods.cpp:142-167 (a[0], c[-1] unused)."""
+    lib = _load()
+    n = len(b)
+    a, b, c, d = (np.ascontiguousarray(v, np.float64) for v in (a, b, c, d))
+    x = np.zeros(n)
+    lib.oo_tridiag(n, _dptr(a), _dptr(b), _dptr(c), _dptr(d), _dptr(x))
+    return x
+
+
+def have_ref():
+    return os.path.exists(REF_BIN)
+
+
+def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
+            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0):
+    """Run the reference driver; returns dict(dx, dy, h, t, p, f, fld)."""
+    if not have_ref():
+        raise RuntimeError("reference driver not built (make -C oracle ref)")
+    nh = num_harmonics(l0, m0)
+    f = np.ascontiguousarray(f, np.complex128).reshape(nh, ny, nx, np_)
+    fld = np.ascontiguousarray(fld, np.complex128).reshape(6, ny, nx)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        fout = os.path.join(td, "out.bin")
+        with open(fin, "wb") as fh:
+            np.array([l0, m0, np_, nx, ny, op, nsteps, nb, 0, 0], np.int32).tofile(fh)
+            np.array([pmax, xmin, xmax, ymin, ymax, h, small_dt, density_np, zeta],
+                     np.float64).tofile(fh)
+            f.tofile(fh)
+            fld.tofile(fh)
+        subprocess.run([REF_BIN, fin, fout], check=True, capture_output=True)
+        raw = np.fromfile(fout, np.float64)
+    hdr = raw[:4]
+    p = raw[4:4 + np_]
+    rest = raw[4 + np_:].view(np.complex128)
+    nf = nh * ny * nx * np_
+    return dict(dx=hdr[0], dy=hdr[1], h=hdr[2], t=hdr[3], p=p,
+                f=rest[:nf].reshape(nh, ny, nx, np_).copy(),
+                fld=rest[nf:].reshape(6, ny, nx).copy())

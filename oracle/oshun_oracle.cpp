@@ -1,0 +1,893 @@
+// OSHUN distribution-function update: CPU restatement used as the ORACLE.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load this library, and only as the checker
+// (or the timed CPU baseline).  The product path (oshun_amd, liboshun_hip.so)
+// never calls it.
+//
+// What it restates (reference = OSHUN-1, files cited as file:line):
+//   * the Vlasov-Maxwell right-hand side  Actions::Adv   def-Actions.cpp:73-94
+//       ElectricField::Exyz / MakeGH / G00  def-VlasovMax.cpp:76-152,346-535
+//       MagneticField::Bxyz                 def-VlasovMax.cpp:547-644
+//       Current::Jxyz                       def-VlasovMax.cpp:656-745
+//       Spatial_Advection::Axy              def-VlasovMax.cpp:757-951
+//       MaxwellEq::Ampere/Faraday           def-VlasovMax.cpp:963-1035
+//       CLEAN_LOW_PC                        def-Actions.cpp:240-253
+//   * classic RK4 Runge_Kutta_4::advance_p1 def-RK.cpp:49-62,110-145
+//   * e-e f00 collisions (Bell's conservative explicit operator, RK4 sub-cycled)
+//       EE_isotropic_conserva               def-FokkerPlanck.cpp:107-267
+//       RungeKutta4_ee / Explicit_EC_FP     def-FokkerPlanck.cpp:484-625
+//   * implicit l>=1 collisions (tridiagonal)
+//       Implicit_FLM_step                   def-FokkerPlanck.cpp:757-1035
+//       Anisotropic_Collisions/Euler_Backward def-FokkerPlanck.cpp:1049-1304
+//       TridiagonalSolve/Thomas_Tridiagonal def-Nmethods.cpp:142-207
+//   * per-step ordering                     main.cpp:79-107
+//
+// Differences from the reference that are deliberate:
+//   * periodic indexing on the interior instead of guard cells (the reference
+//     refreshes guards only after the whole step; with a stencil of width one
+//     per RK stage and four guard cells the interior is identical);
+//   * the field multipliers of Exyz/Bxyz/Axy are formed directly from the
+//     coefficient tables instead of as running products (def-VlasovMax.cpp
+//     :359-484); the rounding differs by a few ulp;
+//   * every (l,m) with l<=l0, m<=min(l,m0) is supported, including m0<2 and
+//     l0<3, which the reference's hard-wired blocks cannot run
+//     (def-Input.cpp:73).  For m0>=2, l0>=3 the set of couplings is the
+//     reference's.
+//   * the y-advection coefficient slip at def-VlasovMax.cpp:918 (divide by
+//     C1[m+1] instead of C1[l-1]) is reproduced when ref_quirks != 0.
+//
+// Layout exchanged through the C ABI (same as the reference minus guards):
+//   f   : complex<double>[nh][ny][nx][np], p fastest, harmonic order ind(l,m)
+//   fld : complex<double>[6][ny][nx]  (Ex,Ey,Ez,Bx,By,Bz)
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+typedef std::complex<double> cd;
+
+namespace {
+
+struct Cfg {
+  int l0, m0, np, nx, ny;
+  double pmin, pmax, dx, dy;
+  int ref_quirks;
+  double density_np, zeta, small_dt;
+  int nb;
+  int coll_ee, coll_flm;
+};
+
+// Harmonic index of (l,m) in the trapezoid, def-State.cpp:417-427.
+inline int hind(int l, int m, int m0) {
+  return (l < m0 + 1) ? (l * (l + 1)) / 2 + m : l * (m0 + 1) - (m0 * (m0 + 1)) / 2 + m;
+}
+
+struct State {
+  int nh, n, nxy;
+  std::vector<cd> f;    // nh * n
+  std::vector<cd> fld;  // 6 * nxy
+  void init(int nh_, int n_, int nxy_) {
+    nh = nh_; n = n_; nxy = nxy_;
+    f.assign((size_t)nh * n, cd(0.0, 0.0));
+    fld.assign((size_t)6 * nxy, cd(0.0, 0.0));
+  }
+  cd* h(int i) { return f.data() + (size_t)i * n; }
+  const cd* h(int i) const { return f.data() + (size_t)i * n; }
+  cd* F(int i) { return fld.data() + (size_t)i * nxy; }
+  const cd* F(int i) const { return fld.data() + (size_t)i * nxy; }
+};
+
+struct Oracle {
+  Cfg c;
+  int nh, np, nx, ny, n, nxy;
+  // p axes
+  std::vector<double> pd;   // Axis<double>(np,pmin,pmax)   matrices.h:860-866
+  double dpd;               // Axis<double>::dx             matrices.h:880
+  std::vector<cd> pc;       // Axis<complex>(np,p0,pN-1)    def-VlasovMax.cpp:88-90
+  cd dpc;
+  // ElectricField tables (def-VlasovMax.cpp:97-151)
+  std::vector<cd> eA1, eA2, eC2, eC4;  // (l0+1)*(m0+1)
+  std::vector<cd> eB1, eB2, eC1, eC3, Hp0;
+  std::vector<cd> invpr;
+  // MagneticField tables (def-VlasovMax.cpp:563-589)
+  std::vector<cd> bA1, bA2, bB1;
+  // Current (def-VlasovMax.cpp:664-674)
+  std::vector<cd> p3og;
+  cd Delta_p, small;
+  // Spatial_Advection (def-VlasovMax.cpp:768-826)
+  std::vector<cd> vr;
+  cd idx, idy;
+  std::vector<cd> aA1, aA2, aC2, aC4, aB1, aB2, aC1, aC3;
+  // Maxwell (def-VlasovMax.cpp:967-973)
+  cd midx, midy;
+
+  // Collision tables
+  std::vector<double> vr_c;                              // v = p/gamma on the double axis
+  std::vector<double> U4, U4m1, U2, U2m1, U1, U1m1, U3, Qn, Pn;
+  double c_kpre, kpre;
+
+  // time bookkeeping (def-RK.cpp:53, def-FokkerPlanck.cpp:572,1262)
+  double t_rk, t_ee, t_eb;
+
+  State Y;
+
+  int l2i(int l, int m) const { return l * (c.m0 + 1) + m; }
+  bool valid(int l, int m) const { return l >= 0 && l <= c.l0 && m >= 0 && m <= c.m0 && m <= l; }
+  int H(int l, int m) const { return hind(l, m, c.m0); }
+  size_t at(int ip, int ix, int iy) const { return ((size_t)iy * nx + ix) * np + ip; }
+  int xm(int ix) const { return ix == 0 ? nx - 1 : ix - 1; }
+  int xp(int ix) const { return ix == nx - 1 ? 0 : ix + 1; }
+  int ym(int iy) const { return iy == 0 ? ny - 1 : iy - 1; }
+  int yp(int iy) const { return iy == ny - 1 ? 0 : iy + 1; }
+
+  explicit Oracle(const Cfg& cfg) : c(cfg) {
+    np = c.np; nx = c.nx; ny = c.ny;
+    nh = ((c.m0 + 1) * (2 * c.l0 - c.m0 + 2)) / 2;
+    n = np * nx * ny; nxy = nx * ny;
+    Y.init(nh, n, nxy);
+    t_rk = t_ee = t_eb = 0.0;
+    build_axes();
+    build_vlasov_tables();
+    build_collision_tables();
+  }
+
+  void build_axes() {
+    // Axis(n, min, max): ax[i] = i; ax *= (max-min)/(n-1); ax += min   (matrices.h:860-866)
+    pd.resize(np);
+    double d = (c.pmax - c.pmin) / static_cast<double>(np - 1);
+    for (int i = 0; i < np; ++i) { double v = i; v *= d; v += c.pmin; pd[i] = v; }
+    dpd = pd[1] - pd[0];
+    pc.resize(np);
+    cd p0(pd[0]), pN(pd[np - 1]);
+    cd dc = (pN - p0) / static_cast<cd>(np - 1);
+    for (int i = 0; i < np; ++i) { cd v(i); v *= dc; v += p0; pc[i] = v; }
+    dpc = pc[1] - pc[0];
+  }
+
+  void build_vlasov_tables() {
+    const int L = c.l0, M = c.m0;
+    const size_t LM = (size_t)(L + 1) * (M + 1);
+    eA1.assign(LM, 0.0); eA2.assign(LM, 0.0); eC2.assign(LM, 0.0); eC4.assign(LM, 0.0);
+    eB1.assign(L + 1, 0.0); eB2.assign(L + 1, 0.0); eC1.assign(L + 1, 0.0); eC3.assign(L + 1, 0.0);
+    Hp0.assign(L + 1, 0.0);
+    cd lc, mc;
+    // def-VlasovMax.cpp:97-105
+    for (int l = 1; l <= L; ++l)
+      for (int m = 0; m <= std::min(M, l); ++m) {
+        lc = cd(l); mc = cd(m);
+        eA1[l2i(l, m)] = (lc + 1.0 - mc) * lc / ((2.0 * lc + 1.0) * (lc + 1.0));
+        eA2[l2i(l, m)] = -(lc + mc) / (2.0 * lc + 1.0);
+      }
+    eA1[l2i(0, 0)] = -1.0;
+    {
+      cd s = 0.5 / dpc;
+      for (size_t i = 0; i < LM; ++i) { eA1[i] *= s; eA2[i] *= s; }
+    }
+    // def-VlasovMax.cpp:112-117
+    for (int l = 1; l <= L; ++l) {
+      lc = cd(l);
+      eB1[l] = (-0.5 / dpc) * lc * lc / (2.0 * lc + 1.0);
+      eB2[l] = (-0.5 / dpc) * lc * (lc + 1.0) / (2.0 * lc + 1.0);
+    }
+    eB2[0] = 1.0;
+    // def-VlasovMax.cpp:122-127
+    for (int l = 1; l <= L; ++l) {
+      lc = cd(l);
+      eC1[l] = (0.25 / dpc) * lc / ((2.0 * lc + 1.0) * (lc + 1.0));
+      eC3[l] = (0.25 / dpc) / (2.0 * lc + 1.0);
+    }
+    eC1[0] = -0.25 / dpc;
+    // def-VlasovMax.cpp:132-139
+    for (int l = 2; l <= L; ++l)
+      for (int m = 2; m <= std::min(M, l); ++m) {
+        lc = cd(l); mc = cd(m);
+        eC2[l2i(l, m)] = (-0.25 / dpc) * lc * (lc - mc + 2.0) * (lc - mc + 1.0) / ((2.0 * lc + 1.0) * (lc + 1.0));
+        eC4[l2i(l, m)] = (-0.25 / dpc) * (lc + mc - 1.0) * (lc + mc) / (2.0 * lc + 1.0);
+      }
+    // def-VlasovMax.cpp:145-147
+    for (int l = 0; l <= L; ++l) Hp0[l] = 1.0 / pc[0];
+    for (int l = 1; l <= L; ++l) Hp0[l] = Hp0[l - 1] * (pc[0] / pc[1]) * (2.0 * l + 1.0) / (2.0 * l - 1);
+    for (int l = 0; l <= L; ++l) Hp0[l] *= -2.0 * dpc;
+    invpr.resize(np);
+    for (int i = 0; i < np; ++i) invpr[i] = 1.0 / pc[i];
+
+    // MagneticField, def-VlasovMax.cpp:563-589
+    bA1.assign(M + 1, 0.0); bA2.assign(LM, 0.0); bB1.assign(L + 1, 0.0);
+    cd c01(0.0, 1.0);
+    for (int m = 0; m <= M; ++m) { mc = cd(m); bA1[m] = (-1.0) * c01 * mc; }
+    for (int l = 1; l <= L; ++l)
+      for (int m = 0; m <= std::min(M, l); ++m) {
+        lc = cd(l); mc = cd(m);
+        bA2[l2i(l, m)] = (-0.5) * (lc + 1.0 - mc) * (lc + mc);
+      }
+    for (int l = 0; l <= L; ++l) { lc = cd(l); bB1[l] = (-1.0) * lc * (lc + 1.0); }
+
+    // Current, def-VlasovMax.cpp:664-674
+    p3og.resize(np);
+    for (int i = 0; i < np; ++i) {
+      cd p = pc[i];
+      p3og[i] = (p * p) * (p / std::sqrt(1.0 + p * p));
+    }
+    Delta_p = cd(dpd);
+    small = cd(pd[0]);
+    small *= small; small *= small; small *= pd[0];
+    small *= 0.2; small *= 1.0 / pd[1];
+
+    // Spatial_Advection, def-VlasovMax.cpp:768-826
+    vr.resize(np);
+    for (int i = 0; i < np; ++i) vr[i] = pc[i] / (std::sqrt(1.0 + pc[i] * pc[i]));
+    idx = cd(c.dx); idy = cd(c.dy);
+    idx = -1.0 / (2.0 * idx);
+    idy = -1.0 / (2.0 * idy);
+    aA1.assign(LM, 0.0); aA2.assign(LM, 0.0); aC2.assign(LM, 0.0); aC4.assign(LM, 0.0);
+    aB1.assign(L + 1, 0.0); aB2.assign(L + 1, 0.0); aC1.assign(L + 1, 0.0); aC3.assign(L + 1, 0.0);
+    for (int l = 0; l <= L; ++l)
+      for (int m = 0; m <= std::min(M, l); ++m) {
+        lc = cd(l); mc = cd(m);
+        aA1[l2i(l, m)] = idx * (-1.0) * (lc - mc + 1.0) / (2.0 * lc + 1.0);
+        aA2[l2i(l, m)] = idx * (-1.0) * (lc + mc) / (2.0 * lc + 1.0);
+      }
+    for (int l = 0; l <= L; ++l) {
+      lc = cd(l);
+      aB1[l] = idy * (lc + 1.0) * lc / (2.0 * lc + 1.0);
+      aB2[l] = (-1.0) * aB1[l];
+    }
+    for (int l = 0; l <= L; ++l) {
+      lc = cd(l);
+      aC1[l] = (-0.5) * idy / (2.0 * lc + 1.0);
+      aC3[l] = (-1.0) * aC1[l];
+    }
+    for (int l = 0; l <= L; ++l)
+      for (int m = 0; m <= std::min(M, l); ++m) {
+        lc = cd(l); mc = cd(m);
+        aC2[l2i(l, m)] = idy * 0.5 * (lc + 2.0 - mc) * (lc - mc + 1.0) / (2.0 * lc + 1.0);
+        aC4[l2i(l, m)] = idy * (-0.5) * (lc + mc - 1.0) * (lc + mc) / (2.0 * lc + 1.0);
+      }
+    // MaxwellEq, def-VlasovMax.cpp:967-973
+    midx = cd(c.dx); midy = cd(c.dy);
+    midx = -1.0 / (2.0 * midx);
+    midy = -1.0 / (2.0 * midy);
+  }
+
+  void build_collision_tables() {
+    // def-FokkerPlanck.cpp:135-173 and :785-803 (identical tables)
+    const double re = 2.8179402894e-13;
+    double kp = std::sqrt(4.0 * M_PI * (c.density_np) * re);
+    c_kpre = 4.0 * M_PI / 3.0 * re * kp;
+    kpre = re * kp;
+    vr_c.assign(np, 0.0);
+    for (int i = 0; i < np; ++i) { vr_c[i] = pd[i]; vr_c[i] = vr_c[i] / (std::sqrt(1.0 + vr_c[i] * vr_c[i])); }
+    U4.assign(np, 0.0); U4m1.assign(np, 0.0); U2.assign(np, 0.0); U2m1.assign(np, 0.0);
+    U1.assign(np, 0.0); U1m1.assign(np, 0.0); U3.assign(np, 0.0); Qn.assign(np, 0.0); Pn.assign(np, 0.0);
+    const std::vector<double>& v = vr_c;
+    for (int i = 1; i < np; ++i) {
+      U4[i] = 0.5 * std::pow(v[i], 4) * (v[i] - v[i - 1]);
+      U4m1[i] = 0.5 * std::pow(v[i - 1], 4) * (v[i] - v[i - 1]);
+      U2[i] = 0.5 * v[i] * v[i] * (v[i] - v[i - 1]);
+      U2m1[i] = 0.5 * v[i - 1] * v[i - 1] * (v[i] - v[i - 1]);
+      U1[i] = 0.5 * v[i] * (v[i] - v[i - 1]);
+      U1m1[i] = 0.5 * v[i - 1] * (v[i] - v[i - 1]);
+    }
+    for (int i = 0; i < np; ++i) U3[i] = std::pow(v[i], 3);
+    Qn[0] = 1.0 / ((v[0] * v[0] * v[1]) / 2.0);
+    for (int i = 1; i < np - 1; ++i) Qn[i] = 1.0 / (v[i] * v[i] * (v[i + 1] - v[i - 1]) / 2.0);
+    for (int i = 0; i < np - 1; ++i) Pn[i] = 1.0 / ((v[i + 1] - v[i]) / 2.0 * (v[i + 1] + v[i]));
+  }
+
+  // ---------------------------------------------------------------- RHS --
+  // SHarmonic::Dp along p for one (x,y) column, def-State.cpp:359-380 with
+  // Matrix3D::Dd1 (matrices.h:804-810): interior f(p-1)-f(p+1); p0 -> 0;
+  // pmax -> f(N-2)-f(N-1).
+  void dp_col(const cd* fcol, cd* out) const {
+    out[0] = 0.0;
+    for (int ip = 1; ip < np - 1; ++ip) out[ip] = fcol[ip - 1] - fcol[ip + 1];
+    out[np - 1] = fcol[np - 2] - fcol[np - 1];
+  }
+
+  // MakeGH (def-VlasovMax.cpp:496-515) for one column of harmonic l.
+  void make_gh_col(int l, const cd* fcol, cd* G, cd* Hh) const {
+    cd lc(l);
+    cd s = -2.0 * (lc + 1.0) * dpc;
+    cd cg = -(2.0 * lc + 1.0) / lc;
+    dp_col(fcol, G);
+    for (int ip = 0; ip < np; ++ip) {
+      cd invpax = invpr[ip] * s;
+      Hh[ip] = fcol[ip] * invpax;
+      Hh[ip] += G[ip];
+      G[ip] *= cg;
+      G[ip] += Hh[ip];
+    }
+    G[0] = 0.0;
+    Hh[0] = fcol[1] * Hp0[l];
+  }
+
+  // G00 (def-VlasovMax.cpp:519-535) for one column of f00.
+  void g00_col(const cd* fcol, cd* G) const {
+    cd p0p1_sq(pc[0] * pc[0] / (pc[1] * pc[1]));
+    cd inv_mp0p1_sq(1.0 / (1.0 - p0p1_sq));
+    cd g_r = -4.0 * dpc * pc[0] / (pc[1] * pc[1]);
+    dp_col(fcol, G);
+    cd f00 = (fcol[0] - fcol[1] * p0p1_sq) * inv_mp0p1_sq;
+    G[0] = (fcol[1] - f00) * g_r;
+  }
+
+  // Coupling list of ElectricField::Exyz (def-VlasovMax.cpp:346-491),
+  // one entry per (source op -> target).  field: 0=Ex, 1=Em, 2=Ep.
+  struct ETerm { int tl, tm; int field; cd coef; bool re; };
+
+  void e_targets_G(int l, int m, std::vector<ETerm>& out) const {
+    // G of source (l,m) feeds l+1 (G00 for l=0)
+    if (l + 1 > c.l0) return;
+    if (l == 0) {
+      out.push_back({1, 0, 0, eA1[l2i(0, 0)], false});
+      if (c.m0 >= 1) out.push_back({1, 1, 1, eC1[0], false});
+      return;
+    }
+    out.push_back({l + 1, m, 0, eA1[l2i(l, m)], false});
+    if (m + 1 <= c.m0) out.push_back({l + 1, m + 1, 1, eC1[l], false});
+    if (m == 1) out.push_back({l + 1, 0, 2, eB1[l], true});
+    else if (m >= 2) out.push_back({l + 1, m - 1, 2, eC2[l2i(l, m)], false});
+  }
+  void e_targets_H(int l, int m, std::vector<ETerm>& out) const {
+    // H of source (l,m), l>=1, feeds l-1
+    if (valid(l - 1, m)) out.push_back({l - 1, m, 0, eA2[l2i(l, m)], false});
+    if (m + 1 <= c.m0 && valid(l - 1, m + 1)) out.push_back({l - 1, m + 1, 1, eC3[l], false});
+    if (m == 1) out.push_back({l - 1, 0, 2, eB2[l], true});
+    else if (m >= 2) out.push_back({l - 1, m - 1, 2, eC4[l2i(l, m)], false});
+  }
+
+  void exyz(const State& Yin, State& Yh) const {
+    std::vector<cd> G(np), Hh(np);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        size_t ixy = (size_t)iy * nx + ix;
+        cd Ex = Yin.F(0)[ixy];
+        cd Ez = Yin.F(2)[ixy], Ey = Yin.F(1)[ixy];
+        cd ii(0.0, 1.0);
+        cd Em = Ez; Em *= (-1.0) * ii; Em += Ey;   // def-VlasovMax.cpp:352
+        cd Ep = Ez; Ep *= ii; Ep += Ey;            // :353
+        cd fl[3] = {Ex, Em, Ep};
+        for (int m = 0; m <= c.m0; ++m)
+          for (int l = m; l <= c.l0; ++l) {
+            const cd* fcol = Yin.h(H(l, m)) + at(0, ix, iy);
+            std::vector<ETerm> tg, th;
+            if (l == 0) g00_col(fcol, G.data());
+            else make_gh_col(l, fcol, G.data(), Hh.data());
+            if (l >= 1) e_targets_H(l, m, th);
+            e_targets_G(l, m, tg);
+            for (const ETerm& t : th) {
+              cd mult = fl[t.field] * t.coef;
+              cd* dst = Yh.h(H(t.tl, t.tm)) + at(0, ix, iy);
+              for (int ip = 0; ip < np; ++ip) {
+                cd v = Hh[ip] * mult;
+                dst[ip] += t.re ? cd(v.real(), 0.0) : v;
+              }
+            }
+            for (const ETerm& t : tg) {
+              cd mult = fl[t.field] * t.coef;
+              cd* dst = Yh.h(H(t.tl, t.tm)) + at(0, ix, iy);
+              for (int ip = 0; ip < np; ++ip) {
+                cd v = G[ip] * mult;
+                dst[ip] += t.re ? cd(v.real(), 0.0) : v;
+              }
+            }
+          }
+      }
+  }
+
+  // Current::Jxyz, def-VlasovMax.cpp:680-745
+  void jxyz(const State& Yin, State& Yh) const {
+    cd c01(0.0, 1.0);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        size_t ixy = (size_t)iy * nx + ix;
+        {
+          const cd* f = Yin.h(H(1, 0)) + at(0, ix, iy);
+          cd j = f[0] * p3og[0];
+          j += f[np - 1] * p3og[np - 1];
+          j *= 0.5;
+          for (int ip = 1; ip < np - 1; ++ip) j += f[ip] * p3og[ip];
+          j *= Delta_p;
+          j += small * (f[1] * p3og[1]);
+          j *= 4.0 * M_PI / 3.0;
+          Yh.F(0)[ixy] += j;
+        }
+        if (c.m0 >= 1) {
+          const cd* f = Yin.h(H(1, 1)) + at(0, ix, iy);
+          cd j = f[0] * p3og[0];
+          j += f[np - 1] * p3og[np - 1];
+          j *= 0.5;
+          for (int ip = 1; ip < np - 1; ++ip) j += f[ip] * p3og[ip];
+          j *= Delta_p;
+          j += small * (f[1] * p3og[1]);
+          j *= 8.0 * M_PI / 3.0;
+          cd jy = cd(j.real(), 0.0);
+          cd jz = j; jz -= jy; jz *= c01;
+          Yh.F(1)[ixy] += jy;
+          Yh.F(2)[ixy] += jz;
+        }
+      }
+  }
+
+  // minus central differences of a field that was scaled first
+  // (tmpEB *= s; tmpEB.Dx(): Matrix2D::Dd1/Dd2, matrices.h:511-531)
+  cd fdx(const cd* g, cd s, int ix, int iy) const {
+    return g[(size_t)iy * nx + xm(ix)] * s - g[(size_t)iy * nx + xp(ix)] * s;
+  }
+  cd fdy(const cd* g, cd s, int ix, int iy) const {
+    return g[(size_t)ym(iy) * nx + ix] * s - g[(size_t)yp(iy) * nx + ix] * s;
+  }
+
+  // MaxwellEq::Ampere, def-VlasovMax.cpp:1009-1035
+  void ampere(const State& Yin, State& Yh) const {
+    const cd *Bx = Yin.F(3), *By = Yin.F(4), *Bz = Yin.F(5);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        size_t k = (size_t)iy * nx + ix;
+        Yh.F(0)[k] += fdy(Bz, midy, ix, iy);
+        Yh.F(1)[k] += fdx(Bz, (-1.0) * midx, ix, iy);
+        Yh.F(2)[k] += fdy(Bx, (-1.0) * midy, ix, iy);
+        Yh.F(2)[k] += fdx(By, midx, ix, iy);
+      }
+  }
+  // MaxwellEq::Faraday, def-VlasovMax.cpp:979-1005
+  void faraday(const State& Yin, State& Yh) const {
+    const cd *Ex = Yin.F(0), *Ey = Yin.F(1), *Ez = Yin.F(2);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        size_t k = (size_t)iy * nx + ix;
+        Yh.F(3)[k] += fdy(Ez, (-1.0) * midy, ix, iy);
+        Yh.F(4)[k] += fdx(Ez, midx, ix, iy);
+        Yh.F(5)[k] += fdy(Ex, midy, ix, iy);
+        Yh.F(5)[k] += fdx(Ey, (-1.0) * midx, ix, iy);
+      }
+  }
+
+  // Spatial_Advection::Axy, def-VlasovMax.cpp:834-951.
+  struct ATerm { int tl, tm; cd coef; bool re; };
+  void axy(const State& Yin, State& Yh) const {
+    // x advection: (l,m) -> (l-1,m) with A2, (l+1,m) with A1    (:842-853)
+    for (int m = 0; m <= c.m0; ++m)
+      for (int l = m; l <= c.l0; ++l) {
+        std::vector<ATerm> t;
+        if (valid(l - 1, m)) t.push_back({l - 1, m, aA2[l2i(l, m)], false});
+        if (l + 1 <= c.l0) t.push_back({l + 1, m, aA1[l2i(l, m)], false});
+        if (t.empty()) continue;
+        const cd* src = Yin.h(H(l, m));
+        for (const ATerm& a : t) {
+          cd* dst = Yh.h(H(a.tl, a.tm));
+          for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+              const cd* fl = src + at(0, xm(ix), iy);
+              const cd* fr = src + at(0, xp(ix), iy);
+              cd* d = dst + at(0, ix, iy);
+              for (int ip = 0; ip < np; ++ip) d[ip] += (fl[ip] - fr[ip]) * (vr[ip] * a.coef);
+            }
+        }
+      }
+    if (c.m0 < 1) return;  // y advection only couples m to m+-1
+    // y advection (:857-947)
+    cd errf(1.0);  // cumulative factor of the :918 coefficient slip
+    for (int m = 0; m <= c.m0; ++m) {
+      cd err_loop(1.0);
+      for (int l = m; l <= c.l0; ++l) {
+        std::vector<ATerm> t;
+        cd q = errf;
+        if (c.ref_quirks && m >= 2 && m < c.m0 && m + 1 < c.l0) {
+          if (l >= m + 3 && l < c.l0) err_loop *= aC1[l - 1] / aC1[m + 1];
+          if (l >= m + 3) q *= err_loop;
+        }
+        if (m + 1 <= c.m0 && valid(l - 1, m + 1)) t.push_back({l - 1, m + 1, aC3[l] * q, false});
+        if (m + 1 <= c.m0 && l + 1 <= c.l0) t.push_back({l + 1, m + 1, aC1[l] * q, false});
+        if (m == 1) {
+          t.push_back({l - 1, 0, aB2[l] * q, true});
+          if (l + 1 <= c.l0) t.push_back({l + 1, 0, aB1[l] * q, true});
+        } else if (m >= 2) {
+          t.push_back({l - 1, m - 1, aC4[l2i(l, m)] * q, false});
+          if (l + 1 <= c.l0) t.push_back({l + 1, m - 1, aC2[l2i(l, m)] * q, false});
+        }
+        const cd* src = Yin.h(H(l, m));
+        for (const ATerm& a : t) {
+          if (!valid(a.tl, a.tm)) continue;
+          cd* dst = Yh.h(H(a.tl, a.tm));
+          for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) {
+              const cd* fl = src + at(0, ix, ym(iy));
+              const cd* fr = src + at(0, ix, yp(iy));
+              cd* d = dst + at(0, ix, iy);
+              for (int ip = 0; ip < np; ++ip) {
+                cd v = (fl[ip] - fr[ip]) * (vr[ip] * a.coef);
+                d[ip] += a.re ? cd(v.real(), 0.0) : v;
+              }
+            }
+        }
+      }
+      if (c.ref_quirks && m >= 2 && m < c.m0 && m + 1 < c.l0) errf *= err_loop;
+    }
+  }
+
+  // MagneticField::Bxyz, def-VlasovMax.cpp:597-644
+  void bxyz(const State& Yin, State& Yh) const {
+    if (c.m0 < 1) return;  // every coupling touches m>=1
+    cd ii(0.0, 1.0);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        size_t k = (size_t)iy * nx + ix;
+        cd Bx = Yin.F(3)[k], By = Yin.F(4)[k], Bz = Yin.F(5)[k];
+        cd Bm = By; Bm *= (-1.0) * ii; Bm += Bz;
+        cd Bp = By; Bp *= ii; Bp += Bz;
+        Bp *= 0.5;
+        for (int m = 0; m <= c.m0; ++m)
+          for (int l = std::max(m, 1); l <= c.l0; ++l) {
+            const cd* f = Yin.h(H(l, m)) + at(0, ix, iy);
+            if (m + 1 <= c.m0 && m + 1 <= l) {
+              cd* d = Yh.h(H(l, m + 1)) + at(0, ix, iy);
+              for (int ip = 0; ip < np; ++ip) d[ip] += f[ip] * Bp;
+            }
+            if (m >= 1) {
+              cd mx = Bx * bA1[m];
+              cd* d = Yh.h(H(l, m)) + at(0, ix, iy);
+              for (int ip = 0; ip < np; ++ip) d[ip] += f[ip] * mx;
+              if (m == 1) {
+                cd mm = Bm * bB1[l];
+                cd* d0 = Yh.h(H(l, 0)) + at(0, ix, iy);
+                for (int ip = 0; ip < np; ++ip) d0[ip] += cd((f[ip] * mm).real(), 0.0);
+              } else {
+                cd mm = Bm * bA2[l2i(l, m)];
+                cd* d1 = Yh.h(H(l, m - 1)) + at(0, ix, iy);
+                for (int ip = 0; ip < np; ++ip) d1[ip] += f[ip] * mm;
+              }
+            }
+          }
+      }
+  }
+
+  // CLEAN_LOW_PC, def-Actions.cpp:240-253
+  void clean_low_pc(State& Yh) const {
+    for (int l = 1; l <= c.l0; ++l)
+      for (int m = 0; m <= std::min(c.m0, l); ++m) {
+        int lim = std::min(l - 1, 80);
+        cd* d = Yh.h(H(l, m));
+        for (int iy = 0; iy < ny; ++iy)
+          for (int ix = 0; ix < nx; ++ix)
+            for (int ip = 0; ip < lim; ++ip) d[at(ip, ix, iy)] = 0.0;
+      }
+  }
+
+  // Actions::Adv, def-Actions.cpp:73-94
+  void adv(const State& Yin, State& Yh) const {
+    std::fill(Yh.f.begin(), Yh.f.end(), cd(0.0, 0.0));
+    std::fill(Yh.fld.begin(), Yh.fld.end(), cd(0.0, 0.0));
+    exyz(Yin, Yh);
+    jxyz(Yin, Yh);
+    ampere(Yin, Yh);
+    axy(Yin, Yh);
+    bxyz(Yin, Yh);
+    faraday(Yin, Yh);
+    clean_low_pc(Yh);
+  }
+
+  // ----------------------------------------------------------- RK4 ------
+  static void axpy_state(State& y, const State& x) {  // y += x
+    for (size_t i = 0; i < y.f.size(); ++i) y.f[i] += x.f[i];
+    for (size_t i = 0; i < y.fld.size(); ++i) y.fld[i] += x.fld[i];
+  }
+  static void scale_state(State& y, double s) {
+    cd cs(s);
+    for (auto& v : y.f) v *= cs;
+    for (auto& v : y.fld) v *= cs;
+  }
+
+  // Runge_Kutta_4::advance_p1, def-RK.cpp:110-145
+  void rk4(double h) {
+    State Y0 = Y, Y1 = Y, Yh;
+    Yh.init(nh, n, nxy);
+    adv(Y1, Yh);
+    scale_state(Yh, 0.5 * h); axpy_state(Y1, Yh);
+    scale_state(Yh, 1.0 / 3.0); axpy_state(Y, Yh);
+
+    adv(Y1, Yh); Y1 = Y0;
+    scale_state(Yh, 0.5 * h); axpy_state(Y1, Yh);
+    scale_state(Yh, 2.0 / 3.0); axpy_state(Y, Yh);
+
+    adv(Y1, Yh);
+    scale_state(Yh, h); axpy_state(Y0, Yh);
+    scale_state(Yh, 1.0 / 3.0); axpy_state(Y, Yh);
+
+    adv(Y0, Yh);
+    scale_state(Yh, h / 6.0); axpy_state(Y, Yh);
+    t_rk += h;
+  }
+
+  // ------------------------------------------------------ collisions ----
+  // LOGee / ZLOGei (NRL), def-FokkerPlanck.cpp:41-95
+  double LOGee(double ne, double Te) const {
+    if (ne > 0.000000001) {
+      Te /= (3.0 * ne);
+      Te *= 511000;
+      ne *= c.density_np;
+      Te = std::log(Te);
+      ne = std::log(ne);
+      double lnee = 23.5 - 0.5 * ne + 1.25 * Te - std::sqrt(0.00001 + 0.0625 * (Te - 2.0) * (Te - 2.0));
+      if (lnee > 2.0) return lnee;
+    }
+    return 2.0;
+  }
+  double ZLOGei(double ne, double Te) const {
+    if (ne > 0.0000001) {
+      Te /= (3.0 * ne);
+      Te *= 511000;
+      ne *= c.density_np;
+      Te = std::log(Te);
+      ne = std::log(ne);
+      double lnei = 24.0 - 0.5 * ne + Te;
+      if (lnei > 2.0) return lnei * c.zeta;
+    }
+    return 2.0 * c.zeta;
+  }
+
+  // EE_isotropic_conserva::G, def-FokkerPlanck.cpp:178-188
+  double Gpar(int n_, const double* fin, const double* J1) const {
+    const std::vector<double>& v = vr_c;
+    double f00 = (fin[0] - fin[1] * (v[0] * v[0]) / (v[1] * v[1])) / (1.0 - (v[0] * v[0]) / (v[1] * v[1]));
+    double i2s = f00 * std::pow(v[n_], 3) / 3.0 + (fin[1] - f00) * std::pow(v[n_], 5) / (v[1] * v[1]) * 0.2;
+    double i4s = f00 * std::pow(v[n_], 5) * 0.2 + (fin[1] - f00) * std::pow(v[n_], 7) / (v[1] * v[1] * 7.0);
+    return fin[n_] * i4s + (std::pow(v[n_], 3) * fin[n_] - 3.0 * i2s) * J1[n_];
+  }
+
+  // EE_isotropic_conserva::Getslope, def-FokkerPlanck.cpp:192-267
+  void getslope(const double* fin, double* fh) const {
+    const int N = np;
+    std::vector<double> I4(N), I2(N), J1(N);
+    I4[0] = 0;
+    for (int k = 1; k < N; ++k) { I4[k] = U4[k] * fin[k] + U4m1[k] * fin[k - 1]; I4[k] += I4[k - 1]; }
+    I2[0] = 0;
+    for (int k = 1; k < N; ++k) { I2[k] = U2[k] * fin[k] + U2m1[k] * fin[k - 1]; I2[k] += I2[k - 1]; }
+    J1[N - 1] = 0;
+    for (int k = N - 2; k > -1; --k) { J1[k] = U1[k + 1] * fin[k + 1] + U1m1[k + 1] * fin[k]; J1[k] += J1[k + 1]; }
+    double Ln_ee = LOGee(4.0 * M_PI * I2[N - 1], 4.0 * M_PI * I4[N - 1]);
+    for (int k = 0; k < N; ++k) {
+      I2[k] *= J1[k];
+      I2[k] *= -3.0;
+      I4[k] += U3[k] * J1[k];
+      I4[k] *= fin[k];
+      I4[k] += I2[k];
+    }
+    for (int k = 0; k < c.nb && k < N; ++k) I4[k] = Gpar(k, fin, J1.data());
+    fh[0] = (-1.0) * I4[0];
+    for (int k = 0; k < N - 1; ++k) I4[k] -= I4[k + 1];
+    fh[0] *= 2.0 / (vr_c[0] * vr_c[0]);
+    for (int k = 0; k < N - 1; ++k) I4[k] *= Pn[k];
+    fh[0] -= I4[0];
+    for (int k = 0; k < N - 1; ++k) I4[k] -= I4[k + 1];
+    fh[0] *= Qn[0];
+    for (int k = 0; k < N - 1; ++k) fh[k + 1] = Qn[k + 1] * I4[k];
+    for (int k = 0; k < N; ++k) fh[k] *= c_kpre * Ln_ee;
+  }
+
+  // RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553
+  void rk4_ee(double* f, double h) const {
+    const int N = np;
+    std::vector<double> f0(f, f + N), f1(f, f + N), fh(N);
+    getslope(f1.data(), fh.data());
+    for (int k = 0; k < N; ++k) { fh[k] *= (0.5 * h); f1[k] += fh[k]; }
+    for (int k = 0; k < N; ++k) { fh[k] *= (1.0 / 3.0); f[k] += fh[k]; }
+    getslope(f1.data(), fh.data());
+    f1 = f0;
+    for (int k = 0; k < N; ++k) { fh[k] *= (0.5 * h); f1[k] += fh[k]; }
+    for (int k = 0; k < N; ++k) { fh[k] *= (2.0 / 3.0); f[k] += fh[k]; }
+    getslope(f1.data(), fh.data());
+    for (int k = 0; k < N; ++k) { fh[k] *= h; f0[k] += fh[k]; }
+    for (int k = 0; k < N; ++k) { fh[k] *= (1.0 / 3.0); f[k] += fh[k]; }
+    getslope(f0.data(), fh.data());
+    for (int k = 0; k < N; ++k) { fh[k] *= (h / 6.0); f[k] += fh[k]; }
+  }
+
+  // Explicit_EC_FP::loop, def-FokkerPlanck.cpp:590-625
+  void collide_ee(double tnew) {
+    int numh = static_cast<int>(static_cast<size_t>(static_cast<int>((tnew - t_ee) / c.small_dt)) + 1);
+    double h = (tnew - t_ee) / static_cast<double>(static_cast<size_t>(numh));
+    std::vector<double> fc(np);
+    cd* f00 = Y.h(0);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        cd* col = f00 + at(0, ix, iy);
+        for (int ip = 0; ip < np; ++ip) fc[ip] = col[ip].real();
+        for (int s = 0; s < numh; ++s) rk4_ee(fc.data(), h);
+        for (int ip = 0; ip < np; ++ip) col[ip] = fc[ip];
+      }
+    t_ee = tnew;
+  }
+
+  // Implicit_FLM_step::reset_coeff, def-FokkerPlanck.cpp:808-939 (tridiagonal part)
+  struct Coef {
+    std::vector<double> sub, diag, sup, scatt;  // Alpha_Tri bands (already * -lnee kpre Dt), scattering term
+  };
+  void reset_coeff(const double* fin, double Dt, Coef& k) const {
+    const int N = np;
+    const std::vector<double>& v = vr_c;
+    std::vector<double> I2(N), I0(N), J1m(N);
+    I2[0] = 0;
+    for (int q = 1; q < N; ++q) { I2[q] = U4[q] * fin[q] + U4m1[q] * fin[q - 1]; I2[q] += I2[q - 1]; }
+    for (int q = 0; q < N; ++q) I2[q] *= 4.0 * M_PI;
+    double I2_temperature = I2[N - 1];
+    for (int q = 0; q < N; ++q) I2[q] /= v[q] * v[q];
+    I0[0] = 0;
+    for (int q = 1; q < N; ++q) { I0[q] = U2[q] * fin[q] + U2m1[q] * fin[q - 1]; I0[q] += I0[q - 1]; }
+    for (int q = 0; q < N; ++q) I0[q] *= 4.0 * M_PI;
+    double I0_density = I0[N - 1];
+    J1m[N - 1] = 0;
+    for (int q = N - 2; q > -1; --q) { J1m[q] = U1[q + 1] * fin[q + 1] + U1m1[q + 1] * fin[q]; J1m[q] += J1m[q + 1]; }
+    for (int q = 0; q < N; ++q) J1m[q] *= 4.0 * M_PI * v[q];
+    double lnee = LOGee(I0_density, I2_temperature);
+    double zlnei = ZLOGei(I0_density, I2_temperature);
+    std::vector<double> T1(N), T2(N);
+    for (int i = 0; i < N; ++i) {
+      T1[i] = I0[i] + (2.0 * J1m[i] - I2[i]) / 3.0;
+      T2[i] = (I2[i] + J1m[i]) / 3.0;
+    }
+    k.scatt.assign(N, 0.0);
+    for (int i = 0; i < N; ++i) k.scatt[i] = T1[i] * lnee;
+    k.scatt[0] = 0.0;
+    double zi = zlnei * I0_density;
+    for (int i = 0; i < N; ++i) k.scatt[i] += zi;
+    for (int i = 0; i < N; ++i) k.scatt[i] /= std::pow(v[i], 3);
+    double kd = kpre * Dt;
+    for (int i = 0; i < N; ++i) k.scatt[i] *= kd;
+    k.sub.assign(N, 0.0); k.diag.assign(N, 0.0); k.sup.assign(N, 0.0);
+    k.diag[0] = 8.0 * M_PI * fin[0];
+    for (int i = 1; i < N - 1; ++i) {
+      double IvDnDm1 = 1.0 / (v[i] * 0.5 * (v[i + 1] - v[i - 1]) * (v[i] - v[i - 1]));
+      double IvDnDp1 = 1.0 / (v[i] * 0.5 * (v[i + 1] - v[i - 1]) * (v[i + 1] - v[i]));
+      double Ivsq2Dn = 1.0 / (v[i] * v[i] * (v[i + 1] - v[i - 1]));
+      k.diag[i] = 8.0 * M_PI * fin[i] - T2[i] * (IvDnDm1 + IvDnDp1);
+      k.sub[i] = T2[i] * IvDnDm1 - T1[i] * Ivsq2Dn;
+      k.sup[i] = T2[i] * IvDnDp1 + T1[i] * Ivsq2Dn;
+    }
+    double s = (-1.0) * lnee * kpre * Dt;
+    for (int i = 0; i < N; ++i) { k.sub[i] *= s; k.diag[i] *= s; k.sup[i] *= s; }
+  }
+
+  // Implicit_FLM_step::advance (tridiagonal) + TridiagonalSolve,
+  // def-FokkerPlanck.cpp:943-1035, def-Nmethods.cpp:142-207
+  void advance_flm(const Coef& k, cd* col, int el) const {
+    const int N = np;
+    std::vector<double> a(N, 0.0), b(N), cc(N, 0.0);
+    for (int i = 1; i < N; ++i) a[i] = k.sub[i];
+    for (int i = 0; i < N; ++i) b[i] = k.diag[i];
+    for (int i = 0; i < N - 1; ++i) cc[i] = k.sup[i];
+    if (el > 1) b[0] = 0.0;
+    double ll1 = static_cast<double>(el);
+    ll1 *= (-0.5) * (ll1 + 1.0);
+    for (int i = 0; i < N; ++i) b[i] += 1.0 - ll1 * k.scatt[i];
+    std::vector<cd> d(col, col + N), x(N);
+    cc[0] /= b[0];
+    d[0] /= b[0];
+    for (int i = 1; i < N; ++i) {
+      double id = 1.0 / (b[i] - cc[i - 1] * a[i]);
+      cc[i] *= id;
+      d[i] -= d[i - 1] * a[i];
+      d[i] *= id;
+    }
+    x[N - 1] = d[N - 1];
+    for (int i = N - 2; i > -1; --i) { x[i] = d[i]; x[i] -= cc[i] * x[i + 1]; }
+    for (int i = 0; i < N; ++i) col[i] = x[i];
+  }
+
+  // Anisotropic_Collisions::f1_loop / flm_loop, def-FokkerPlanck.cpp:1112-1244
+  // which = 1: l = 1 (m = 0,1);  which = 2: l = 2..l0.
+  void collide_flm(double Dt, int which) {
+    std::vector<double> f00(np);
+    Coef k;
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const cd* c0 = Y.h(0) + at(0, ix, iy);
+        for (int ip = 0; ip < np; ++ip) f00[ip] = c0[ip].real();
+        reset_coeff(f00.data(), Dt, k);
+        if (which == 1) {
+          for (int m = 0; m <= std::min(1, c.m0); ++m) advance_flm(k, Y.h(H(1, m)) + at(0, ix, iy), 1);
+        } else {
+          for (int l = 2; l <= c.l0; ++l)
+            for (int m = 0; m <= std::min(c.m0, l); ++m) advance_flm(k, Y.h(H(l, m)) + at(0, ix, iy), l);
+        }
+      }
+  }
+
+  // main_loop step body, main.cpp:79-107
+  void step(double h) {
+    rk4(h);
+    double t = t_rk;
+    if (c.coll_ee) collide_ee(t); else t_ee = t;
+    if (c.coll_flm) {
+      if (c.l0 >= 1) collide_flm(t - t_eb, 1);   // Euler_Backward::advance_1
+      collide_flm(t - t_eb, 2);                  // Euler_Backward::advance_lm
+    }
+    t_eb = t;
+  }
+};
+
+void put_state(State& s, const double* f, const double* fld) {
+  if (f) std::memcpy((void*)s.f.data(), f, s.f.size() * sizeof(cd));
+  if (fld) std::memcpy((void*)s.fld.data(), fld, s.fld.size() * sizeof(cd));
+}
+void get_state(const State& s, double* f, double* fld) {
+  if (f) std::memcpy(f, s.f.data(), s.f.size() * sizeof(cd));
+  if (fld) std::memcpy(fld, s.fld.data(), s.fld.size() * sizeof(cd));
+}
+
+}  // namespace
+
+extern "C" {
+
+// config: ints {l0,m0,np,nx,ny,ref_quirks,nb,coll_ee,coll_flm}
+//         doubles {pmin,pmax,dx,dy,density_np,zeta,small_dt}
+void* oo_create(const int* iv, const double* dv) {
+  Cfg c;
+  c.l0 = iv[0]; c.m0 = iv[1]; c.np = iv[2]; c.nx = iv[3]; c.ny = iv[4];
+  c.ref_quirks = iv[5]; c.nb = iv[6]; c.coll_ee = iv[7]; c.coll_flm = iv[8];
+  c.pmin = dv[0]; c.pmax = dv[1]; c.dx = dv[2]; c.dy = dv[3];
+  c.density_np = dv[4]; c.zeta = dv[5]; c.small_dt = dv[6];
+  if (c.l0 < 1 || c.m0 < 0 || c.m0 > c.l0 || c.np < 4 || c.nx < 3 || c.ny < 1) return nullptr;
+  return new Oracle(c);
+}
+void oo_destroy(void* o) { delete static_cast<Oracle*>(o); }
+int oo_num_harmonics(void* o) { return static_cast<Oracle*>(o)->nh; }
+void oo_set_state(void* o, const double* f, const double* fld) { put_state(static_cast<Oracle*>(o)->Y, f, fld); }
+void oo_get_state(void* o, double* f, double* fld) { get_state(static_cast<Oracle*>(o)->Y, f, fld); }
+void oo_set_time(void* o, double t) {
+  Oracle* q = static_cast<Oracle*>(o);
+  q->t_rk = q->t_ee = q->t_eb = t;
+}
+double oo_time(void* o) { return static_cast<Oracle*>(o)->t_rk; }
+
+// Actions::Adv on the current state; writes the slope.
+void oo_rhs(void* o, double* f_out, double* fld_out) {
+  Oracle* q = static_cast<Oracle*>(o);
+  State Yh; Yh.init(q->nh, q->n, q->nxy);
+  q->adv(q->Y, Yh);
+  get_state(Yh, f_out, fld_out);
+}
+void oo_rk4(void* o, double h) { static_cast<Oracle*>(o)->rk4(h); }
+void oo_collide_ee(void* o, double tnew) { static_cast<Oracle*>(o)->collide_ee(tnew); }
+void oo_collide_flm(void* o, double tnew, int which) {
+  Oracle* q = static_cast<Oracle*>(o);
+  q->collide_flm(tnew - q->t_eb, which);
+  if (which == 2) q->t_eb = tnew;
+}
+void oo_step(void* o, double h, int nsteps) {
+  Oracle* q = static_cast<Oracle*>(o);
+  for (int s = 0; s < nsteps; ++s) q->step(h);
+}
+// Single-column pieces for closed-form tests.
+void oo_ee_getslope(void* o, const double* fin, double* fh) { static_cast<Oracle*>(o)->getslope(fin, fh); }
+void oo_flm_solve(void* o, const double* f00, double Dt, int el, double* col_re_im) {
+  Oracle* q = static_cast<Oracle*>(o);
+  Oracle::Coef k;
+  q->reset_coeff(f00, Dt, k);
+  q->advance_flm(k, reinterpret_cast<cd*>(col_re_im), el);
+}
+void oo_flm_coeffs(void* o, const double* f00, double Dt, double* sub, double* diag, double* sup, double* scatt) {
+  Oracle* q = static_cast<Oracle*>(o);
+  Oracle::Coef k;
+  q->reset_coeff(f00, Dt, k);
+  for (int i = 0; i < q->np; ++i) { sub[i] = k.sub[i]; diag[i] = k.diag[i]; sup[i] = k.sup[i]; scatt[i] = k.scatt[i]; }
+}
+void oo_tridiag(int n, const double* a, const double* b, const double* c, const double* d, double* x) {
+  std::vector<double> cc(c, c + n), dd(d, d + n);
+  cc[0] /= b[0]; dd[0] /= b[0];
+  for (int i = 1; i < n; ++i) {
+    double id = 1.0 / (b[i] - cc[i - 1] * a[i]);
+    cc[i] *= id; dd[i] -= dd[i - 1] * a[i]; dd[i] *= id;
+  }
+  x[n - 1] = dd[n - 1];
+  for (int i = n - 2; i > -1; --i) x[i] = dd[i] - cc[i] * x[i + 1];
+}
+// Axes used by the restatement: pd (double axis), pc (complex axis, real part).
+void oo_axes(void* o, double* pd, double* pc, double* vr_c) {
+  Oracle* q = static_cast<Oracle*>(o);
+  for (int i = 0; i < q->np; ++i) { pd[i] = q->pd[i]; pc[i] = q->pc[i].real(); vr_c[i] = q->vr_c[i]; }
+}
+
+}  // extern "C"

@@ -1,0 +1,164 @@
+"""Write the golden fixtures of the reference's hot path.
+
+Runs oracle/_ref/oshun_ref -- a driver linked with the unmodified reference
+translation units (see oracle/Makefile) -- on small seeded states and stores
+inputs and outputs as .npz files next to this script.  Run from the repo root:
+
+    make -C oracle all ref && python tests/golden/make_golden.py
+
+Each fixture holds: the integer/float configuration, the interior input state
+(f: complex128[nh,ny,nx,np], fld: complex128[6,ny,nx]), the axis and cell sizes
+the reference actually used, and the interior output state.
+
+Cases (the reference needs l0 >= 3 and m0 >= 2, def-Input.cpp:73):
+  rhs_2d_m3      Actions::Adv slope, l0=4 m0=3, 2D, random complex state
+  rhs_quirk_l7   Actions::Adv slope, l0=7 m0=3 (exercises def-VlasovMax.cpp:918)
+  step_1d_m0     3 full steps (RK4 + e-e + f1 + flm), l0=8 m0=0 physics run
+                 through the reference as an m0=2, y-uniform stand-in with all
+                 m>=1 harmonics and Ey, Ez, B zero; only m=0 is stored
+  step_2d_c3     2 full steps, l0=4 m0=4, 2D, uniform Bz (a small C3 shape)
+  rk4_1d_nocoll  4 RK4 steps, collisions off (density 1e-30), l0=3 m0=2
+  ee_only        3 e-e calls (Explicit_EC_FP::loop)
+  flm_only       3 implicit f_lm calls (Euler_Backward::advance_1/advance_lm)
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle.oracle import hind, num_harmonics, run_ref  # noqa: E402
+
+
+def p_axis(np_, pmax):
+    pmin = pmax / (2.0 * np_)
+    d = (pmax - pmin) / float(np_ - 1)
+    return np.arange(np_) * d + pmin
+
+
+def maxwellian_state(l0, m0, np_, nx, ny, pmax, seed, amp=1e-3, bz=0.0, eyz=True):
+    """Perturbed Maxwellian f00 (def-Setup.cpp:498-520 formula) + small f_lm."""
+    rng = np.random.default_rng(seed)
+    nh = num_harmonics(l0, m0)
+    p = p_axis(np_, pmax)
+    x = np.arange(nx) / nx
+    y = np.arange(ny) / max(ny, 1)
+    T = (1.0 + 0.1 * np.cos(2 * np.pi * x))[None, :] * (1.0 + 0.05 * np.sin(2 * np.pi * y))[:, None]
+    pt = np.sqrt(T / 511.0)
+    n = (1.0 + 0.05 * np.sin(2 * np.pi * x))[None, :] * np.ones((ny, 1))
+    f = np.zeros((nh, ny, nx, np_), np.complex128)
+    f[0] = (n / (np.sqrt(2 * np.pi) * 2 * np.pi * pt ** 3))[..., None] * \
+        np.exp(-p[None, None, :] ** 2 / (2 * pt[..., None] ** 2))
+    for l in range(1, l0 + 1):
+        for m in range(min(l, m0) + 1):
+            r = rng.uniform(-1, 1, (ny, nx, np_))
+            if m > 0:
+                r = r + 1j * rng.uniform(-1, 1, (ny, nx, np_))
+            f[hind(l, m, m0)] = amp * r * f[0].real
+    fld = np.zeros((6, ny, nx), np.complex128)
+    fld[0] = 1e-4 * np.sin(2 * np.pi * x)[None, :] * np.ones((ny, 1))
+    if eyz:
+        fld[1] = 2e-5 * np.cos(2 * np.pi * x)[None, :]
+        fld[2] = -1e-5 * np.sin(2 * np.pi * y)[:, None]
+    fld[5] = bz
+    return f, fld
+
+
+def random_state(l0, m0, np_, nx, ny, seed):
+    rng = np.random.default_rng(seed)
+    nh = num_harmonics(l0, m0)
+    f = rng.uniform(-1, 1, (nh, ny, nx, np_)) + 1j * rng.uniform(-1, 1, (nh, ny, nx, np_))
+    for l in range(l0 + 1):
+        f[hind(l, 0, m0)] = f[hind(l, 0, m0)].real
+    fld = rng.uniform(-1, 1, (6, ny, nx)) + 0j
+    return f, fld
+
+
+def save(name, cfg, fin, fldin, res, keep_m0_only=False, m0_phys=None):
+    l0, m0 = cfg["l0"], cfg["m0"]
+    fout = res["f"]
+    if keep_m0_only:
+        idx = [hind(l, 0, m0) for l in range(l0 + 1)]
+        fin = fin[idx].real.copy()
+        fout = fout[idx].real.copy()
+        fldin = fldin[0:1].real.copy()
+        fldout = res["fld"][0:1].real.copy()
+        cfg = dict(cfg, m0=m0_phys, standin_m0=m0)
+    else:
+        fldout = res["fld"]
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, f_in=fin, fld_in=fldin, f_out=fout, fld_out=fldout, p=res["p"],
+                        dx=res["dx"], dy=res["dy"], h=res["h"], t=res["t"],
+                        **{"cfg_" + k: v for k, v in cfg.items()})
+    print(f"{name}: {os.path.getsize(path)} bytes")
+
+
+def main():
+    # 1) RHS, 2D complex
+    c = dict(l0=4, m0=3, np=16, nx=6, ny=4, pmax=0.3, xmin=-2.0, xmax=3.0, ymin=-1.0, ymax=2.0,
+             op=0, nsteps=0, nb=4, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0)
+    f, fld = random_state(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], 1234)
+    r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], f, fld, op=0, pmax=c["pmax"],
+                xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"])
+    save("rhs_2d_m3", c, f, fld, r)
+
+    # 2) RHS with the y-advection coefficient slip
+    c = dict(c, l0=7, m0=3, np=10, nx=5, ny=4)
+    f, fld = random_state(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], 99)
+    r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], f, fld, op=0, pmax=c["pmax"],
+                xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"])
+    save("rhs_quirk_l7", c, f, fld, r)
+
+    # 3) full steps, 1D m0=0 physics through the m0=2 stand-in
+    c = dict(l0=8, m0=2, np=32, nx=12, ny=1, pmax=0.25, xmin=0.0, xmax=100.0, ymin=0.0, ymax=1.0,
+             op=1, nsteps=3, nb=4, h=0.5, small_dt=0.2, density_np=1e21, zeta=1.0)
+    f, fld = maxwellian_state(c["l0"], c["m0"], c["np"], c["nx"], 1, c["pmax"], 7, eyz=False)
+    for l in range(c["l0"] + 1):
+        for m in range(1, min(l, c["m0"]) + 1):
+            f[hind(l, m, c["m0"])] = 0.0
+    fld[1:] = 0.0
+    r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], 1, f, fld, op=1, nsteps=c["nsteps"],
+                pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"],
+                h=c["h"], small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"])
+    # the stand-in's m>=1 harmonics and Ey, Ez, B must stay exactly zero
+    for l in range(c["l0"] + 1):
+        for m in range(1, min(l, c["m0"]) + 1):
+            assert not np.any(r["f"][hind(l, m, c["m0"])]), (l, m)
+    assert not np.any(r["fld"][1:])
+    save("step_1d_m0", c, f, fld, r, keep_m0_only=True, m0_phys=0)
+
+    # 4) full steps, 2D, l0=m0=4, uniform Bz, e-e and e-i on
+    c = dict(l0=4, m0=4, np=12, nx=6, ny=6, pmax=0.25, xmin=0.0, xmax=60.0, ymin=0.0, ymax=60.0,
+             op=1, nsteps=2, nb=4, h=0.4, small_dt=0.15, density_np=1e22, zeta=2.0)
+    f, fld = maxwellian_state(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], c["pmax"], 11, bz=2e-3)
+    r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], f, fld, op=1, nsteps=c["nsteps"],
+                pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"],
+                h=c["h"], small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"],
+                zeta=c["zeta"])
+    save("step_2d_c3", c, f, fld, r)
+
+    # 5) RK4 only, collisions off
+    c = dict(l0=3, m0=2, np=24, nx=10, ny=1, pmax=0.3, xmin=-50.0, xmax=50.0, ymin=0.0, ymax=1.0,
+             op=2, nsteps=4, nb=4, h=0.3, small_dt=1.0, density_np=1e-30, zeta=1.0)
+    f, fld = maxwellian_state(c["l0"], c["m0"], c["np"], c["nx"], 1, c["pmax"], 5)
+    r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], 1, f, fld, op=2, nsteps=c["nsteps"],
+                pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"],
+                h=c["h"], density_np=c["density_np"])
+    save("rk4_1d_nocoll", c, f, fld, r)
+
+    # 6) e-e only and 7) f_lm only, Np=64
+    for name, op in (("ee_only", 3), ("flm_only", 4)):
+        c = dict(l0=3, m0=2, np=64, nx=4, ny=1, pmax=0.3, xmin=0.0, xmax=10.0, ymin=0.0, ymax=1.0,
+                 op=op, nsteps=3, nb=6, h=0.7, small_dt=0.25, density_np=1e23, zeta=1.0)
+        f, fld = maxwellian_state(c["l0"], c["m0"], c["np"], c["nx"], 1, c["pmax"], 3 + op)
+        r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], 1, f, fld, op=op, nsteps=c["nsteps"],
+                    pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"],
+                    ymax=c["ymax"], h=c["h"], small_dt=c["small_dt"],
+                    density_np=c["density_np"], nb=c["nb"])
+        save(name, c, f, fld, r)
+
+
+if __name__ == "__main__":
+    main()
