@@ -1,0 +1,202 @@
+"""Benchmark: phase-space cell-updates/s per RK4 + Fokker-Planck step on one MI355X.
+
+Workload (the target shape of the distribution-function update): 1D periodic,
+l0 = 8, m0 = 0 (9 real harmonics), Np = 256 to pmax = 0.3 m_e c, Nx = 4096,
+kλ_D = 0.3 Langmuir perturbation over 16 wavelengths, Maxwellian T = 1 keV,
+collisions on (e-e f00 + implicit l >= 1, n = 1e21 cm^-3, Z = 1), h = 0.01/ω_p,
+e-e sub-step = h.  A step = RK4 (4 Vlasov-Maxwell stages) + e-e + f1 + f_lm,
+i.e. main.cpp:79-107 of the reference.  Synthetic initial data, resident in HBM
+before timing.
+
+value      = N_h*Np*Nx / t_step for the plain FP64 path (dtype f64)
+fp32_split = the same for FP32 f storage + FP32 Vlasov RHS with FP64 fields and
+             collisions
+roofline   = dominant kernel (vlasov_stage): algorithmic bytes per launch
+             (16 S_f over the 4 stage launches, S_f = stored f bytes) / mean
+             launch duration from HIP events on the library's stream
+cpu_baseline = the reference (oracle/_ref, built from the reference sources)
+             on a bounded sample of the same workload, 1 core, or the oracle's
+             CPU restatement when that driver is absent
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+L0, M0, NP, NX = 8, 0, 256, 4096
+PMAX = 0.3
+H = 0.01
+DENSITY = 1e21
+NB = 6
+
+
+def target_state(nx=NX, l0=L0, np_=NP, seed=1234):
+    pt = np.sqrt(1.0 / 511.0)                 # T = 1 keV
+    lam_d = pt                                # c/ω_p units
+    k = 0.3 / lam_d
+    L = 16 * 2 * np.pi / k
+    dx = L / nx
+    pmin = PMAX / (2.0 * np_)
+    p = np.arange(np_) * ((PMAX - pmin) / (np_ - 1)) + pmin
+    x = np.arange(nx) * dx
+    n = 1.0 + 1e-3 * np.sin(k * x)
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f00 = n[:, None] * np.exp(-p[None, :] ** 2 / (2 * pt ** 2)) / (np.sqrt(2 * np.pi) * 2 * np.pi * pt ** 3)
+    f[0, 0] = f00
+    rng = np.random.default_rng(seed)
+    for l in range(1, l0 + 1):
+        f[l, 0] = 1e-6 * rng.uniform(-1, 1, (nx, np_)) * f00
+    fld = np.zeros((6, 1, nx), np.complex128)
+    return f, fld, dx
+
+
+def make_ctx(precision, dx):
+    import oshun_amd
+    return oshun_amd.Context(L0, M0, NP, NX, 1, pmax=PMAX, dx=dx, precision=precision, density_np=DENSITY,
+                             zeta=1.0, small_dt=H, nb_algorithms=NB)
+
+
+def run_path(precision, steps, warmup, f, fld, dx):
+    ctx = make_ctx(precision, dx)
+    ctx.upload(f, fld)
+    ctx.step(H, warmup)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(H, steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    # per-kernel durations: HIP events on the library's stream over `steps` more steps
+    ctx.profile(True)
+    ctx.step(H, steps)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    fo, _ = ctx.download()
+    if not np.all(np.isfinite(fo.real)):
+        raise RuntimeError("non-finite state after the timed steps")
+    return ctx, ms, prof
+
+
+def cpu_baseline(budget_s=20.0):
+    """The reference on a bounded sample of the workload (same l0, Np, h, collisions; Nx = 64)."""
+    from oracle import oracle as orc
+    nx = 64
+    f, fld, dx = target_state(nx=nx)
+    xmin, xmax = 0.0, dx * (nx - 1)
+    if orc.have_ref():
+        # the reference needs m0 >= 2: run the m0 = 0 physics as a y-uniform m0 = 2 stand-in
+        nh2 = orc.num_harmonics(L0, 2)
+        f2 = np.zeros((nh2, 1, nx, NP), np.complex128)
+        for l in range(L0 + 1):
+            f2[orc.hind(l, 0, 2)] = f[l]
+        nsteps = 1
+        t0 = time.perf_counter()
+        orc.run_ref(L0, 2, NP, nx, 1, f2, fld, op=1, nsteps=nsteps, nb=NB, pmax=PMAX, xmin=xmin, xmax=xmax,
+                    h=H, small_dt=H, density_np=DENSITY)
+        t1 = time.perf_counter()
+        setup = t1 - t0
+        nsteps = max(1, min(6, int(budget_s / max(setup, 1e-3))))
+        t0 = time.perf_counter()
+        orc.run_ref(L0, 2, NP, nx, 1, f2, fld, op=1, nsteps=nsteps + 1, nb=NB, pmax=PMAX, xmin=xmin,
+                    xmax=xmax, h=H, small_dt=H, density_np=DENSITY)
+        t1 = time.perf_counter()
+        per_step = max((t1 - t0) - setup, 1e-9) / nsteps
+        cells = (L0 + 1) * NP * nx
+        return {"value": cells / per_step, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"OSHUN-1 classes (oracle/_ref), l0=8 as m0=2 y-uniform stand-in (24 harmonics "
+                          f"stored, 9 useful), Np=256, Nx=64 (+4 guards each side, 9 y rows incl. guards), "
+                          f"{nsteps} RK4+FP steps minus a 1-step run; value counts the 9 useful harmonics",
+                "s_per_step": per_step}
+    # fall back to the oracle's restatement
+    o = orc.Oracle(orc.Config(l0=L0, m0=M0, np=NP, nx=nx, pmax=PMAX, dx=dx, density_np=DENSITY, small_dt=H,
+                              nb=NB))
+    o.set_state(f, fld)
+    t0 = time.perf_counter()
+    o.step(H, 1)
+    dt1 = time.perf_counter() - t0
+    n = max(1, min(50, int(budget_s / max(dt1, 1e-3))))
+    t0 = time.perf_counter()
+    o.step(H, n)
+    per_step = (time.perf_counter() - t0) / n
+    cells = (L0 + 1) * NP * nx
+    return {"value": cells / per_step, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle CPU restatement, l0=8 m0=0 Np=256 Nx=64, {n} steps", "s_per_step": per_step}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
+              "a single run shards along x with one halo exchange per step (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import oshun_amd
+
+    f, fld, dx = target_state()
+    cells = (L0 + 1) * NP * NX
+    ctx64, ms64, prof64 = run_path(oshun_amd.FP64, args.steps, args.warmup, f, fld, dx)
+    S64 = ctx64.state_bytes()
+    ctx32, ms32, prof32 = run_path(oshun_amd.FP32_SPLIT, args.steps, args.warmup, f, fld, dx)
+    S32 = ctx32.state_bytes()
+
+    def roof(prof, S):
+        ms, n = prof["vlasov_stage"]
+        avg_s = ms / max(n, 1) * 1e-3
+        bytes_per_launch = 16.0 * S / 4.0
+        ach = bytes_per_launch / avg_s
+        return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": ach / HBM_PEAK, "traffic": None, "kernel": "vlasov_stage",
+                "kernel_avg_us": avg_s * 1e6, "bytes_per_launch": bytes_per_launch}
+
+    def breakdown(prof, steps):
+        return {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps}
+                for k, v in prof.items() if v[1]}
+
+    r64 = roof(prof64, S64)
+    r32 = roof(prof32, S32)
+    step_frac64 = 16.0 * S64 / (ms64 * 1e-3) / HBM_PEAK
+    step_frac32 = 16.0 * S32 / (ms32 * 1e-3) / HBM_PEAK
+    cpu = None if args.no_cpu else cpu_baseline()
+    value = cells / (ms64 * 1e-3)
+    line = {
+        "metric": "phase-space cell-updates/s per RK4+FP step (FP64 vs FP32 split), % HBM roofline",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Maxwellian T=1 keV, n=1+1e-3 sin kx, f_l>=1 seeded 1e-6 noise)",
+        "config": {"workload": "1D periodic Langmuir/collisional, l0=8 m0=0 (9 harmonics), Np=256 pmax=0.3, "
+                               "Nx=4096, k lambda_D=0.3 x16 wavelengths, h=0.01, e-e+e-i collisions on "
+                               "(n=1e21 cm^-3, Z=1, small_dt=h, NB=6)",
+                   "cells": cells, "state_bytes_fp64": S64, "state_bytes_fp32": S32},
+        "roofline": r64,
+        "step_roofline_frac": step_frac64,
+        "fp32_split": {"value": cells / (ms32 * 1e-3), "ms_per_step": ms32, "dtype": "f32 f / f64 fields+collisions",
+                       "speedup_vs_fp64": ms64 / ms32, "roofline": r32, "step_roofline_frac": step_frac32},
+        "kernels_fp64": breakdown(prof64, args.steps),
+        "kernels_fp32_split": breakdown(prof32, args.steps),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

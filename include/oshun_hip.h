@@ -1,0 +1,120 @@
+/*
+ * oshun_hip.h -- C ABI of the MI355X (gfx950) distribution-function update.
+ *
+ * One context owns the device-resident state of one OSHUN run (f_lm and the
+ * six field components), its coefficient tables and one HIP stream.  Calls on
+ * one context are serialised by the caller; separate contexts are independent.
+ * No C++ types, no exceptions and no torch types cross this boundary.  Every
+ * function returns 0 on success and a negative OSHUN_ERR_* code on failure;
+ * oshun_last_error() then describes the failure.
+ *
+ * The entry points replace, one for one, the reference's interfaces on the
+ * hot path (OSHUN-1, file:line):
+ *   Runge_Kutta_4::advance_p1            def-RK.cpp:110-145, decl-RK.h:48-79
+ *   Actions::Adv                         def-Actions.cpp:73-94, decl-actions.h:47-69
+ *   Explicit_EC_FP::loop                 def-FokkerPlanck.cpp:590-625, decl-fokkerplanck.h:140-163
+ *   Euler_Backward::advance_1/advance_lm def-FokkerPlanck.cpp:1270-1304, decl-fokkerplanck.h:289-311
+ *   main_loop step body                  main.cpp:79-107
+ *   Stat storage (upload/download)       decl-state.h:302-334, def-State.cpp:413-427
+ *
+ * Host layouts.  A harmonic is complex<double>, p fastest, then x, then y,
+ * exactly a reference SHarmonic (matrices.h:713-716) when nbc = RKLevel guard
+ * cells are included on each side of x and y; nbc = 0 means interior only.
+ * Harmonics are in the reference's DFunc order ind(l,m) (def-State.cpp:422-426).
+ * Fields are complex<double>[ny+2nbc][nx+2nbc] in the order Ex,Ey,Ez,Bx,By,Bz.
+ * The state is real for m = 0 and for the fields; imaginary parts of those
+ * are ignored on upload and written as 0 on download.
+ */
+#ifndef OSHUN_HIP_H
+#define OSHUN_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSHUN_OK 0
+#define OSHUN_ERR_ARG (-1)
+#define OSHUN_ERR_HIP (-2)
+#define OSHUN_ERR_ALLOC (-3)
+#define OSHUN_ERR_STATE (-4)
+
+/* precision of the stored distribution function and of the Vlasov RHS */
+#define OSHUN_FP64 0       /* everything in double                                 */
+#define OSHUN_FP32_SPLIT 1 /* f_lm + Vlasov RHS in float; fields, collisions double */
+
+/* collision switches (bit mask) */
+#define OSHUN_COLL_EE 1  /* e-e on f00            (Explicit_EC_FP)             */
+#define OSHUN_COLL_FLM 2 /* implicit l >= 1 step  (Euler_Backward)             */
+
+typedef struct oshun_grid {
+  int l0, m0;           /* harmonic truncation, l <= l0, m <= min(l, m0)         */
+  int nump, numx, numy; /* interior cells: momentum, x, y (numy = 1 for 1D)      */
+  double pmin, pmax;    /* momentum axis p_i = pmin + i (pmax-pmin)/(nump-1)      */
+                        /* (matrices.h:860-866; decks use pmin = pmax/(2 nump))  */
+  double dx, dy;        /* cell sizes (the reference's x.dx(), y.dx())            */
+  int precision;        /* OSHUN_FP64 or OSHUN_FP32_SPLIT                         */
+} oshun_grid;
+
+typedef struct oshun_phys {
+  double density_np;  /* reference density [cm^-3] (Coulomb logs, collision rate) */
+  double zeta;        /* ion charge Z                                             */
+  double small_dt;    /* e-e sub-step target: numh = int(Dt/small_dt)+1          */
+  int nb_algorithms;  /* low-velocity cells with the parabolic G (NB)           */
+  int collisions;     /* OR of OSHUN_COLL_EE, OSHUN_COLL_FLM                      */
+  int ref_quirks;     /* 1: reproduce def-VlasovMax.cpp:918 (y-advection slip)   */
+} oshun_phys;
+
+typedef struct oshun_ctx oshun_ctx;
+
+int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out);
+int oshun_destroy(oshun_ctx* ctx);
+const char* oshun_last_error(const oshun_ctx* ctx);
+int oshun_num_harmonics(const oshun_ctx* ctx);
+
+/* Stat <-> device.  One host pointer per harmonic and per field (the shape
+ * of a reference Stat, decl-state.h:302-334); guards are skipped on upload
+ * and refilled with periodic images on download. */
+int oshun_upload_harmonics(oshun_ctx* ctx, const double* const* sh, const double* const* fld, int nbc);
+int oshun_download_harmonics(oshun_ctx* ctx, double* const* sh, double* const* fld, int nbc);
+/* Same with all harmonics (resp. all fields) contiguous in one buffer. */
+int oshun_upload_state(oshun_ctx* ctx, const double* f, const double* fld, int nbc);
+int oshun_download_state(oshun_ctx* ctx, double* f, double* fld, int nbc);
+
+/* Clock shared by the RK4 driver and both collision operators
+ * (def-RK.cpp:53, def-FokkerPlanck.cpp:572,1262). */
+int oshun_set_time(oshun_ctx* ctx, double t);
+int oshun_get_time(const oshun_ctx* ctx, double* t);
+
+/* Actions::Adv of the current state; the slope goes to host buffers laid out
+ * like oshun_download_state with nbc = 0. */
+int oshun_rhs(oshun_ctx* ctx, double* f_slope, double* fld_slope);
+/* Runge_Kutta_4::advance_p1: one classic RK4 step of size h; t += h. */
+int oshun_rk4_step(oshun_ctx* ctx, double h);
+/* Explicit_EC_FP::loop(tnew): e-e relaxation of f00 over tnew - t_ee. */
+int oshun_collide_ee(oshun_ctx* ctx, double tnew);
+/* Euler_Backward::advance_1 (which = 1: l = 1) and advance_lm (which = 2:
+ * l >= 2, then t_flm = tnew). */
+int oshun_collide_flm(oshun_ctx* ctx, double tnew, int which);
+/* nsteps x main.cpp:79-107 (RK4, e-e, f1, flm; guards are implicit). */
+int oshun_step(oshun_ctx* ctx, double h, int nsteps);
+
+/* Stream and device views, for callers that keep data on the GPU. */
+int oshun_synchronize(oshun_ctx* ctx);
+int oshun_get_stream(const oshun_ctx* ctx, void** hip_stream);
+/* Device state: f planes (float or double per precision) and 6 double field
+ * planes.  Plane layout [y][p][x] (x fastest); see oshun_plane_info. */
+int oshun_device_state(oshun_ctx* ctx, void** f_dev, double** fld_dev);
+/* ncomp real planes; comp_l/comp_m/comp_im describe each plane (arrays of
+ * length ncomp, may be NULL); plane_elems = ny*np*nx. */
+int oshun_plane_info(const oshun_ctx* ctx, int* ncomp, long long* plane_elems, int* comp_l, int* comp_m,
+                     int* comp_im);
+
+/* Per-kernel timing with HIP events on the context's stream (off by default).
+ * Kernel classes: 0 Vlasov stage, 1 field stage, 2 collisions. */
+int oshun_profile_enable(oshun_ctx* ctx, int on);
+int oshun_profile_read(oshun_ctx* ctx, double* ms_total, long long* launches, int nclasses);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OSHUN_HIP_H */

@@ -1,0 +1,276 @@
+"""MI355X-native distribution-function update of OSHUN (Vlasov-Maxwell RK4 + Fokker-Planck).
+
+Python host mirror over the C ABI of ``liboshun_hip.so`` (declared in
+include/oshun_hip.h).  The classes keep the reference's names and call
+pattern (OSHUN-1, decl-RK.h:48-79, decl-fokkerplanck.h:140-163,289-311):
+
+    ctx = Context(grid, phys)           # device state (the reference's Stat Y)
+    rk = Runge_Kutta_4(ctx)             # rk.advance_p1()     -> t += h
+    ee = Explicit_EC_FP(ctx)            # ee.loop(rk.time())
+    eb = Euler_Backward(ctx)            # eb.advance_1(t); eb.advance_lm(t)
+
+or ``ctx.step(h, nsteps)`` for the whole step body of main.cpp:79-107.
+
+Host arrays use the reference layout: complex128[nh, ny(+2nbc), nx(+2nbc), np]
+for f (p fastest, harmonic order ind(l, m)) and complex128[6, ny, nx] for
+Ex, Ey, Ez, Bx, By, Bz.  The library is required: nothing here computes on the
+CPU, and a missing or unloadable library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Grid", "Phys", "Context", "Runge_Kutta_4", "Explicit_EC_FP", "Euler_Backward", "num_harmonics",
+           "hind", "deck_pmin", "FP64", "FP32_SPLIT", "COLL_EE", "COLL_FLM", "library_path", "OshunError"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "liboshun_hip.so"
+
+FP64 = 0
+FP32_SPLIT = 1
+COLL_EE = 1
+COLL_FLM = 2
+
+
+class OshunError(RuntimeError):
+    pass
+
+
+def library_path():
+    return os.path.join(HERE, _LIB_NAME)
+
+
+def num_harmonics(l0, m0):
+    """N_h of the (l, m) trapezoid, def-State.cpp:417."""
+    return ((m0 + 1) * (2 * l0 - m0 + 2)) // 2
+
+
+def hind(l, m, m0):
+    """ind(l, m) of def-State.cpp:422-426."""
+    if l < m0 + 1:
+        return l * (l + 1) // 2 + m
+    return l * (m0 + 1) - m0 * (m0 + 1) // 2 + m
+
+
+def deck_pmin(pmax, nump):
+    """The reference decks' pmin = pmax / (2 nump) (def-Input.cpp:79)."""
+    return pmax / (2.0 * nump)
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("l0", ctypes.c_int), ("m0", ctypes.c_int), ("nump", ctypes.c_int), ("numx", ctypes.c_int),
+                ("numy", ctypes.c_int), ("pmin", ctypes.c_double), ("pmax", ctypes.c_double),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("precision", ctypes.c_int)]
+
+
+class Phys(ctypes.Structure):
+    _fields_ = [("density_np", ctypes.c_double), ("zeta", ctypes.c_double), ("small_dt", ctypes.c_double),
+                ("nb_algorithms", ctypes.c_int), ("collisions", ctypes.c_int), ("ref_quirks", ctypes.c_int)]
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise OshunError(f"{path} is missing: build it with `python -m oshun_amd._build` (hipcc, gfx950)")
+    lib = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    D = ctypes.c_double
+    I = ctypes.c_int
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.oshun_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Phys), ctypes.POINTER(P)]
+    lib.oshun_destroy.argtypes = [P]
+    lib.oshun_last_error.argtypes = [P]
+    lib.oshun_last_error.restype = ctypes.c_char_p
+    lib.oshun_num_harmonics.argtypes = [P]
+    lib.oshun_upload_state.argtypes = [P, dp, dp, I]
+    lib.oshun_download_state.argtypes = [P, dp, dp, I]
+    lib.oshun_upload_harmonics.argtypes = [P, ctypes.POINTER(dp), ctypes.POINTER(dp), I]
+    lib.oshun_download_harmonics.argtypes = [P, ctypes.POINTER(dp), ctypes.POINTER(dp), I]
+    lib.oshun_set_time.argtypes = [P, D]
+    lib.oshun_get_time.argtypes = [P, dp]
+    lib.oshun_rhs.argtypes = [P, dp, dp]
+    lib.oshun_rk4_step.argtypes = [P, D]
+    lib.oshun_collide_ee.argtypes = [P, D]
+    lib.oshun_collide_flm.argtypes = [P, D, I]
+    lib.oshun_step.argtypes = [P, D, I]
+    lib.oshun_synchronize.argtypes = [P]
+    lib.oshun_get_stream.argtypes = [P, ctypes.POINTER(P)]
+    lib.oshun_device_state.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
+    lib.oshun_plane_info.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(ctypes.c_longlong),
+                                     ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]
+    lib.oshun_profile_enable.argtypes = [P, I]
+    lib.oshun_profile_read.argtypes = [P, dp, ctypes.POINTER(ctypes.c_longlong), I]
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Context:
+    """One device-resident OSHUN state (the reference's ``Stat Y`` plus its operators)."""
+
+    def __init__(self, l0, m0, nump, numx, numy=1, pmax=0.3, pmin=None, dx=1.0, dy=1.0, precision=FP64,
+                 density_np=1e21, zeta=1.0, small_dt=1.0, nb_algorithms=4, collisions=COLL_EE | COLL_FLM,
+                 ref_quirks=1):
+        lib = _load()
+        if pmin is None:
+            pmin = deck_pmin(pmax, nump)
+        self.grid = Grid(l0, m0, nump, numx, numy, pmin, pmax, dx, dy, precision)
+        self.phys = Phys(density_np, zeta, small_dt, nb_algorithms, collisions, ref_quirks)
+        h = ctypes.c_void_p()
+        rc = lib.oshun_create(ctypes.byref(self.grid), ctypes.byref(self.phys), ctypes.byref(h))
+        if rc != 0:
+            raise OshunError(f"oshun_create failed ({rc}) for {l0=} {m0=} {nump=} {numx=} {numy=}")
+        self._h = h
+        self._lib = lib
+        self.l0, self.m0, self.np, self.nx, self.ny = l0, m0, nump, numx, numy
+        self.nh = num_harmonics(l0, m0)
+        self.precision = precision
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self._lib.oshun_last_error(self._h)
+            raise OshunError(f"oshun call failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.oshun_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ------------------------------------------------------------
+    def f_shape(self, nbc=0):
+        return (self.nh, self.ny + 2 * nbc, self.nx + 2 * nbc, self.np)
+
+    def fld_shape(self, nbc=0):
+        return (6, self.ny + 2 * nbc, self.nx + 2 * nbc)
+
+    def upload(self, f, fld, nbc=0):
+        f = np.ascontiguousarray(f, np.complex128).reshape(self.f_shape(nbc))
+        fld = np.ascontiguousarray(fld, np.complex128).reshape(self.fld_shape(nbc))
+        self._chk(self._lib.oshun_upload_state(self._h, _dptr(f.view(np.float64)), _dptr(fld.view(np.float64)),
+                                               nbc))
+
+    def download(self, nbc=0):
+        f = np.zeros(self.f_shape(nbc), np.complex128)
+        fld = np.zeros(self.fld_shape(nbc), np.complex128)
+        self._chk(self._lib.oshun_download_state(self._h, _dptr(f.view(np.float64)),
+                                                 _dptr(fld.view(np.float64)), nbc))
+        return f, fld
+
+    def set_time(self, t):
+        self._chk(self._lib.oshun_set_time(self._h, float(t)))
+
+    def time(self):
+        t = ctypes.c_double()
+        self._chk(self._lib.oshun_get_time(self._h, ctypes.byref(t)))
+        return t.value
+
+    # ---- operators ----------------------------------------------------------
+    def rhs(self):
+        """Actions::Adv of the current state (def-Actions.cpp:73-94)."""
+        f = np.zeros(self.f_shape(), np.complex128)
+        fld = np.zeros(self.fld_shape(), np.complex128)
+        self._chk(self._lib.oshun_rhs(self._h, _dptr(f.view(np.float64)), _dptr(fld.view(np.float64))))
+        return f, fld
+
+    def rk4_step(self, h):
+        self._chk(self._lib.oshun_rk4_step(self._h, float(h)))
+
+    def collide_ee(self, tnew):
+        self._chk(self._lib.oshun_collide_ee(self._h, float(tnew)))
+
+    def collide_flm(self, tnew, which):
+        self._chk(self._lib.oshun_collide_flm(self._h, float(tnew), int(which)))
+
+    def step(self, h, nsteps=1):
+        self._chk(self._lib.oshun_step(self._h, float(h), int(nsteps)))
+
+    def synchronize(self):
+        self._chk(self._lib.oshun_synchronize(self._h))
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        self._chk(self._lib.oshun_get_stream(self._h, ctypes.byref(s)))
+        return s.value
+
+    def plane_info(self):
+        n = ctypes.c_int()
+        pe = ctypes.c_longlong()
+        self._chk(self._lib.oshun_plane_info(self._h, ctypes.byref(n), ctypes.byref(pe), None, None, None))
+        L = (ctypes.c_int * n.value)()
+        M = (ctypes.c_int * n.value)()
+        IM = (ctypes.c_int * n.value)()
+        self._chk(self._lib.oshun_plane_info(self._h, ctypes.byref(n), ctypes.byref(pe), L, M, IM))
+        return dict(ncomp=n.value, plane=pe.value, l=list(L), m=list(M), im=list(IM))
+
+    def state_bytes(self):
+        """Bytes of the stored distribution function (the S_f of the roofline)."""
+        info = self.plane_info()
+        esz = 4 if self.precision == FP32_SPLIT else 8
+        return info["ncomp"] * info["plane"] * esz
+
+    def profile(self, on=True):
+        self._chk(self._lib.oshun_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_longlong * 4)()
+        self._chk(self._lib.oshun_profile_read(self._h, ms, n, 4))
+        names = ["vlasov_stage", "field_stage", "collide", "other"]
+        return {names[k]: (ms[k], n[k]) for k in range(4)}
+
+
+class Runge_Kutta_4:
+    """Explicit_Method facade (decl-RK.h:48-79): advance_p1() is one RK4 step of size h."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = float(h)
+
+    def advance_p1(self):
+        self.ctx.rk4_step(self.h)
+        return self
+
+    def advance_p2(self):
+        return self
+
+    def time(self):
+        return self.ctx.time()
+
+
+class Explicit_EC_FP:
+    """e-e collisions of f00 (decl-fokkerplanck.h:140-163)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def loop(self, tnew):
+        self.ctx.collide_ee(tnew)
+
+
+class Euler_Backward:
+    """Implicit l >= 1 collisions (decl-fokkerplanck.h:289-311)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def advance_1(self, tnew):
+        self.ctx.collide_flm(tnew, 1)
+
+    def advance_lm(self, tnew):
+        self.ctx.collide_flm(tnew, 2)

@@ -1,0 +1,612 @@
+// C ABI of the MI355X distribution-function update (declared in
+// include/oshun_hip.h).  One oshun_ctx = one device-resident OSHUN state.
+//
+// Step structure per call of oshun_step (main.cpp:79-107):
+//   4 x { vlasov_stage<T>, field_stage<T> }   Runge_Kutta_4::advance_p1
+//   collide<T,NPL>                            Explicit_EC_FP::loop +
+//                                             Euler_Backward::advance_1/_lm
+// Guard cells do not exist on the device: stencils index periodically, which
+// reproduces the reference's interior (its guards are refreshed only after a
+// whole step, and every RK stage widens the stencil by one cell of four).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "oshun_hip.h"
+
+using oshun::HostTables;
+using oshun::Term;
+
+struct ProfRec {
+  int cls;
+  hipEvent_t e0, e1;
+};
+
+struct oshun_ctx {
+  oshun_grid g;
+  oshun_phys ph;
+  HostTables T;
+  int nc = 0, np = 0, nx = 0, ny = 0, nxy = 0, npb = 0;
+  long long plane = 0;
+  bool fp32 = false;
+  size_t esz = 8;
+  void *S = nullptr, *A = nullptr, *B1 = nullptr, *B2 = nullptr;
+  double *Sf = nullptr, *Af = nullptr, *B1f = nullptr, *B2f = nullptr;
+  double* jpart = nullptr;
+  Term* d_terms = nullptr;
+  int* d_tstart = nullptr;
+  int* d_clean = nullptr;
+  int* d_comp_l = nullptr;
+  void* d_vt = nullptr;  // Vlasov tables in storage precision: invpax | vtab | cg | hp0
+  double* d_p3og = nullptr;
+  double* d_coll = nullptr;
+  oshun::CollTables ct;
+  hipStream_t stream = nullptr;
+  double t_rk = 0.0, t_ee = 0.0, t_eb = 0.0;
+  std::string err;
+  bool prof = false;
+  std::vector<ProfRec> prec;
+  double prof_ms[4] = {0, 0, 0, 0};
+  long long prof_n[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(oshun_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                      \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return fail(ctx, OSHUN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename V>
+int upload_vec(oshun_ctx* c, V** dst, const std::vector<V>& src) {
+  size_t n = std::max<size_t>(src.size(), 1) * sizeof(V);
+  HIPCHK(c, hipMalloc((void**)dst, n));
+  if (!src.empty()) HIPCHK(c, hipMemcpy(*dst, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice));
+  return OSHUN_OK;
+}
+
+void prof_begin(oshun_ctx* c, int cls, ProfRec* r) {
+  if (!c->prof) return;
+  r->cls = cls;
+  hipEventCreate(&r->e0);
+  hipEventCreate(&r->e1);
+  hipEventRecord(r->e0, c->stream);
+}
+void prof_end(oshun_ctx* c, ProfRec* r) {
+  if (!c->prof) return;
+  hipEventRecord(r->e1, c->stream);
+  c->prec.push_back(*r);
+}
+
+template <typename T>
+const T* vt_ptr(const oshun_ctx* c, int which) {
+  // invpax (l0+1)*np | vtab np | cg l0+1 | hp0 l0+1
+  const T* b = static_cast<const T*>(c->d_vt);
+  const size_t L = c->T.l0 + 1, np = c->np;
+  switch (which) {
+    case 0: return b;
+    case 1: return b + L * np;
+    case 2: return b + L * np + np;
+    default: return b + L * np + np + L;
+  }
+}
+
+template <typename T>
+int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* out, void* acc, const double* fin,
+                 double h) {
+  oshun::StageArgs<T> a;
+  a.in = static_cast<const T*>(in);
+  a.y0 = static_cast<const T*>(y0);
+  a.out = static_cast<T*>(out);
+  a.acc = static_cast<T*>(acc);
+  a.fin = fin;
+  a.jpart = c->jpart;
+  a.terms = c->d_terms;
+  a.tstart = c->d_tstart;
+  a.clean = c->d_clean;
+  a.invpax = vt_ptr<T>(c, 0);
+  a.vtab = vt_ptr<T>(c, 1);
+  a.cg = vt_ptr<T>(c, 2);
+  a.hp0 = vt_ptr<T>(c, 3);
+  a.p3og = c->d_p3og;
+  a.p0p1_sq = static_cast<T>(c->T.p0p1_sq);
+  a.inv_mp0p1_sq = static_cast<T>(c->T.inv_mp0p1_sq);
+  a.g_r = static_cast<T>(c->T.g_r);
+  a.c05h = static_cast<T>(0.5 * h);
+  a.hh = static_cast<T>(h);
+  a.h6 = static_cast<T>(h / 6.0);
+  a.third = static_cast<T>(1.0 / 3.0);
+  a.twothird = static_cast<T>(2.0 / 3.0);
+  a.np = c->np; a.nx = c->nx; a.ny = c->ny; a.nc = c->nc;
+  a.plane = c->plane;
+  a.mode = mode;
+  a.c10 = c->T.comp_re[oshun::hind(1, 0, c->T.m0)];
+  a.c11r = c->T.m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, c->T.m0)] : -1;
+  a.c11i = c->T.m0 >= 1 ? c->T.comp_im[oshun::hind(1, 1, c->T.m0)] : -1;
+  dim3 block(oshun::kTileX, oshun::kTileP, 1);
+  dim3 grid((c->nx + oshun::kTileX - 1) / oshun::kTileX, c->npb, c->ny);
+  ProfRec r;
+  prof_begin(c, 0, &r);
+  hipLaunchKernelGGL(oshun::vlasov_stage<T>, grid, block, 0, c->stream, a);
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  return OSHUN_OK;
+}
+
+template <typename T>
+int launch_field(oshun_ctx* c, int mode, const double* fin, const double* f0, double* fout, double* facc,
+                 const void* fpl, double h) {
+  oshun::FieldArgs<T> a;
+  a.fin = fin; a.f0 = f0; a.fout = fout; a.facc = facc;
+  a.jpart = c->jpart;
+  a.fpl = static_cast<const T*>(fpl);
+  a.npb = c->npb; a.np = c->np; a.nx = c->nx; a.ny = c->ny; a.mode = mode;
+  a.c10 = c->T.comp_re[oshun::hind(1, 0, c->T.m0)];
+  a.c11r = c->T.m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, c->T.m0)] : -1;
+  a.c11i = c->T.m0 >= 1 ? c->T.comp_im[oshun::hind(1, 1, c->T.m0)] : -1;
+  a.plane = c->plane;
+  a.p3og1 = c->T.p3og[1];
+  a.small = c->T.small;
+  a.delta_p = c->T.delta_p;
+  a.midx = c->T.midx; a.midy = c->T.midy;
+  a.c05h = 0.5 * h; a.hh = h; a.h6 = h / 6.0; a.third = 1.0 / 3.0; a.twothird = 2.0 / 3.0;
+  const int threads = 256;
+  dim3 grid((c->nxy + threads - 1) / threads);
+  ProfRec r;
+  prof_begin(c, 1, &r);
+  hipLaunchKernelGGL(oshun::field_stage<T>, grid, dim3(threads), 0, c->stream, a);
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  return OSHUN_OK;
+}
+
+template <typename T>
+int launch_stage_pair(oshun_ctx* c, int mode, void* in, void* y0, void* out, double* fin, double* f0, double* fout,
+                      double h) {
+  int rc = launch_stage<T>(c, mode, in, y0, out, c->A, fin, h);
+  if (rc) return rc;
+  return launch_field<T>(c, mode, fin, f0, fout, c->Af, in, h);
+}
+
+template <typename T>
+int rk4(oshun_ctx* c, double h) {
+  // Runge_Kutta_4::advance_p1, def-RK.cpp:110-145 (Y0 = S, Y1 = B1/B2 ping-pong, Y = A)
+  int rc;
+  if ((rc = launch_stage_pair<T>(c, 1, c->S, c->S, c->B1, c->Sf, c->Sf, c->B1f, h))) return rc;
+  if ((rc = launch_stage_pair<T>(c, 2, c->B1, c->S, c->B2, c->B1f, c->Sf, c->B2f, h))) return rc;
+  if ((rc = launch_stage_pair<T>(c, 3, c->B2, c->S, c->B1, c->B2f, c->Sf, c->B1f, h))) return rc;
+  if ((rc = launch_stage_pair<T>(c, 4, c->B1, nullptr, c->S, c->B1f, nullptr, c->Sf, h))) return rc;
+  c->t_rk += h;
+  return OSHUN_OK;
+}
+
+template <typename T, int NPL>
+int launch_collide_npl(oshun_ctx* c, oshun::CollArgs<T>& a) {
+  ProfRec r;
+  prof_begin(c, 2, &r);
+  hipLaunchKernelGGL((oshun::collide<T, NPL>), dim3(c->nxy), dim3(64), 0, c->stream, a);
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  return OSHUN_OK;
+}
+
+template <typename T>
+int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_ee, double dt_flm) {
+  oshun::CollArgs<T> a;
+  a.f = static_cast<T*>(c->S);
+  a.t = c->ct;
+  a.comp_l = c->d_comp_l;
+  a.nc = c->nc; a.np = c->np; a.nx = c->nx; a.ny = c->ny; a.nb = c->ph.nb_algorithms; a.l0 = c->T.l0;
+  a.plane = c->plane;
+  a.do_ee = do_ee; a.do_f1 = do_f1; a.do_flm = do_flm; a.numh = numh;
+  a.h_ee = h_ee; a.dt_flm = dt_flm;
+  a.c_kpre = c->T.c_kpre; a.kpre = c->T.kpre; a.density_np = c->ph.density_np; a.zeta = c->ph.zeta;
+  a.v0sq = c->T.v0sq; a.v1sq = c->T.v1sq; a.g_den = c->T.g_den;
+  if (c->np <= 64) return launch_collide_npl<T, 1>(c, a);
+  if (c->np <= 128) return launch_collide_npl<T, 2>(c, a);
+  if (c->np <= 256) return launch_collide_npl<T, 4>(c, a);
+  return launch_collide_npl<T, 8>(c, a);
+}
+
+int do_collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_ee, double dt_flm) {
+  return c->fp32 ? collide<float>(c, do_ee, do_f1, do_flm, numh, h_ee, dt_flm)
+                 : collide<double>(c, do_ee, do_f1, do_flm, numh, h_ee, dt_flm);
+}
+
+// numh and sub-step of Explicit_EC_FP::loop, def-FokkerPlanck.cpp:597-599
+void ee_substeps(const oshun_ctx* c, double tnew, int* numh, double* h) {
+  size_t n = static_cast<size_t>(static_cast<int>((tnew - c->t_ee) / c->ph.small_dt)) + 1;
+  *numh = static_cast<int>(n);
+  *h = (tnew - c->t_ee) / static_cast<double>(n);
+}
+
+int full_step(oshun_ctx* c, double h) {
+  int rc = c->fp32 ? rk4<float>(c, h) : rk4<double>(c, h);
+  if (rc) return rc;
+  const double t = c->t_rk;
+  const int coll = c->ph.collisions;
+  const int do_ee = (coll & OSHUN_COLL_EE) ? 1 : 0;
+  const int do_flm = (coll & OSHUN_COLL_FLM) ? 1 : 0;
+  int numh = 0;
+  double hee = 0.0;
+  if (do_ee) ee_substeps(c, t, &numh, &hee);
+  if (do_ee || do_flm) {
+    rc = do_collide(c, do_ee, do_flm, do_flm, numh, hee, t - c->t_eb);
+    if (rc) return rc;
+  }
+  c->t_ee = t;
+  c->t_eb = t;
+  return OSHUN_OK;
+}
+
+// host <-> device repacking ---------------------------------------------------
+template <typename T>
+void pack(const oshun_ctx* c, const double* const* sh, int nbc, std::vector<T>& out) {
+  const int np = c->np, nx = c->nx, ny = c->ny;
+  const int nxg = nx + 2 * nbc;
+  out.assign((size_t)c->nc * c->plane, T(0));
+  for (int h = 0; h < c->T.nh; ++h) {
+    const double* src = sh[h];
+    const int cr = c->T.comp_re[h], ci = c->T.comp_im[h];
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const double* col = src + 2 * (((size_t)(y + nbc) * nxg + (x + nbc)) * np);
+        for (int p = 0; p < np; ++p) {
+          const size_t o = ((size_t)y * np + p) * nx + x;
+          out[(size_t)cr * c->plane + o] = static_cast<T>(col[2 * p]);
+          if (ci >= 0) out[(size_t)ci * c->plane + o] = static_cast<T>(col[2 * p + 1]);
+        }
+      }
+  }
+}
+
+template <typename T>
+void unpack(const oshun_ctx* c, const std::vector<T>& in, double* const* sh, int nbc) {
+  const int np = c->np, nx = c->nx, ny = c->ny;
+  const int nxg = nx + 2 * nbc, nyg = ny + 2 * nbc;
+  for (int h = 0; h < c->T.nh; ++h) {
+    double* dst = sh[h];
+    const int cr = c->T.comp_re[h], ci = c->T.comp_im[h];
+    for (int yg = 0; yg < nyg; ++yg)
+      for (int xg = 0; xg < nxg; ++xg) {
+        const int y = (((yg - nbc) % ny) + ny) % ny, x = (((xg - nbc) % nx) + nx) % nx;
+        double* col = dst + 2 * (((size_t)yg * nxg + xg) * np);
+        for (int p = 0; p < np; ++p) {
+          const size_t o = ((size_t)y * np + p) * nx + x;
+          col[2 * p] = static_cast<double>(in[(size_t)cr * c->plane + o]);
+          col[2 * p + 1] = ci >= 0 ? static_cast<double>(in[(size_t)ci * c->plane + o]) : 0.0;
+        }
+      }
+  }
+}
+
+void pack_fields(const oshun_ctx* c, const double* const* fld, int nbc, std::vector<double>& out) {
+  const int nx = c->nx, ny = c->ny, nxg = nx + 2 * nbc;
+  out.assign((size_t)6 * c->nxy, 0.0);
+  for (int k = 0; k < 6; ++k)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x)
+        out[(size_t)k * c->nxy + (size_t)y * nx + x] = fld[k][2 * ((size_t)(y + nbc) * nxg + (x + nbc))];
+}
+void unpack_fields(const oshun_ctx* c, const std::vector<double>& in, double* const* fld, int nbc) {
+  const int nx = c->nx, ny = c->ny, nxg = nx + 2 * nbc, nyg = ny + 2 * nbc;
+  for (int k = 0; k < 6; ++k)
+    for (int yg = 0; yg < nyg; ++yg)
+      for (int xg = 0; xg < nxg; ++xg) {
+        const int y = (((yg - nbc) % ny) + ny) % ny, x = (((xg - nbc) % nx) + nx) % nx;
+        double* d = fld[k] + 2 * ((size_t)yg * nxg + xg);
+        d[0] = in[(size_t)k * c->nxy + (size_t)y * nx + x];
+        d[1] = 0.0;
+      }
+}
+
+int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* const* sh, double* const* fld,
+                 int nbc) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)c->nc * c->plane;
+  if (sh) {
+    if (c->fp32) {
+      std::vector<float> h(n);
+      HIPCHK(c, hipMemcpy(h.data(), fsrc, n * sizeof(float), hipMemcpyDeviceToHost));
+      unpack<float>(c, h, sh, nbc);
+    } else {
+      std::vector<double> h(n);
+      HIPCHK(c, hipMemcpy(h.data(), fsrc, n * sizeof(double), hipMemcpyDeviceToHost));
+      unpack<double>(c, h, sh, nbc);
+    }
+  }
+  if (fld) {
+    std::vector<double> h((size_t)6 * c->nxy);
+    HIPCHK(c, hipMemcpy(h.data(), fldsrc, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    unpack_fields(c, h, fld, nbc);
+  }
+  return OSHUN_OK;
+}
+
+void free_all(oshun_ctx* c) {
+  void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->jpart, c->d_terms, c->d_tstart,
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
+  c->prec.clear();
+  if (c->stream) hipStreamDestroy(c->stream);
+}
+
+bool sane(const oshun_ctx* c) { return c && c->S; }
+
+}  // namespace
+
+extern "C" {
+
+int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out) {
+  if (!grid || !phys || !out) return OSHUN_ERR_ARG;
+  *out = nullptr;
+  const oshun_grid& g = *grid;
+  if (g.l0 < 1 || g.m0 < 0 || g.m0 > g.l0 || g.nump < 4 || g.nump > 512 || g.numx < 3 || g.numy < 1 ||
+      !(g.pmax > g.pmin) || !(g.pmin > 0.0) || !(g.dx > 0.0) || !(g.dy > 0.0) ||
+      (g.precision != OSHUN_FP64 && g.precision != OSHUN_FP32_SPLIT))
+    return OSHUN_ERR_ARG;
+  if (phys->nb_algorithms < 0 || phys->nb_algorithms > g.nump || !(phys->small_dt > 0.0) ||
+      !(phys->density_np > 0.0))
+    return OSHUN_ERR_ARG;
+  oshun_ctx* c = new oshun_ctx();
+  c->g = g;
+  c->ph = *phys;
+  c->fp32 = g.precision == OSHUN_FP32_SPLIT;
+  c->esz = c->fp32 ? 4 : 8;
+  oshun::build_tables(c->T, g.l0, g.m0, g.nump, g.numy, g.pmin, g.pmax, g.dx, g.dy, phys->density_np,
+                      phys->nb_algorithms, phys->ref_quirks);
+  c->nc = c->T.nc; c->np = g.nump; c->nx = g.numx; c->ny = g.numy; c->nxy = g.numx * g.numy;
+  c->npb = (g.nump + oshun::kTileP - 1) / oshun::kTileP;
+  c->plane = (long long)g.numx * g.numy * g.nump;
+  auto bail = [&](int rc) { free_all(c); std::string m = c->err; delete c; (void)m; return rc; };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(OSHUN_ERR_HIP);
+  const size_t fbytes = (size_t)c->nc * c->plane * c->esz;
+  const size_t fldbytes = (size_t)6 * c->nxy * sizeof(double);
+  void** fb[] = {&c->S, &c->A, &c->B1, &c->B2};
+  for (void** p : fb)
+    if (hipMalloc(p, fbytes) != hipSuccess) return bail(OSHUN_ERR_ALLOC);
+  double** fl[] = {&c->Sf, &c->Af, &c->B1f, &c->B2f};
+  for (double** p : fl)
+    if (hipMalloc((void**)p, fldbytes) != hipSuccess) return bail(OSHUN_ERR_ALLOC);
+  for (void** p : fb)
+    if (hipMemset(*p, 0, fbytes) != hipSuccess) return bail(OSHUN_ERR_HIP);
+  for (double** p : fl)
+    if (hipMemset(*p, 0, fldbytes) != hipSuccess) return bail(OSHUN_ERR_HIP);
+  if (hipMalloc((void**)&c->jpart, (size_t)c->npb * 3 * c->nxy * sizeof(double)) != hipSuccess)
+    return bail(OSHUN_ERR_ALLOC);
+  if (upload_vec(c, &c->d_terms, c->T.terms) || upload_vec(c, &c->d_tstart, c->T.tstart) ||
+      upload_vec(c, &c->d_clean, c->T.clean) || upload_vec(c, &c->d_comp_l, c->T.comp_l) ||
+      upload_vec(c, &c->d_p3og, c->T.p3og))
+    return bail(OSHUN_ERR_HIP);
+  {
+    std::vector<double> vt;
+    vt.insert(vt.end(), c->T.invpax.begin(), c->T.invpax.end());
+    vt.insert(vt.end(), c->T.vtab.begin(), c->T.vtab.end());
+    vt.insert(vt.end(), c->T.cg.begin(), c->T.cg.end());
+    vt.insert(vt.end(), c->T.hp0.begin(), c->T.hp0.end());
+    if (c->fp32) {
+      std::vector<float> vf(vt.begin(), vt.end());
+      float* d = nullptr;
+      if (upload_vec(c, &d, vf)) return bail(OSHUN_ERR_HIP);
+      c->d_vt = d;
+    } else {
+      double* d = nullptr;
+      if (upload_vec(c, &d, vt)) return bail(OSHUN_ERR_HIP);
+      c->d_vt = d;
+    }
+  }
+  {
+    const HostTables& T = c->T;
+    const std::vector<double>* tabs[] = {&T.U4, &T.U4m1, &T.U2, &T.U2m1, &T.U1s, &T.U1m1s, &T.U3, &T.Qn,
+                                         &T.Pn, &T.vc,  &T.ivdm, &T.ivdp, &T.ivs2, &T.pw3, &T.pw5, &T.pw7};
+    std::vector<double> all;
+    for (auto* t : tabs) all.insert(all.end(), t->begin(), t->end());
+    if (upload_vec(c, &c->d_coll, all)) return bail(OSHUN_ERR_HIP);
+    const double* b = c->d_coll;
+    const size_t n = c->np;
+    c->ct.U4 = b; c->ct.U4m1 = b + n; c->ct.U2 = b + 2 * n; c->ct.U2m1 = b + 3 * n; c->ct.U1s = b + 4 * n;
+    c->ct.U1m1s = b + 5 * n; c->ct.U3 = b + 6 * n; c->ct.Qn = b + 7 * n; c->ct.Pn = b + 8 * n;
+    c->ct.vc = b + 9 * n; c->ct.ivdm = b + 10 * n; c->ct.ivdp = b + 11 * n; c->ct.ivs2 = b + 12 * n;
+    c->ct.pw3 = b + 13 * n; c->ct.pw5 = b + 14 * n; c->ct.pw7 = b + 15 * n;
+  }
+  *out = c;
+  return OSHUN_OK;
+}
+
+int oshun_destroy(oshun_ctx* ctx) {
+  if (!ctx) return OSHUN_ERR_ARG;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  free_all(ctx);
+  delete ctx;
+  return OSHUN_OK;
+}
+
+const char* oshun_last_error(const oshun_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int oshun_num_harmonics(const oshun_ctx* ctx) { return ctx ? ctx->T.nh : OSHUN_ERR_ARG; }
+
+int oshun_upload_harmonics(oshun_ctx* c, const double* const* sh, const double* const* fld, int nbc) {
+  if (!sane(c) || nbc < 0) return OSHUN_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (sh) {
+    for (int h = 0; h < c->T.nh; ++h)
+      if (!sh[h]) return fail(c, OSHUN_ERR_ARG, "null harmonic pointer");
+    if (c->fp32) {
+      std::vector<float> v;
+      pack<float>(c, sh, nbc, v);
+      HIPCHK(c, hipMemcpy(c->S, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+      std::vector<double> v;
+      pack<double>(c, sh, nbc, v);
+      HIPCHK(c, hipMemcpy(c->S, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+  }
+  if (fld) {
+    for (int k = 0; k < 6; ++k)
+      if (!fld[k]) return fail(c, OSHUN_ERR_ARG, "null field pointer");
+    std::vector<double> v;
+    pack_fields(c, fld, nbc, v);
+    HIPCHK(c, hipMemcpy(c->Sf, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return OSHUN_OK;
+}
+
+int oshun_download_harmonics(oshun_ctx* c, double* const* sh, double* const* fld, int nbc) {
+  if (!sane(c) || nbc < 0) return OSHUN_ERR_ARG;
+  return download_raw(c, c->S, c->Sf, sh, fld, nbc);
+}
+
+static void split_ptrs(const oshun_ctx* c, const double* f, const double* fld, int nbc,
+                       std::vector<const double*>& hp, std::vector<const double*>& fp) {
+  const size_t hsz = 2 * (size_t)c->np * (c->nx + 2 * nbc) * (c->ny + 2 * nbc);
+  const size_t fsz = 2 * (size_t)(c->nx + 2 * nbc) * (c->ny + 2 * nbc);
+  hp.clear(); fp.clear();
+  if (f) for (int h = 0; h < c->T.nh; ++h) hp.push_back(f + h * hsz);
+  if (fld) for (int k = 0; k < 6; ++k) fp.push_back(fld + k * fsz);
+}
+
+int oshun_upload_state(oshun_ctx* c, const double* f, const double* fld, int nbc) {
+  if (!sane(c) || nbc < 0) return OSHUN_ERR_ARG;
+  std::vector<const double*> hp, fp;
+  split_ptrs(c, f, fld, nbc, hp, fp);
+  return oshun_upload_harmonics(c, f ? hp.data() : nullptr, fld ? fp.data() : nullptr, nbc);
+}
+
+int oshun_download_state(oshun_ctx* c, double* f, double* fld, int nbc) {
+  if (!sane(c) || nbc < 0) return OSHUN_ERR_ARG;
+  std::vector<const double*> hp, fp;
+  split_ptrs(c, f, fld, nbc, hp, fp);
+  return oshun_download_harmonics(c, f ? const_cast<double* const*>(hp.data()) : nullptr,
+                                  fld ? const_cast<double* const*>(fp.data()) : nullptr, nbc);
+}
+
+int oshun_set_time(oshun_ctx* c, double t) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  c->t_rk = c->t_ee = c->t_eb = t;
+  return OSHUN_OK;
+}
+
+int oshun_get_time(const oshun_ctx* c, double* t) {
+  if (!sane(c) || !t) return OSHUN_ERR_ARG;
+  *t = c->t_rk;
+  return OSHUN_OK;
+}
+
+int oshun_rhs(oshun_ctx* c, double* f_slope, double* fld_slope) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  int rc = c->fp32 ? launch_stage_pair<float>(c, 0, c->S, c->S, c->B1, c->Sf, c->Sf, c->B1f, 0.0)
+                   : launch_stage_pair<double>(c, 0, c->S, c->S, c->B1, c->Sf, c->Sf, c->B1f, 0.0);
+  if (rc) return rc;
+  std::vector<const double*> hp, fp;
+  split_ptrs(c, f_slope, fld_slope, 0, hp, fp);
+  return download_raw(c, c->B1, c->B1f, f_slope ? const_cast<double* const*>(hp.data()) : nullptr,
+                      fld_slope ? const_cast<double* const*>(fp.data()) : nullptr, 0);
+}
+
+int oshun_rk4_step(oshun_ctx* c, double h) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  return c->fp32 ? rk4<float>(c, h) : rk4<double>(c, h);
+}
+
+int oshun_collide_ee(oshun_ctx* c, double tnew) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  int numh;
+  double h;
+  ee_substeps(c, tnew, &numh, &h);
+  int rc = do_collide(c, 1, 0, 0, numh, h, 0.0);
+  if (rc) return rc;
+  c->t_ee = tnew;
+  return OSHUN_OK;
+}
+
+int oshun_collide_flm(oshun_ctx* c, double tnew, int which) {
+  if (!sane(c) || (which != 1 && which != 2)) return OSHUN_ERR_ARG;
+  int rc = do_collide(c, 0, which == 1, which == 2, 0, 0.0, tnew - c->t_eb);
+  if (rc) return rc;
+  if (which == 2) c->t_eb = tnew;
+  return OSHUN_OK;
+}
+
+int oshun_step(oshun_ctx* c, double h, int nsteps) {
+  if (!sane(c) || nsteps < 0) return OSHUN_ERR_ARG;
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = full_step(c, h);
+    if (rc) return rc;
+  }
+  return OSHUN_OK;
+}
+
+int oshun_synchronize(oshun_ctx* c) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return OSHUN_OK;
+}
+
+int oshun_get_stream(const oshun_ctx* c, void** s) {
+  if (!sane(c) || !s) return OSHUN_ERR_ARG;
+  *s = c->stream;
+  return OSHUN_OK;
+}
+
+int oshun_device_state(oshun_ctx* c, void** f_dev, double** fld_dev) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  if (f_dev) *f_dev = c->S;
+  if (fld_dev) *fld_dev = c->Sf;
+  return OSHUN_OK;
+}
+
+int oshun_plane_info(const oshun_ctx* c, int* ncomp, long long* plane_elems, int* comp_l, int* comp_m,
+                     int* comp_im) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  if (ncomp) *ncomp = c->nc;
+  if (plane_elems) *plane_elems = c->plane;
+  for (int i = 0; i < c->nc; ++i) {
+    if (comp_l) comp_l[i] = c->T.comp_l[i];
+    if (comp_m) comp_m[i] = c->T.comp_m[i];
+    if (comp_im) comp_im[i] = c->T.comp_isim[i];
+  }
+  return OSHUN_OK;
+}
+
+int oshun_profile_enable(oshun_ctx* c, int on) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  c->prof = on != 0;
+  return OSHUN_OK;
+}
+
+int oshun_profile_read(oshun_ctx* c, double* ms_total, long long* launches, int nclasses) {
+  if (!sane(c) || nclasses < 0) return OSHUN_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto& r : c->prec) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess && r.cls >= 0 && r.cls < 4) {
+      c->prof_ms[r.cls] += ms;
+      c->prof_n[r.cls] += 1;
+    }
+    hipEventDestroy(r.e0);
+    hipEventDestroy(r.e1);
+  }
+  c->prec.clear();
+  for (int k = 0; k < nclasses && k < 4; ++k) {
+    if (ms_total) ms_total[k] = c->prof_ms[k];
+    if (launches) launches[k] = c->prof_n[k];
+    c->prof_ms[k] = 0.0;
+    c->prof_n[k] = 0;
+  }
+  return OSHUN_OK;
+}
+
+}  // extern "C"
