@@ -137,6 +137,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp64", "fp32"], default=None,
+                    help="profiling aid: run one path only and print its timings (no JSON contract line)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="profiling aid: also launch one plain copy of S_f bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -146,6 +150,15 @@ def main():
 
     f, fld, dx = target_state()
     cells = (L0 + 1) * NP * NX
+    if args.only:
+        prec = oshun_amd.FP64 if args.only == "fp64" else oshun_amd.FP32_SPLIT
+        ctx, ms, prof = run_path(prec, args.steps, args.warmup, f, fld, dx)
+        if args.copy_calib:
+            ctx.diag_copy(ctx.state_bytes())
+            ctx.synchronize()
+        print(json.dumps({"path": args.only, "ms_per_step": ms, "state_bytes": ctx.state_bytes(),
+                          "kernels": {k: v for k, v in prof.items()}}))
+        return
     ctx64, ms64, prof64 = run_path(oshun_amd.FP64, args.steps, args.warmup, f, fld, dx)
     S64 = ctx64.state_bytes()
     ctx32, ms32, prof32 = run_path(oshun_amd.FP32_SPLIT, args.steps, args.warmup, f, fld, dx)

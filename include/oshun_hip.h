@@ -114,6 +114,11 @@ int oshun_plane_info(const oshun_ctx* ctx, int* ncomp, long long* plane_elems, i
 int oshun_profile_enable(oshun_ctx* ctx, int on);
 int oshun_profile_read(oshun_ctx* ctx, double* ms_total, long long* launches, int nclasses);
 
+/* Diagnostics: a plain device copy of nbytes (<= the stored f size) between
+ * two scratch buffers of the context, used to calibrate hardware byte
+ * counters (FETCH_SIZE / WRITE_SIZE) against a kernel of known traffic. */
+int oshun_diag_copy(oshun_ctx* ctx, long long nbytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,7 @@ def _load():
                                      ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]
     lib.oshun_profile_enable.argtypes = [P, I]
     lib.oshun_profile_read.argtypes = [P, dp, ctypes.POINTER(ctypes.c_longlong), I]
+    lib.oshun_diag_copy.argtypes = [P, ctypes.c_longlong]
     _lib = lib
     return lib
 
@@ -224,6 +225,10 @@ class Context:
         esz = 4 if self.precision == FP32_SPLIT else 8
         return info["ncomp"] * info["plane"] * esz
 
+    def diag_copy(self, nbytes):
+        """Plain device copy of nbytes between scratch buffers (counter calibration)."""
+        self._chk(self._lib.oshun_diag_copy(self._h, int(nbytes)))
+
     def profile(self, on=True):
         self._chk(self._lib.oshun_profile_enable(self._h, 1 if on else 0))
 
@@ -231,7 +236,7 @@ class Context:
         ms = (ctypes.c_double * 4)()
         n = (ctypes.c_longlong * 4)()
         self._chk(self._lib.oshun_profile_read(self._h, ms, n, 4))
-        names = ["vlasov_stage", "field_stage", "collide", "other"]
+        names = ["vlasov_stage", "field_stage", "collide", "diag_copy"]
         return {names[k]: (ms[k], n[k]) for k in range(4)}
 
 

@@ -172,6 +172,136 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage(StageArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// 1D, m0 = 0 specialisation (every real plane is one harmonic l, no y, only
+// Ex couples).  L0 is a compile-time constant so the l loops unroll and every
+// operand stays in registers.  One thread per (x, p): the 5-point stencil of
+// all L0+1 planes (p-1, p, p+1, x-1, x+1) and the RK operands are loaded up
+// front, so a wavefront keeps ~5 (L0+1) independent 512-byte loads in flight;
+// the p+-1 and x+-1 reloads are served by L1/L2.  The sums are those of the
+// coupling program in the same order: E-field terms of source l-1 then l+1
+// (def-VlasovMax.cpp:358-379), x-advection of source l-1 then l+1 (:842-853).
+constexpr int kRows1D = 4;  // p rows per block (block = 64 x 4)
+
+template <typename T>
+struct Stage1DArgs {
+  const T* in;
+  const T* y0;
+  T* out;
+  T* acc;
+  const double* fin;  // Ex plane
+  double* jpart;      // [npb][3][nx]
+  const T* invpax;    // (l0+1) x np
+  const T* vtab;
+  const T* cg;
+  const T* hp0;
+  const double* p3og;
+  const double* eA1;  // E-field G coefficient of source l   (l = 0: G00)
+  const double* eA2;  // E-field H coefficient of source l
+  const double* xA1;  // x-advection coefficient of source l towards l+1
+  const double* xA2;  // x-advection coefficient of source l towards l-1
+  T p0p1_sq, inv_mp0p1_sq, g_r;
+  T c05h, hh, h6, third, twothird;
+  int np, nx, mode;
+  long long plane;
+};
+
+template <typename T, int L0>
+__global__ __launch_bounds__(64 * kRows1D) void vlasov_stage_1d(Stage1DArgs<T> a) {
+  constexpr int NL = L0 + 1;
+  const int lane = threadIdx.x;
+  const int row = threadIdx.y;
+  const int x = blockIdx.x * 64 + lane;
+  const int p = blockIdx.y * kRows1D + row;
+  const int np = a.np, nx = a.nx;
+  __shared__ double jsh[kRows1D][64];
+  double jv = 0.0;
+  if (x < nx && p < np) {
+    const long long idx = (long long)p * nx + x;
+    const long long xm = (x == 0) ? (nx - 1) : -1;
+    const long long xp = (x == nx - 1) ? -(long long)(nx - 1) : 1;
+    const long long up = (p > 0) ? -(long long)nx : 0;          // row p-1 (unused at p = 0)
+    const long long dn = (p < np - 1) ? (long long)nx : 0;      // row p+1 (unused at np-1)
+    T fm[NL], fc[NL], fn[NL], fl[NL], fr[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const T* s = a.in + (long long)l * a.plane + idx;
+      fc[l] = s[0];
+      fm[l] = s[up];
+      fn[l] = s[dn];
+      fl[l] = s[xm];
+      fr[l] = s[xp];
+    }
+    T y0v[NL], accv[NL];
+    if (a.mode == 2 || a.mode == 3) {
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        y0v[l] = a.y0[(long long)l * a.plane + idx];
+        accv[l] = a.acc[(long long)l * a.plane + idx];
+      }
+    } else if (a.mode == 4) {
+#pragma unroll
+      for (int l = 0; l < NL; ++l) accv[l] = a.acc[(long long)l * a.plane + idx];
+    }
+    const double Ex = a.fin[x];
+    const T v = a.vtab[p];
+    T G[NL], Hh[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      if (p == 0) {
+        if (l == 0) {
+          const T fe = (fc[0] - fn[0] * a.p0p1_sq) * a.inv_mp0p1_sq;   // G00, def-VlasovMax.cpp:531-532
+          G[0] = (fn[0] - fe) * a.g_r;
+          Hh[0] = T(0);
+        } else {
+          G[l] = T(0);
+          Hh[l] = fn[l] * a.hp0[l];                                    // H(p0), :513-514
+        }
+      } else {
+        const T dpf = (p == np - 1) ? (fm[l] - fc[l]) : (fm[l] - fn[l]);  // Dp, def-State.cpp:359-380
+        if (l == 0) {
+          G[0] = dpf;
+          Hh[0] = T(0);
+        } else {
+          T hv = fc[l] * a.invpax[l * np + p];                           // MakeGH, :504-509
+          hv = hv + dpf;
+          Hh[l] = hv;
+          G[l] = dpf * a.cg[l] + hv;
+        }
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      T acc = T(0);
+      if (l >= 1) acc += G[l - 1] * static_cast<T>(Ex * a.eA1[l - 1]);
+      if (l + 1 < NL) acc += Hh[l + 1] * static_cast<T>(Ex * a.eA2[l + 1]);
+      if (l >= 1) acc += (fl[l - 1] - fr[l - 1]) * (v * static_cast<T>(a.xA1[l - 1]));
+      if (l + 1 < NL) acc += (fl[l + 1] - fr[l + 1]) * (v * static_cast<T>(a.xA2[l + 1]));
+      if (p < (l >= 1 ? min(l - 1, 80) : 0)) acc = T(0);  // CLEAN_LOW_PC, def-Actions.cpp:240-253
+      const long long o = (long long)l * a.plane + idx;
+      switch (a.mode) {  // def-RK.cpp:116-138
+        case 0: a.out[o] = acc; break;
+        case 1: { const T kh = acc * a.c05h; a.out[o] = fc[l] + kh; a.acc[o] = fc[l] + kh * a.third; } break;
+        case 2: { const T kh = acc * a.c05h; a.out[o] = y0v[l] + kh; a.acc[o] = accv[l] + kh * a.twothird; } break;
+        case 3: { const T kh = acc * a.hh; a.out[o] = y0v[l] + kh; a.acc[o] = accv[l] + kh * a.third; } break;
+        default: { const T kh = acc * a.h6; a.out[o] = accv[l] + kh; } break;
+      }
+    }
+    if (NL > 1) {  // Current::Jxyz integrand, trapezoid weights
+      const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
+      jv = w * (static_cast<double>(fc[1]) * a.p3og[p]);
+    }
+  }
+  jsh[row][lane] = jv;
+  __syncthreads();
+  if (row == 0 && x < nx) {
+    double s2 = jsh[0][lane];
+#pragma unroll
+    for (int k = 1; k < kRows1D; ++k) s2 += jsh[k][lane];
+    a.jpart[((long long)blockIdx.y * 3) * nx + x] = s2;
+  }
+}
+
 template <typename T>
 struct FieldArgs {
   const double* fin;
@@ -198,11 +328,43 @@ __device__ __forceinline__ void rk_combine(int mode, double k, double y0, double
   }
 }
 
+// partial sums of the current: 4 rows of 64 lanes split the p blocks, then
+// the four row sums are added in LDS (fixed order, deterministic)
+__device__ __forceinline__ void sum_parts(const double* jpart, int npb, int nxy, int i, int comp, int row,
+                                          double (&sh)[4][64], double* out) {
+  // up to 8 independent loads in flight per pass (npb <= 128 for Np <= 512)
+  double s = 0.0;
+  if (i < nxy) {
+    for (int b0 = row; b0 < npb; b0 += 32) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int b = b0 + 4 * k;
+        v[k] = (b < npb) ? jpart[((long long)b * 3 + comp) * nxy + i] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s += v[k];
+    }
+  }
+  sh[row][threadIdx.x] = s;
+  __syncthreads();
+  *out = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+  __syncthreads();
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void field_stage(FieldArgs<T> a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  const int row = threadIdx.y;
   const int nx = a.nx, ny = a.ny, nxy = nx * ny;
-  if (i >= nxy) return;
+  __shared__ double sh[4][64];
+  double jx = 0.0, jr = 0.0, ji = 0.0;
+  if (a.c10 >= 0) sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &jx);
+  if (a.c11r >= 0) {
+    sum_parts(a.jpart, a.npb, nxy, i, 1, row, sh, &jr);
+    sum_parts(a.jpart, a.npb, nxy, i, 2, row, sh, &ji);
+  }
+  if (i >= nxy || row != 0) return;
   const int x = i % nx, y = i / nx;
   const int xm = y * nx + (x == 0 ? nx - 1 : x - 1), xp = y * nx + (x == nx - 1 ? 0 : x + 1);
   const int ym = (y == 0 ? ny - 1 : y - 1) * nx + x, yp = (y == ny - 1 ? 0 : y + 1) * nx + x;
@@ -215,8 +377,7 @@ __global__ __launch_bounds__(256) void field_stage(FieldArgs<T> a) {
   double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   // Current::Jxyz (def-VlasovMax.cpp:680-745)
   if (a.c10 >= 0) {
-    double j = 0.0;
-    for (int b = 0; b < a.npb; ++b) j += a.jpart[((long long)b * 3 + 0) * nxy + i];
+    double j = jx;
     j *= a.delta_p;
     const long long o1 = (long long)a.c10 * a.plane + ((long long)y * a.np + 1) * nx + x;
     j += a.small * (static_cast<double>(a.fpl[o1]) * a.p3og1);
@@ -224,11 +385,6 @@ __global__ __launch_bounds__(256) void field_stage(FieldArgs<T> a) {
     k[0] += j;
   }
   if (a.c11r >= 0) {
-    double jr = 0.0, ji = 0.0;
-    for (int b = 0; b < a.npb; ++b) {
-      jr += a.jpart[((long long)b * 3 + 1) * nxy + i];
-      ji += a.jpart[((long long)b * 3 + 2) * nxy + i];
-    }
     jr *= a.delta_p;
     ji *= a.delta_p;
     const long long o1 = ((long long)y * a.np + 1) * nx + x;
@@ -304,24 +460,71 @@ __device__ __forceinline__ double zlog_ei(double ne, double Te, double density_n
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-// inclusive prefix over the 64 lanes of per-lane totals
-__device__ __forceinline__ double wave_scan_incl(double v) {
+// ---- wavefront data movement without LDS -----------------------------------
+// DPP row shifts move data inside each 16-lane row; readlane carries values
+// across the four rows.  Doubles move as two 32-bit halves.
+template <int N> constexpr int row_shl() { return 0x100 + N; }
+template <int N> constexpr int row_shr() { return 0x110 + N; }
+
+// lane i takes v of lane i -+ N inside its row; lanes without a source get
+// `fill` (selected explicitly, as rocPRIM's DPP scans do)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v, double fill) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), CTRL, 0xF, 0xF, false);
+  const double r = __hiloint2double(hi, lo);
+  const int rl = threadIdx.x & 15;
+  constexpr bool right = CTRL >= 0x110;  // row_shr: source is lane - N
+  constexpr int n = right ? CTRL - 0x110 : CTRL - 0x100;
+  const bool has_src = right ? (rl >= n) : (rl + n <= 15);
+  return has_src ? r : fill;
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+// value held by lane-1 (lane 0 gets fill)
+__device__ __forceinline__ double lane_prev(double v, double fill) {
+  double e = dpp_mov<row_shr<1>()>(v, fill);
+  const double b15 = lane_read(v, 15), b31 = lane_read(v, 31), b47 = lane_read(v, 47);
   const int lane = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
+  e = (lane == 16) ? b15 : (lane == 32) ? b31 : (lane == 48) ? b47 : e;
+  return e;
+}
+// value held by lane+1 (lane 63 gets fill)
+__device__ __forceinline__ double lane_next(double v, double fill) {
+  double e = dpp_mov<row_shl<1>()>(v, fill);
+  const double b16 = lane_read(v, 16), b32 = lane_read(v, 32), b48 = lane_read(v, 48);
+  const int lane = lane_id();
+  e = (lane == 15) ? b16 : (lane == 31) ? b32 : (lane == 47) ? b48 : e;
+  return e;
+}
+
+// inclusive prefix sum over the 64 lanes
+__device__ __forceinline__ double wave_scan_incl(double v) {
+  v += dpp_mov<row_shr<1>()>(v, 0.0);
+  v += dpp_mov<row_shr<2>()>(v, 0.0);
+  v += dpp_mov<row_shr<4>()>(v, 0.0);
+  v += dpp_mov<row_shr<8>()>(v, 0.0);
+  const double r0 = lane_read(v, 15), r1 = lane_read(v, 31), r2 = lane_read(v, 47);
+  const int row = lane_id() >> 4;
+  if (row == 1) v += r0;
+  else if (row == 2) v += (r0 + r1);
+  else if (row == 3) v += ((r0 + r1) + r2);
   return v;
 }
-// inclusive suffix over lanes
+// inclusive suffix sum over lanes
 __device__ __forceinline__ double wave_scan_incl_rev(double v) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double t = __shfl_down(v, o, 64);
-    if (lane + o < 64) v += t;
-  }
+  v += dpp_mov<row_shl<1>()>(v, 0.0);
+  v += dpp_mov<row_shl<2>()>(v, 0.0);
+  v += dpp_mov<row_shl<4>()>(v, 0.0);
+  v += dpp_mov<row_shl<8>()>(v, 0.0);
+  const double r1 = lane_read(v, 16), r2 = lane_read(v, 32), r3 = lane_read(v, 48);
+  const int row = lane_id() >> 4;
+  if (row == 2) v += r3;
+  else if (row == 1) v += (r2 + r3);
+  else if (row == 0) v += (r1 + (r2 + r3));
   return v;
 }
 
@@ -332,11 +535,10 @@ __device__ __forceinline__ void prefix_p(const double (&t)[NPL], double (&out)[N
 #pragma unroll
   for (int i = 0; i < NPL; ++i) { s += t[i]; out[i] = s; }
   const double inc = wave_scan_incl(s);
-  double ex = __shfl_up(inc, 1, 64);
-  if (lane_id() == 0) ex = 0.0;
+  const double ex = lane_prev(inc, 0.0);
 #pragma unroll
   for (int i = 0; i < NPL; ++i) out[i] += ex;
-  total = __shfl(inc, 63, 64);
+  total = lane_read(inc, 63);
 }
 // Suffix of t over p (inclusive).
 template <int NPL>
@@ -345,22 +547,21 @@ __device__ __forceinline__ void suffix_p(const double (&t)[NPL], double (&out)[N
 #pragma unroll
   for (int i = NPL - 1; i >= 0; --i) { s += t[i]; out[i] = s; }
   const double inc = wave_scan_incl_rev(s);
-  double ex = __shfl_down(inc, 1, 64);
-  if (lane_id() == 63) ex = 0.0;
+  const double ex = lane_next(inc, 0.0);
 #pragma unroll
   for (int i = 0; i < NPL; ++i) out[i] += ex;
 }
 // value of row p+1 (next row) for each held row
 template <int NPL>
 __device__ __forceinline__ void next_row(const double (&v)[NPL], double (&nx)[NPL]) {
-  const double first_of_next = __shfl_down(v[0], 1, 64);
+  const double first_of_next = lane_next(v[0], 0.0);
 #pragma unroll
   for (int i = 0; i < NPL - 1; ++i) nx[i] = v[i + 1];
   nx[NPL - 1] = first_of_next;
 }
 template <int NPL>
 __device__ __forceinline__ void prev_row(const double (&v)[NPL], double (&pv)[NPL]) {
-  const double last_of_prev = __shfl_up(v[NPL - 1], 1, 64);
+  const double last_of_prev = lane_prev(v[NPL - 1], 0.0);
 #pragma unroll
   for (int i = NPL - 1; i > 0; --i) pv[i] = v[i - 1];
   pv[0] = last_of_prev;
@@ -395,8 +596,8 @@ __device__ __forceinline__ void ee_slope(const CollArgs<T>& a, const EETab<NPL>&
   suffix_p<NPL>(t1, J1);
   const double lnee = log_ee(4.0 * M_PI * tot2, 4.0 * M_PI * tot4, a.density_np);
   // f(p0), f(p1) for the parabolic G at low v
-  const double f0 = __shfl(f[0], 0, 64);
-  const double f1 = (NPL >= 2) ? __shfl(f[NPL >= 2 ? 1 : 0], 0, 64) : __shfl(f[0], 1, 64);
+  const double f0 = lane_read(f[0], 0);
+  const double f1 = (NPL >= 2) ? lane_read(f[NPL >= 2 ? 1 : 0], 0) : lane_read(f[0], 1);
   const double f00 = (f0 - f1 * a.v0sq / a.v1sq) / a.g_den;
   double G[NPL];
 #pragma unroll
@@ -422,7 +623,7 @@ __device__ __forceinline__ void ee_slope(const CollArgs<T>& a, const EETab<NPL>&
     const int p = lane * NPL + i;
     D1[i] = (p < np - 1) ? G[i] - Gn[i] : G[i];
   }
-  const double G0 = __shfl(G[0], 0, 64);
+  const double G0 = lane_read(G[0], 0);
   double fh0 = (-1.0) * G0;
   fh0 *= 2.0 / a.v0sq;
   double D2[NPL];
@@ -431,7 +632,7 @@ __device__ __forceinline__ void ee_slope(const CollArgs<T>& a, const EETab<NPL>&
     const int p = lane * NPL + i;
     D2[i] = (p < np - 1) ? D1[i] * tb.Pn[i] : D1[i];
   }
-  fh0 -= __shfl(D2[0], 0, 64);
+  fh0 -= lane_read(D2[0], 0);
   double D2n[NPL], D3[NPL];
   next_row<NPL>(D2, D2n);
 #pragma unroll
@@ -482,18 +683,35 @@ __device__ __forceinline__ void tri_pivots(const double (&A)[NPL], const double 
     const Mob Mi = {0.0, C[i], -A[i], B[i]};
     P = mob_norm(mob_mul(Mi, P));
   }
-  // inclusive scan: S_j = P_j S_{j-1}
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    Mob q;
-    q.a = __shfl_up(P.a, o, 64); q.b = __shfl_up(P.b, o, 64);
-    q.c = __shfl_up(P.c, o, 64); q.d = __shfl_up(P.d, o, 64);
-    if (lane >= o) P = mob_norm(mob_mul(P, q));
+  // inclusive scan S_j = P_j S_{j-1}: four DPP steps inside each row ...
+#define OSHUN_MOB_STEP(N)                                                                    \
+  {                                                                                          \
+    Mob q;                                                                                   \
+    q.a = dpp_mov<row_shr<N>()>(P.a, 1.0); q.b = dpp_mov<row_shr<N>()>(P.b, 0.0);            \
+    q.c = dpp_mov<row_shr<N>()>(P.c, 0.0); q.d = dpp_mov<row_shr<N>()>(P.d, 1.0);            \
+    P = mob_norm(mob_mul(P, q));                                                             \
+  }
+  OSHUN_MOB_STEP(1)
+  OSHUN_MOB_STEP(2)
+  OSHUN_MOB_STEP(4)
+  OSHUN_MOB_STEP(8)
+#undef OSHUN_MOB_STEP
+  // ... then the products of the earlier rows
+  {
+    auto rd = [](const Mob& m, int l) {
+      return Mob{lane_read(m.a, l), lane_read(m.b, l), lane_read(m.c, l), lane_read(m.d, l)};
+    };
+    const Mob R0 = rd(P, 15), R1 = rd(P, 31), R2 = rd(P, 47);
+    const Mob C2 = mob_norm(mob_mul(R1, R0));
+    const Mob C3 = mob_norm(mob_mul(R2, C2));
+    const int row = lane >> 4;
+    if (row == 1) P = mob_norm(mob_mul(P, R0));
+    else if (row == 2) P = mob_norm(mob_mul(P, C2));
+    else if (row == 3) P = mob_norm(mob_mul(P, C3));
   }
   Mob E;
-  E.a = __shfl_up(P.a, 1, 64); E.b = __shfl_up(P.b, 1, 64);
-  E.c = __shfl_up(P.c, 1, 64); E.d = __shfl_up(P.d, 1, 64);
-  if (lane == 0) E = {1.0, 0.0, 0.0, 1.0};
+  E.a = lane_prev(P.a, 1.0); E.b = lane_prev(P.b, 0.0);
+  E.c = lane_prev(P.c, 0.0); E.d = lane_prev(P.d, 1.0);
   // state vector before this lane's first row: E @ [0, 1]
   double Pv = E.b, Qv = E.d;
 #pragma unroll
@@ -526,35 +744,84 @@ __device__ __forceinline__ void tri_solve(const double (&A)[NPL], const double (
     be = ali[i] * be + bei[i];
     al = ali[i] * al;
   }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double qa = __shfl_up(al, o, 64), qb = __shfl_up(be, o, 64);
-    if (lane >= o) { be = al * qb + be; al = al * qa; }
+#define OSHUN_AFF_FWD(N)                                                                         \
+  {                                                                                              \
+    const double qa = dpp_mov<row_shr<N>()>(al, 1.0), qb = dpp_mov<row_shr<N>()>(be, 0.0);       \
+    be = al * qb + be;                                                                           \
+    al = al * qa;                                                                                \
   }
-  double y = __shfl_up(be, 1, 64);
-  if (lane == 0) y = 0.0;
+  OSHUN_AFF_FWD(1)
+  OSHUN_AFF_FWD(2)
+  OSHUN_AFF_FWD(4)
+  OSHUN_AFF_FWD(8)
+#undef OSHUN_AFF_FWD
+  {
+    const double a15 = lane_read(al, 15), b15 = lane_read(be, 15);
+    const double a31 = lane_read(al, 31), b31 = lane_read(be, 31);
+    const double a47 = lane_read(al, 47), b47 = lane_read(be, 47);
+    const double c2a = a31 * a15, c2b = a31 * b15 + b31;  // rows 1 o 0
+    const double c3a = a47 * c2a, c3b = a47 * c2b + b47;  // rows 2 o 1 o 0
+    const int row = lane >> 4;
+    double ca = 1.0, cb = 0.0;
+    if (row == 1) { ca = a15; cb = b15; }
+    else if (row == 2) { ca = c2a; cb = c2b; }
+    else if (row == 3) { ca = c3a; cb = c3b; }
+    if (row > 0) { be = al * cb + be; al = al * ca; }
+  }
+  double y = lane_prev(be, 0.0);
 #pragma unroll
   for (int i = 0; i < NPL; ++i) { y = ali[i] * y + bei[i]; d[i] = y; }
   // backward affine scan x -> ga*x + de
   double ga = 1.0, de = 0.0;
 #pragma unroll
   for (int i = NPL - 1; i >= 0; --i) { de = -cp[i] * de + d[i]; ga = -cp[i] * ga; }
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const double qa = __shfl_down(ga, o, 64), qb = __shfl_down(de, o, 64);
-    if (lane + o < 64) { de = ga * qb + de; ga = ga * qa; }
+#define OSHUN_AFF_BWD(N)                                                                         \
+  {                                                                                              \
+    const double qa = dpp_mov<row_shl<N>()>(ga, 1.0), qb = dpp_mov<row_shl<N>()>(de, 0.0);       \
+    de = ga * qb + de;                                                                           \
+    ga = ga * qa;                                                                                \
   }
-  double x = __shfl_down(de, 1, 64);
-  if (lane == 63) x = 0.0;
+  OSHUN_AFF_BWD(1)
+  OSHUN_AFF_BWD(2)
+  OSHUN_AFF_BWD(4)
+  OSHUN_AFF_BWD(8)
+#undef OSHUN_AFF_BWD
+  {
+    const double a16 = lane_read(ga, 16), b16 = lane_read(de, 16);
+    const double a32 = lane_read(ga, 32), b32 = lane_read(de, 32);
+    const double a48 = lane_read(ga, 48), b48 = lane_read(de, 48);
+    const double c1a = a32 * a48, c1b = a32 * b48 + b32;  // rows 2 o 3
+    const double c0a = a16 * c1a, c0b = a16 * c1b + b16;  // rows 1 o 2 o 3
+    const int row = lane >> 4;
+    double ca = 1.0, cb = 0.0;
+    if (row == 2) { ca = a48; cb = b48; }
+    else if (row == 1) { ca = c1a; cb = c1b; }
+    else if (row == 0) { ca = c0a; cb = c0b; }
+    if (row < 3) { de = ga * cb + de; ga = ga * ca; }
+  }
+  double x = lane_next(de, 0.0);
 #pragma unroll
   for (int i = NPL - 1; i >= 0; --i) { x = d[i] - cp[i] * x; d[i] = x; }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD
+// works on one contiguous run of tiles and neighbouring x columns share its L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
+  constexpr int kXcd = 8;
+  if (nblocks % kXcd != 0) return bid;
+  return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
+}
+
+constexpr int kCollCols = 8;  // x columns (one wavefront each) per workgroup
+
 template <typename T, int NPL>
-__global__ __launch_bounds__(64) void collide(CollArgs<T> a) {
-  const int col = blockIdx.x;
+__global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
   const int nx = a.nx, np = a.np;
-  const int x = col % nx, y = col / nx;
+  const int ntx = (nx + kCollCols - 1) / kCollCols;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int y = tile / ntx;
+  const int x = (tile % ntx) * kCollCols + (threadIdx.x >> 6);
+  if (x >= nx) return;  // whole wavefront leaves together
   const int lane = lane_id();
   const long long base = (long long)y * np * nx + x;
   EETab<NPL> tb;
@@ -692,6 +959,12 @@ __global__ __launch_bounds__(64) void collide(CollArgs<T> a) {
       }
     }
   }
+}
+
+// plain copy, 16 B per lane per iteration (counter calibration)
+__global__ __launch_bounds__(256) void diag_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                 long long n) {
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) dst[i] = src[i];
 }
 
 }  // namespace oshun

@@ -44,6 +44,8 @@ struct oshun_ctx {
   int* d_comp_l = nullptr;
   void* d_vt = nullptr;  // Vlasov tables in storage precision: invpax | vtab | cg | hp0
   double* d_p3og = nullptr;
+  double* d_e1d = nullptr;  // 1D m0 = 0 coefficient rows: eA1 | eA2 | xA1 | xA2
+  bool use1d = false;       // specialised 1D m0 = 0 Vlasov kernel
   double* d_coll = nullptr;
   oshun::CollTables ct;
   hipStream_t stream = nullptr;
@@ -102,9 +104,70 @@ const T* vt_ptr(const oshun_ctx* c, int which) {
   }
 }
 
+constexpr int kMaxL0Special = 12;
+
+template <typename T, int L0>
+void launch_1d(oshun_ctx* c, const oshun::Stage1DArgs<T>& a, dim3 grid, dim3 block) {
+  hipLaunchKernelGGL((oshun::vlasov_stage_1d<T, L0>), grid, block, 0, c->stream, a);
+}
+
+template <typename T>
+int launch_stage_1d(oshun_ctx* c, int mode, const void* in, const void* y0, void* out, void* acc,
+                    const double* fin, double h) {
+  oshun::Stage1DArgs<T> a;
+  a.in = static_cast<const T*>(in);
+  a.y0 = static_cast<const T*>(y0);
+  a.out = static_cast<T*>(out);
+  a.acc = static_cast<T*>(acc);
+  a.fin = fin;
+  a.jpart = c->jpart;
+  a.invpax = vt_ptr<T>(c, 0);
+  a.vtab = vt_ptr<T>(c, 1);
+  a.cg = vt_ptr<T>(c, 2);
+  a.hp0 = vt_ptr<T>(c, 3);
+  a.p3og = c->d_p3og;
+  const size_t L = c->T.l0 + 1;
+  a.eA1 = c->d_e1d;
+  a.eA2 = c->d_e1d + L;
+  a.xA1 = c->d_e1d + 2 * L;
+  a.xA2 = c->d_e1d + 3 * L;
+  a.p0p1_sq = static_cast<T>(c->T.p0p1_sq);
+  a.inv_mp0p1_sq = static_cast<T>(c->T.inv_mp0p1_sq);
+  a.g_r = static_cast<T>(c->T.g_r);
+  a.c05h = static_cast<T>(0.5 * h);
+  a.hh = static_cast<T>(h);
+  a.h6 = static_cast<T>(h / 6.0);
+  a.third = static_cast<T>(1.0 / 3.0);
+  a.twothird = static_cast<T>(2.0 / 3.0);
+  a.np = c->np; a.nx = c->nx; a.mode = mode;
+  a.plane = c->plane;
+  dim3 block(64, oshun::kRows1D, 1);
+  dim3 grid((c->nx + 63) / 64, c->npb, 1);
+  ProfRec r;
+  prof_begin(c, 0, &r);
+  switch (c->T.l0) {
+    case 1: launch_1d<T, 1>(c, a, grid, block); break;
+    case 2: launch_1d<T, 2>(c, a, grid, block); break;
+    case 3: launch_1d<T, 3>(c, a, grid, block); break;
+    case 4: launch_1d<T, 4>(c, a, grid, block); break;
+    case 5: launch_1d<T, 5>(c, a, grid, block); break;
+    case 6: launch_1d<T, 6>(c, a, grid, block); break;
+    case 7: launch_1d<T, 7>(c, a, grid, block); break;
+    case 8: launch_1d<T, 8>(c, a, grid, block); break;
+    case 9: launch_1d<T, 9>(c, a, grid, block); break;
+    case 10: launch_1d<T, 10>(c, a, grid, block); break;
+    case 11: launch_1d<T, 11>(c, a, grid, block); break;
+    default: launch_1d<T, 12>(c, a, grid, block); break;
+  }
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  return OSHUN_OK;
+}
+
 template <typename T>
 int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* out, void* acc, const double* fin,
                  double h) {
+  if (c->use1d) return launch_stage_1d<T>(c, mode, in, y0, out, acc, fin, h);
   oshun::StageArgs<T> a;
   a.in = static_cast<const T*>(in);
   a.y0 = static_cast<const T*>(y0);
@@ -161,11 +224,10 @@ int launch_field(oshun_ctx* c, int mode, const double* fin, const double* f0, do
   a.delta_p = c->T.delta_p;
   a.midx = c->T.midx; a.midy = c->T.midy;
   a.c05h = 0.5 * h; a.hh = h; a.h6 = h / 6.0; a.third = 1.0 / 3.0; a.twothird = 2.0 / 3.0;
-  const int threads = 256;
-  dim3 grid((c->nxy + threads - 1) / threads);
+  dim3 grid((c->nxy + 63) / 64);
   ProfRec r;
   prof_begin(c, 1, &r);
-  hipLaunchKernelGGL(oshun::field_stage<T>, grid, dim3(threads), 0, c->stream, a);
+  hipLaunchKernelGGL(oshun::field_stage<T>, grid, dim3(64, 4), 0, c->stream, a);
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
@@ -195,7 +257,8 @@ template <typename T, int NPL>
 int launch_collide_npl(oshun_ctx* c, oshun::CollArgs<T>& a) {
   ProfRec r;
   prof_begin(c, 2, &r);
-  hipLaunchKernelGGL((oshun::collide<T, NPL>), dim3(c->nxy), dim3(64), 0, c->stream, a);
+  const int ntx = (c->nx + oshun::kCollCols - 1) / oshun::kCollCols;
+  hipLaunchKernelGGL((oshun::collide<T, NPL>), dim3(ntx * c->ny), dim3(64 * oshun::kCollCols), 0, c->stream, a);
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
@@ -336,7 +399,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -369,7 +432,9 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
   oshun::build_tables(c->T, g.l0, g.m0, g.nump, g.numy, g.pmin, g.pmax, g.dx, g.dy, phys->density_np,
                       phys->nb_algorithms, phys->ref_quirks);
   c->nc = c->T.nc; c->np = g.nump; c->nx = g.numx; c->ny = g.numy; c->nxy = g.numx * g.numy;
-  c->npb = (g.nump + oshun::kTileP - 1) / oshun::kTileP;
+  c->use1d = (g.m0 == 0 && g.numy == 1 && g.l0 <= kMaxL0Special);
+  const int prows = c->use1d ? oshun::kRows1D : oshun::kTileP;
+  c->npb = (g.nump + prows - 1) / prows;
   c->plane = (long long)g.numx * g.numy * g.nump;
   auto bail = [&](int rc) { free_all(c); std::string m = c->err; delete c; (void)m; return rc; };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(OSHUN_ERR_HIP);
@@ -391,6 +456,14 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
       upload_vec(c, &c->d_clean, c->T.clean) || upload_vec(c, &c->d_comp_l, c->T.comp_l) ||
       upload_vec(c, &c->d_p3og, c->T.p3og))
     return bail(OSHUN_ERR_HIP);
+  {
+    std::vector<double> e;
+    e.insert(e.end(), c->T.e1d_A1.begin(), c->T.e1d_A1.end());
+    e.insert(e.end(), c->T.e1d_A2.begin(), c->T.e1d_A2.end());
+    e.insert(e.end(), c->T.x1d_A1.begin(), c->T.x1d_A1.end());
+    e.insert(e.end(), c->T.x1d_A2.begin(), c->T.x1d_A2.end());
+    if (upload_vec(c, &c->d_e1d, e)) return bail(OSHUN_ERR_HIP);
+  }
   {
     std::vector<double> vt;
     vt.insert(vt.end(), c->T.invpax.begin(), c->T.invpax.end());
@@ -606,6 +679,20 @@ int oshun_profile_read(oshun_ctx* c, double* ms_total, long long* launches, int 
     c->prof_ms[k] = 0.0;
     c->prof_n[k] = 0;
   }
+  return OSHUN_OK;
+}
+
+int oshun_diag_copy(oshun_ctx* c, long long nbytes) {
+  if (!sane(c) || nbytes < 16) return OSHUN_ERR_ARG;
+  const long long cap = (long long)c->nc * c->plane * (long long)c->esz;
+  if (nbytes > cap) return fail(c, OSHUN_ERR_ARG, "diag copy larger than the state buffer");
+  const long long n = nbytes / 16;
+  ProfRec r;
+  prof_begin(c, 3, &r);
+  hipLaunchKernelGGL(oshun::diag_copy, dim3(2048), dim3(256), 0, c->stream, static_cast<const double2*>(c->B1),
+                     static_cast<double2*>(c->B2), n);
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
 

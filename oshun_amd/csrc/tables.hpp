@@ -55,6 +55,8 @@ struct HostTables {
   double p0p1_sq, inv_mp0p1_sq, g_r;
   double small, delta_p;
   double midx, midy;  // MaxwellEq -1/(2 dx), -1/(2 dy)
+  // 1D m0 = 0 coefficient rows (per source l): E-field G / H, x-advection to l+1 / l-1
+  std::vector<double> e1d_A1, e1d_A2, x1d_A1, x1d_A2;
   std::vector<Term> terms;
   std::vector<int> tstart;  // nc+1
   std::vector<int> clean;   // per plane: rows p < clean are zeroed (CLEAN_LOW_PC)
@@ -207,6 +209,15 @@ inline void build_tables(HostTables& T, int l0, int m0, int np, int ny, double p
     a = -1.0 / (2.0 * a);
     b = -1.0 / (2.0 * b);
     T.midx = a.real(); T.midy = b.real();
+  }
+
+  T.e1d_A1.assign(L + 1, 0.0); T.e1d_A2.assign(L + 1, 0.0);
+  T.x1d_A1.assign(L + 1, 0.0); T.x1d_A2.assign(L + 1, 0.0);
+  for (int l = 0; l <= L; ++l) {
+    T.e1d_A1[l] = eA1[LI(l, 0)].real();
+    T.e1d_A2[l] = eA2[LI(l, 0)].real();
+    T.x1d_A1[l] = aA1[LI(l, 0)].real();
+    T.x1d_A2[l] = aA2[LI(l, 0)].real();
   }
 
   // ---- coupling program
