@@ -178,6 +178,16 @@ def main():
                 for k, v in prof.items() if v[1]}
 
     r64 = roof(prof64, S64)
+    # HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE) on
+    # this workload, calibrated on a plain copy (profiles/traffic_*.json)
+    tpath = os.path.join(ROOT, "profiles", "traffic_l8_np256_nx4096_fp64.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        for name, v in tr.get("kernels", {}).items():
+            if name.startswith("oshun::vlasov_stage"):
+                r64["traffic"] = v["read_bytes"] + v["write_bytes"]
+                r64["traffic_source"] = os.path.relpath(tpath, ROOT)
     r32 = roof(prof32, S32)
     step_frac64 = 16.0 * S64 / (ms64 * 1e-3) / HBM_PEAK
     step_frac32 = 16.0 * S32 / (ms32 * 1e-3) / HBM_PEAK

@@ -98,6 +98,13 @@ int oshun_collide_flm(oshun_ctx* ctx, double tnew, int which);
 /* nsteps x main.cpp:79-107 (RK4, e-e, f1, flm; guards are implicit). */
 int oshun_step(oshun_ctx* ctx, double h, int nsteps);
 
+/* Nonrelativistic moments of the current state in double
+ * (Parallel_Output::nonrelativistic_moments, def-Parallel.cpp:1677-2134;
+ * the reference computes them in float).  out = double[7][ny][nx]:
+ * density, Vx, Vy, VxVx, VyVy, VzVz, Qx.  Used for conservation reports and
+ * the heat-flux objective. */
+int oshun_moments(oshun_ctx* ctx, double* out);
+
 /* Stream and device views, for callers that keep data on the GPU. */
 int oshun_synchronize(oshun_ctx* ctx);
 int oshun_get_stream(const oshun_ctx* ctx, void** hip_stream);

@@ -119,6 +119,7 @@ def _load():
         lib.oo_flm_coeffs.argtypes = [P, dp, D, dp, dp, dp, dp]
         lib.oo_tridiag.argtypes = [I, dp, dp, dp, dp, dp]
         lib.oo_axes.argtypes = [P, dp, dp, dp]
+        lib.oo_moments.argtypes = [P, dp]
         _lib = lib
     return _lib
 
@@ -201,6 +202,12 @@ class Oracle:
         out = [np.zeros(n) for _ in range(4)]
         self._lib.oo_flm_coeffs(self._h, _dptr(f00), float(dt), *[_dptr(o) for o in out])
         return out  # sub, diag, sup, scatt
+
+    def moments(self):
+        """density, Vx, Vy, VxVx, VyVy, VzVz, Qx per (y, x) (def-Parallel.cpp:1677-2134, FP64)."""
+        out = np.zeros((7, self.cfg.ny, self.cfg.nx))
+        self._lib.oo_moments(self._h, _dptr(out))
+        return out
 
     def axes(self):
         n = self.cfg.np

@@ -795,6 +795,83 @@ struct Oracle {
       }
   }
 
+  // Nonrelativistic moments of Parallel_Output::nonrelativistic_moments,
+  // def-Parallel.cpp:1677-2134, in double instead of float.  out[k][y][x]:
+  // k = 0 density, 1 Vx, 2 Vy, 3 VxVx, 4 VyVy, 5 VzVz, 6 Qx.
+  // VyVy takes its 16pi/5 term from f20 as the reference does (:1875-1886).
+  void moments(double* out) const {
+    const std::vector<double>& pr = pd;
+    const int N = np;
+    const double dp = dpd;
+    std::vector<double> ig(N);
+    for (int i = 0; i < N; ++i) ig[i] = 1.0 / std::sqrt(1.0 + pr[i] * pr[i]);
+    const double r = pr[0] * pr[0] / (pr[1] * pr[1]);
+    const double inv = 1.0 / (1.0 - r);
+    auto trap = [&](const cd* f, int p0, double (*w)(double, double), bool real_part) {
+      (void)real_part;
+      double s = (p0 == 0 ? f[0].real() * w(pr[0], ig[0]) : 0.0);
+      s += f[N - 1].real() * w(pr[N - 1], ig[N - 1]);
+      s *= 0.5;
+      for (int i = 1; i < N - 1; ++i) s += w(pr[i], ig[i]) * f[i].real();
+      return s;
+    };
+    auto w2 = [](double p, double g) { (void)g; return p * p; };
+    auto w3 = [](double p, double g) { return std::pow(p, 3) * g; };
+    auto w4 = [](double p, double g) { return std::pow(p, 4) * std::pow(g, 2); };
+    auto w5 = [](double p, double g) { return std::pow(p, 5) * std::pow(g, 3); };
+    const size_t nxy_ = (size_t)nx * ny;
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const size_t k = (size_t)iy * nx + ix;
+        const cd* f0 = Y.h(H(0, 0)) + at(0, ix, iy);
+        const cd* f10 = Y.h(H(1, 0)) + at(0, ix, iy);
+        const cd* f11 = c.m0 >= 1 ? Y.h(H(1, 1)) + at(0, ix, iy) : nullptr;
+        const cd* f20 = c.l0 >= 2 ? Y.h(H(2, 0)) + at(0, ix, iy) : nullptr;
+        // density, :1699-1726
+        double n = trap(f0, 0, w2, true);
+        double c00 = f0[1].real() / 5.0 * r + (f0[0].real() - f0[1].real() * r) * inv * (1.0 / 3.0 - 1.0 / 5.0 * r);
+        c00 *= pr[0] * (pr[0] / dp) * pr[0];
+        n += c00;
+        n *= 4.0 * M_PI * dp;
+        if (n < 4.0 * 1.1920928955078125e-07) n = 1.0;
+        // Vx, Vy, :1731-1800
+        double vx = trap(f10, 0, w3, true) + f10[1].real() * (std::pow(pr[0], 5) / (dp * pr[1] * 5.0));
+        vx *= (4.0 * M_PI / 3.0) * dp;
+        vx /= n;
+        double vy = 0.0;
+        if (f11) {
+          vy = trap(f11, 0, w3, true) + f11[1].real() * (std::pow(pr[0], 5) / (dp * pr[1] * 5.0));
+          vy *= (8.0 * M_PI / 3.0) * dp;
+          vy /= n;
+        }
+        // VxVx, VyVy, VzVz, :1805-1916
+        double m1 = trap(f0, 0, w4, true);
+        double c01 = f0[1].real() / 7.0 * r + (f0[0].real() - f0[1].real() * r) * inv * (1.0 / 5.0 - 1.0 / 7.0 * r);
+        c01 *= std::pow(pr[0], 4) * (pr[0] / dp);
+        m1 += c01;
+        m1 *= 4.0 * M_PI / 3.0 * dp;
+        double m2 = f20 ? trap(f20, 1, w4, true) * (8.0 * M_PI / 15.0 * dp) : 0.0;
+        const double vxvx = (m1 + m2) / n;
+        m2 *= 0.5;
+        const double m1b = m1 - m2;
+        const double t22 = f20 ? trap(f20, 1, w4, true) * (16.0 * M_PI / 5.0 * dp) : 0.0;
+        const double vyvy = (t22 + m1b) / n;
+        const double vzvz = (m1b - t22) / n;
+        // Qx, :2094-2134 (VxVy = 0 here: it needs f21)
+        double q = 0.0;
+        q -= vx * (vxvx + vyvy + vzvz);
+        q -= 2.0 * (vxvx * vx);
+        q += 2.0 * (vx * vx + vy * vy) * vx;
+        q *= n;
+        double m5 = trap(f10, 0, w5, true) + f10[1].real() * (std::pow(pr[0], 7) / (dp * pr[1] * 7.0));
+        m5 *= (4.0 * M_PI / 3.0) * dp;
+        q += m5;
+        q *= 0.5;
+        const double vals[7] = {n, vx, vy, vxvx, vyvy, vzvz, q};
+        for (int j = 0; j < 7; ++j) out[j * nxy_ + k] = vals[j];
+      }
+  }
+
   // main_loop step body, main.cpp:79-107
   void step(double h) {
     rk4(h);
@@ -860,6 +937,7 @@ void oo_step(void* o, double h, int nsteps) {
   Oracle* q = static_cast<Oracle*>(o);
   for (int s = 0; s < nsteps; ++s) q->step(h);
 }
+void oo_moments(void* o, double* out) { static_cast<Oracle*>(o)->moments(out); }
 // Single-column pieces for closed-form tests.
 void oo_ee_getslope(void* o, const double* fin, double* fh) { static_cast<Oracle*>(o)->getslope(fin, fh); }
 void oo_flm_solve(void* o, const double* f00, double Dt, int el, double* col_re_im) {

@@ -108,6 +108,7 @@ def _load():
     lib.oshun_profile_enable.argtypes = [P, I]
     lib.oshun_profile_read.argtypes = [P, dp, ctypes.POINTER(ctypes.c_longlong), I]
     lib.oshun_diag_copy.argtypes = [P, ctypes.c_longlong]
+    lib.oshun_moments.argtypes = [P, dp]
     _lib = lib
     return lib
 
@@ -173,6 +174,12 @@ class Context:
                                                  _dptr(fld.view(np.float64)), nbc))
         return f, fld
 
+    def fields(self):
+        """Ex, Ey, Ez, Bx, By, Bz as float64[6, ny, nx] (no distribution transfer)."""
+        fld = np.zeros(self.fld_shape(), np.complex128)
+        self._chk(self._lib.oshun_download_state(self._h, None, _dptr(fld.view(np.float64)), 0))
+        return fld.real.copy()
+
     def set_time(self, t):
         self._chk(self._lib.oshun_set_time(self._h, float(t)))
 
@@ -224,6 +231,12 @@ class Context:
         info = self.plane_info()
         esz = 4 if self.precision == FP32_SPLIT else 8
         return info["ncomp"] * info["plane"] * esz
+
+    def moments(self):
+        """density, Vx, Vy, VxVx, VyVy, VzVz, Qx per (y, x), FP64 (def-Parallel.cpp:1677-2134)."""
+        out = np.zeros((7, self.ny, self.nx))
+        self._chk(self._lib.oshun_moments(self._h, _dptr(out)))
+        return out
 
     def diag_copy(self, nbytes):
         """Plain device copy of nbytes between scratch buffers (counter calibration)."""

@@ -961,6 +961,85 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
   }
 }
 
+// Nonrelativistic moments, Parallel_Output::nonrelativistic_moments
+// (def-Parallel.cpp:1677-2134) in double: one thread per (x,y), trapezoid
+// sums along p with coalesced loads along x.  out[k][y][x], k = density, Vx,
+// Vy, VxVx, VyVy, VzVz, Qx.
+template <typename T>
+struct MomArgs {
+  const T* f;
+  double* out;
+  const double *w2, *w3, *w4, *w5;
+  int np, nx, ny, c00, c10, c11r, c20;
+  long long plane;
+  double dp, r, inv, c5, c7, c00k, c04k;
+};
+
+template <typename T>
+__device__ __forceinline__ double trap_p(const T* f, long long base, int nx, int np, const double* w, bool skip0) {
+  double s = skip0 ? 0.0 : static_cast<double>(f[base]) * w[0];
+  s += static_cast<double>(f[base + (long long)(np - 1) * nx]) * w[np - 1];
+  s *= 0.5;
+  for (int p = 1; p < np - 1; ++p) s += w[p] * static_cast<double>(f[base + (long long)p * nx]);
+  return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void moments_kernel(MomArgs<T> a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int nxy = a.nx * a.ny;
+  if (i >= nxy) return;
+  const int x = i % a.nx, y = i / a.nx;
+  const long long base = (long long)y * a.np * a.nx + x;
+  const int nx = a.nx, np = a.np;
+  const T* f0 = a.f + (long long)a.c00 * a.plane;
+  const T* f10 = a.f + (long long)a.c10 * a.plane;
+  const double f0_0 = static_cast<double>(f0[base]), f0_1 = static_cast<double>(f0[base + nx]);
+  double n = trap_p<T>(f0, base, nx, np, a.w2, false);
+  double c = f0_1 / 5.0 * a.r + (f0_0 - f0_1 * a.r) * a.inv * (1.0 / 3.0 - 1.0 / 5.0 * a.r);
+  c *= a.c00k;
+  n += c;
+  n *= 4.0 * M_PI * a.dp;
+  if (n < 4.0 * 1.1920928955078125e-07) n = 1.0;
+  const double f10_1 = static_cast<double>(f10[base + nx]);
+  double vx = trap_p<T>(f10, base, nx, np, a.w3, false) + f10_1 * a.c5;
+  vx *= (4.0 * M_PI / 3.0) * a.dp;
+  vx /= n;
+  double vy = 0.0;
+  if (a.c11r >= 0) {
+    const T* f11 = a.f + (long long)a.c11r * a.plane;
+    vy = trap_p<T>(f11, base, nx, np, a.w3, false) + static_cast<double>(f11[base + nx]) * a.c5;
+    vy *= (8.0 * M_PI / 3.0) * a.dp;
+    vy /= n;
+  }
+  double m1 = trap_p<T>(f0, base, nx, np, a.w4, false);
+  double c1 = f0_1 / 7.0 * a.r + (f0_0 - f0_1 * a.r) * a.inv * (1.0 / 5.0 - 1.0 / 7.0 * a.r);
+  c1 *= a.c04k;
+  m1 += c1;
+  m1 *= 4.0 * M_PI / 3.0 * a.dp;
+  double t20 = 0.0;
+  if (a.c20 >= 0) t20 = trap_p<T>(a.f + (long long)a.c20 * a.plane, base, nx, np, a.w4, true);
+  double m2 = t20 * (8.0 * M_PI / 15.0 * a.dp);
+  const double vxvx = (m1 + m2) / n;
+  m2 *= 0.5;
+  const double m1b = m1 - m2;
+  const double t22 = t20 * (16.0 * M_PI / 5.0 * a.dp);  // f20 as in def-Parallel.cpp:1875-1886
+  const double vyvy = (t22 + m1b) / n;
+  const double vzvz = (m1b - t22) / n;
+  double q = 0.0;
+  q -= vx * (vxvx + vyvy + vzvz);
+  q -= 2.0 * (vxvx * vx);
+  q += 2.0 * (vx * vx + vy * vy) * vx;
+  q *= n;
+  double m5 = trap_p<T>(f10, base, nx, np, a.w5, false) + f10_1 * a.c7;
+  m5 *= (4.0 * M_PI / 3.0) * a.dp;
+  q += m5;
+  q *= 0.5;
+  const double vals[7] = {n, vx, vy, vxvx, vyvy, vzvz, q};
+#pragma unroll
+  for (int k = 0; k < 7; ++k) a.out[(long long)k * nxy + i] = vals[k];
+}
+
 // plain copy, 16 B per lane per iteration (counter calibration)
 __global__ __launch_bounds__(256) void diag_copy(const double2* __restrict__ src, double2* __restrict__ dst,
                                                  long long n) {

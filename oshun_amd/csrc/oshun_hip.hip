@@ -45,6 +45,8 @@ struct oshun_ctx {
   void* d_vt = nullptr;  // Vlasov tables in storage precision: invpax | vtab | cg | hp0
   double* d_p3og = nullptr;
   double* d_e1d = nullptr;  // 1D m0 = 0 coefficient rows: eA1 | eA2 | xA1 | xA2
+  double* d_mw = nullptr;   // moment weights w2 | w3 | w4 | w5
+  double* d_mom = nullptr;  // moments output, 7 planes of nxy
   bool use1d = false;       // specialised 1D m0 = 0 Vlasov kernel
   double* d_coll = nullptr;
   oshun::CollTables ct;
@@ -399,7 +401,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -410,6 +412,28 @@ void free_all(oshun_ctx* c) {
 bool sane(const oshun_ctx* c) { return c && c->S; }
 
 }  // namespace
+
+template <typename T>
+static int launch_moments(oshun_ctx* c) {
+  oshun::MomArgs<T> a;
+  a.f = static_cast<const T*>(c->S);
+  a.out = c->d_mom;
+  const size_t n = c->np;
+  a.w2 = c->d_mw; a.w3 = c->d_mw + n; a.w4 = c->d_mw + 2 * n; a.w5 = c->d_mw + 3 * n;
+  a.np = c->np; a.nx = c->nx; a.ny = c->ny;
+  const int m0 = c->T.m0;
+  a.c00 = c->T.comp_re[0];
+  a.c10 = c->T.comp_re[oshun::hind(1, 0, m0)];
+  a.c11r = m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, m0)] : -1;
+  a.c20 = c->T.l0 >= 2 ? c->T.comp_re[oshun::hind(2, 0, m0)] : -1;
+  a.plane = c->plane;
+  a.dp = c->T.dpd; a.r = c->T.m_r; a.inv = c->T.m_inv; a.c5 = c->T.m_c5; a.c7 = c->T.m_c7;
+  a.c00k = c->T.m_c00; a.c04k = c->T.m_c04;
+  hipLaunchKernelGGL(oshun::moments_kernel<T>, dim3((c->nxy + 255) / 256), dim3(256), 0, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  return OSHUN_OK;
+}
+
 
 extern "C" {
 
@@ -463,6 +487,12 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     e.insert(e.end(), c->T.x1d_A1.begin(), c->T.x1d_A1.end());
     e.insert(e.end(), c->T.x1d_A2.begin(), c->T.x1d_A2.end());
     if (upload_vec(c, &c->d_e1d, e)) return bail(OSHUN_ERR_HIP);
+  }
+  {
+    std::vector<double> w;
+    for (auto* t : {&c->T.mw2, &c->T.mw3, &c->T.mw4, &c->T.mw5}) w.insert(w.end(), t->begin(), t->end());
+    if (upload_vec(c, &c->d_mw, w)) return bail(OSHUN_ERR_HIP);
+    if (hipMalloc((void**)&c->d_mom, (size_t)7 * c->nxy * sizeof(double)) != hipSuccess) return bail(OSHUN_ERR_ALLOC);
   }
   {
     std::vector<double> vt;
@@ -679,6 +709,15 @@ int oshun_profile_read(oshun_ctx* c, double* ms_total, long long* launches, int 
     c->prof_ms[k] = 0.0;
     c->prof_n[k] = 0;
   }
+  return OSHUN_OK;
+}
+
+int oshun_moments(oshun_ctx* c, double* out) {
+  if (!sane(c) || !out) return OSHUN_ERR_ARG;
+  int rc = c->fp32 ? launch_moments<float>(c) : launch_moments<double>(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, c->d_mom, (size_t)7 * c->nxy * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return OSHUN_OK;
 }
 

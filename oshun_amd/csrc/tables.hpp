@@ -64,6 +64,9 @@ struct HostTables {
   std::vector<double> vc, U4, U4m1, U2, U2m1, U1s, U1m1s, U3, Qn, Pn;
   std::vector<double> ivdm, ivdp, ivs2;  // tridiagonal geometric factors
   std::vector<double> pw3, pw5, pw7;     // pow(v,3|5|7) for the parabolic G
+  // moment weights on the double axis (def-Parallel.cpp:1699-2134): p^2, p^3/g, p^4/g^2, p^5/g^3
+  std::vector<double> mw2, mw3, mw4, mw5;
+  double m_r, m_inv, m_c5, m_c7, m_c04, m_c00;
   double c_kpre, kpre, v0sq, v1sq, g_den;
 };
 
@@ -370,6 +373,24 @@ inline void build_tables(HostTables& T, int l0, int m0, int np, int ny, double p
   T.pw3.assign(np, 0.0); T.pw5.assign(np, 0.0); T.pw7.assign(np, 0.0);
   for (int i = 0; i < std::min(nb, np); ++i) {
     T.pw3[i] = std::pow(v[i], 3); T.pw5[i] = std::pow(v[i], 5); T.pw7[i] = std::pow(v[i], 7);
+  }
+  {
+    const auto& pr = T.pd;
+    T.mw2.resize(np); T.mw3.resize(np); T.mw4.resize(np); T.mw5.resize(np);
+    for (int i = 0; i < np; ++i) {
+      const double ig = 1.0 / std::sqrt(1.0 + pr[i] * pr[i]);
+      T.mw2[i] = pr[i] * pr[i];
+      T.mw3[i] = std::pow(pr[i], 3) * ig;
+      T.mw4[i] = std::pow(pr[i], 4) * std::pow(ig, 2);
+      T.mw5[i] = std::pow(pr[i], 5) * std::pow(ig, 3);
+    }
+    const double dp = T.dpd;
+    T.m_r = pr[0] * pr[0] / (pr[1] * pr[1]);
+    T.m_inv = 1.0 / (1.0 - T.m_r);
+    T.m_c5 = std::pow(pr[0], 5) / (dp * pr[1] * 5.0);
+    T.m_c7 = std::pow(pr[0], 7) / (dp * pr[1] * 7.0);
+    T.m_c00 = pr[0] * (pr[0] / dp) * pr[0];
+    T.m_c04 = std::pow(pr[0], 4) * (pr[0] / dp);
   }
   T.v0sq = v[0] * v[0];
   T.v1sq = v[1] * v[1];

@@ -187,3 +187,41 @@ def test_fp32_split_close_to_fp64():
     print("fp32 split rel L2: f00", e_f00, "Ex", e_ex)
     assert e_f00 < 1e-5
     assert e_ex < 1e-4
+
+
+def test_moments_vs_oracle():
+    import oshun_amd
+    l0, m0, np_, nx, ny = 3, 1, 128, 16, 3
+    f, fld = _maxwellian_state(l0, m0, np_, nx, ny, 0.3, seed=9)
+    o = Oracle(Config(l0=l0, m0=m0, np=np_, nx=nx, ny=ny, pmax=0.3))
+    o.set_state(f, fld)
+    mo = o.moments()
+    ctx = oshun_amd.Context(l0, m0, np_, nx, ny, pmax=0.3)
+    ctx.upload(f, fld)
+    mg = ctx.moments()
+    for k in range(7):
+        assert rel(mg[k], mo[k]) < TOL, k
+
+
+def test_conservation_at_target_size():
+    """Full target shape (l0=8, Np=256, Nx=4096, collisions on), 20 steps:
+    total number and energy are conserved (size-independent properties)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    import oshun_amd
+    f, fld, dx = bench.target_state()
+    ctx = bench.make_ctx(oshun_amd.FP64, dx)
+    ctx.upload(f, fld)
+    m0 = ctx.moments()
+    ctx.step(bench.H, 20)
+    m1 = ctx.moments()
+    N0, N1 = m0[0].sum(), m1[0].sum()
+    E0 = (m0[0] * (m0[3] + m0[4] + m0[5])).sum()
+    E1 = (m1[0] * (m1[3] + m1[4] + m1[5])).sum()
+    print("dN/N", (N1 - N0) / N0, "dE/E", (E1 - E0) / E0)
+    assert abs(N1 - N0) < 1e-9 * N0
+    assert abs(E1 - E0) < 1e-6 * E0
+    g, gf = ctx.download()
+    assert np.all(np.isfinite(g.real))

@@ -136,3 +136,25 @@ def test_density_conserved_by_full_step():
     # advection moves f00 in x only; e-e conserves its own discrete number.
     # The p-end closures leak a little; the bound is loose but catches sign slips.
     assert abs(n1 - n0) < 1e-6 * abs(n0)
+
+
+def test_moments_closed_forms():
+    """FP64 moments (def-Parallel.cpp:1677-2134) on a drifting Maxwellian:
+    n = 1, <v_x^2> = T/m up to the p/gamma correction, Vx = u, and Qx = 0 at rest."""
+    np_, nx = 256, 4
+    o = Oracle(Config(l0=2, m0=0, np=np_, nx=nx, pmax=0.3))
+    pd, _, _ = o.axes()
+    pt = np.sqrt(1.0 / 511.0)
+    f = np.zeros((3, 1, nx, np_), np.complex128)
+    f[0, 0] = _maxwellian(pd)[None, :]
+    o.set_state(f, np.zeros((6, 1, nx)))
+    m = o.moments()
+    assert abs(m[0].mean() - 1.0) < 1e-5
+    assert abs(m[3].mean() / pt ** 2 - 1.0) < 0.02          # relativistic 1/gamma^2 weight
+    assert np.allclose(m[3], m[4]) and np.allclose(m[4], m[5])  # isotropic
+    assert np.abs(m[1]).max() == 0.0 and np.abs(m[6]).max() == 0.0
+    u = 0.01 * pt
+    f[1, 0] = u * pd / pt ** 2 * f[0, 0].real                 # first harmonic of a slow drift
+    o.set_state(f, np.zeros((6, 1, nx)))
+    m = o.moments()
+    assert abs(m[1].mean() / u - 1.0) < 0.01
