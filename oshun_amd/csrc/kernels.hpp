@@ -429,6 +429,8 @@ struct CollArgs {
   int do_ee, do_f1, do_flm, numh;
   double h_ee, dt_flm;
   double c_kpre, kpre, density_np, zeta, v0sq, v1sq, g_den;
+  const double* rt;             // collide_rows tables, interleaved for its NPL
+  double r01, inv_gden, c2v0;   // v0^2/v1^2, 1/g_den, 2/v0^2
 };
 
 __device__ __forceinline__ double log_ee(double ne, double Te, double density_np) {
@@ -956,6 +958,360 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
       for (int i = 0; i < NPL; ++i) {
         const int p = lane * NPL + i;
         if (p < np) fc[base + (long long)p * nx] = static_cast<T>(d[i]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// collide_rows: the collision step with 16 lanes per (x,y) column, i.e. one
+// 16-lane DPP row per column and four columns per wavefront.  Each lane owns
+// NPL consecutive p rows and runs the reference's recurrences over them; the
+// only cross-lane steps are 4-step DPP scans inside the row and row_newbcast
+// for single values.  The p tables sit in LDS in an interleaved order (row i
+// of every lane is one 128-byte line), all p < np masks are folded into the
+// tables (zero where the reference's loops skip a row), and divisions by
+// tabulated or uniform quantities are multiplications by reciprocals.
+constexpr int kRowCols = 16;  // columns per workgroup (4 wavefronts x 4 rows)
+constexpr int kRowP = 256;    // LDS table length: 16 lanes x NPL rows
+
+template <int N>
+__device__ __forceinline__ double row_bcast(double v) {  // lane N of the 16-lane row, to the whole row
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + N, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + N, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double row_scan(double v) {  // inclusive prefix over the row
+  v += dpp_mov<row_shr<1>()>(v, 0.0);
+  v += dpp_mov<row_shr<2>()>(v, 0.0);
+  v += dpp_mov<row_shr<4>()>(v, 0.0);
+  v += dpp_mov<row_shr<8>()>(v, 0.0);
+  return v;
+}
+__device__ __forceinline__ double row_scan_rev(double v) {  // inclusive suffix over the row
+  v += dpp_mov<row_shl<1>()>(v, 0.0);
+  v += dpp_mov<row_shl<2>()>(v, 0.0);
+  v += dpp_mov<row_shl<4>()>(v, 0.0);
+  v += dpp_mov<row_shl<8>()>(v, 0.0);
+  return v;
+}
+// 1/d by v_rcp_f64 and two Newton steps (within an ulp or so of the correctly rounded quotient)
+__device__ __forceinline__ double rcp_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+}
+
+// LDS/host table rows of collide_rows; element p of row k sits at
+// k*kRowP + (p % NPL)*16 + p / NPL.
+enum {
+  RT_U4 = 0,  // 0.5 v^4 dv          (0 at p = 0 and p >= np)
+  RT_U4M1,    // 0.5 v_{p-1}^4 dv    (same zeros)
+  RT_U2,      // 0.5 v^2 dv
+  RT_U2M1,
+  RT_U1S,     // U1[p+1]            (0 at p >= np-1)
+  RT_U1M1S,   // U1m1[p+1]
+  RT_U3,      // v^3                 (1 beyond np)
+  RT_PN,      // Pn, 1 at np-1, 0 beyond
+  RT_QN,      // Qn, 0 at np-1 and beyond
+  RT_VC,      // v                   (1 beyond np)
+  RT_IV2,     // 1/v^2               (1 beyond np)
+  RT_IU3,     // 1/v^3               (0 beyond np)
+  RT_IVDM,    // IvDnDm1             (0 at p = 0 and p >= np-1)
+  RT_IVDP,    // IvDnDp1
+  RT_IVS2,    // Ivsq2Dn
+  RT_ROW,     // 1 for p < np-1, else 0 (Alpha_Tri row np-1 is empty)
+  RT_P3,      // v^3 for p < NB (low-p G)
+  RT_A2,      // v^3/3
+  RT_B2,      // 0.2 v^5/v1^2
+  RT_A4,      // 0.2 v^5
+  RT_B4,      // v^7/(7 v1^2)
+  RT_COUNT
+};
+
+template <int NPL>
+__device__ __forceinline__ double rt(const double* tb, int k, int i, int seg) {
+  return tb[k * kRowP + i * 16 + seg];
+}
+
+// EE_isotropic_conserva::Getslope (def-FokkerPlanck.cpp:192-267) for the NPL
+// rows p = seg*NPL + i of one lane; f in, slope out (same array)
+template <typename T, int NPL>
+__device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double* tb, int seg, double (&f)[NPL]) {
+  // re-read the tables from LDS on every call instead of letting the
+  // compiler keep all 9*NPL of them in registers across the subcycles
+  __asm__ volatile("" ::: "memory");
+  const double fprev = dpp_mov<row_shr<1>()>(f[NPL - 1], 0.0);
+  const double fnext = dpp_mov<row_shl<1>()>(f[0], 0.0);
+  // trapezoid prefix sums I4, I2 (from p = 1) and suffix J1, lane-local first
+  double I4[NPL], I2[NPL], J1[NPL];
+  double s4 = 0.0, s2 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    const double fm = (i > 0) ? f[i - 1] : fprev;
+    s4 += rt<NPL>(tb, RT_U4, i, seg) * f[i] + rt<NPL>(tb, RT_U4M1, i, seg) * fm;
+    s2 += rt<NPL>(tb, RT_U2, i, seg) * f[i] + rt<NPL>(tb, RT_U2M1, i, seg) * fm;
+    I4[i] = s4;
+    I2[i] = s2;
+  }
+#pragma unroll
+  for (int i = NPL - 1; i >= 0; --i) {
+    const double fn = (i < NPL - 1) ? f[i + 1] : fnext;
+    s1 += rt<NPL>(tb, RT_U1S, i, seg) * fn + rt<NPL>(tb, RT_U1M1S, i, seg) * f[i];
+    J1[i] = s1;
+  }
+  const double inc4 = row_scan(s4), inc2 = row_scan(s2), inc1 = row_scan_rev(s1);
+  const double ex4 = dpp_mov<row_shr<1>()>(inc4, 0.0), ex2 = dpp_mov<row_shr<1>()>(inc2, 0.0);
+  const double ex1 = dpp_mov<row_shl<1>()>(inc1, 0.0);
+  const double lnee = log_ee(4.0 * M_PI * row_bcast<15>(inc2), 4.0 * M_PI * row_bcast<15>(inc4), a.density_np);
+  const double f0 = row_bcast<0>(f[0]);
+  const double f1 = row_bcast<0>(f[1]);
+  const double f00 = (f0 - f1 * a.r01) * a.inv_gden;
+  double G[NPL];
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    const double j1 = J1[i] + ex1;
+    double i2 = (I2[i] + ex2) * j1;
+    i2 *= -3.0;
+    double g = (I4[i] + ex4) + rt<NPL>(tb, RT_U3, i, seg) * j1;
+    g *= f[i];
+    g += i2;
+    if (i < a.nb) {  // parabolic G for p < NB (EE_isotropic_conserva::G, :178-188); only lane 0 has such rows
+      const double p3 = tb[RT_P3 * kRowP + i * 16];
+      const double i2s = f00 * tb[RT_A2 * kRowP + i * 16] + (f1 - f00) * tb[RT_B2 * kRowP + i * 16];
+      const double i4s = f00 * tb[RT_A4 * kRowP + i * 16] + (f1 - f00) * tb[RT_B4 * kRowP + i * 16];
+      const double gl = f[i] * i4s + (p3 * f[i] - 3.0 * i2s) * j1;
+      g = (seg == 0) ? gl : g;
+    }
+    G[i] = g;
+  }
+  // -DG, -DG/(v Dv), DDG/(v Dv), DDG/(v^3 DDv)   (:247-260)
+  const double Gnext = dpp_mov<row_shl<1>()>(G[0], 0.0);
+  double D2[NPL];
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    const double gn = (i < NPL - 1) ? G[i + 1] : Gnext;
+    D2[i] = (G[i] - gn) * rt<NPL>(tb, RT_PN, i, seg);
+  }
+  double fh0 = (-1.0) * row_bcast<0>(G[0]);
+  fh0 *= a.c2v0;
+  fh0 -= row_bcast<0>(D2[0]);
+  fh0 *= tb[RT_QN * kRowP];
+  const double D2next = dpp_mov<row_shl<1>()>(D2[0], 0.0);
+  double D3[NPL];
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) D3[i] = D2[i] - ((i < NPL - 1) ? D2[i + 1] : D2next);
+  const double D3prev = dpp_mov<row_shr<1>()>(D3[NPL - 1], 0.0);
+  const double sc = a.c_kpre * lnee;
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    double v = rt<NPL>(tb, RT_QN, i, seg) * ((i > 0) ? D3[i - 1] : D3prev);
+    if (i == 0) v = (seg == 0) ? fh0 : v;
+    f[i] = v * sc;
+  }
+}
+
+// MODE 1: e-e subcycles on f_00 (Explicit_EC_FP::loop); MODE 2: implicit
+// l >= 1 step (Euler_Backward::advance_1/advance_lm).  Separate launches.
+template <typename T, int NPL, int MODE>
+__global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
+  __shared__ double tb[RT_COUNT * kRowP];
+  for (int i = threadIdx.x; i < RT_COUNT * kRowP; i += blockDim.x) tb[i] = a.rt[i];
+  __syncthreads();
+  const int nx = a.nx, np = a.np;
+  const int ntx = (nx + kRowCols - 1) / kRowCols;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int y = tile / ntx;
+  const int xc = (tile % ntx) * kRowCols + (threadIdx.x >> 4);
+  const bool colv = xc < nx;
+  const int x = colv ? xc : 0;  // idle rows compute on column 0 and store nothing
+  const int seg = threadIdx.x & 15;
+  const int p0 = seg * NPL;
+  const long long base = (long long)y * np * nx + x;
+  double f[NPL];
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    const int p = p0 + i;
+    f[i] = (p < np) ? static_cast<double>(a.f[base + (long long)p * nx]) : 0.0;
+  }
+  if (MODE == 1) {
+    // RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553, numh times (:590-625)
+    const double h = a.h_ee;
+    for (int s = 0; s < a.numh; ++s) {
+      // f: running sum, f0: start of the subcycle, fs: stage input, overwritten by its slope
+      double f0[NPL], fs[NPL];
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) { f0[i] = f[i]; fs[i] = f[i]; }
+      ee_slope_rows<T, NPL>(a, tb, seg, fs);
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) { fs[i] *= (0.5 * h); const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (1.0 / 3.0); }
+      ee_slope_rows<T, NPL>(a, tb, seg, fs);
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) { fs[i] *= (0.5 * h); const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (2.0 / 3.0); }
+      ee_slope_rows<T, NPL>(a, tb, seg, fs);
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) { fs[i] *= h; const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (1.0 / 3.0); }
+      ee_slope_rows<T, NPL>(a, tb, seg, fs);
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) { fs[i] *= (h / 6.0); f[i] += fs[i]; }
+    }
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      const int p = p0 + i;
+      if (colv && p < np) a.f[base + (long long)p * nx] = static_cast<T>(f[i]);
+    }
+    return;
+  }
+  // Implicit_FLM_step::reset_coeff, def-FokkerPlanck.cpp:808-939
+  double Asub[NPL], Bdia[NPL], Csup[NPL], Sct[NPL];
+  {
+    const double fprev = dpp_mov<row_shr<1>()>(f[NPL - 1], 0.0);
+    const double fnext = dpp_mov<row_shl<1>()>(f[0], 0.0);
+    double I2[NPL], I0[NPL], J1[NPL];
+    double s4 = 0.0, s2 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      const double fm = (i > 0) ? f[i - 1] : fprev;
+      s4 += rt<NPL>(tb, RT_U4, i, seg) * f[i] + rt<NPL>(tb, RT_U4M1, i, seg) * fm;
+      s2 += rt<NPL>(tb, RT_U2, i, seg) * f[i] + rt<NPL>(tb, RT_U2M1, i, seg) * fm;
+      I2[i] = s4;
+      I0[i] = s2;
+    }
+#pragma unroll
+    for (int i = NPL - 1; i >= 0; --i) {
+      const double fn = (i < NPL - 1) ? f[i + 1] : fnext;
+      s1 += rt<NPL>(tb, RT_U1S, i, seg) * fn + rt<NPL>(tb, RT_U1M1S, i, seg) * f[i];
+      J1[i] = s1;
+    }
+    const double inc4 = row_scan(s4), inc2 = row_scan(s2), inc1 = row_scan_rev(s1);
+    const double ex4 = dpp_mov<row_shr<1>()>(inc4, 0.0), ex2 = dpp_mov<row_shr<1>()>(inc2, 0.0);
+    const double ex1 = dpp_mov<row_shl<1>()>(inc1, 0.0);
+    const double I2_temperature = row_bcast<15>(inc4) * (4.0 * M_PI);
+    const double I0_density = row_bcast<15>(inc2) * (4.0 * M_PI);
+    const double lnee = log_ee(I0_density, I2_temperature, a.density_np);
+    const double zi = zlog_ei(I0_density, I2_temperature, a.density_np, a.zeta) * I0_density;
+    const double kd = a.kpre * a.dt_flm;
+    const double sc = (-1.0) * lnee * a.kpre * a.dt_flm;
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      double i2 = (I2[i] + ex4) * (4.0 * M_PI);
+      i2 *= rt<NPL>(tb, RT_IV2, i, seg);
+      const double i0 = (I0[i] + ex2) * (4.0 * M_PI);
+      const double j1 = (J1[i] + ex1) * (4.0 * M_PI * rt<NPL>(tb, RT_VC, i, seg));
+      const double T1 = i0 + (2.0 * j1 - i2) * (1.0 / 3.0);
+      const double T2 = (i2 + j1) * (1.0 / 3.0);
+      double sct = T1 * lnee;
+      if (i == 0) sct = (seg == 0) ? 0.0 : sct;  // Scattering_Term[0] has no e-e part (:874)
+      sct += zi;
+      sct *= rt<NPL>(tb, RT_IU3, i, seg);
+      Sct[i] = sct * kd;
+      const double dm = rt<NPL>(tb, RT_IVDM, i, seg), dpp = rt<NPL>(tb, RT_IVDP, i, seg);
+      const double s2v = rt<NPL>(tb, RT_IVS2, i, seg);
+      const double dg = 8.0 * M_PI * f[i] * rt<NPL>(tb, RT_ROW, i, seg) - T2 * (dm + dpp);
+      Asub[i] = (T2 * dm - T1 * s2v) * sc;
+      Bdia[i] = dg * sc;
+      Csup[i] = (T2 * dpp + T1 * s2v) * sc;
+    }
+  }
+  // Implicit_FLM_step::advance for every l >= 1 plane, def-FokkerPlanck.cpp:943-1035,
+  // TridiagonalSolve def-Nmethods.cpp:142-167 with the lane boundaries joined by scans
+  const int lmin = a.do_f1 ? 1 : 2;
+  const int lmax = a.do_flm ? a.l0 : 1;
+  for (int l = lmin; l <= lmax; ++l) {
+    double ll1 = static_cast<double>(l);
+    ll1 *= (-0.5) * (ll1 + 1.0);
+    auto bl = [&](int i) {
+      double bd = Bdia[i];
+      if (i == 0 && l > 1) bd = (seg == 0) ? 0.0 : bd;  // row 0 of Alpha_Tri dropped for l > 1 (:953-955)
+      return bd + (1.0 - ll1 * Sct[i]);
+    };
+    // pivots: c'_i = P_i/Q_i with (P,Q)_i = M_i (P,Q)_{i-1}, M_i = [[0, c_i], [-a_i, b_i]]
+    Mob P = {1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      const Mob Mi = {0.0, Csup[i], -Asub[i], bl(i)};
+      P = mob_mul(Mi, P);
+      if ((i & 3) == 3) P = mob_norm(P);  // exact power-of-two rescaling against overflow
+    }
+#define OSHUN_RMOB(N)                                                                    \
+    {                                                                                    \
+      Mob q;                                                                             \
+      q.a = dpp_mov<row_shr<N>()>(P.a, 1.0); q.b = dpp_mov<row_shr<N>()>(P.b, 0.0);      \
+      q.c = dpp_mov<row_shr<N>()>(P.c, 0.0); q.d = dpp_mov<row_shr<N>()>(P.d, 1.0);      \
+      P = mob_norm(mob_mul(P, q));                                                       \
+    }
+    OSHUN_RMOB(1)
+    OSHUN_RMOB(2)
+    OSHUN_RMOB(4)
+    OSHUN_RMOB(8)
+#undef OSHUN_RMOB
+    // c' entering the lane from the exclusive prefix, then the reference's recurrence
+    double cprev = dpp_mov<row_shr<1>()>(P.b, 0.0) * rcp_nr(dpp_mov<row_shr<1>()>(P.d, 1.0));
+    double w[NPL];  // c'_i = Csup_i w_i is recomputed where needed
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      w[i] = rcp_nr(bl(i) - cprev * Asub[i]);
+      cprev = Csup[i] * w[i];
+    }
+    for (int c = 1; c < a.nc; ++c) {
+      if (a.comp_l[c] != l) continue;
+      T* fc = a.f + (long long)c * a.plane;
+      double d[NPL];
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        const int p = p0 + i;
+        d[i] = (p < np) ? static_cast<double>(fc[base + (long long)p * nx]) : 0.0;
+      }
+      // forward substitution d'_i = (d_i - a_i d'_{i-1}) w_i: affine map per lane, scanned
+      double al = 1.0, be = 0.0;
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        const double aw = -Asub[i] * w[i];
+        be = aw * be + w[i] * d[i];
+        al = aw * al;
+      }
+#define OSHUN_RAFF(N)                                                                              \
+      {                                                                                            \
+        const double qa = dpp_mov<row_shr<N>()>(al, 1.0), qb = dpp_mov<row_shr<N>()>(be, 0.0);     \
+        be = al * qb + be;                                                                         \
+        al = al * qa;                                                                              \
+      }
+      OSHUN_RAFF(1)
+      OSHUN_RAFF(2)
+      OSHUN_RAFF(4)
+      OSHUN_RAFF(8)
+#undef OSHUN_RAFF
+      double yv = dpp_mov<row_shr<1>()>(be, 0.0);
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) { yv = (d[i] - yv * Asub[i]) * w[i]; d[i] = yv; }
+      // back substitution x_i = d'_i - c'_i x_{i+1}: affine map per lane, reverse scan
+      double ga = 1.0, de = 0.0;
+#pragma unroll
+      for (int i = NPL - 1; i >= 0; --i) {
+        const double cp = Csup[i] * w[i];
+        de = -cp * de + d[i];
+        ga = -cp * ga;
+      }
+#define OSHUN_RAFB(N)                                                                              \
+      {                                                                                            \
+        const double qa = dpp_mov<row_shl<N>()>(ga, 1.0), qb = dpp_mov<row_shl<N>()>(de, 0.0);     \
+        de = ga * qb + de;                                                                         \
+        ga = ga * qa;                                                                              \
+      }
+      OSHUN_RAFB(1)
+      OSHUN_RAFB(2)
+      OSHUN_RAFB(4)
+      OSHUN_RAFB(8)
+#undef OSHUN_RAFB
+      double xv = dpp_mov<row_shl<1>()>(de, 0.0);
+#pragma unroll
+      for (int i = NPL - 1; i >= 0; --i) { xv = d[i] - (Csup[i] * w[i]) * xv; d[i] = xv; }
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        const int p = p0 + i;
+        if (colv && p < np) fc[base + (long long)p * nx] = static_cast<T>(d[i]);
       }
     }
   }

@@ -49,6 +49,8 @@ struct oshun_ctx {
   double* d_mom = nullptr;  // moments output, 7 planes of nxy
   bool use1d = false;       // specialised 1D m0 = 0 Vlasov kernel
   double* d_coll = nullptr;
+  double* d_rt = nullptr;   // collide_rows tables (interleaved for rows_npl), null if that kernel is not used
+  int rows_npl = 0;
   oshun::CollTables ct;
   hipStream_t stream = nullptr;
   double t_rk = 0.0, t_ee = 0.0, t_eb = 0.0;
@@ -266,6 +268,19 @@ int launch_collide_npl(oshun_ctx* c, oshun::CollArgs<T>& a) {
   return OSHUN_OK;
 }
 
+template <typename T, int NPL>
+int launch_collide_rows(oshun_ctx* c, oshun::CollArgs<T>& a) {
+  ProfRec r;
+  prof_begin(c, 2, &r);
+  const int ntx = (c->nx + oshun::kRowCols - 1) / oshun::kRowCols;
+  if (a.do_ee) hipLaunchKernelGGL((oshun::collide_rows<T, NPL, 1>), dim3(ntx * c->ny), dim3(256), 0, c->stream, a);
+  if (a.do_f1 || a.do_flm)
+    hipLaunchKernelGGL((oshun::collide_rows<T, NPL, 2>), dim3(ntx * c->ny), dim3(256), 0, c->stream, a);
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  return OSHUN_OK;
+}
+
 template <typename T>
 int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_ee, double dt_flm) {
   oshun::CollArgs<T> a;
@@ -278,6 +293,14 @@ int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_e
   a.h_ee = h_ee; a.dt_flm = dt_flm;
   a.c_kpre = c->T.c_kpre; a.kpre = c->T.kpre; a.density_np = c->ph.density_np; a.zeta = c->ph.zeta;
   a.v0sq = c->T.v0sq; a.v1sq = c->T.v1sq; a.g_den = c->T.g_den;
+  a.rt = c->d_rt;
+  a.r01 = c->T.v0sq / c->T.v1sq; a.inv_gden = 1.0 / c->T.g_den; a.c2v0 = 2.0 / c->T.v0sq;
+  switch (c->rows_npl) {
+    case 4: return launch_collide_rows<T, 4>(c, a);
+    case 8: return launch_collide_rows<T, 8>(c, a);
+    case 16: return launch_collide_rows<T, 16>(c, a);
+    default: break;
+  }
   if (c->np <= 64) return launch_collide_npl<T, 1>(c, a);
   if (c->np <= 128) return launch_collide_npl<T, 2>(c, a);
   if (c->np <= 256) return launch_collide_npl<T, 4>(c, a);
@@ -401,7 +424,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -524,6 +547,45 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     c->ct.U1m1s = b + 5 * n; c->ct.U3 = b + 6 * n; c->ct.Qn = b + 7 * n; c->ct.Pn = b + 8 * n;
     c->ct.vc = b + 9 * n; c->ct.ivdm = b + 10 * n; c->ct.ivdp = b + 11 * n; c->ct.ivs2 = b + 12 * n;
     c->ct.pw3 = b + 13 * n; c->ct.pw5 = b + 14 * n; c->ct.pw7 = b + 15 * n;
+  }
+  // collide_rows: 16 lanes x NPL rows per column; needs NPL >= NB (the
+  // parabolic low-p rows stay in lane 0) and 16*NPL >= np
+  for (int npl : {4, 8, 16})
+    if (16 * npl >= c->np && npl >= c->ph.nb_algorithms && c->np >= 3) { c->rows_npl = npl; break; }
+  if (c->rows_npl) {
+    const HostTables& T = c->T;
+    const int np = c->np, npl = c->rows_npl, nb = std::min(c->ph.nb_algorithms, np);
+    const double* v = T.vc.data();
+    std::vector<double> rt((size_t)oshun::RT_COUNT * oshun::kRowP, 0.0);
+    auto put = [&](int k, int p, double val) { rt[(size_t)k * oshun::kRowP + (p % npl) * 16 + p / npl] = val; };
+    for (int p = 0; p < 16 * npl; ++p) {
+      const bool in = p < np, mid = p >= 1 && p < np - 1;
+      put(oshun::RT_U4, p, in ? T.U4[p] : 0.0);  // U4[0] = 0: the sums start at p = 1
+      put(oshun::RT_U4M1, p, in ? T.U4m1[p] : 0.0);
+      put(oshun::RT_U2, p, in ? T.U2[p] : 0.0);
+      put(oshun::RT_U2M1, p, in ? T.U2m1[p] : 0.0);
+      put(oshun::RT_U1S, p, p < np - 1 ? T.U1s[p] : 0.0);
+      put(oshun::RT_U1M1S, p, p < np - 1 ? T.U1m1s[p] : 0.0);
+      put(oshun::RT_U3, p, in ? T.U3[p] : 1.0);
+      put(oshun::RT_PN, p, p < np - 1 ? T.Pn[p] : (p == np - 1 ? 1.0 : 0.0));
+      put(oshun::RT_QN, p, p < np - 1 ? T.Qn[p] : 0.0);
+      put(oshun::RT_VC, p, in ? v[p] : 1.0);
+      put(oshun::RT_IV2, p, in ? 1.0 / (v[p] * v[p]) : 1.0);
+      put(oshun::RT_IU3, p, in ? 1.0 / std::pow(v[p], 3) : 0.0);
+      put(oshun::RT_IVDM, p, mid ? T.ivdm[p] : 0.0);
+      put(oshun::RT_IVDP, p, mid ? T.ivdp[p] : 0.0);
+      put(oshun::RT_IVS2, p, mid ? T.ivs2[p] : 0.0);
+      put(oshun::RT_ROW, p, p < np - 1 ? 1.0 : 0.0);
+      if (p < nb) {
+        const double v1sq = T.v1sq;
+        put(oshun::RT_P3, p, T.pw3[p]);
+        put(oshun::RT_A2, p, T.pw3[p] / 3.0);
+        put(oshun::RT_B2, p, T.pw5[p] / v1sq * 0.2);
+        put(oshun::RT_A4, p, T.pw5[p] * 0.2);
+        put(oshun::RT_B4, p, T.pw7[p] / (v1sq * 7.0));
+      }
+    }
+    if (upload_vec(c, &c->d_rt, rt)) return bail(OSHUN_ERR_HIP);
   }
   *out = c;
   return OSHUN_OK;
