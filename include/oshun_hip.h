@@ -40,7 +40,10 @@ extern "C" {
 
 /* precision of the stored distribution function and of the Vlasov RHS */
 #define OSHUN_FP64 0       /* everything in double                                 */
-#define OSHUN_FP32_SPLIT 1 /* f_lm + Vlasov RHS in float; fields, collisions double */
+#define OSHUN_FP32_SPLIT 1 /* f_lm + Vlasov RHS in float; fields, collisions double.
+                              The float plane of f00 holds f00 - bg(p), bg the FP64
+                              x,y-mean of f00 at the last upload, so rounding acts on
+                              the deviation; uploads/downloads/moments add bg back. */
 
 /* collision switches (bit mask) */
 #define OSHUN_COLL_EE 1  /* e-e on f00            (Explicit_EC_FP)             */
@@ -109,7 +112,8 @@ int oshun_moments(oshun_ctx* ctx, double* out);
 int oshun_synchronize(oshun_ctx* ctx);
 int oshun_get_stream(const oshun_ctx* ctx, void** hip_stream);
 /* Device state: f planes (float or double per precision) and 6 double field
- * planes.  Plane layout [y][p][x] (x fastest); see oshun_plane_info. */
+ * planes.  Plane layout [y][p][x] (x fastest); see oshun_plane_info.  In
+ * OSHUN_FP32_SPLIT plane 0 is f00 minus the background (see above). */
 int oshun_device_state(oshun_ctx* ctx, void** f_dev, double** fld_dev);
 /* ncomp real planes; comp_l/comp_m/comp_im describe each plane (arrays of
  * length ncomp, may be NULL); plane_elems = ny*np*nx. */

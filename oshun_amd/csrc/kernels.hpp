@@ -43,6 +43,7 @@ struct StageArgs {
   const T* cg;
   const T* hp0;
   const double* p3og;
+  const T* bgG;  // G00 of the f00 background (plane 0 stores f00 minus it), or null
   T p0p1_sq, inv_mp0p1_sq, g_r;
   T c05h, hh, h6, third, twothird;
   int np, nx, ny, nc;
@@ -110,7 +111,8 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage(StageArgs<T> a) {
       for (int t = a.tstart[tc]; t < t1; ++t) {
         const Term tm = a.terms[t];
         const T* s = a.in + (long long)tm.src * a.plane;
-        const T val = op_value<T>(tm, s, idx, p, np, nx, xm_off, xp_off, ym_off, yp_off, a);
+        T val = op_value<T>(tm, s, idx, p, np, nx, xm_off, xp_off, ym_off, yp_off, a);
+        if (tm.op == OP_G00 && a.bgG) val += a.bgG[p];  // G00 is linear: G00(bg + df) = G00(bg) + G00(df)
         T mult;
         switch (tm.mul) {  // wave-uniform; keeps F[] in registers
           case MUL_EX: mult = static_cast<T>(F[0] * tm.c); break;
@@ -200,6 +202,7 @@ struct Stage1DArgs {
   const double* eA2;  // E-field H coefficient of source l
   const double* xA1;  // x-advection coefficient of source l towards l+1
   const double* xA2;  // x-advection coefficient of source l towards l-1
+  const T* bgG;       // G00 of the f00 background (plane 0 stores f00 minus it), or null
   T p0p1_sq, inv_mp0p1_sq, g_r;
   T c05h, hh, h6, third, twothird;
   int np, nx, mode;
@@ -245,6 +248,7 @@ __global__ __launch_bounds__(64 * kRows1D) void vlasov_stage_1d(Stage1DArgs<T> a
     }
     const double Ex = a.fin[x];
     const T v = a.vtab[p];
+    const T bgg = a.bgG ? a.bgG[p] : T(0);
     T G[NL], Hh[NL];
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
@@ -252,6 +256,7 @@ __global__ __launch_bounds__(64 * kRows1D) void vlasov_stage_1d(Stage1DArgs<T> a
         if (l == 0) {
           const T fe = (fc[0] - fn[0] * a.p0p1_sq) * a.inv_mp0p1_sq;   // G00, def-VlasovMax.cpp:531-532
           G[0] = (fn[0] - fe) * a.g_r;
+          if (a.bgG) G[0] += bgg;
           Hh[0] = T(0);
         } else {
           G[l] = T(0);
@@ -261,6 +266,7 @@ __global__ __launch_bounds__(64 * kRows1D) void vlasov_stage_1d(Stage1DArgs<T> a
         const T dpf = (p == np - 1) ? (fm[l] - fc[l]) : (fm[l] - fn[l]);  // Dp, def-State.cpp:359-380
         if (l == 0) {
           G[0] = dpf;
+          if (a.bgG) G[0] += bgg;
           Hh[0] = T(0);
         } else {
           T hv = fc[l] * a.invpax[l * np + p];                           // MakeGH, :504-509
@@ -430,6 +436,7 @@ struct CollArgs {
   double h_ee, dt_flm;
   double c_kpre, kpre, density_np, zeta, v0sq, v1sq, g_den;
   const double* rt;             // collide_rows tables, interleaved for its NPL
+  const double* bg;             // f00 background per p (plane 0 stores f00 minus it), or null
   double r01, inv_gden, c2v0;   // v0^2/v1^2, 1/g_den, 2/v0^2
 };
 
@@ -841,7 +848,7 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
     tb.U3[i] = in ? a.t.U3[p] : 1.0;
     tb.Qn[i] = in ? a.t.Qn[p] : 0.0;
     tb.Pn[i] = in ? a.t.Pn[p] : 0.0;
-    f[i] = in ? static_cast<double>(a.f[base + (long long)p * nx]) : 0.0;
+    f[i] = in ? static_cast<double>(a.f[base + (long long)p * nx]) + (a.bg ? a.bg[p] : 0.0) : 0.0;
   }
   if (a.do_ee) {
     // Explicit_EC_FP::loop + RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553,590-625
@@ -866,12 +873,16 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
       const int p = lane * NPL + i;
-      if (p < np) a.f[base + (long long)p * nx] = static_cast<T>(f[i]);
+      if (p < np) a.f[base + (long long)p * nx] = static_cast<T>(f[i] - (a.bg ? a.bg[p] : 0.0));
     }
     // the implicit step reads f00 as stored
     if (sizeof(T) < sizeof(double)) {
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) f[i] = static_cast<double>(static_cast<T>(f[i]));
+      for (int i = 0; i < NPL; ++i) {
+        const int p = lane * NPL + i;
+        const double b = (a.bg && p < np) ? a.bg[p] : 0.0;
+        f[i] = static_cast<double>(static_cast<T>(f[i] - b)) + b;
+      }
     }
   }
   if (!(a.do_f1 || a.do_flm)) return;
@@ -1134,7 +1145,7 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const int p = p0 + i;
-    f[i] = (p < np) ? static_cast<double>(a.f[base + (long long)p * nx]) : 0.0;
+    f[i] = (p < np) ? static_cast<double>(a.f[base + (long long)p * nx]) + (a.bg ? a.bg[p] : 0.0) : 0.0;
   }
   if (MODE == 1) {
     // RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553, numh times (:590-625)
@@ -1160,7 +1171,7 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
       const int p = p0 + i;
-      if (colv && p < np) a.f[base + (long long)p * nx] = static_cast<T>(f[i]);
+      if (colv && p < np) a.f[base + (long long)p * nx] = static_cast<T>(f[i] - (a.bg ? a.bg[p] : 0.0));
     }
     return;
   }
@@ -1326,17 +1337,20 @@ struct MomArgs {
   const T* f;
   double* out;
   const double *w2, *w3, *w4, *w5;
+  const double* bg;  // f00 background per p added to plane 0, or null
   int np, nx, ny, c00, c10, c11r, c20;
   long long plane;
   double dp, r, inv, c5, c7, c00k, c04k;
 };
 
 template <typename T>
-__device__ __forceinline__ double trap_p(const T* f, long long base, int nx, int np, const double* w, bool skip0) {
-  double s = skip0 ? 0.0 : static_cast<double>(f[base]) * w[0];
-  s += static_cast<double>(f[base + (long long)(np - 1) * nx]) * w[np - 1];
+__device__ __forceinline__ double trap_p(const T* f, long long base, int nx, int np, const double* w, bool skip0,
+                                         const double* bg = nullptr) {
+  auto val = [&](int p) { return static_cast<double>(f[base + (long long)p * nx]) + (bg ? bg[p] : 0.0); };
+  double s = skip0 ? 0.0 : val(0) * w[0];
+  s += val(np - 1) * w[np - 1];
   s *= 0.5;
-  for (int p = 1; p < np - 1; ++p) s += w[p] * static_cast<double>(f[base + (long long)p * nx]);
+  for (int p = 1; p < np - 1; ++p) s += w[p] * val(p);
   return s;
 }
 
@@ -1350,8 +1364,9 @@ __global__ __launch_bounds__(256) void moments_kernel(MomArgs<T> a) {
   const int nx = a.nx, np = a.np;
   const T* f0 = a.f + (long long)a.c00 * a.plane;
   const T* f10 = a.f + (long long)a.c10 * a.plane;
-  const double f0_0 = static_cast<double>(f0[base]), f0_1 = static_cast<double>(f0[base + nx]);
-  double n = trap_p<T>(f0, base, nx, np, a.w2, false);
+  const double f0_0 = static_cast<double>(f0[base]) + (a.bg ? a.bg[0] : 0.0);
+  const double f0_1 = static_cast<double>(f0[base + nx]) + (a.bg ? a.bg[1] : 0.0);
+  double n = trap_p<T>(f0, base, nx, np, a.w2, false, a.bg);
   double c = f0_1 / 5.0 * a.r + (f0_0 - f0_1 * a.r) * a.inv * (1.0 / 3.0 - 1.0 / 5.0 * a.r);
   c *= a.c00k;
   n += c;
@@ -1368,7 +1383,7 @@ __global__ __launch_bounds__(256) void moments_kernel(MomArgs<T> a) {
     vy *= (8.0 * M_PI / 3.0) * a.dp;
     vy /= n;
   }
-  double m1 = trap_p<T>(f0, base, nx, np, a.w4, false);
+  double m1 = trap_p<T>(f0, base, nx, np, a.w4, false, a.bg);
   double c1 = f0_1 / 7.0 * a.r + (f0_0 - f0_1 * a.r) * a.inv * (1.0 / 5.0 - 1.0 / 7.0 * a.r);
   c1 *= a.c04k;
   m1 += c1;

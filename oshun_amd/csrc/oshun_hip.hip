@@ -50,6 +50,12 @@ struct oshun_ctx {
   bool use1d = false;       // specialised 1D m0 = 0 Vlasov kernel
   double* d_coll = nullptr;
   double* d_rt = nullptr;   // collide_rows tables (interleaved for rows_npl), null if that kernel is not used
+  // FP32 split: plane 0 stores f00 - bg(p), bg an FP64 background per p
+  // (the x,y mean of f00 at upload), so that FP32 rounding acts on the
+  // deviation only; d_bgG holds G00(bg) for the Vlasov kernels.
+  double* d_bg = nullptr;
+  float* d_bgG = nullptr;
+  std::vector<double> bg;
   int rows_npl = 0;
   oshun::CollTables ct;
   hipStream_t stream = nullptr;
@@ -115,6 +121,14 @@ void launch_1d(oshun_ctx* c, const oshun::Stage1DArgs<T>& a, dim3 grid, dim3 blo
   hipLaunchKernelGGL((oshun::vlasov_stage_1d<T, L0>), grid, block, 0, c->stream, a);
 }
 
+// G00 of the FP32 background table, null in FP64 (no background there)
+template <typename T>
+const T* bg_g00(const oshun_ctx* c);
+template <>
+const double* bg_g00<double>(const oshun_ctx*) { return nullptr; }
+template <>
+const float* bg_g00<float>(const oshun_ctx* c) { return c->d_bgG; }
+
 template <typename T>
 int launch_stage_1d(oshun_ctx* c, int mode, const void* in, const void* y0, void* out, void* acc,
                     const double* fin, double h) {
@@ -135,6 +149,7 @@ int launch_stage_1d(oshun_ctx* c, int mode, const void* in, const void* y0, void
   a.eA2 = c->d_e1d + L;
   a.xA1 = c->d_e1d + 2 * L;
   a.xA2 = c->d_e1d + 3 * L;
+  a.bgG = bg_g00<T>(c);
   a.p0p1_sq = static_cast<T>(c->T.p0p1_sq);
   a.inv_mp0p1_sq = static_cast<T>(c->T.inv_mp0p1_sq);
   a.g_r = static_cast<T>(c->T.g_r);
@@ -187,6 +202,7 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
   a.cg = vt_ptr<T>(c, 2);
   a.hp0 = vt_ptr<T>(c, 3);
   a.p3og = c->d_p3og;
+  a.bgG = bg_g00<T>(c);
   a.p0p1_sq = static_cast<T>(c->T.p0p1_sq);
   a.inv_mp0p1_sq = static_cast<T>(c->T.inv_mp0p1_sq);
   a.g_r = static_cast<T>(c->T.g_r);
@@ -294,6 +310,7 @@ int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_e
   a.c_kpre = c->T.c_kpre; a.kpre = c->T.kpre; a.density_np = c->ph.density_np; a.zeta = c->ph.zeta;
   a.v0sq = c->T.v0sq; a.v1sq = c->T.v1sq; a.g_den = c->T.g_den;
   a.rt = c->d_rt;
+  a.bg = c->d_bg;
   a.r01 = c->T.v0sq / c->T.v1sq; a.inv_gden = 1.0 / c->T.g_den; a.c2v0 = 2.0 / c->T.v0sq;
   switch (c->rows_npl) {
     case 4: return launch_collide_rows<T, 4>(c, a);
@@ -338,6 +355,29 @@ int full_step(oshun_ctx* c, double h) {
   return OSHUN_OK;
 }
 
+// FP32 split background -------------------------------------------------------
+// bg(p) = mean over the interior x,y of Re f00 of the harmonic array sh0
+void set_background(oshun_ctx* c, const double* sh0, int nbc) {
+  const int np = c->np, nx = c->nx, ny = c->ny, nxg = nx + 2 * nbc;
+  c->bg.assign(np, 0.0);
+  for (int y = 0; y < ny; ++y)
+    for (int x = 0; x < nx; ++x) {
+      const double* col = sh0 + 2 * (((size_t)(y + nbc) * nxg + (x + nbc)) * np);
+      for (int p = 0; p < np; ++p) c->bg[p] += col[2 * p];
+    }
+  for (int p = 0; p < np; ++p) c->bg[p] /= static_cast<double>(nx) * ny;
+}
+// G00 of the background with the formulas of the Vlasov kernels (FP64, then rounded)
+void background_g00(const oshun_ctx* c, std::vector<float>& g) {
+  const int np = c->np;
+  const std::vector<double>& b = c->bg;
+  g.assign(np, 0.0f);
+  const double fe = (b[0] - b[1] * c->T.p0p1_sq) * c->T.inv_mp0p1_sq;  // def-VlasovMax.cpp:531-532
+  g[0] = static_cast<float>((b[1] - fe) * c->T.g_r);
+  for (int p = 1; p < np; ++p)
+    g[p] = static_cast<float>(p == np - 1 ? b[p - 1] - b[p] : b[p - 1] - b[p + 1]);  // def-State.cpp:359-380
+}
+
 // host <-> device repacking ---------------------------------------------------
 template <typename T>
 void pack(const oshun_ctx* c, const double* const* sh, int nbc, std::vector<T>& out) {
@@ -352,7 +392,8 @@ void pack(const oshun_ctx* c, const double* const* sh, int nbc, std::vector<T>& 
         const double* col = src + 2 * (((size_t)(y + nbc) * nxg + (x + nbc)) * np);
         for (int p = 0; p < np; ++p) {
           const size_t o = ((size_t)y * np + p) * nx + x;
-          out[(size_t)cr * c->plane + o] = static_cast<T>(col[2 * p]);
+          const double b = (h == 0 && !c->bg.empty()) ? c->bg[p] : 0.0;
+          out[(size_t)cr * c->plane + o] = static_cast<T>(col[2 * p] - b);
           if (ci >= 0) out[(size_t)ci * c->plane + o] = static_cast<T>(col[2 * p + 1]);
         }
       }
@@ -360,7 +401,7 @@ void pack(const oshun_ctx* c, const double* const* sh, int nbc, std::vector<T>& 
 }
 
 template <typename T>
-void unpack(const oshun_ctx* c, const std::vector<T>& in, double* const* sh, int nbc) {
+void unpack(const oshun_ctx* c, const std::vector<T>& in, double* const* sh, int nbc, bool state) {
   const int np = c->np, nx = c->nx, ny = c->ny;
   const int nxg = nx + 2 * nbc, nyg = ny + 2 * nbc;
   for (int h = 0; h < c->T.nh; ++h) {
@@ -372,7 +413,8 @@ void unpack(const oshun_ctx* c, const std::vector<T>& in, double* const* sh, int
         double* col = dst + 2 * (((size_t)yg * nxg + xg) * np);
         for (int p = 0; p < np; ++p) {
           const size_t o = ((size_t)y * np + p) * nx + x;
-          col[2 * p] = static_cast<double>(in[(size_t)cr * c->plane + o]);
+          const double b = (state && h == 0 && !c->bg.empty()) ? c->bg[p] : 0.0;
+          col[2 * p] = static_cast<double>(in[(size_t)cr * c->plane + o]) + b;
           col[2 * p + 1] = ci >= 0 ? static_cast<double>(in[(size_t)ci * c->plane + o]) : 0.0;
         }
       }
@@ -399,19 +441,20 @@ void unpack_fields(const oshun_ctx* c, const std::vector<double>& in, double* co
       }
 }
 
+// state: fsrc holds a state (plane 0 relative to the background), not a slope
 int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* const* sh, double* const* fld,
-                 int nbc) {
+                 int nbc, bool state) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const size_t n = (size_t)c->nc * c->plane;
   if (sh) {
     if (c->fp32) {
       std::vector<float> h(n);
       HIPCHK(c, hipMemcpy(h.data(), fsrc, n * sizeof(float), hipMemcpyDeviceToHost));
-      unpack<float>(c, h, sh, nbc);
+      unpack<float>(c, h, sh, nbc, state);
     } else {
       std::vector<double> h(n);
       HIPCHK(c, hipMemcpy(h.data(), fsrc, n * sizeof(double), hipMemcpyDeviceToHost));
-      unpack<double>(c, h, sh, nbc);
+      unpack<double>(c, h, sh, nbc, state);
     }
   }
   if (fld) {
@@ -424,7 +467,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_bg, c->d_bgG};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -443,6 +486,7 @@ static int launch_moments(oshun_ctx* c) {
   a.out = c->d_mom;
   const size_t n = c->np;
   a.w2 = c->d_mw; a.w3 = c->d_mw + n; a.w4 = c->d_mw + 2 * n; a.w5 = c->d_mw + 3 * n;
+  a.bg = c->d_bg;
   a.np = c->np; a.nx = c->nx; a.ny = c->ny;
   const int m0 = c->T.m0;
   a.c00 = c->T.comp_re[0];
@@ -499,6 +543,11 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     if (hipMemset(*p, 0, fldbytes) != hipSuccess) return bail(OSHUN_ERR_HIP);
   if (hipMalloc((void**)&c->jpart, (size_t)c->npb * 3 * c->nxy * sizeof(double)) != hipSuccess)
     return bail(OSHUN_ERR_ALLOC);
+  if (c->fp32) {
+    c->bg.assign(c->np, 0.0);
+    if (upload_vec(c, &c->d_bg, c->bg) || upload_vec(c, &c->d_bgG, std::vector<float>(c->np, 0.0f)))
+      return bail(OSHUN_ERR_HIP);
+  }
   if (upload_vec(c, &c->d_terms, c->T.terms) || upload_vec(c, &c->d_tstart, c->T.tstart) ||
       upload_vec(c, &c->d_clean, c->T.clean) || upload_vec(c, &c->d_comp_l, c->T.comp_l) ||
       upload_vec(c, &c->d_p3og, c->T.p3og))
@@ -610,9 +659,14 @@ int oshun_upload_harmonics(oshun_ctx* c, const double* const* sh, const double* 
     for (int h = 0; h < c->T.nh; ++h)
       if (!sh[h]) return fail(c, OSHUN_ERR_ARG, "null harmonic pointer");
     if (c->fp32) {
+      set_background(c, sh[0], nbc);
       std::vector<float> v;
       pack<float>(c, sh, nbc, v);
       HIPCHK(c, hipMemcpy(c->S, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->d_bg, c->bg.data(), c->np * sizeof(double), hipMemcpyHostToDevice));
+      std::vector<float> g(c->np);
+      background_g00(c, g);
+      HIPCHK(c, hipMemcpy(c->d_bgG, g.data(), c->np * sizeof(float), hipMemcpyHostToDevice));
     } else {
       std::vector<double> v;
       pack<double>(c, sh, nbc, v);
@@ -631,7 +685,7 @@ int oshun_upload_harmonics(oshun_ctx* c, const double* const* sh, const double* 
 
 int oshun_download_harmonics(oshun_ctx* c, double* const* sh, double* const* fld, int nbc) {
   if (!sane(c) || nbc < 0) return OSHUN_ERR_ARG;
-  return download_raw(c, c->S, c->Sf, sh, fld, nbc);
+  return download_raw(c, c->S, c->Sf, sh, fld, nbc, true);
 }
 
 static void split_ptrs(const oshun_ctx* c, const double* f, const double* fld, int nbc,
@@ -678,7 +732,7 @@ int oshun_rhs(oshun_ctx* c, double* f_slope, double* fld_slope) {
   std::vector<const double*> hp, fp;
   split_ptrs(c, f_slope, fld_slope, 0, hp, fp);
   return download_raw(c, c->B1, c->B1f, f_slope ? const_cast<double* const*>(hp.data()) : nullptr,
-                      fld_slope ? const_cast<double* const*>(fp.data()) : nullptr, 0);
+                      fld_slope ? const_cast<double* const*>(fp.data()) : nullptr, 0, false);
 }
 
 int oshun_rk4_step(oshun_ctx* c, double h) {

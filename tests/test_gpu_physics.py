@@ -7,7 +7,8 @@ The E_x mode must oscillate at the real part and decay at the imaginary part
 of the root of the Maxwellian dispersion relation 1 + (1 + zeta Z(zeta))/(k lambda_D)^2 = 0
 (textbook Landau damping; the root is computed here with scipy's Faddeeva
 function).  The FP32-split path is compared with FP64 on the same run and its
-error per field is printed and bounded.
+error per field is printed and bounded (f00 is stored in float relative to
+its FP64 x-mean, so float rounding acts on the 1e-3 deviation only).
 """
 import numpy as np
 import pytest
@@ -97,6 +98,9 @@ def test_langmuir_wave_frequency_and_landau_damping():
     print("FP32-split relative L2 error after 2000 steps:", {k: f"{v:.3e}" for k, v in e.items()})
     _, _, g32, w32, _ = fit_damped(t[sel], a32[sel], wr, wi)
     print(f"FP32 split fit omega={w32:.5f} gamma={g32:.5f}")
-    assert e["f00"] < 1e-5
-    assert e["Ex"] < 1e-4
+    # target <= 1e-5 in f00 and E; the background-relative FP32 storage of
+    # f00 gives about 1e-9 and 1e-7 here
+    assert e["f00"] < 1e-7
+    assert e["Ex"] < 1e-6
+    assert e["f10"] < 1e-4
     assert abs(g32 - g) / abs(g) < 0.01
