@@ -334,36 +334,41 @@ __device__ __forceinline__ void rk_combine(int mode, double k, double y0, double
   }
 }
 
-// partial sums of the current: 4 rows of 64 lanes split the p blocks, then
-// the four row sums are added in LDS (fixed order, deterministic)
+// partial sums of the current: kFieldRows rows of kFieldX lanes split the p
+// blocks, then the row sums are added in LDS in a fixed tree (deterministic)
+constexpr int kFieldX = 16, kFieldRows = 16;
 __device__ __forceinline__ void sum_parts(const double* jpart, int npb, int nxy, int i, int comp, int row,
-                                          double (&sh)[4][64], double* out) {
-  // up to 8 independent loads in flight per pass (npb <= 128 for Np <= 512)
+                                          double (&sh)[kFieldRows][kFieldX], double* out) {
   double s = 0.0;
   if (i < nxy) {
-    for (int b0 = row; b0 < npb; b0 += 32) {
-      double v[8];
+    for (int b0 = row; b0 < npb; b0 += 4 * kFieldRows) {
+      double v[4];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int b = b0 + 4 * k;
+      for (int k = 0; k < 4; ++k) {
+        const int b = b0 + kFieldRows * k;
         v[k] = (b < npb) ? jpart[((long long)b * 3 + comp) * nxy + i] : 0.0;
       }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) s += v[k];
+      s += (v[0] + v[1]) + (v[2] + v[3]);
     }
   }
-  sh[row][threadIdx.x] = s;
+  const int lx = threadIdx.x;
+  sh[row][lx] = s;
   __syncthreads();
-  *out = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
+#pragma unroll
+  for (int w = kFieldRows / 2; w >= 1; w /= 2) {
+    if (row < w) sh[row][lx] += sh[row + w][lx];
+    __syncthreads();
+  }
+  *out = sh[0][lx];
   __syncthreads();
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void field_stage(FieldArgs<T> a) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
+__global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage(FieldArgs<T> a) {
+  const int i = blockIdx.x * kFieldX + threadIdx.x;
   const int row = threadIdx.y;
   const int nx = a.nx, ny = a.ny, nxy = nx * ny;
-  __shared__ double sh[4][64];
+  __shared__ double sh[kFieldRows][kFieldX];
   double jx = 0.0, jr = 0.0, ji = 0.0;
   if (a.c10 >= 0) sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &jx);
   if (a.c11r >= 0) {

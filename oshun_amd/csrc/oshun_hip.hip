@@ -244,10 +244,10 @@ int launch_field(oshun_ctx* c, int mode, const double* fin, const double* f0, do
   a.delta_p = c->T.delta_p;
   a.midx = c->T.midx; a.midy = c->T.midy;
   a.c05h = 0.5 * h; a.hh = h; a.h6 = h / 6.0; a.third = 1.0 / 3.0; a.twothird = 2.0 / 3.0;
-  dim3 grid((c->nxy + 63) / 64);
+  dim3 grid((c->nxy + oshun::kFieldX - 1) / oshun::kFieldX);
   ProfRec r;
   prof_begin(c, 1, &r);
-  hipLaunchKernelGGL(oshun::field_stage<T>, grid, dim3(64, 4), 0, c->stream, a);
+  hipLaunchKernelGGL(oshun::field_stage<T>, grid, dim3(oshun::kFieldX, oshun::kFieldRows), 0, c->stream, a);
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
