@@ -203,17 +203,78 @@ struct Stage1DArgs {
   const double* xA1;  // x-advection coefficient of source l towards l+1
   const double* xA2;  // x-advection coefficient of source l towards l-1
   const T* bgG;       // G00 of the f00 background (plane 0 stores f00 minus it), or null
+  const T* xv1;       // v(p) * xA1[l] at [l][p]
+  const T* xv2;       // v(p) * xA2[l] at [l][p]
   T p0p1_sq, inv_mp0p1_sq, g_r;
   T c05h, hh, h6, third, twothird;
   int np, nx, mode;
   long long plane;
+  // stage pairs (vlasov_pair_1d) only
+  const double* fin2;  // Ex of the second stage's input
+  double* jpA;         // [ptile][3][nx] current partials of the second stage's input
+  double* jpB;         // same for the pair's output (first pair only)
+  double* f1A;         // f10(p1) of the second stage's input, per x
 };
+
+// Vlasov RHS of every l at one (x,p) of the 1D m0 = 0 system from the values
+// of all planes at (p-1, p, p+1) and (x-1, x+1):
+// Actions::Adv (def-Actions.cpp:73-94) = ElectricField::Exyz + Spatial_Advection::Axy
+// restricted to m = 0, with CLEAN_LOW_PC (def-Actions.cpp:240-253).
+template <typename T, int NL>
+__device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, double Ex, T v, T bgg,
+                                             const T (&fm)[NL], const T (&fc)[NL], const T (&fn)[NL],
+                                             const T (&fl)[NL], const T (&fr)[NL], T (&k)[NL]) {
+  const int np = a.np;
+  T G[NL], Hh[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    if (p == 0) {
+      if (l == 0) {
+        const T fe = (fc[0] - fn[0] * a.p0p1_sq) * a.inv_mp0p1_sq;   // G00, def-VlasovMax.cpp:531-532
+        G[0] = (fn[0] - fe) * a.g_r;
+        if (a.bgG) G[0] += bgg;
+        Hh[0] = T(0);
+      } else {
+        G[l] = T(0);
+        Hh[l] = fn[l] * a.hp0[l];                                    // H(p0), :513-514
+      }
+    } else {
+      // SHarmonic::Dp, def-State.cpp:359-380: f(p-1) - f(p+1), and f(N-2) - f(N-1)
+      // at pmax, which is the same expression because callers pass fn = fc there
+      const T dpf = fm[l] - fn[l];
+      if (l == 0) {
+        G[0] = dpf;
+        if (a.bgG) G[0] += bgg;
+        Hh[0] = T(0);
+      } else {
+        T hv = fc[l] * a.invpax[l * np + p];                           // MakeGH, :504-509
+        hv = hv + dpf;
+        Hh[l] = hv;
+        G[l] = dpf * a.cg[l] + hv;
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    T acc = T(0);
+    if (l >= 1) acc += G[l - 1] * static_cast<T>(Ex * a.eA1[l - 1]);
+    if (l + 1 < NL) acc += Hh[l + 1] * static_cast<T>(Ex * a.eA2[l + 1]);
+    if (l >= 1) acc += (fl[l - 1] - fr[l - 1]) * a.xv1[(l - 1) * np + p];  // v * xA1, wave-uniform
+    if (l + 1 < NL) acc += (fl[l + 1] - fr[l + 1]) * a.xv2[(l + 1) * np + p];
+    k[l] = acc;
+  }
+  if (p < NL - 2) {  // CLEAN_LOW_PC, def-Actions.cpp:240-253 (rows p < min(l-1, 80) of l >= 1)
+#pragma unroll
+    for (int l = 2; l < NL; ++l)
+      if (p < min(l - 1, 80)) k[l] = T(0);
+  }
+}
 
 template <typename T, int L0>
 __global__ __launch_bounds__(64 * kRows1D) void vlasov_stage_1d(Stage1DArgs<T> a) {
   constexpr int NL = L0 + 1;
   const int lane = threadIdx.x;
-  const int row = threadIdx.y;
+  const int row = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one row per wavefront
   const int x = blockIdx.x * 64 + lane;
   const int p = blockIdx.y * kRows1D + row;
   const int np = a.np, nx = a.nx;
@@ -246,44 +307,12 @@ __global__ __launch_bounds__(64 * kRows1D) void vlasov_stage_1d(Stage1DArgs<T> a
 #pragma unroll
       for (int l = 0; l < NL; ++l) accv[l] = a.acc[(long long)l * a.plane + idx];
     }
-    const double Ex = a.fin[x];
-    const T v = a.vtab[p];
     const T bgg = a.bgG ? a.bgG[p] : T(0);
-    T G[NL], Hh[NL];
+    T kv[NL];
+    rhs_point_1d<T, NL>(a, p, a.fin[x], a.vtab[p], bgg, fm, fc, fn, fl, fr, kv);
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
-      if (p == 0) {
-        if (l == 0) {
-          const T fe = (fc[0] - fn[0] * a.p0p1_sq) * a.inv_mp0p1_sq;   // G00, def-VlasovMax.cpp:531-532
-          G[0] = (fn[0] - fe) * a.g_r;
-          if (a.bgG) G[0] += bgg;
-          Hh[0] = T(0);
-        } else {
-          G[l] = T(0);
-          Hh[l] = fn[l] * a.hp0[l];                                    // H(p0), :513-514
-        }
-      } else {
-        const T dpf = (p == np - 1) ? (fm[l] - fc[l]) : (fm[l] - fn[l]);  // Dp, def-State.cpp:359-380
-        if (l == 0) {
-          G[0] = dpf;
-          if (a.bgG) G[0] += bgg;
-          Hh[0] = T(0);
-        } else {
-          T hv = fc[l] * a.invpax[l * np + p];                           // MakeGH, :504-509
-          hv = hv + dpf;
-          Hh[l] = hv;
-          G[l] = dpf * a.cg[l] + hv;
-        }
-      }
-    }
-#pragma unroll
-    for (int l = 0; l < NL; ++l) {
-      T acc = T(0);
-      if (l >= 1) acc += G[l - 1] * static_cast<T>(Ex * a.eA1[l - 1]);
-      if (l + 1 < NL) acc += Hh[l + 1] * static_cast<T>(Ex * a.eA2[l + 1]);
-      if (l >= 1) acc += (fl[l - 1] - fr[l - 1]) * (v * static_cast<T>(a.xA1[l - 1]));
-      if (l + 1 < NL) acc += (fl[l + 1] - fr[l + 1]) * (v * static_cast<T>(a.xA2[l + 1]));
-      if (p < (l >= 1 ? min(l - 1, 80) : 0)) acc = T(0);  // CLEAN_LOW_PC, def-Actions.cpp:240-253
+      const T acc = kv[l];
       const long long o = (long long)l * a.plane + idx;
       switch (a.mode) {  // def-RK.cpp:116-138
         case 0: a.out[o] = acc; break;
@@ -308,6 +337,205 @@ __global__ __launch_bounds__(64 * kRows1D) void vlasov_stage_1d(Stage1DArgs<T> a
   }
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD
+// works on one contiguous run of tiles and neighbouring x columns share its L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
+  constexpr int kXcd = 8;
+  if (nblocks % kXcd != 0) return bid;
+  return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
+}
+
+// ---------------------------------------------------------------------------
+// Two RK4 stages in one pass (1D, m0 = 0).  A workgroup owns kPairX x columns
+// and TP = 4R-2 p rows.  Stage one is evaluated on the tile plus one column
+// and one row on each side and kept in LDS; stage two reads its stencil from
+// there.  Per RK4 step the state crosses HBM 7 times (read Y0; write Y2 and
+// the accumulator; read Y2, Y0 and the accumulator; write Y) instead of 16
+// for four separate stages.  The stage arithmetic is the same as in
+// vlasov_stage_1d (def-RK.cpp:116-138): PAIR 0 = stages 1+2, PAIR 1 = 3+4.
+constexpr int kPairX = 62;  // interior columns; lanes 0 and 63 are the halo columns
+
+// raw buffer access: 32-bit byte offsets, plane offset in an SGPR, and
+// hardware range checking against the buffer size
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, int voff, int soff);
+template <>
+__device__ __forceinline__ void bstore<double>(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+template <>
+__device__ __forceinline__ void bstore<float>(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+}
+
+template <typename T, int L0, int R, int PAIR>
+__global__ __launch_bounds__(256) void vlasov_pair_1d(Stage1DArgs<T> a) {
+  constexpr int NL = L0 + 1;
+  constexpr int TP = 4 * R - 2;  // interior rows
+  constexpr int S1 = TP + 2;     // rows of the first stage
+  constexpr int SZ = sizeof(T);
+  __shared__ T ys[S1][NL][64];
+  __shared__ double jsh[2][4][64];
+  const int lane = threadIdx.x;
+  const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);  // one row group per wavefront
+  const int nx = a.nx, np = a.np;
+  // p-tiles of one x strip are consecutive and each XCD gets a contiguous run
+  // of tiles, so the halo rows shared by neighbouring p-tiles are L2 hits
+  const int npt = (np + TP - 1) / TP;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int x0 = (tile / npt) * kPairX;
+  const int pt = tile % npt;
+  const int p0 = pt * TP;
+  const int xe = ((x0 - 1 + lane) % nx + nx) % nx;
+  const int xm = (xe == 0) ? nx - 1 : xe - 1, xp = (xe == nx - 1) ? 0 : xe + 1;
+  const bool inner = lane >= 1 && lane <= kPairX && x0 + lane - 1 < nx;
+  const unsigned nbytes = static_cast<unsigned>(a.plane * NL * SZ);
+  const int pb = static_cast<int>(a.plane) * SZ;  // plane stride in bytes
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(a.in, nbytes);
+  const __amdgpu_buffer_rsrc_t rout = buf_rsrc(a.out, nbytes);
+  const __amdgpu_buffer_rsrc_t racc = buf_rsrc(a.acc, nbytes);
+  const __amdgpu_buffer_rsrc_t ry0 = buf_rsrc(PAIR == 0 ? a.in : a.y0, nbytes);
+  // Y0 of the tile points stays in registers for stage 2 (a re-read would
+  // come from HBM: the tile's inputs are out of L2 by then)
+  constexpr bool kKeepY0 = true;
+  T accr[R][NL], y0r[R][NL];
+  // ---- first stage on rows p0-1 .. p0+TP, columns x0-1 .. x0+kPairX
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int s = ty + 4 * r;
+    const int p = p0 - 1 + s;  // wave-uniform
+    const bool mid = s >= 1 && s <= TP && inner;  // a point of the tile itself
+    if (p >= 0 && p < np) {
+      const int row = p * nx;
+      const int vc = (row + xe) * SZ, vl = (row + xm) * SZ, vr = (row + xp) * SZ;
+      const int vm = (p > 0) ? vc - nx * SZ : vc;        // row p-1 (unused at p = 0)
+      const int vn = (p < np - 1) ? vc + nx * SZ : vc;   // row p+1; f(N-1) itself at pmax
+      T fm[NL], fc[NL], fn[NL], fl[NL], fr[NL];
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        fc[l] = bload<T>(rin, vc, l * pb);
+        fm[l] = bload<T>(rin, vm, l * pb);
+        fn[l] = bload<T>(rin, vn, l * pb);
+        fl[l] = bload<T>(rin, vl, l * pb);
+        fr[l] = bload<T>(rin, vr, l * pb);
+      }
+      T kv[NL];
+      rhs_point_1d<T, NL>(a, p, a.fin[xe], a.vtab[p], a.bgG ? a.bgG[p] : T(0), fm, fc, fn, fl, fr, kv);
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        T y;
+        if (PAIR == 0) {  // stage 1: Y1 = Y0 + h/2 k1, acc = Y0 + h/6 k1
+          const T kh = kv[l] * a.c05h;
+          y = fc[l] + kh;
+          accr[r][l] = fc[l] + kh * a.third;
+          y0r[r][l] = fc[l];
+        } else {          // stage 3: Y3 = Y0 + h k3, acc += h/3 k3
+          const T kh = kv[l] * a.hh;
+          y = bload<T>(ry0, vc, l * pb) + kh;
+          accr[r][l] = mid ? bload<T>(racc, vc, l * pb) + kh * a.third : T(0);
+          y0r[r][l] = T(0);
+        }
+        ys[s][l][lane] = y;
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        ys[s][l][lane] = T(0);
+        accr[r][l] = T(0);
+        y0r[r][l] = T(0);
+      }
+    }
+  }
+  __syncthreads();
+  // ---- second stage on the tile, stencil from LDS
+  double jin = 0.0, jout = 0.0;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int s = ty + 4 * r;
+    const int p = p0 - 1 + s;  // wave-uniform
+    if (s >= 1 && s <= TP && p < np && inner) {
+      // rows outside 0..np-1 hold zeros and are not read: p = 0 uses no f(p-1),
+      // and at pmax f(p+1) is replaced by f(p) as in the first stage
+      T fm[NL], fc[NL], fn[NL], fl[NL], fr[NL];
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        fc[l] = ys[s][l][lane];
+        fm[l] = ys[s - 1][l][lane];
+        fn[l] = (p < np - 1) ? ys[s + 1][l][lane] : fc[l];
+        fl[l] = ys[s][l][lane - 1];
+        fr[l] = ys[s][l][lane + 1];
+      }
+      T kv[NL];
+      rhs_point_1d<T, NL>(a, p, a.fin2[xe], a.vtab[p], a.bgG ? a.bgG[p] : T(0), fm, fc, fn, fl, fr, kv);
+      const int vc = (p * nx + xe) * SZ;
+      const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        if (PAIR == 0) {  // stage 2: Y2 = Y0 + h/2 k2, acc += h/3 k2
+          const T kh = kv[l] * a.c05h;
+          const T y2 = (kKeepY0 ? y0r[r][l] : bload<T>(rin, vc, l * pb)) + kh;
+          bstore<T>(y2, rout, vc, l * pb);
+          bstore<T>(accr[r][l] + kh * a.twothird, racc, vc, l * pb);
+          if (l == 1) jout += w * (static_cast<double>(y2) * a.p3og[p]);
+        } else {          // stage 4: Y = acc + h/6 k4
+          const T kh = kv[l] * a.h6;
+          bstore<T>(accr[r][l] + kh, rout, vc, l * pb);
+        }
+      }
+      if (NL > 1) {
+        jin += w * (static_cast<double>(fc[1]) * a.p3og[p]);
+        if (p == 1) a.f1A[xe] = static_cast<double>(fc[1]);
+      }
+    }
+  }
+  jsh[0][ty][lane] = jin;
+  jsh[1][ty][lane] = jout;
+  __syncthreads();
+  if (ty == 0 && inner) {
+    const long long o = ((long long)pt * 3) * nx + xe;
+    a.jpA[o] = ((jsh[0][0][lane] + jsh[0][1][lane]) + jsh[0][2][lane]) + jsh[0][3][lane];
+    if (PAIR == 0) a.jpB[o] = ((jsh[1][0][lane] + jsh[1][1][lane]) + jsh[1][2][lane]) + jsh[1][3][lane];
+  }
+}
+
+// Current::Jxyz integrand of f10 summed over 64-row blocks of p: [pb][3][nx], component 0
+template <typename T>
+__global__ __launch_bounds__(256) void current_1d(const T* __restrict__ f10, const double* __restrict__ p3og,
+                                                  double* __restrict__ jp, int np, int nx) {
+  __shared__ double sh[4][64];
+  const int lane = threadIdx.x, ty = threadIdx.y;
+  const int x = blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (x < nx) {
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+      const int p = blockIdx.y * 64 + ty + 4 * i;
+      if (p < np) {
+        const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
+        s += w * (static_cast<double>(f10[(long long)p * nx + x]) * p3og[p]);
+      }
+    }
+  }
+  sh[ty][lane] = s;
+  __syncthreads();
+  if (ty == 0 && x < nx)
+    jp[((long long)blockIdx.y * 3) * nx + x] = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+}
+
 template <typename T>
 struct FieldArgs {
   const double* fin;
@@ -316,6 +544,7 @@ struct FieldArgs {
   double* facc;
   const double* jpart;
   const T* fpl;  // distribution planes of the stage input
+  const double* f1p1;  // if set: f10 at p1 per x of the stage input (instead of fpl)
   int npb, np, nx, ny, mode;
   int c10, c11r, c11i;
   long long plane;
@@ -391,7 +620,7 @@ __global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage(FieldArgs<T>
     double j = jx;
     j *= a.delta_p;
     const long long o1 = (long long)a.c10 * a.plane + ((long long)y * a.np + 1) * nx + x;
-    j += a.small * (static_cast<double>(a.fpl[o1]) * a.p3og1);
+    j += a.small * ((a.f1p1 ? a.f1p1[i] : static_cast<double>(a.fpl[o1])) * a.p3og1);
     j *= 4.0 * M_PI / 3.0;
     k[0] += j;
   }
@@ -816,14 +1045,6 @@ __device__ __forceinline__ void tri_solve(const double (&A)[NPL], const double (
   double x = lane_next(de, 0.0);
 #pragma unroll
   for (int i = NPL - 1; i >= 0; --i) { x = d[i] - cp[i] * x; d[i] = x; }
-}
-
-// Workgroups are dealt to the 8 XCDs round-robin; remap so that each XCD
-// works on one contiguous run of tiles and neighbouring x columns share its L2.
-__device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
-  constexpr int kXcd = 8;
-  if (nblocks % kXcd != 0) return bid;
-  return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
 }
 
 constexpr int kCollCols = 8;  // x columns (one wavefront each) per workgroup

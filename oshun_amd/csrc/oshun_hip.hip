@@ -55,6 +55,14 @@ struct oshun_ctx {
   // deviation only; d_bgG holds G00(bg) for the Vlasov kernels.
   double* d_bg = nullptr;
   float* d_bgG = nullptr;
+  // 1D m0 = 0: RK4 as two fused stage pairs (vlasov_pair_1d)
+  bool use_pair = false;
+  int npt = 0;                 // p tiles of the pair kernels
+  int npb0 = 0;                // 64-row blocks of current_1d
+  double* jp0 = nullptr;       // current partials of Y0 [npb0][3][nx]
+  double* jp1 = nullptr;       // of the second stage input of a pair [npt][3][nx]
+  double* jp2 = nullptr;       // of Y2 [npt][3][nx]
+  double* f1y = nullptr;       // f10(p1) of the second stage input, per x
   std::vector<double> bg;
   int rows_npl = 0;
   oshun::CollTables ct;
@@ -103,14 +111,16 @@ void prof_end(oshun_ctx* c, ProfRec* r) {
 
 template <typename T>
 const T* vt_ptr(const oshun_ctx* c, int which) {
-  // invpax (l0+1)*np | vtab np | cg l0+1 | hp0 l0+1
+  // invpax (l0+1)*np | vtab np | cg l0+1 | hp0 l0+1 | xv1 (l0+1)*np | xv2 (l0+1)*np
   const T* b = static_cast<const T*>(c->d_vt);
   const size_t L = c->T.l0 + 1, np = c->np;
   switch (which) {
     case 0: return b;
     case 1: return b + L * np;
     case 2: return b + L * np + np;
-    default: return b + L * np + np + L;
+    case 3: return b + L * np + np + L;
+    case 4: return b + L * np + np + 2 * L;
+    default: return b + 2 * L * np + np + 2 * L;
   }
 }
 
@@ -130,9 +140,8 @@ template <>
 const float* bg_g00<float>(const oshun_ctx* c) { return c->d_bgG; }
 
 template <typename T>
-int launch_stage_1d(oshun_ctx* c, int mode, const void* in, const void* y0, void* out, void* acc,
-                    const double* fin, double h) {
-  oshun::Stage1DArgs<T> a;
+void fill_1d_args(oshun_ctx* c, oshun::Stage1DArgs<T>& a, int mode, const void* in, const void* y0, void* out,
+                  void* acc, const double* fin, double h) {
   a.in = static_cast<const T*>(in);
   a.y0 = static_cast<const T*>(y0);
   a.out = static_cast<T*>(out);
@@ -143,6 +152,8 @@ int launch_stage_1d(oshun_ctx* c, int mode, const void* in, const void* y0, void
   a.vtab = vt_ptr<T>(c, 1);
   a.cg = vt_ptr<T>(c, 2);
   a.hp0 = vt_ptr<T>(c, 3);
+  a.xv1 = vt_ptr<T>(c, 4);
+  a.xv2 = vt_ptr<T>(c, 5);
   a.p3og = c->d_p3og;
   const size_t L = c->T.l0 + 1;
   a.eA1 = c->d_e1d;
@@ -160,6 +171,14 @@ int launch_stage_1d(oshun_ctx* c, int mode, const void* in, const void* y0, void
   a.twothird = static_cast<T>(2.0 / 3.0);
   a.np = c->np; a.nx = c->nx; a.mode = mode;
   a.plane = c->plane;
+  a.fin2 = nullptr; a.jpA = nullptr; a.jpB = nullptr; a.f1A = nullptr;
+}
+
+template <typename T>
+int launch_stage_1d(oshun_ctx* c, int mode, const void* in, const void* y0, void* out, void* acc,
+                    const double* fin, double h) {
+  oshun::Stage1DArgs<T> a;
+  fill_1d_args<T>(c, a, mode, in, y0, out, acc, fin, h);
   dim3 block(64, oshun::kRows1D, 1);
   dim3 grid((c->nx + 63) / 64, c->npb, 1);
   ProfRec r;
@@ -229,12 +248,13 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
 
 template <typename T>
 int launch_field(oshun_ctx* c, int mode, const double* fin, const double* f0, double* fout, double* facc,
-                 const void* fpl, double h) {
+                 const void* fpl, double h, const double* jp = nullptr, int npb = 0, const double* f1p1 = nullptr) {
   oshun::FieldArgs<T> a;
   a.fin = fin; a.f0 = f0; a.fout = fout; a.facc = facc;
-  a.jpart = c->jpart;
+  a.jpart = jp ? jp : c->jpart;
   a.fpl = static_cast<const T*>(fpl);
-  a.npb = c->npb; a.np = c->np; a.nx = c->nx; a.ny = c->ny; a.mode = mode;
+  a.f1p1 = f1p1;
+  a.npb = jp ? npb : c->npb; a.np = c->np; a.nx = c->nx; a.ny = c->ny; a.mode = mode;
   a.c10 = c->T.comp_re[oshun::hind(1, 0, c->T.m0)];
   a.c11r = c->T.m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, c->T.m0)] : -1;
   a.c11i = c->T.m0 >= 1 ? c->T.comp_im[oshun::hind(1, 1, c->T.m0)] : -1;
@@ -271,6 +291,83 @@ int rk4(oshun_ctx* c, double h) {
   if ((rc = launch_stage_pair<T>(c, 4, c->B1, nullptr, c->S, c->B1f, nullptr, c->Sf, h))) return rc;
   c->t_rk += h;
   return OSHUN_OK;
+}
+
+constexpr int kPairR = 3;  // rows per thread of the pair kernels: 4*3-2 = 10 p rows per tile
+constexpr int kPairTP = 4 * kPairR - 2;
+
+template <typename T, int L0>
+void launch_pair_l0(oshun_ctx* c, int pair, const oshun::Stage1DArgs<T>& a, dim3 grid) {
+  if (pair == 0)
+    hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, kPairR, 0>), grid, dim3(64, 4), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, kPairR, 1>), grid, dim3(64, 4), 0, c->stream, a);
+}
+
+// RK4 stages 1+2 (pair 0) or 3+4 (pair 1) of the 1D m0 = 0 system
+template <typename T>
+int launch_pair(oshun_ctx* c, int pair, const void* in, const void* y0, void* out, void* acc, const double* fin,
+                const double* fin2, double* jpA, double* jpB, double h) {
+  oshun::Stage1DArgs<T> a;
+  fill_1d_args<T>(c, a, 0, in, y0, out, acc, fin, h);
+  a.fin2 = fin2;
+  a.jpA = jpA;
+  a.jpB = jpB;
+  a.f1A = c->f1y;
+  const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
+  dim3 grid(ntx * c->npt);
+  ProfRec r;
+  prof_begin(c, 0, &r);
+  switch (c->T.l0) {
+    case 1: launch_pair_l0<T, 1>(c, pair, a, grid); break;
+    case 2: launch_pair_l0<T, 2>(c, pair, a, grid); break;
+    case 3: launch_pair_l0<T, 3>(c, pair, a, grid); break;
+    case 4: launch_pair_l0<T, 4>(c, pair, a, grid); break;
+    case 5: launch_pair_l0<T, 5>(c, pair, a, grid); break;
+    case 6: launch_pair_l0<T, 6>(c, pair, a, grid); break;
+    case 7: launch_pair_l0<T, 7>(c, pair, a, grid); break;
+    case 8: launch_pair_l0<T, 8>(c, pair, a, grid); break;
+    case 9: launch_pair_l0<T, 9>(c, pair, a, grid); break;
+    case 10: launch_pair_l0<T, 10>(c, pair, a, grid); break;
+    case 11: launch_pair_l0<T, 11>(c, pair, a, grid); break;
+    default: launch_pair_l0<T, 12>(c, pair, a, grid); break;
+  }
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  return OSHUN_OK;
+}
+
+// Runge_Kutta_4::advance_p1 (def-RK.cpp:110-145) for 1D m0 = 0 with the
+// stages fused in pairs.  Y0 = S; Y2 -> B2; accumulator A; Y -> B1, then S
+// and B1 swap (the second pair reads Y0 around its tile while other tiles
+// write Y, so Y cannot go to S in place).  The field stages are unchanged:
+// each uses the current of its stage input.
+template <typename T>
+int rk4_pair(oshun_ctx* c, double h) {
+  int rc;
+  {
+    ProfRec r;
+    prof_begin(c, 1, &r);
+    const T* f10 = static_cast<const T*>(c->S) + (size_t)c->T.comp_re[oshun::hind(1, 0, 0)] * c->plane;
+    hipLaunchKernelGGL(oshun::current_1d<T>, dim3((c->nx + 63) / 64, c->npb0), dim3(64, 4), 0, c->stream, f10,
+                       c->d_p3og, c->jp0, c->np, c->nx);
+    prof_end(c, &r);
+    HIPCHK(c, hipGetLastError());
+  }
+  if ((rc = launch_field<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, c->S, h, c->jp0, c->npb0))) return rc;
+  if ((rc = launch_pair<T>(c, 0, c->S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h))) return rc;
+  if ((rc = launch_field<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, c->npt, c->f1y))) return rc;
+  if ((rc = launch_field<T>(c, 3, c->B2f, c->Sf, c->B1f, c->Af, c->B2, h, c->jp2, c->npt))) return rc;
+  if ((rc = launch_pair<T>(c, 1, c->B2, c->S, c->B1, c->A, c->B2f, c->B1f, c->jp1, nullptr, h))) return rc;
+  if ((rc = launch_field<T>(c, 4, c->B1f, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, c->npt, c->f1y))) return rc;
+  std::swap(c->S, c->B1);
+  c->t_rk += h;
+  return OSHUN_OK;
+}
+
+int do_rk4(oshun_ctx* c, double h) {
+  if (c->use_pair) return c->fp32 ? rk4_pair<float>(c, h) : rk4_pair<double>(c, h);
+  return c->fp32 ? rk4<float>(c, h) : rk4<double>(c, h);
 }
 
 template <typename T, int NPL>
@@ -337,7 +434,7 @@ void ee_substeps(const oshun_ctx* c, double tnew, int* numh, double* h) {
 }
 
 int full_step(oshun_ctx* c, double h) {
-  int rc = c->fp32 ? rk4<float>(c, h) : rk4<double>(c, h);
+  int rc = do_rk4(c, h);
   if (rc) return rc;
   const double t = c->t_rk;
   const int coll = c->ph.collisions;
@@ -467,7 +564,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_bg, c->d_bgG};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_bg, c->d_bgG, c->jp0, c->jp1, c->jp2, c->f1y};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -543,6 +640,17 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     if (hipMemset(*p, 0, fldbytes) != hipSuccess) return bail(OSHUN_ERR_HIP);
   if (hipMalloc((void**)&c->jpart, (size_t)c->npb * 3 * c->nxy * sizeof(double)) != hipSuccess)
     return bail(OSHUN_ERR_ALLOC);
+  c->use_pair = c->use1d;
+  if (c->use_pair) {
+    c->npt = (c->np + kPairTP - 1) / kPairTP;
+    c->npb0 = (c->np + 63) / 64;
+    const size_t nx = c->nx;
+    if (hipMalloc((void**)&c->jp0, (size_t)c->npb0 * 3 * nx * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->jp1, (size_t)c->npt * 3 * nx * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->jp2, (size_t)c->npt * 3 * nx * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->f1y, nx * sizeof(double)) != hipSuccess)
+      return bail(OSHUN_ERR_ALLOC);
+  }
   if (c->fp32) {
     c->bg.assign(c->np, 0.0);
     if (upload_vec(c, &c->d_bg, c->bg) || upload_vec(c, &c->d_bgG, std::vector<float>(c->np, 0.0f)))
@@ -572,12 +680,29 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     vt.insert(vt.end(), c->T.vtab.begin(), c->T.vtab.end());
     vt.insert(vt.end(), c->T.cg.begin(), c->T.cg.end());
     vt.insert(vt.end(), c->T.hp0.begin(), c->T.hp0.end());
+    // x-advection multipliers v(p)*A of the 1D kernels, formed in the storage precision
+    auto xv = [&](auto zero, std::vector<decltype(zero)>& out) {
+      using V = decltype(zero);
+      const size_t L = c->T.l0 + 1, np = c->np;
+      out.assign(2 * L * np, V(0));
+      for (size_t l = 0; l < L; ++l)
+        for (size_t q = 0; q < np; ++q) {
+          const V v = static_cast<V>(c->T.vtab[q]);
+          out[l * np + q] = v * static_cast<V>(c->T.x1d_A1[l]);
+          out[(L + l) * np + q] = v * static_cast<V>(c->T.x1d_A2[l]);
+        }
+    };
     if (c->fp32) {
-      std::vector<float> vf(vt.begin(), vt.end());
+      std::vector<float> vf(vt.begin(), vt.end()), x2;
+      xv(0.0f, x2);
+      vf.insert(vf.end(), x2.begin(), x2.end());
       float* d = nullptr;
       if (upload_vec(c, &d, vf)) return bail(OSHUN_ERR_HIP);
       c->d_vt = d;
     } else {
+      std::vector<double> x2;
+      xv(0.0, x2);
+      vt.insert(vt.end(), x2.begin(), x2.end());
       double* d = nullptr;
       if (upload_vec(c, &d, vt)) return bail(OSHUN_ERR_HIP);
       c->d_vt = d;
@@ -737,7 +862,7 @@ int oshun_rhs(oshun_ctx* c, double* f_slope, double* fld_slope) {
 
 int oshun_rk4_step(oshun_ctx* c, double h) {
   if (!sane(c)) return OSHUN_ERR_ARG;
-  return c->fp32 ? rk4<float>(c, h) : rk4<double>(c, h);
+  return do_rk4(c, h);
 }
 
 int oshun_collide_ee(oshun_ctx* c, double tnew) {
