@@ -1201,16 +1201,16 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
 }
 
 // ---------------------------------------------------------------------------
-// collide_rows: the collision step with 16 lanes per (x,y) column, i.e. one
-// 16-lane DPP row per column and four columns per wavefront.  Each lane owns
-// NPL consecutive p rows and runs the reference's recurrences over them; the
-// only cross-lane steps are 4-step DPP scans inside the row and row_newbcast
-// for single values.  The p tables sit in LDS in an interleaved order (row i
-// of every lane is one 128-byte line), all p < np masks are folded into the
-// tables (zero where the reference's loops skip a row), and divisions by
-// tabulated or uniform quantities are multiplications by reciprocals.
-constexpr int kRowCols = 16;  // columns per workgroup (4 wavefronts x 4 rows)
-constexpr int kRowP = 256;    // LDS table length: 16 lanes x NPL rows
+// collide_rows: the collision step with SEGW = 16 or 32 lanes per (x,y)
+// column (one or two 16-lane DPP rows), 64/SEGW columns per wavefront.  Each
+// lane owns NPL consecutive p rows and runs the reference's recurrences over
+// them; the lanes of a column are joined by 4-step DPP scans inside each row
+// plus, for 32 lanes, one exchange between the two rows (ds_swizzle, lane
+// xor 16).  The p tables sit in LDS in an interleaved order (row i of every
+// lane is one contiguous line), all p < np masks are folded into the tables
+// (zero where the reference's loops skip a row), and divisions by tabulated
+// or uniform quantities are multiplications by reciprocals.
+constexpr int kRowP = 256;    // LDS table length: SEGW lanes x NPL rows
 
 template <int N>
 __device__ __forceinline__ double row_bcast(double v) {  // lane N of the 16-lane row, to the whole row
@@ -1232,6 +1232,65 @@ __device__ __forceinline__ double row_scan_rev(double v) {  // inclusive suffix 
   v += dpp_mov<row_shl<8>()>(v, 0.0);
   return v;
 }
+// the value of lane ^ 16 (the other row of a 32-lane column); all lanes active
+__device__ __forceinline__ double xor16(double v) {
+  const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), 0x401F);  // bit mode: and 0x1f, xor 0x10
+  const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), 0x401F);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ bool upper_row() { return (threadIdx.x & 16) != 0; }
+
+// column-wide (SEGW lanes) versions of the row primitives
+template <int SEGW>
+__device__ __forceinline__ double seg_scan(double v) {  // inclusive prefix
+  v = row_scan(v);
+  if (SEGW == 32) {
+    const double t = xor16(row_bcast<15>(v));
+    v += upper_row() ? t : 0.0;
+  }
+  return v;
+}
+template <int SEGW>
+__device__ __forceinline__ double seg_scan_rev(double v) {  // inclusive suffix
+  v = row_scan_rev(v);
+  if (SEGW == 32) {
+    const double t = xor16(row_bcast<0>(v));
+    v += upper_row() ? 0.0 : t;
+  }
+  return v;
+}
+template <int SEGW>
+__device__ __forceinline__ double seg_last(double v) {  // the column's last lane, to all its lanes
+  const double t = row_bcast<15>(v);
+  if (SEGW == 16) return t;
+  const double o = xor16(t);
+  return upper_row() ? t : o;
+}
+template <int SEGW>
+__device__ __forceinline__ double seg_first(double v) {  // the column's lane 0, to all its lanes
+  const double b = row_bcast<0>(v);
+  if (SEGW == 16) return b;
+  const double o = xor16(b);
+  return upper_row() ? o : b;
+}
+template <int SEGW>
+__device__ __forceinline__ double seg_prev(double v, double fill) {  // lane - 1 (fill at lane 0)
+  double r = dpp_mov<row_shr<1>()>(v, fill);
+  if (SEGW == 32) {
+    const double o = xor16(row_bcast<15>(v));
+    r = ((threadIdx.x & 31) == 16) ? o : r;
+  }
+  return r;
+}
+template <int SEGW>
+__device__ __forceinline__ double seg_next(double v, double fill) {  // lane + 1 (fill at the last lane)
+  double r = dpp_mov<row_shl<1>()>(v, fill);
+  if (SEGW == 32) {
+    const double o = xor16(row_bcast<0>(v));
+    r = ((threadIdx.x & 31) == 15) ? o : r;
+  }
+  return r;
+}
 // 1/d by v_rcp_f64 and two Newton steps (within an ulp or so of the correctly rounded quotient)
 __device__ __forceinline__ double rcp_nr(double d) {
   double r = __builtin_amdgcn_rcp(d);
@@ -1242,7 +1301,7 @@ __device__ __forceinline__ double rcp_nr(double d) {
 }
 
 // LDS/host table rows of collide_rows; element p of row k sits at
-// k*kRowP + (p % NPL)*16 + p / NPL.
+// k*kRowP + (p % NPL)*SEGW + p / NPL.
 enum {
   RT_U4 = 0,  // 0.5 v^4 dv          (0 at p = 0 and p >= np)
   RT_U4M1,    // 0.5 v_{p-1}^4 dv    (same zeros)
@@ -1268,43 +1327,43 @@ enum {
   RT_COUNT
 };
 
-template <int NPL>
+template <int SEGW>
 __device__ __forceinline__ double rt(const double* tb, int k, int i, int seg) {
-  return tb[k * kRowP + i * 16 + seg];
+  return tb[k * kRowP + i * SEGW + seg];
 }
 
 // EE_isotropic_conserva::Getslope (def-FokkerPlanck.cpp:192-267) for the NPL
 // rows p = seg*NPL + i of one lane; f in, slope out (same array)
-template <typename T, int NPL>
+template <typename T, int NPL, int SEGW>
 __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double* tb, int seg, double (&f)[NPL]) {
   // re-read the tables from LDS on every call instead of letting the
   // compiler keep all 9*NPL of them in registers across the subcycles
   __asm__ volatile("" ::: "memory");
-  const double fprev = dpp_mov<row_shr<1>()>(f[NPL - 1], 0.0);
-  const double fnext = dpp_mov<row_shl<1>()>(f[0], 0.0);
+  const double fprev = seg_prev<SEGW>(f[NPL - 1], 0.0);
+  const double fnext = seg_next<SEGW>(f[0], 0.0);
   // trapezoid prefix sums I4, I2 (from p = 1) and suffix J1, lane-local first
   double I4[NPL], I2[NPL], J1[NPL];
   double s4 = 0.0, s2 = 0.0, s1 = 0.0;
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const double fm = (i > 0) ? f[i - 1] : fprev;
-    s4 += rt<NPL>(tb, RT_U4, i, seg) * f[i] + rt<NPL>(tb, RT_U4M1, i, seg) * fm;
-    s2 += rt<NPL>(tb, RT_U2, i, seg) * f[i] + rt<NPL>(tb, RT_U2M1, i, seg) * fm;
+    s4 += rt<SEGW>(tb, RT_U4, i, seg) * f[i] + rt<SEGW>(tb, RT_U4M1, i, seg) * fm;
+    s2 += rt<SEGW>(tb, RT_U2, i, seg) * f[i] + rt<SEGW>(tb, RT_U2M1, i, seg) * fm;
     I4[i] = s4;
     I2[i] = s2;
   }
 #pragma unroll
   for (int i = NPL - 1; i >= 0; --i) {
     const double fn = (i < NPL - 1) ? f[i + 1] : fnext;
-    s1 += rt<NPL>(tb, RT_U1S, i, seg) * fn + rt<NPL>(tb, RT_U1M1S, i, seg) * f[i];
+    s1 += rt<SEGW>(tb, RT_U1S, i, seg) * fn + rt<SEGW>(tb, RT_U1M1S, i, seg) * f[i];
     J1[i] = s1;
   }
-  const double inc4 = row_scan(s4), inc2 = row_scan(s2), inc1 = row_scan_rev(s1);
-  const double ex4 = dpp_mov<row_shr<1>()>(inc4, 0.0), ex2 = dpp_mov<row_shr<1>()>(inc2, 0.0);
-  const double ex1 = dpp_mov<row_shl<1>()>(inc1, 0.0);
-  const double lnee = log_ee(4.0 * M_PI * row_bcast<15>(inc2), 4.0 * M_PI * row_bcast<15>(inc4), a.density_np);
-  const double f0 = row_bcast<0>(f[0]);
-  const double f1 = row_bcast<0>(f[1]);
+  const double inc4 = seg_scan<SEGW>(s4), inc2 = seg_scan<SEGW>(s2), inc1 = seg_scan_rev<SEGW>(s1);
+  const double ex4 = seg_prev<SEGW>(inc4, 0.0), ex2 = seg_prev<SEGW>(inc2, 0.0);
+  const double ex1 = seg_next<SEGW>(inc1, 0.0);
+  const double lnee = log_ee(4.0 * M_PI * seg_last<SEGW>(inc2), 4.0 * M_PI * seg_last<SEGW>(inc4), a.density_np);
+  const double f0 = seg_first<SEGW>(f[0]);
+  const double f1 = seg_first<SEGW>(f[1]);
   const double f00 = (f0 - f1 * a.r01) * a.inv_gden;
   double G[NPL];
 #pragma unroll
@@ -1312,39 +1371,39 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
     const double j1 = J1[i] + ex1;
     double i2 = (I2[i] + ex2) * j1;
     i2 *= -3.0;
-    double g = (I4[i] + ex4) + rt<NPL>(tb, RT_U3, i, seg) * j1;
+    double g = (I4[i] + ex4) + rt<SEGW>(tb, RT_U3, i, seg) * j1;
     g *= f[i];
     g += i2;
     if (i < a.nb) {  // parabolic G for p < NB (EE_isotropic_conserva::G, :178-188); only lane 0 has such rows
-      const double p3 = tb[RT_P3 * kRowP + i * 16];
-      const double i2s = f00 * tb[RT_A2 * kRowP + i * 16] + (f1 - f00) * tb[RT_B2 * kRowP + i * 16];
-      const double i4s = f00 * tb[RT_A4 * kRowP + i * 16] + (f1 - f00) * tb[RT_B4 * kRowP + i * 16];
+      const double p3 = tb[RT_P3 * kRowP + i * SEGW];
+      const double i2s = f00 * tb[RT_A2 * kRowP + i * SEGW] + (f1 - f00) * tb[RT_B2 * kRowP + i * SEGW];
+      const double i4s = f00 * tb[RT_A4 * kRowP + i * SEGW] + (f1 - f00) * tb[RT_B4 * kRowP + i * SEGW];
       const double gl = f[i] * i4s + (p3 * f[i] - 3.0 * i2s) * j1;
       g = (seg == 0) ? gl : g;
     }
     G[i] = g;
   }
   // -DG, -DG/(v Dv), DDG/(v Dv), DDG/(v^3 DDv)   (:247-260)
-  const double Gnext = dpp_mov<row_shl<1>()>(G[0], 0.0);
+  const double Gnext = seg_next<SEGW>(G[0], 0.0);
   double D2[NPL];
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const double gn = (i < NPL - 1) ? G[i + 1] : Gnext;
-    D2[i] = (G[i] - gn) * rt<NPL>(tb, RT_PN, i, seg);
+    D2[i] = (G[i] - gn) * rt<SEGW>(tb, RT_PN, i, seg);
   }
-  double fh0 = (-1.0) * row_bcast<0>(G[0]);
+  double fh0 = (-1.0) * seg_first<SEGW>(G[0]);
   fh0 *= a.c2v0;
-  fh0 -= row_bcast<0>(D2[0]);
+  fh0 -= seg_first<SEGW>(D2[0]);
   fh0 *= tb[RT_QN * kRowP];
-  const double D2next = dpp_mov<row_shl<1>()>(D2[0], 0.0);
+  const double D2next = seg_next<SEGW>(D2[0], 0.0);
   double D3[NPL];
 #pragma unroll
   for (int i = 0; i < NPL; ++i) D3[i] = D2[i] - ((i < NPL - 1) ? D2[i + 1] : D2next);
-  const double D3prev = dpp_mov<row_shr<1>()>(D3[NPL - 1], 0.0);
+  const double D3prev = seg_prev<SEGW>(D3[NPL - 1], 0.0);
   const double sc = a.c_kpre * lnee;
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
-    double v = rt<NPL>(tb, RT_QN, i, seg) * ((i > 0) ? D3[i - 1] : D3prev);
+    double v = rt<SEGW>(tb, RT_QN, i, seg) * ((i > 0) ? D3[i - 1] : D3prev);
     if (i == 0) v = (seg == 0) ? fh0 : v;
     f[i] = v * sc;
   }
@@ -1352,19 +1411,20 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 
 // MODE 1: e-e subcycles on f_00 (Explicit_EC_FP::loop); MODE 2: implicit
 // l >= 1 step (Euler_Backward::advance_1/advance_lm).  Separate launches.
-template <typename T, int NPL, int MODE>
+template <typename T, int NPL, int SEGW, int MODE>
 __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
+  constexpr int kCols = 256 / SEGW;  // columns per workgroup
   __shared__ double tb[RT_COUNT * kRowP];
   for (int i = threadIdx.x; i < RT_COUNT * kRowP; i += blockDim.x) tb[i] = a.rt[i];
   __syncthreads();
   const int nx = a.nx, np = a.np;
-  const int ntx = (nx + kRowCols - 1) / kRowCols;
+  const int ntx = (nx + kCols - 1) / kCols;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int y = tile / ntx;
-  const int xc = (tile % ntx) * kRowCols + (threadIdx.x >> 4);
+  const int xc = (tile % ntx) * kCols + threadIdx.x / SEGW;
   const bool colv = xc < nx;
   const int x = colv ? xc : 0;  // idle rows compute on column 0 and store nothing
-  const int seg = threadIdx.x & 15;
+  const int seg = threadIdx.x & (SEGW - 1);
   const int p0 = seg * NPL;
   const long long base = (long long)y * np * nx + x;
   double f[NPL];
@@ -1381,16 +1441,16 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
       double f0[NPL], fs[NPL];
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { f0[i] = f[i]; fs[i] = f[i]; }
-      ee_slope_rows<T, NPL>(a, tb, seg, fs);
+      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs);
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { fs[i] *= (0.5 * h); const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (1.0 / 3.0); }
-      ee_slope_rows<T, NPL>(a, tb, seg, fs);
+      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs);
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { fs[i] *= (0.5 * h); const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (2.0 / 3.0); }
-      ee_slope_rows<T, NPL>(a, tb, seg, fs);
+      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs);
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { fs[i] *= h; const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (1.0 / 3.0); }
-      ee_slope_rows<T, NPL>(a, tb, seg, fs);
+      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs);
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { fs[i] *= (h / 6.0); f[i] += fs[i]; }
     }
@@ -1404,29 +1464,29 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
   // Implicit_FLM_step::reset_coeff, def-FokkerPlanck.cpp:808-939
   double Asub[NPL], Bdia[NPL], Csup[NPL], Sct[NPL];
   {
-    const double fprev = dpp_mov<row_shr<1>()>(f[NPL - 1], 0.0);
-    const double fnext = dpp_mov<row_shl<1>()>(f[0], 0.0);
+    const double fprev = seg_prev<SEGW>(f[NPL - 1], 0.0);
+    const double fnext = seg_next<SEGW>(f[0], 0.0);
     double I2[NPL], I0[NPL], J1[NPL];
     double s4 = 0.0, s2 = 0.0, s1 = 0.0;
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
       const double fm = (i > 0) ? f[i - 1] : fprev;
-      s4 += rt<NPL>(tb, RT_U4, i, seg) * f[i] + rt<NPL>(tb, RT_U4M1, i, seg) * fm;
-      s2 += rt<NPL>(tb, RT_U2, i, seg) * f[i] + rt<NPL>(tb, RT_U2M1, i, seg) * fm;
+      s4 += rt<SEGW>(tb, RT_U4, i, seg) * f[i] + rt<SEGW>(tb, RT_U4M1, i, seg) * fm;
+      s2 += rt<SEGW>(tb, RT_U2, i, seg) * f[i] + rt<SEGW>(tb, RT_U2M1, i, seg) * fm;
       I2[i] = s4;
       I0[i] = s2;
     }
 #pragma unroll
     for (int i = NPL - 1; i >= 0; --i) {
       const double fn = (i < NPL - 1) ? f[i + 1] : fnext;
-      s1 += rt<NPL>(tb, RT_U1S, i, seg) * fn + rt<NPL>(tb, RT_U1M1S, i, seg) * f[i];
+      s1 += rt<SEGW>(tb, RT_U1S, i, seg) * fn + rt<SEGW>(tb, RT_U1M1S, i, seg) * f[i];
       J1[i] = s1;
     }
-    const double inc4 = row_scan(s4), inc2 = row_scan(s2), inc1 = row_scan_rev(s1);
-    const double ex4 = dpp_mov<row_shr<1>()>(inc4, 0.0), ex2 = dpp_mov<row_shr<1>()>(inc2, 0.0);
-    const double ex1 = dpp_mov<row_shl<1>()>(inc1, 0.0);
-    const double I2_temperature = row_bcast<15>(inc4) * (4.0 * M_PI);
-    const double I0_density = row_bcast<15>(inc2) * (4.0 * M_PI);
+    const double inc4 = seg_scan<SEGW>(s4), inc2 = seg_scan<SEGW>(s2), inc1 = seg_scan_rev<SEGW>(s1);
+    const double ex4 = seg_prev<SEGW>(inc4, 0.0), ex2 = seg_prev<SEGW>(inc2, 0.0);
+    const double ex1 = seg_next<SEGW>(inc1, 0.0);
+    const double I2_temperature = seg_last<SEGW>(inc4) * (4.0 * M_PI);
+    const double I0_density = seg_last<SEGW>(inc2) * (4.0 * M_PI);
     const double lnee = log_ee(I0_density, I2_temperature, a.density_np);
     const double zi = zlog_ei(I0_density, I2_temperature, a.density_np, a.zeta) * I0_density;
     const double kd = a.kpre * a.dt_flm;
@@ -1434,19 +1494,19 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
       double i2 = (I2[i] + ex4) * (4.0 * M_PI);
-      i2 *= rt<NPL>(tb, RT_IV2, i, seg);
+      i2 *= rt<SEGW>(tb, RT_IV2, i, seg);
       const double i0 = (I0[i] + ex2) * (4.0 * M_PI);
-      const double j1 = (J1[i] + ex1) * (4.0 * M_PI * rt<NPL>(tb, RT_VC, i, seg));
+      const double j1 = (J1[i] + ex1) * (4.0 * M_PI * rt<SEGW>(tb, RT_VC, i, seg));
       const double T1 = i0 + (2.0 * j1 - i2) * (1.0 / 3.0);
       const double T2 = (i2 + j1) * (1.0 / 3.0);
       double sct = T1 * lnee;
       if (i == 0) sct = (seg == 0) ? 0.0 : sct;  // Scattering_Term[0] has no e-e part (:874)
       sct += zi;
-      sct *= rt<NPL>(tb, RT_IU3, i, seg);
+      sct *= rt<SEGW>(tb, RT_IU3, i, seg);
       Sct[i] = sct * kd;
-      const double dm = rt<NPL>(tb, RT_IVDM, i, seg), dpp = rt<NPL>(tb, RT_IVDP, i, seg);
-      const double s2v = rt<NPL>(tb, RT_IVS2, i, seg);
-      const double dg = 8.0 * M_PI * f[i] * rt<NPL>(tb, RT_ROW, i, seg) - T2 * (dm + dpp);
+      const double dm = rt<SEGW>(tb, RT_IVDM, i, seg), dpp = rt<SEGW>(tb, RT_IVDP, i, seg);
+      const double s2v = rt<SEGW>(tb, RT_IVS2, i, seg);
+      const double dg = 8.0 * M_PI * f[i] * rt<SEGW>(tb, RT_ROW, i, seg) - T2 * (dm + dpp);
       Asub[i] = (T2 * dm - T1 * s2v) * sc;
       Bdia[i] = dg * sc;
       Csup[i] = (T2 * dpp + T1 * s2v) * sc;
@@ -1484,8 +1544,15 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
     OSHUN_RMOB(4)
     OSHUN_RMOB(8)
 #undef OSHUN_RMOB
+    if (SEGW == 32) {  // the upper row continues from the lower row's product
+      Mob q;
+      q.a = xor16(row_bcast<15>(P.a)); q.b = xor16(row_bcast<15>(P.b));
+      q.c = xor16(row_bcast<15>(P.c)); q.d = xor16(row_bcast<15>(P.d));
+      const Mob Pq = mob_norm(mob_mul(P, q));
+      if (upper_row()) P = Pq;
+    }
     // c' entering the lane from the exclusive prefix, then the reference's recurrence
-    double cprev = dpp_mov<row_shr<1>()>(P.b, 0.0) * rcp_nr(dpp_mov<row_shr<1>()>(P.d, 1.0));
+    double cprev = seg_prev<SEGW>(P.b, 0.0) * rcp_nr(seg_prev<SEGW>(P.d, 1.0));
     double w[NPL];  // c'_i = Csup_i w_i is recomputed where needed
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
@@ -1520,7 +1587,12 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
       OSHUN_RAFF(4)
       OSHUN_RAFF(8)
 #undef OSHUN_RAFF
-      double yv = dpp_mov<row_shr<1>()>(be, 0.0);
+      if (SEGW == 32) {
+        const double qa = xor16(row_bcast<15>(al)), qb = xor16(row_bcast<15>(be));
+        const double be2 = al * qb + be, al2 = al * qa;
+        if (upper_row()) { be = be2; al = al2; }
+      }
+      double yv = seg_prev<SEGW>(be, 0.0);
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { yv = (d[i] - yv * Asub[i]) * w[i]; d[i] = yv; }
       // back substitution x_i = d'_i - c'_i x_{i+1}: affine map per lane, reverse scan
@@ -1542,7 +1614,12 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
       OSHUN_RAFB(4)
       OSHUN_RAFB(8)
 #undef OSHUN_RAFB
-      double xv = dpp_mov<row_shl<1>()>(de, 0.0);
+      if (SEGW == 32) {
+        const double qa = xor16(row_bcast<0>(ga)), qb = xor16(row_bcast<0>(de));
+        const double de2 = ga * qb + de, ga2 = ga * qa;
+        if (!upper_row()) { de = de2; ga = ga2; }
+      }
+      double xv = seg_next<SEGW>(de, 0.0);
 #pragma unroll
       for (int i = NPL - 1; i >= 0; --i) { xv = d[i] - (Csup[i] * w[i]) * xv; d[i] = xv; }
 #pragma unroll

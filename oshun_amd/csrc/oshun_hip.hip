@@ -49,7 +49,8 @@ struct oshun_ctx {
   double* d_mom = nullptr;  // moments output, 7 planes of nxy
   bool use1d = false;       // specialised 1D m0 = 0 Vlasov kernel
   double* d_coll = nullptr;
-  double* d_rt = nullptr;   // collide_rows tables (interleaved for rows_npl), null if that kernel is not used
+  double* d_rt = nullptr;   // collide_rows tables of the e-e launch (interleaved for rows_npl/rows_segw)
+  double* d_rt2 = nullptr;  // of the implicit l >= 1 launch (rows_npl2/rows_segw2)
   // FP32 split: plane 0 stores f00 - bg(p), bg an FP64 background per p
   // (the x,y mean of f00 at upload), so that FP32 rounding acts on the
   // deviation only; d_bgG holds G00(bg) for the Vlasov kernels.
@@ -65,6 +66,8 @@ struct oshun_ctx {
   double* f1y = nullptr;       // f10(p1) of the second stage input, per x
   std::vector<double> bg;
   int rows_npl = 0;
+  int rows_segw = 16;       // lanes per column of collide_rows (16 or 32)
+  int rows_npl2 = 0, rows_segw2 = 16;
   oshun::CollTables ct;
   hipStream_t stream = nullptr;
   double t_rk = 0.0, t_ee = 0.0, t_eb = 0.0;
@@ -381,15 +384,11 @@ int launch_collide_npl(oshun_ctx* c, oshun::CollArgs<T>& a) {
   return OSHUN_OK;
 }
 
-template <typename T, int NPL>
+template <typename T, int NPL, int SEGW, int MODE>
 int launch_collide_rows(oshun_ctx* c, oshun::CollArgs<T>& a) {
-  ProfRec r;
-  prof_begin(c, 2, &r);
-  const int ntx = (c->nx + oshun::kRowCols - 1) / oshun::kRowCols;
-  if (a.do_ee) hipLaunchKernelGGL((oshun::collide_rows<T, NPL, 1>), dim3(ntx * c->ny), dim3(256), 0, c->stream, a);
-  if (a.do_f1 || a.do_flm)
-    hipLaunchKernelGGL((oshun::collide_rows<T, NPL, 2>), dim3(ntx * c->ny), dim3(256), 0, c->stream, a);
-  prof_end(c, &r);
+  const int cols = 256 / SEGW;
+  const int ntx = (c->nx + cols - 1) / cols;
+  hipLaunchKernelGGL((oshun::collide_rows<T, NPL, SEGW, MODE>), dim3(ntx * c->ny), dim3(256), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
@@ -409,11 +408,30 @@ int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_e
   a.rt = c->d_rt;
   a.bg = c->d_bg;
   a.r01 = c->T.v0sq / c->T.v1sq; a.inv_gden = 1.0 / c->T.g_den; a.c2v0 = 2.0 / c->T.v0sq;
-  switch (c->rows_npl) {
-    case 4: return launch_collide_rows<T, 4>(c, a);
-    case 8: return launch_collide_rows<T, 8>(c, a);
-    case 16: return launch_collide_rows<T, 16>(c, a);
-    default: break;
+  if (c->rows_npl) {
+    ProfRec r;
+    prof_begin(c, 2, &r);
+    int rc = OSHUN_OK;
+    if (a.do_ee) {
+      a.rt = c->d_rt;
+      switch (c->rows_segw * 100 + c->rows_npl) {
+        case 1604: rc = launch_collide_rows<T, 4, 16, 1>(c, a); break;
+        case 1608: rc = launch_collide_rows<T, 8, 16, 1>(c, a); break;
+        case 1616: rc = launch_collide_rows<T, 16, 16, 1>(c, a); break;
+        default: rc = launch_collide_rows<T, 8, 32, 1>(c, a); break;
+      }
+    }
+    if (rc == OSHUN_OK && (a.do_f1 || a.do_flm)) {
+      a.rt = c->d_rt2;
+      switch (c->rows_segw2 * 100 + c->rows_npl2) {
+        case 1604: rc = launch_collide_rows<T, 4, 16, 2>(c, a); break;
+        case 1608: rc = launch_collide_rows<T, 8, 16, 2>(c, a); break;
+        case 1616: rc = launch_collide_rows<T, 16, 16, 2>(c, a); break;
+        default: rc = launch_collide_rows<T, 8, 32, 2>(c, a); break;
+      }
+    }
+    prof_end(c, &r);
+    return rc;
   }
   if (c->np <= 64) return launch_collide_npl<T, 1>(c, a);
   if (c->np <= 128) return launch_collide_npl<T, 2>(c, a);
@@ -564,7 +582,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_bg, c->d_bgG, c->jp0, c->jp1, c->jp2, c->f1y};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp0, c->jp1, c->jp2, c->f1y};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -722,17 +740,34 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     c->ct.vc = b + 9 * n; c->ct.ivdm = b + 10 * n; c->ct.ivdp = b + 11 * n; c->ct.ivs2 = b + 12 * n;
     c->ct.pw3 = b + 13 * n; c->ct.pw5 = b + 14 * n; c->ct.pw7 = b + 15 * n;
   }
-  // collide_rows: 16 lanes x NPL rows per column; needs NPL >= NB (the
-  // parabolic low-p rows stay in lane 0) and 16*NPL >= np
-  for (int npl : {4, 8, 16})
-    if (16 * npl >= c->np && npl >= c->ph.nb_algorithms && c->np >= 3) { c->rows_npl = npl; break; }
-  if (c->rows_npl) {
+  // collide_rows: SEGW lanes x NPL rows per column; needs NPL >= NB (the
+  // parabolic low-p rows stay in lane 0) and SEGW*NPL >= np.  Above np = 128
+  // both launches use 32 lanes (two DPP rows) per column: at Nx = 4096 that
+  // keeps two wavefronts per SIMD in flight instead of one.  The two launches
+  // keep separate tables so that their shapes can differ.
+  {
+    const int nb = c->ph.nb_algorithms;
+    auto pick = [&](const int (*cand)[2], int n, int* segw, int* npl) {
+      for (int k = 0; k < n; ++k)
+        if (cand[k][0] * cand[k][1] >= c->np && cand[k][1] >= nb && c->np >= 3) {
+          *segw = cand[k][0];
+          *npl = cand[k][1];
+          return;
+        }
+    };
+    const int ee[3][2] = {{16, 4}, {16, 8}, {32, 8}};
+    const int im[3][2] = {{16, 4}, {16, 8}, {32, 8}};
+    pick(ee, 3, &c->rows_segw, &c->rows_npl);
+    pick(im, 3, &c->rows_segw2, &c->rows_npl2);
+    if (!c->rows_npl || !c->rows_npl2) c->rows_npl = c->rows_npl2 = 0;  // both or neither
+  }
+  auto build_rt = [&](int segw, int npl, std::vector<double>& rt) {
     const HostTables& T = c->T;
-    const int np = c->np, npl = c->rows_npl, nb = std::min(c->ph.nb_algorithms, np);
+    const int np = c->np, nb = std::min(c->ph.nb_algorithms, np);
     const double* v = T.vc.data();
-    std::vector<double> rt((size_t)oshun::RT_COUNT * oshun::kRowP, 0.0);
-    auto put = [&](int k, int p, double val) { rt[(size_t)k * oshun::kRowP + (p % npl) * 16 + p / npl] = val; };
-    for (int p = 0; p < 16 * npl; ++p) {
+    rt.assign((size_t)oshun::RT_COUNT * oshun::kRowP, 0.0);
+    auto put = [&](int k, int p, double val) { rt[(size_t)k * oshun::kRowP + (p % npl) * segw + p / npl] = val; };
+    for (int p = 0; p < segw * npl; ++p) {
       const bool in = p < np, mid = p >= 1 && p < np - 1;
       put(oshun::RT_U4, p, in ? T.U4[p] : 0.0);  // U4[0] = 0: the sums start at p = 1
       put(oshun::RT_U4M1, p, in ? T.U4m1[p] : 0.0);
@@ -759,7 +794,13 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
         put(oshun::RT_B4, p, T.pw7[p] / (v1sq * 7.0));
       }
     }
+  };
+  if (c->rows_npl) {
+    std::vector<double> rt;
+    build_rt(c->rows_segw, c->rows_npl, rt);
     if (upload_vec(c, &c->d_rt, rt)) return bail(OSHUN_ERR_HIP);
+    build_rt(c->rows_segw2, c->rows_npl2, rt);
+    if (upload_vec(c, &c->d_rt2, rt)) return bail(OSHUN_ERR_HIP);
   }
   *out = c;
   return OSHUN_OK;
