@@ -11,9 +11,12 @@ before timing.
 value      = N_h*Np*Nx / t_step for the plain FP64 path (dtype f64)
 fp32_split = the same for FP32 f storage + FP32 Vlasov RHS with FP64 fields and
              collisions
-roofline   = dominant kernel (vlasov_stage): algorithmic bytes per launch
-             (16 S_f over the 4 stage launches, S_f = stored f bytes) / mean
-             launch duration from HIP events on the library's stream
+roofline   = dominant kernel (vlasov_pair_1d, two RK4 stages per launch):
+             algorithmic bytes per launch (16 S_f per step shared by the
+             step's Vlasov launches, S_f = stored f bytes, i.e. 8 S_f per
+             pair launch) / mean launch duration from HIP events on the
+             library's stream; traffic = mean HBM bytes per pair launch
+             measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
 cpu_baseline = the reference (oracle/_ref, built from the reference sources)
              on a bounded sample of the same workload, 1 core, or the oracle's
              CPU restatement when that driver is absent
@@ -164,31 +167,35 @@ def main():
     ctx32, ms32, prof32 = run_path(oshun_amd.FP32_SPLIT, args.steps, args.warmup, f, fld, dx)
     S32 = ctx32.state_bytes()
 
-    def roof(prof, S):
+    def roof(prof, S, steps):
+        # the Vlasov launches of one RK4 step (two fused stage pairs on the 1D
+        # path) share the 16 S_f algorithmic bytes of the step
         ms, n = prof["vlasov_stage"]
         avg_s = ms / max(n, 1) * 1e-3
-        bytes_per_launch = 16.0 * S / 4.0
+        per_step = n / steps
+        bytes_per_launch = 16.0 * S / per_step
         ach = bytes_per_launch / avg_s
         return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": None, "kernel": "vlasov_stage",
-                "kernel_avg_us": avg_s * 1e6, "bytes_per_launch": bytes_per_launch}
+                "frac": ach / HBM_PEAK, "traffic": None, "kernel": "vlasov_pair_1d",
+                "kernel_avg_us": avg_s * 1e6, "bytes_per_launch": bytes_per_launch,
+                "launches_per_step": per_step}
 
     def breakdown(prof, steps):
         return {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps}
                 for k, v in prof.items() if v[1]}
 
-    r64 = roof(prof64, S64)
+    r64 = roof(prof64, S64, args.steps)
     # HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE) on
     # this workload, calibrated on a plain copy (profiles/traffic_*.json)
     tpath = os.path.join(ROOT, "profiles", "traffic_l8_np256_nx4096_fp64.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             tr = json.load(fh)
-        for name, v in tr.get("kernels", {}).items():
-            if name.startswith("oshun::vlasov_stage"):
-                r64["traffic"] = v["read_bytes"] + v["write_bytes"]
-                r64["traffic_source"] = os.path.relpath(tpath, ROOT)
-    r32 = roof(prof32, S32)
+        pairs = [v for name, v in tr.get("kernels", {}).items() if name.startswith("oshun::vlasov_pair_1d")]
+        if pairs:  # mean HBM bytes per launch over the two pair kernels
+            r64["traffic"] = sum(v["read_bytes"] + v["write_bytes"] for v in pairs) / len(pairs)
+            r64["traffic_source"] = os.path.relpath(tpath, ROOT)
+    r32 = roof(prof32, S32, args.steps)
     step_frac64 = 16.0 * S64 / (ms64 * 1e-3) / HBM_PEAK
     step_frac32 = 16.0 * S32 / (ms32 * 1e-3) / HBM_PEAK
     cpu = None if args.no_cpu else cpu_baseline()

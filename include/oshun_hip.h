@@ -130,6 +130,29 @@ int oshun_profile_read(oshun_ctx* ctx, double* ms_total, long long* launches, in
  * counters (FETCH_SIZE / WRITE_SIZE) against a kernel of known traffic. */
 int oshun_diag_copy(oshun_ctx* ctx, long long nbytes);
 
+/* ---- Batched independent runs ------------------------------------------
+ * Many 1D runs (m0 = 0, ny = 1, l0 <= 8, one precision for all) of uneven
+ * Nx, Np and l0 advanced together: every step is one hipGraph launch whose
+ * kernels each cover the tiles of all runs.  No counterpart in the
+ * reference, which runs one deck per process; each run here behaves as
+ * main.cpp:79-107 (Runge_Kutta_4::advance_p1, Explicit_EC_FP::loop,
+ * Euler_Backward::advance_1/advance_lm) with its own step h[i].
+ * oshun_batch_run returns the run's context for upload/download/moments;
+ * its device buffers are owned by the batch. */
+typedef struct oshun_batch oshun_batch;
+int oshun_batch_create(const oshun_grid* grids, const oshun_phys* phys, const double* h, int nruns,
+                       oshun_batch** out);
+int oshun_batch_destroy(oshun_batch* batch);
+const char* oshun_batch_last_error(const oshun_batch* batch);
+int oshun_batch_size(const oshun_batch* batch);
+oshun_ctx* oshun_batch_run(oshun_batch* batch, int i);
+int oshun_batch_step(oshun_batch* batch, int nsteps);
+int oshun_batch_synchronize(oshun_batch* batch);
+/* Times `steps` steps with each candidate tile height of the stage-pair
+ * kernels on the current state (restored afterwards) and keeps the fastest;
+ * reports its p rows per tile and ms per step. */
+int oshun_batch_autotune(oshun_batch* batch, int steps, int* chosen_rows, double* ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Grid", "Phys", "Context", "Runge_Kutta_4", "Explicit_EC_FP", "Euler_Backward", "num_harmonics",
+__all__ = ["Grid", "Phys", "Context", "Batch", "Runge_Kutta_4", "Explicit_EC_FP", "Euler_Backward", "num_harmonics",
            "hind", "deck_pmin", "FP64", "FP32_SPLIT", "COLL_EE", "COLL_FLM", "library_path", "OshunError"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -109,6 +109,16 @@ def _load():
     lib.oshun_profile_read.argtypes = [P, dp, ctypes.POINTER(ctypes.c_longlong), I]
     lib.oshun_diag_copy.argtypes = [P, ctypes.c_longlong]
     lib.oshun_moments.argtypes = [P, dp]
+    lib.oshun_batch_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Phys), dp, I, ctypes.POINTER(P)]
+    lib.oshun_batch_destroy.argtypes = [P]
+    lib.oshun_batch_last_error.argtypes = [P]
+    lib.oshun_batch_last_error.restype = ctypes.c_char_p
+    lib.oshun_batch_size.argtypes = [P]
+    lib.oshun_batch_run.argtypes = [P, I]
+    lib.oshun_batch_run.restype = P
+    lib.oshun_batch_step.argtypes = [P, I]
+    lib.oshun_batch_synchronize.argtypes = [P]
+    lib.oshun_batch_autotune.argtypes = [P, I, ctypes.POINTER(I), dp]
     _lib = lib
     return lib
 
@@ -134,9 +144,24 @@ class Context:
             raise OshunError(f"oshun_create failed ({rc}) for {l0=} {m0=} {nump=} {numx=} {numy=}")
         self._h = h
         self._lib = lib
+        self._owned = True
         self.l0, self.m0, self.np, self.nx, self.ny = l0, m0, nump, numx, numy
         self.nh = num_harmonics(l0, m0)
         self.precision = precision
+
+    @classmethod
+    def _view(cls, lib, handle, grid, phys, owner):
+        """A non-owning view of a context that another object (a Batch) owns."""
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p(handle)
+        self._lib = lib
+        self._owned = False
+        self._owner = owner
+        self.grid, self.phys = grid, phys
+        self.l0, self.m0, self.np, self.nx, self.ny = grid.l0, grid.m0, grid.nump, grid.numx, grid.numy
+        self.nh = num_harmonics(grid.l0, grid.m0)
+        self.precision = grid.precision
+        return self
 
     def _chk(self, rc):
         if rc != 0:
@@ -145,7 +170,8 @@ class Context:
 
     def close(self):
         if getattr(self, "_h", None):
-            self._lib.oshun_destroy(self._h)
+            if getattr(self, "_owned", True):
+                self._lib.oshun_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -292,3 +318,76 @@ class Euler_Backward:
 
     def advance_lm(self, tnew):
         self.ctx.collide_flm(tnew, 2)
+
+
+class Batch:
+    """Independent 1D runs (m0 = 0) of uneven shape advanced together, one
+    hipGraph launch per step (oshun_batch_* in include/oshun_hip.h).
+
+    ``runs`` is a list of keyword dicts for Context (l0, nump, numx, pmax, dx,
+    precision, density_np, small_dt, nb_algorithms, collisions, ...); ``h`` the
+    RK4 step of each run.  ``batch.run(i)`` is a Context view of run i for
+    upload / download / moments; ``batch.step(n)`` advances every run n steps.
+    """
+
+    def __init__(self, runs, h):
+        lib = _load()
+        n = len(runs)
+        grids = (Grid * n)()
+        phys = (Phys * n)()
+        for i, r in enumerate(runs):
+            nump = r["nump"]
+            pmax = r.get("pmax", 0.3)
+            pmin = r.get("pmin", deck_pmin(pmax, nump))
+            grids[i] = Grid(r["l0"], r.get("m0", 0), nump, r["numx"], 1, pmin, pmax, r.get("dx", 1.0), 1.0,
+                            r.get("precision", FP64))
+            phys[i] = Phys(r.get("density_np", 1e21), r.get("zeta", 1.0), r.get("small_dt", 1.0),
+                           r.get("nb_algorithms", 4), r.get("collisions", COLL_EE | COLL_FLM),
+                           r.get("ref_quirks", 1))
+        hs = np.ascontiguousarray(h, np.float64)
+        hb = ctypes.c_void_p()
+        rc = lib.oshun_batch_create(grids, phys, _dptr(hs), n, ctypes.byref(hb))
+        if rc != 0:
+            raise OshunError(f"oshun_batch_create failed ({rc})")
+        self._h = hb
+        self._lib = lib
+        self.grids, self.phys = grids, phys
+        self.h = hs
+        self._runs = [Context._view(lib, lib.oshun_batch_run(hb, i), grids[i], phys[i], self) for i in range(n)]
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self._lib.oshun_batch_last_error(self._h)
+            raise OshunError(f"oshun batch call failed ({rc}): {msg.decode() if msg else ''}")
+
+    def __len__(self):
+        return len(self._runs)
+
+    def run(self, i):
+        return self._runs[i]
+
+    def step(self, nsteps=1):
+        self._chk(self._lib.oshun_batch_step(self._h, int(nsteps)))
+
+    def synchronize(self):
+        self._chk(self._lib.oshun_batch_synchronize(self._h))
+
+    def autotune(self, steps=3):
+        """Times each stage-pair tile height on the batch (state restored); returns (p rows, ms/step)."""
+        rows = ctypes.c_int()
+        ms = ctypes.c_double()
+        self._chk(self._lib.oshun_batch_autotune(self._h, int(steps), ctypes.byref(rows), ctypes.byref(ms)))
+        return rows.value, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for r in self._runs:
+                r._h = None
+            self._lib.oshun_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

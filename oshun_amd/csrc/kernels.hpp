@@ -345,6 +345,19 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblocks) {
   return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
 }
 
+// Batched launches: the workgroups of one launch cover the tiles of many
+// independent runs.  ts[r] is the first global tile of run r (ts[nruns] =
+// total); a workgroup finds its run by binary search and reads that run's
+// arguments from device memory.
+__device__ __forceinline__ int batch_run(const int* __restrict__ ts, int nruns, int g) {
+  int lo = 0, hi = nruns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ts[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // ---------------------------------------------------------------------------
 // Two RK4 stages in one pass (1D, m0 = 0).  A workgroup owns kPairX x columns
 // and TP = 4R-2 p rows.  Stage one is evaluated on the tile plus one column
@@ -383,7 +396,7 @@ __device__ __forceinline__ void bstore<float>(float v, __amdgpu_buffer_rsrc_t r,
 }
 
 template <typename T, int L0, int R, int PAIR>
-__global__ __launch_bounds__(256) void vlasov_pair_1d(Stage1DArgs<T> a) {
+__device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int tile) {
   constexpr int NL = L0 + 1;
   constexpr int TP = 4 * R - 2;  // interior rows
   constexpr int S1 = TP + 2;     // rows of the first stage
@@ -396,7 +409,6 @@ __global__ __launch_bounds__(256) void vlasov_pair_1d(Stage1DArgs<T> a) {
   // p-tiles of one x strip are consecutive and each XCD gets a contiguous run
   // of tiles, so the halo rows shared by neighbouring p-tiles are L2 hits
   const int npt = (np + TP - 1) / TP;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int x0 = (tile / npt) * kPairX;
   const int pt = tile % npt;
   const int p0 = pt * TP;
@@ -512,28 +524,60 @@ __global__ __launch_bounds__(256) void vlasov_pair_1d(Stage1DArgs<T> a) {
   }
 }
 
+template <typename T, int L0, int R, int PAIR>
+__global__ __launch_bounds__(256) void vlasov_pair_1d(Stage1DArgs<T> a) {
+  pair_body<T, L0, R, PAIR>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+template <typename T, int L0, int R, int PAIR>
+__global__ __launch_bounds__(256) void vlasov_pair_1d_batch(const Stage1DArgs<T>* __restrict__ args,
+                                                            const int* __restrict__ ts, int nruns) {
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
+  const int r = batch_run(ts, nruns, g);
+  pair_body<T, L0, R, PAIR>(args[r], g - ts[r]);
+}
+
 // Current::Jxyz integrand of f10 summed over 64-row blocks of p: [pb][3][nx], component 0
 template <typename T>
-__global__ __launch_bounds__(256) void current_1d(const T* __restrict__ f10, const double* __restrict__ p3og,
-                                                  double* __restrict__ jp, int np, int nx) {
+struct CurArgs {
+  const T* f10;
+  const double* p3og;
+  double* jp;
+  int np, nx;
+};
+template <typename T>
+__device__ __forceinline__ void current_body(const CurArgs<T>& a, const int tile) {
   __shared__ double sh[4][64];
   const int lane = threadIdx.x, ty = threadIdx.y;
-  const int x = blockIdx.x * 64 + lane;
+  const int nbx = (a.nx + 63) / 64;
+  const int bx = tile % nbx, by = tile / nbx;
+  const int np = a.np, nx = a.nx;
+  const int x = bx * 64 + lane;
   double s = 0.0;
   if (x < nx) {
 #pragma unroll 4
     for (int i = 0; i < 16; ++i) {
-      const int p = blockIdx.y * 64 + ty + 4 * i;
+      const int p = by * 64 + ty + 4 * i;
       if (p < np) {
         const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
-        s += w * (static_cast<double>(f10[(long long)p * nx + x]) * p3og[p]);
+        s += w * (static_cast<double>(a.f10[(long long)p * nx + x]) * a.p3og[p]);
       }
     }
   }
   sh[ty][lane] = s;
   __syncthreads();
   if (ty == 0 && x < nx)
-    jp[((long long)blockIdx.y * 3) * nx + x] = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+    a.jp[((long long)by * 3) * nx + x] = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+}
+template <typename T>
+__global__ __launch_bounds__(256) void current_1d(CurArgs<T> a) {
+  current_body<T>(a, blockIdx.x);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void current_1d_batch(const CurArgs<T>* __restrict__ args,
+                                                        const int* __restrict__ ts, int nruns) {
+  const int g = blockIdx.x;
+  const int r = batch_run(ts, nruns, g);
+  current_body<T>(args[r], g - ts[r]);
 }
 
 template <typename T>
@@ -593,8 +637,8 @@ __device__ __forceinline__ void sum_parts(const double* jpart, int npb, int nxy,
 }
 
 template <typename T>
-__global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage(FieldArgs<T> a) {
-  const int i = blockIdx.x * kFieldX + threadIdx.x;
+__device__ __forceinline__ void field_body(const FieldArgs<T>& a, const int block) {
+  const int i = block * kFieldX + threadIdx.x;
   const int row = threadIdx.y;
   const int nx = a.nx, ny = a.ny, nxy = nx * ny;
   __shared__ double sh[kFieldRows][kFieldX];
@@ -654,6 +698,18 @@ __global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage(FieldArgs<T>
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage(FieldArgs<T> a) {
+  field_body<T>(a, blockIdx.x);
+}
+template <typename T>
+__global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage_batch(const FieldArgs<T>* __restrict__ args,
+                                                                         const int* __restrict__ ts, int nruns) {
+  const int g = blockIdx.x;
+  const int r = batch_run(ts, nruns, g);
+  field_body<T>(args[r], g - ts[r]);
+}
+
 // ------------------------------------------------------------------ collide
 struct CollTables {
   const double *U4, *U4m1, *U2, *U2m1, *U1s, *U1m1s, *U3, *Qn, *Pn, *vc, *ivdm, *ivdp, *ivs2, *pw3, *pw5, *pw7;
@@ -671,6 +727,8 @@ struct CollArgs {
   double c_kpre, kpre, density_np, zeta, v0sq, v1sq, g_den;
   const double* rt;             // collide_rows tables, interleaved for its NPL
   const double* bg;             // f00 background per p (plane 0 stores f00 minus it), or null
+  const double* tclk;           // batched runs: the run's clock before the step (else null)
+  double hstep, small_dt;       // batched runs: RK4 step and e-e sub-step target
   double r01, inv_gden, c2v0;   // v0^2/v1^2, 1/g_den, 2/v0^2
 };
 
@@ -1412,14 +1470,13 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 // MODE 1: e-e subcycles on f_00 (Explicit_EC_FP::loop); MODE 2: implicit
 // l >= 1 step (Euler_Backward::advance_1/advance_lm).  Separate launches.
 template <typename T, int NPL, int SEGW, int MODE>
-__global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
+__device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int tile) {
   constexpr int kCols = 256 / SEGW;  // columns per workgroup
   __shared__ double tb[RT_COUNT * kRowP];
   for (int i = threadIdx.x; i < RT_COUNT * kRowP; i += blockDim.x) tb[i] = a.rt[i];
   __syncthreads();
   const int nx = a.nx, np = a.np;
   const int ntx = (nx + kCols - 1) / kCols;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int y = tile / ntx;
   const int xc = (tile % ntx) * kCols + threadIdx.x / SEGW;
   const bool colv = xc < nx;
@@ -1427,6 +1484,14 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
   const int seg = threadIdx.x & (SEGW - 1);
   const int p0 = seg * NPL;
   const long long base = (long long)y * np * nx + x;
+  int numh = a.numh;
+  double h_ee = a.h_ee, dt_flm = a.dt_flm;
+  if (a.tclk) {  // batched runs: the sub-steps of Explicit_EC_FP::loop from the run's clock (:597-599)
+    const double t = *a.tclk, tnew = t + a.hstep;
+    numh = static_cast<int>((tnew - t) / a.small_dt) + 1;
+    h_ee = (tnew - t) / static_cast<double>(numh);
+    dt_flm = tnew - t;
+  }
   double f[NPL];
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
@@ -1435,8 +1500,8 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
   }
   if (MODE == 1) {
     // RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553, numh times (:590-625)
-    const double h = a.h_ee;
-    for (int s = 0; s < a.numh; ++s) {
+    const double h = h_ee;
+    for (int s = 0; s < numh; ++s) {
       // f: running sum, f0: start of the subcycle, fs: stage input, overwritten by its slope
       double f0[NPL], fs[NPL];
 #pragma unroll
@@ -1489,8 +1554,8 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
     const double I0_density = seg_last<SEGW>(inc2) * (4.0 * M_PI);
     const double lnee = log_ee(I0_density, I2_temperature, a.density_np);
     const double zi = zlog_ei(I0_density, I2_temperature, a.density_np, a.zeta) * I0_density;
-    const double kd = a.kpre * a.dt_flm;
-    const double sc = (-1.0) * lnee * a.kpre * a.dt_flm;
+    const double kd = a.kpre * dt_flm;
+    const double sc = (-1.0) * lnee * a.kpre * dt_flm;
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
       double i2 = (I2[i] + ex4) * (4.0 * M_PI);
@@ -1631,6 +1696,24 @@ __global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
   }
 }
 
+template <typename T, int NPL, int SEGW, int MODE>
+__global__ __launch_bounds__(64 * 4) void collide_rows(CollArgs<T> a) {
+  collide_body<T, NPL, SEGW, MODE>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+template <typename T, int NPL, int SEGW, int MODE>
+__global__ __launch_bounds__(64 * 4) void collide_rows_batch(const CollArgs<T>* __restrict__ args,
+                                                             const int* __restrict__ ts, int nruns) {
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
+  const int r = batch_run(ts, nruns, g);
+  collide_body<T, NPL, SEGW, MODE>(args[r], g - ts[r]);
+}
+
+// batched runs: every run's clock advances by its step after the collisions
+__global__ __launch_bounds__(256) void batch_clock(double* __restrict__ t, const double* __restrict__ h, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) t[i] += h[i];
+}
+
 // Nonrelativistic moments, Parallel_Output::nonrelativistic_moments
 // (def-Parallel.cpp:1677-2134) in double: one thread per (x,y), trapezoid
 // sums along p with coalesced loads along x.  out[k][y][x], k = density, Vx,
@@ -1714,8 +1797,8 @@ __global__ __launch_bounds__(256) void moments_kernel(MomArgs<T> a) {
   for (int k = 0; k < 7; ++k) a.out[(long long)k * nxy + i] = vals[k];
 }
 
-// plain copy, 16 B per lane per iteration (counter calibration)
-__global__ __launch_bounds__(256) void diag_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+// plain copy, 8 B per lane per iteration like the Vlasov and collision kernels (counter calibration)
+__global__ __launch_bounds__(256) void diag_copy(const double* __restrict__ src, double* __restrict__ dst,
                                                  long long n) {
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) dst[i] = src[i];
 }

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -250,8 +251,9 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
 }
 
 template <typename T>
-int launch_field(oshun_ctx* c, int mode, const double* fin, const double* f0, double* fout, double* facc,
-                 const void* fpl, double h, const double* jp = nullptr, int npb = 0, const double* f1p1 = nullptr) {
+oshun::FieldArgs<T> field_args(oshun_ctx* c, int mode, const double* fin, const double* f0, double* fout,
+                               double* facc, const void* fpl, double h, const double* jp, int npb,
+                               const double* f1p1) {
   oshun::FieldArgs<T> a;
   a.fin = fin; a.f0 = f0; a.fout = fout; a.facc = facc;
   a.jpart = jp ? jp : c->jpart;
@@ -267,6 +269,13 @@ int launch_field(oshun_ctx* c, int mode, const double* fin, const double* f0, do
   a.delta_p = c->T.delta_p;
   a.midx = c->T.midx; a.midy = c->T.midy;
   a.c05h = 0.5 * h; a.hh = h; a.h6 = h / 6.0; a.third = 1.0 / 3.0; a.twothird = 2.0 / 3.0;
+  return a;
+}
+
+template <typename T>
+int launch_field(oshun_ctx* c, int mode, const double* fin, const double* f0, double* fout, double* facc,
+                 const void* fpl, double h, const double* jp = nullptr, int npb = 0, const double* f1p1 = nullptr) {
+  const oshun::FieldArgs<T> a = field_args<T>(c, mode, fin, f0, fout, facc, fpl, h, jp, npb, f1p1);
   dim3 grid((c->nxy + oshun::kFieldX - 1) / oshun::kFieldX);
   ProfRec r;
   prof_begin(c, 1, &r);
@@ -298,6 +307,7 @@ int rk4(oshun_ctx* c, double h) {
 
 constexpr int kPairR = 3;  // rows per thread of the pair kernels: 4*3-2 = 10 p rows per tile
 constexpr int kPairTP = 4 * kPairR - 2;
+constexpr int kPairTPMin = 4 * 2 - 2;  // smallest tile height the batched path can choose
 
 template <typename T, int L0>
 void launch_pair_l0(oshun_ctx* c, int pair, const oshun::Stage1DArgs<T>& a, dim3 grid) {
@@ -309,14 +319,21 @@ void launch_pair_l0(oshun_ctx* c, int pair, const oshun::Stage1DArgs<T>& a, dim3
 
 // RK4 stages 1+2 (pair 0) or 3+4 (pair 1) of the 1D m0 = 0 system
 template <typename T>
-int launch_pair(oshun_ctx* c, int pair, const void* in, const void* y0, void* out, void* acc, const double* fin,
-                const double* fin2, double* jpA, double* jpB, double h) {
+oshun::Stage1DArgs<T> pair_args(oshun_ctx* c, const void* in, const void* y0, void* out, void* acc,
+                                const double* fin, const double* fin2, double* jpA, double* jpB, double h) {
   oshun::Stage1DArgs<T> a;
   fill_1d_args<T>(c, a, 0, in, y0, out, acc, fin, h);
   a.fin2 = fin2;
   a.jpA = jpA;
   a.jpB = jpB;
   a.f1A = c->f1y;
+  return a;
+}
+
+template <typename T>
+int launch_pair(oshun_ctx* c, int pair, const void* in, const void* y0, void* out, void* acc, const double* fin,
+                const double* fin2, double* jpA, double* jpB, double h) {
+  const oshun::Stage1DArgs<T> a = pair_args<T>(c, in, y0, out, acc, fin, fin2, jpA, jpB, h);
   const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
   dim3 grid(ntx * c->npt);
   ProfRec r;
@@ -352,8 +369,8 @@ int rk4_pair(oshun_ctx* c, double h) {
     ProfRec r;
     prof_begin(c, 1, &r);
     const T* f10 = static_cast<const T*>(c->S) + (size_t)c->T.comp_re[oshun::hind(1, 0, 0)] * c->plane;
-    hipLaunchKernelGGL(oshun::current_1d<T>, dim3((c->nx + 63) / 64, c->npb0), dim3(64, 4), 0, c->stream, f10,
-                       c->d_p3og, c->jp0, c->np, c->nx);
+    oshun::CurArgs<T> ca{f10, c->d_p3og, c->jp0, c->np, c->nx};
+    hipLaunchKernelGGL(oshun::current_1d<T>, dim3(((c->nx + 63) / 64) * c->npb0), dim3(64, 4), 0, c->stream, ca);
     prof_end(c, &r);
     HIPCHK(c, hipGetLastError());
   }
@@ -394,7 +411,8 @@ int launch_collide_rows(oshun_ctx* c, oshun::CollArgs<T>& a) {
 }
 
 template <typename T>
-int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_ee, double dt_flm) {
+oshun::CollArgs<T> coll_args(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_ee,
+                             double dt_flm) {
   oshun::CollArgs<T> a;
   a.f = static_cast<T*>(c->S);
   a.t = c->ct;
@@ -407,7 +425,14 @@ int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_e
   a.v0sq = c->T.v0sq; a.v1sq = c->T.v1sq; a.g_den = c->T.g_den;
   a.rt = c->d_rt;
   a.bg = c->d_bg;
+  a.tclk = nullptr; a.hstep = 0.0; a.small_dt = c->ph.small_dt;
   a.r01 = c->T.v0sq / c->T.v1sq; a.inv_gden = 1.0 / c->T.g_den; a.c2v0 = 2.0 / c->T.v0sq;
+  return a;
+}
+
+template <typename T>
+int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_ee, double dt_flm) {
+  oshun::CollArgs<T> a = coll_args<T>(c, do_ee, do_f1, do_flm, numh, h_ee, dt_flm);
   if (c->rows_npl) {
     ProfRec r;
     prof_begin(c, 2, &r);
@@ -661,11 +686,12 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
   c->use_pair = c->use1d;
   if (c->use_pair) {
     c->npt = (c->np + kPairTP - 1) / kPairTP;
+    const int npt_max = (c->np + kPairTPMin - 1) / kPairTPMin;  // partial buffers fit every tile height
     c->npb0 = (c->np + 63) / 64;
     const size_t nx = c->nx;
     if (hipMalloc((void**)&c->jp0, (size_t)c->npb0 * 3 * nx * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&c->jp1, (size_t)c->npt * 3 * nx * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&c->jp2, (size_t)c->npt * 3 * nx * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->jp1, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->jp2, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->f1y, nx * sizeof(double)) != hipSuccess)
       return bail(OSHUN_ERR_ALLOC);
   }
@@ -1007,14 +1033,17 @@ int oshun_diag_copy(oshun_ctx* c, long long nbytes) {
   if (!sane(c) || nbytes < 16) return OSHUN_ERR_ARG;
   const long long cap = (long long)c->nc * c->plane * (long long)c->esz;
   if (nbytes > cap) return fail(c, OSHUN_ERR_ARG, "diag copy larger than the state buffer");
-  const long long n = nbytes / 16;
+  const long long n = nbytes / 8;
   ProfRec r;
   prof_begin(c, 3, &r);
-  hipLaunchKernelGGL(oshun::diag_copy, dim3(2048), dim3(256), 0, c->stream, static_cast<const double2*>(c->B1),
-                     static_cast<double2*>(c->B2), n);
+  hipLaunchKernelGGL(oshun::diag_copy, dim3(2048), dim3(256), 0, c->stream, static_cast<const double*>(c->B1),
+                     static_cast<double*>(c->B2), n);
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
 
 }  // extern "C"
+
+// batched runs (oshun_batch_*)
+#include "batch.hpp"
