@@ -134,6 +134,73 @@ def cpu_baseline(budget_s=20.0):
             "sample": f"oracle CPU restatement, l0=8 m0=0 Np=256 Nx=64, {n} steps", "s_per_step": per_step}
 
 
+def ensemble_specs(nruns=512, seed=1234):
+    """The batched-ensemble workload: per run, drawn in this order from
+    numpy.random.default_rng(seed): Nx ~ U{32..1024}, Np ~ U{64..256},
+    l0 ~ U{1..6}, T0 ~ log-uniform [0.1, 5] keV, perturbation amplitude
+    ~ U[0.01, 0.2].  1D periodic, m0 = 0, FP32 Vlasov RHS with FP64 fields and
+    collisions (e-e + implicit l >= 1 at n = 1e21 cm^-3), h = e-e sub-step =
+    0.01/omega_p, pmax = 6 p_t (at least 0.05), dx = 0.5 c/omega_p, initial
+    state a Maxwellian with T(x) = T0 (1 + a cos(2 pi x / L))."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(nruns):
+        nx = int(rng.integers(32, 1025))
+        np_ = int(rng.integers(64, 257))
+        l0 = int(rng.integers(1, 7))
+        t0 = float(np.exp(rng.uniform(np.log(0.1), np.log(5.0))))
+        amp = float(rng.uniform(0.01, 0.2))
+        specs.append((nx, np_, l0, t0, amp))
+    return specs
+
+
+def ensemble_state(nx, np_, l0, t0, amp, pmax, seed):
+    pmin = pmax / (2.0 * np_)
+    p = np.arange(np_) * ((pmax - pmin) / (np_ - 1)) + pmin
+    T = (t0 / 511.0) * (1.0 + amp * np.cos(2 * np.pi * np.arange(nx) / nx))
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f[0, 0] = np.exp(-p[None, :] ** 2 / (2 * T[:, None])) / (np.sqrt(2 * np.pi) * 2 * np.pi * T[:, None] ** 1.5)
+    rng = np.random.default_rng(seed)
+    for l in range(1, l0 + 1):
+        f[l, 0] = 1e-6 * rng.uniform(-1, 1, (nx, np_)) * f[0, 0].real
+    return f, np.zeros((6, 1, nx), np.complex128)
+
+
+def run_ensemble(steps, warmup, nruns=512):
+    import oshun_amd
+    specs = ensemble_specs(nruns)
+    runs, hs = [], []
+    for nx, np_, l0, t0, amp in specs:
+        pmax = max(0.05, 6.0 * np.sqrt(t0 / 511.0))
+        runs.append(dict(l0=l0, nump=np_, numx=nx, pmax=pmax, dx=0.5, precision=oshun_amd.FP32_SPLIT,
+                         density_np=DENSITY, small_dt=H, nb_algorithms=NB))
+        hs.append(H)
+    b = oshun_amd.Batch(runs, hs)
+    for i, (nx, np_, l0, t0, amp) in enumerate(specs):
+        b.run(i).upload(*ensemble_state(nx, np_, l0, t0, amp, runs[i]["pmax"], seed=i))
+    rows, tuned_ms = b.autotune(3)
+    b.step(warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    dt = time.perf_counter() - t0
+    for i in (0, len(specs) // 2, len(specs) - 1):
+        f, _ = b.run(i).download()
+        if not np.all(np.isfinite(f.real)):
+            raise RuntimeError(f"non-finite state in run {i}")
+    cells = sum((l0 + 1) * np_ * nx for nx, np_, l0, _, _ in specs)
+    return {"metric": "runs*steps/s (batched ensemble, one hipGraph launch per step)",
+            "value": nruns * steps / dt, "unit": "run-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": dt * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 f / f64 fields+collisions", "data": "synthetic (seeded draws, see config)",
+            "config": {"workload": f"{nruns} independent 1D runs, Nx U[32,1024], Np U[64,256], l0 U{{1..6}}, "
+                                   "T0 log-U[0.1,5] keV, cosine T perturbation U[0.01,0.2], seed 1234, "
+                                   "collisions on, h=0.01",
+                       "cells": cells, "autotuned_p_rows_per_tile": rows, "autotune_ms_per_step": tuned_ms},
+            "cell_updates_per_s": cells * steps / dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,13 +211,18 @@ def main():
                     help="profiling aid: run one path only and print its timings (no JSON contract line)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="profiling aid: also launch one plain copy of S_f bytes (counter calibration)")
+    ap.add_argument("--batch", action="store_true",
+                    help="the batched ensemble (512 independent 1D runs) instead of the single-run workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
-              "a single run shards along x with one halo exchange per step (see DESIGN.md)", file=sys.stderr)
+              "a single run shards along x with one halo exchange per step", file=sys.stderr)
         sys.exit(2)
     import oshun_amd
 
+    if args.batch:
+        print(json.dumps(run_ensemble(args.steps, args.warmup)))
+        return
     f, fld, dx = target_state()
     cells = (L0 + 1) * NP * NX
     if args.only:
