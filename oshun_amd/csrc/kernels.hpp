@@ -767,18 +767,14 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 template <int N> constexpr int row_shl() { return 0x100 + N; }
 template <int N> constexpr int row_shr() { return 0x110 + N; }
 
-// lane i takes v of lane i -+ N inside its row; lanes without a source get
-// `fill` (selected explicitly, as rocPRIM's DPP scans do)
+// lane i takes v of lane i -+ N inside its row; lanes without a source keep
+// `fill` (the DPP move's old value: with bound_ctrl off, a lane whose source is
+// outside its row is not written)
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v, double fill) {
   const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), CTRL, 0xF, 0xF, false);
   const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), CTRL, 0xF, 0xF, false);
-  const double r = __hiloint2double(hi, lo);
-  const int rl = threadIdx.x & 15;
-  constexpr bool right = CTRL >= 0x110;  // row_shr: source is lane - N
-  constexpr int n = right ? CTRL - 0x110 : CTRL - 0x100;
-  const bool has_src = right ? (rl >= n) : (rl + n <= 15);
-  return has_src ? r : fill;
+  return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double lane_read(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -1578,120 +1574,147 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     }
   }
   // Implicit_FLM_step::advance for every l >= 1 plane, def-FokkerPlanck.cpp:943-1035,
-  // TridiagonalSolve def-Nmethods.cpp:142-167 with the lane boundaries joined by scans
+  // TridiagonalSolve def-Nmethods.cpp:142-167 with the lane boundaries joined by
+  // scans.  Planes come in ind(l,m) order (l non-decreasing): the pivots are
+  // formed when l changes, and the next plane's loads are in flight while the
+  // current plane is solved.  Rows past np (and idle columns, for stores) use
+  // an offset outside the plane: the buffer access returns 0 / is dropped.
   const int lmin = a.do_f1 ? 1 : 2;
   const int lmax = a.do_flm ? a.l0 : 1;
-  for (int l = lmin; l <= lmax; ++l) {
-    double ll1 = static_cast<double>(l);
-    ll1 *= (-0.5) * (ll1 + 1.0);
-    auto bl = [&](int i) {
-      double bd = Bdia[i];
-      if (i == 0 && l > 1) bd = (seg == 0) ? 0.0 : bd;  // row 0 of Alpha_Tri dropped for l > 1 (:953-955)
-      return bd + (1.0 - ll1 * Sct[i]);
-    };
-    // pivots: c'_i = P_i/Q_i with (P,Q)_i = M_i (P,Q)_{i-1}, M_i = [[0, c_i], [-a_i, b_i]]
-    Mob P = {1.0, 0.0, 0.0, 1.0};
+  const unsigned pbytes = static_cast<unsigned>(a.plane * sizeof(T));
+  constexpr int kOut = 0x7ffffff0;
+  int lo[NPL], so[NPL];
 #pragma unroll
-    for (int i = 0; i < NPL; ++i) {
-      const Mob Mi = {0.0, Csup[i], -Asub[i], bl(i)};
-      P = mob_mul(Mi, P);
-      if ((i & 3) == 3) P = mob_norm(P);  // exact power-of-two rescaling against overflow
+  for (int i = 0; i < NPL; ++i) {
+    const int p = p0 + i;
+    lo[i] = (p < np) ? static_cast<int>((base + (long long)p * nx) * (long long)sizeof(T)) : kOut;
+    so[i] = colv ? lo[i] : kOut;
+  }
+  int c = 1;
+  while (c < a.nc && a.comp_l[c] < lmin) ++c;
+  int cend = c;
+  while (cend < a.nc && a.comp_l[cend] <= lmax) ++cend;
+  double dn[NPL], w[NPL];
+  if (c < cend) {
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)c * a.plane, pbytes);
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) dn[i] = static_cast<double>(bload<T>(r, lo[i], 0));
+  }
+  int lcur = -1;
+  for (; c < cend; ++c) {
+    double d[NPL];
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) d[i] = dn[i];
+    if (c + 1 < cend) {  // next plane
+      const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)(c + 1) * a.plane, pbytes);
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) dn[i] = static_cast<double>(bload<T>(r, lo[i], 0));
     }
+    const int l = a.comp_l[c];
+    if (l != lcur) {
+      lcur = l;
+      double ll1 = static_cast<double>(l);
+      ll1 *= (-0.5) * (ll1 + 1.0);
+      double bl[NPL];  // diagonal of this l
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        double bd = Bdia[i];
+        if (i == 0 && l > 1) bd = (seg == 0) ? 0.0 : bd;  // row 0 of Alpha_Tri dropped for l > 1 (:953-955)
+        bl[i] = bd + (1.0 - ll1 * Sct[i]);
+      }
+      // pivots: c'_i = P_i/Q_i with (P,Q)_i = M_i (P,Q)_{i-1}, M_i = [[0, c_i], [-a_i, b_i]]
+      Mob P = {1.0, 0.0, 0.0, 1.0};
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {  // P <- [[0, c], [-a, b]] P
+        const Mob Q = {Csup[i] * P.c, Csup[i] * P.d, fma(bl[i], P.c, -Asub[i] * P.a), fma(bl[i], P.d, -Asub[i] * P.b)};
+        P = Q;
+        if ((i & 3) == 3) P = mob_norm(P);  // exact power-of-two rescaling against overflow
+      }
 #define OSHUN_RMOB(N)                                                                    \
-    {                                                                                    \
-      Mob q;                                                                             \
-      q.a = dpp_mov<row_shr<N>()>(P.a, 1.0); q.b = dpp_mov<row_shr<N>()>(P.b, 0.0);      \
-      q.c = dpp_mov<row_shr<N>()>(P.c, 0.0); q.d = dpp_mov<row_shr<N>()>(P.d, 1.0);      \
-      P = mob_norm(mob_mul(P, q));                                                       \
-    }
-    OSHUN_RMOB(1)
-    OSHUN_RMOB(2)
-    OSHUN_RMOB(4)
-    OSHUN_RMOB(8)
+      {                                                                                    \
+        Mob q;                                                                             \
+        q.a = dpp_mov<row_shr<N>()>(P.a, 1.0); q.b = dpp_mov<row_shr<N>()>(P.b, 0.0);      \
+        q.c = dpp_mov<row_shr<N>()>(P.c, 0.0); q.d = dpp_mov<row_shr<N>()>(P.d, 1.0);      \
+        P = mob_mul(P, q);  /* normalised lane products: no range issue over 5 steps */    \
+      }
+      OSHUN_RMOB(1)
+      OSHUN_RMOB(2)
+      OSHUN_RMOB(4)
+      OSHUN_RMOB(8)
 #undef OSHUN_RMOB
-    if (SEGW == 32) {  // the upper row continues from the lower row's product
-      Mob q;
-      q.a = xor16(row_bcast<15>(P.a)); q.b = xor16(row_bcast<15>(P.b));
-      q.c = xor16(row_bcast<15>(P.c)); q.d = xor16(row_bcast<15>(P.d));
-      const Mob Pq = mob_norm(mob_mul(P, q));
-      if (upper_row()) P = Pq;
+      if (SEGW == 32) {  // the upper row continues from the lower row's product
+        Mob q;
+        q.a = xor16(row_bcast<15>(P.a)); q.b = xor16(row_bcast<15>(P.b));
+        q.c = xor16(row_bcast<15>(P.c)); q.d = xor16(row_bcast<15>(P.d));
+        const Mob Pq = mob_mul(P, q);
+        if (upper_row()) P = Pq;
+      }
+      // c' entering the lane from the exclusive prefix, then the reference's recurrence
+      double cprev = seg_prev<SEGW>(P.b, 0.0) * rcp_nr(seg_prev<SEGW>(P.d, 1.0));
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        w[i] = rcp_nr(bl[i] - cprev * Asub[i]);
+        cprev = Csup[i] * w[i];
+      }
     }
-    // c' entering the lane from the exclusive prefix, then the reference's recurrence
-    double cprev = seg_prev<SEGW>(P.b, 0.0) * rcp_nr(seg_prev<SEGW>(P.d, 1.0));
-    double w[NPL];  // c'_i = Csup_i w_i is recomputed where needed
+    // forward substitution d'_i = (d_i - a_i d'_{i-1}) w_i: affine map per lane, scanned
+    double al = 1.0, be = 0.0;
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
-      w[i] = rcp_nr(bl(i) - cprev * Asub[i]);
-      cprev = Csup[i] * w[i];
+      const double aw = -Asub[i] * w[i];
+      be = aw * be + w[i] * d[i];
+      al = aw * al;
     }
-    for (int c = 1; c < a.nc; ++c) {
-      if (a.comp_l[c] != l) continue;
-      T* fc = a.f + (long long)c * a.plane;
-      double d[NPL];
-#pragma unroll
-      for (int i = 0; i < NPL; ++i) {
-        const int p = p0 + i;
-        d[i] = (p < np) ? static_cast<double>(fc[base + (long long)p * nx]) : 0.0;
-      }
-      // forward substitution d'_i = (d_i - a_i d'_{i-1}) w_i: affine map per lane, scanned
-      double al = 1.0, be = 0.0;
-#pragma unroll
-      for (int i = 0; i < NPL; ++i) {
-        const double aw = -Asub[i] * w[i];
-        be = aw * be + w[i] * d[i];
-        al = aw * al;
-      }
 #define OSHUN_RAFF(N)                                                                              \
-      {                                                                                            \
-        const double qa = dpp_mov<row_shr<N>()>(al, 1.0), qb = dpp_mov<row_shr<N>()>(be, 0.0);     \
-        be = al * qb + be;                                                                         \
-        al = al * qa;                                                                              \
-      }
-      OSHUN_RAFF(1)
-      OSHUN_RAFF(2)
-      OSHUN_RAFF(4)
-      OSHUN_RAFF(8)
+    {                                                                                            \
+      const double qa = dpp_mov<row_shr<N>()>(al, 1.0), qb = dpp_mov<row_shr<N>()>(be, 0.0);     \
+      be = al * qb + be;                                                                         \
+      al = al * qa;                                                                              \
+    }
+    OSHUN_RAFF(1)
+    OSHUN_RAFF(2)
+    OSHUN_RAFF(4)
+    OSHUN_RAFF(8)
 #undef OSHUN_RAFF
-      if (SEGW == 32) {
-        const double qa = xor16(row_bcast<15>(al)), qb = xor16(row_bcast<15>(be));
-        const double be2 = al * qb + be, al2 = al * qa;
-        if (upper_row()) { be = be2; al = al2; }
-      }
-      double yv = seg_prev<SEGW>(be, 0.0);
+    if (SEGW == 32) {
+      const double qa = xor16(row_bcast<15>(al)), qb = xor16(row_bcast<15>(be));
+      const double be2 = al * qb + be, al2 = al * qa;
+      if (upper_row()) { be = be2; al = al2; }
+    }
+    double yv = seg_prev<SEGW>(be, 0.0);
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) { yv = (d[i] - yv * Asub[i]) * w[i]; d[i] = yv; }
-      // back substitution x_i = d'_i - c'_i x_{i+1}: affine map per lane, reverse scan
-      double ga = 1.0, de = 0.0;
+    for (int i = 0; i < NPL; ++i) { yv = (d[i] - yv * Asub[i]) * w[i]; d[i] = yv; }
+    // back substitution x_i = d'_i - c'_i x_{i+1}: affine map per lane, reverse scan
+    double ga = 1.0, de = 0.0;
 #pragma unroll
-      for (int i = NPL - 1; i >= 0; --i) {
-        const double cp = Csup[i] * w[i];
-        de = -cp * de + d[i];
-        ga = -cp * ga;
-      }
+    for (int i = NPL - 1; i >= 0; --i) {
+      const double cp = Csup[i] * w[i];
+      de = -cp * de + d[i];
+      ga = -cp * ga;
+    }
 #define OSHUN_RAFB(N)                                                                              \
-      {                                                                                            \
-        const double qa = dpp_mov<row_shl<N>()>(ga, 1.0), qb = dpp_mov<row_shl<N>()>(de, 0.0);     \
-        de = ga * qb + de;                                                                         \
-        ga = ga * qa;                                                                              \
-      }
-      OSHUN_RAFB(1)
-      OSHUN_RAFB(2)
-      OSHUN_RAFB(4)
-      OSHUN_RAFB(8)
+    {                                                                                            \
+      const double qa = dpp_mov<row_shl<N>()>(ga, 1.0), qb = dpp_mov<row_shl<N>()>(de, 0.0);     \
+      de = ga * qb + de;                                                                         \
+      ga = ga * qa;                                                                              \
+    }
+    OSHUN_RAFB(1)
+    OSHUN_RAFB(2)
+    OSHUN_RAFB(4)
+    OSHUN_RAFB(8)
 #undef OSHUN_RAFB
-      if (SEGW == 32) {
-        const double qa = xor16(row_bcast<0>(ga)), qb = xor16(row_bcast<0>(de));
-        const double de2 = ga * qb + de, ga2 = ga * qa;
-        if (!upper_row()) { de = de2; ga = ga2; }
-      }
-      double xv = seg_next<SEGW>(de, 0.0);
+    if (SEGW == 32) {
+      const double qa = xor16(row_bcast<0>(ga)), qb = xor16(row_bcast<0>(de));
+      const double de2 = ga * qb + de, ga2 = ga * qa;
+      if (!upper_row()) { de = de2; ga = ga2; }
+    }
+    double xv = seg_next<SEGW>(de, 0.0);
 #pragma unroll
-      for (int i = NPL - 1; i >= 0; --i) { xv = d[i] - (Csup[i] * w[i]) * xv; d[i] = xv; }
+    for (int i = NPL - 1; i >= 0; --i) { xv = d[i] - (Csup[i] * w[i]) * xv; d[i] = xv; }
+    {
+      const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)c * a.plane, pbytes);
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) {
-        const int p = p0 + i;
-        if (colv && p < np) fc[base + (long long)p * nx] = static_cast<T>(d[i]);
-      }
+      for (int i = 0; i < NPL; ++i) bstore<T>(static_cast<T>(d[i]), r, so[i], 0);
     }
   }
 }

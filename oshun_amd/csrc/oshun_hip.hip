@@ -442,7 +442,6 @@ int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_e
       switch (c->rows_segw * 100 + c->rows_npl) {
         case 1604: rc = launch_collide_rows<T, 4, 16, 1>(c, a); break;
         case 1608: rc = launch_collide_rows<T, 8, 16, 1>(c, a); break;
-        case 1616: rc = launch_collide_rows<T, 16, 16, 1>(c, a); break;
         default: rc = launch_collide_rows<T, 8, 32, 1>(c, a); break;
       }
     }
@@ -451,7 +450,6 @@ int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_e
       switch (c->rows_segw2 * 100 + c->rows_npl2) {
         case 1604: rc = launch_collide_rows<T, 4, 16, 2>(c, a); break;
         case 1608: rc = launch_collide_rows<T, 8, 16, 2>(c, a); break;
-        case 1616: rc = launch_collide_rows<T, 16, 16, 2>(c, a); break;
         default: rc = launch_collide_rows<T, 8, 32, 2>(c, a); break;
       }
     }
