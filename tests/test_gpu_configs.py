@@ -1,0 +1,155 @@
+"""The reference's other configurations as parity and property tests at their
+real sizes.
+
+C1  plain heat-flow relaxation: Nx = 64 on L = 100 lambda_ei, Np = 64 to
+    pmax = 0.25, l0 = 2, m0 = 0, T(x) = 1 keV (1 + 0.1 cos(2 pi x/L)), n = 1,
+    collisions on (Z = 1), 100 RK4 + FP steps of 0.1/omega_p: GPU vs the
+    oracle at full size (FP64 <= 1e-12) and the FP32-split error per field.
+C5  shape of the adjoint configuration: Nx = 512, Np = 192, l0 = 3, m0 = 0,
+    T0 = 1 keV with a seeded 8-mode Fourier perturbation (<= 5%): 5 steps vs
+    the oracle at full size.
+C3  2D magnetised hot spot at full size: 256 x 256, Np = 128 to pmax = 0.25,
+    l0 = m0 = 4 (complex harmonics), T = 0.5 + 1.5 exp(-r^2/(2 (20 lambda_ei)^2))
+    keV, n = 1, uniform Bz with omega_c/nu_ei = 0.5, collisions on: number and
+    energy conservation over 2 steps and the FP32-split error per field
+    (the oracle would need hours at this size).
+
+lambda_ei and nu_ei follow the NRL formulary (nu_ei = 2.91e-6 Z n lnL T^-1.5
+s^-1, T in eV, n in cm^-3, lnL = 10) at the reference density 1e21 cm^-3;
+lengths are in c/omega_p, times in 1/omega_p.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Config, Oracle, num_harmonics
+
+pytestmark = pytest.mark.gpu
+
+DENSITY = 1e21
+
+
+def nu_ei_over_wp(T_kev, Z=1.0, lnl=10.0):
+    nu = 2.91e-6 * Z * DENSITY * lnl * (T_kev * 1e3) ** -1.5
+    wp = 5.64e4 * np.sqrt(DENSITY)
+    return nu / wp
+
+
+def lambda_ei(T_kev):
+    return np.sqrt(T_kev / 511.0) / nu_ei_over_wp(T_kev)
+
+
+def maxwellian(np_, pmax, T_kev, n=1.0):
+    """f00 on the deck axis for temperatures T_kev[...] (def-Setup.cpp:500-533)."""
+    pmin = pmax / (2.0 * np_)
+    p = np.arange(np_) * ((pmax - pmin) / (np_ - 1)) + pmin
+    pt2 = np.asarray(T_kev)[..., None] / 511.0
+    return n * np.exp(-p ** 2 / (2 * pt2)) / (np.sqrt(2 * np.pi) * 2 * np.pi * pt2 ** 1.5)
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def l2rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_c1_heat_flow_full_size_vs_oracle():
+    import oshun_amd
+    print("case C1 heat flow, Nx=64 Np=64 l0=2, 100 steps", flush=True)
+    nx, np_, l0, pmax, h, steps = 64, 64, 2, 0.25, 0.1, 100
+    L = 100.0 * lambda_ei(1.0)
+    dx = L / nx
+    x = np.arange(nx) * dx
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f[0, 0] = maxwellian(np_, pmax, 1.0 * (1.0 + 0.1 * np.cos(2 * np.pi * x / L)))
+    fld = np.zeros((6, 1, nx), np.complex128)
+    kw = dict(pmax=pmax, dx=dx, density_np=DENSITY, small_dt=h)
+    o = Oracle(Config(l0=l0, m0=0, np=np_, nx=nx, ny=1, nb=4, **kw))
+    o.set_state(f, fld)
+    o.step(h, steps)
+    fo, fldo = o.get_state()
+    out = {}
+    for prec in (oshun_amd.FP64, oshun_amd.FP32_SPLIT):
+        ctx = oshun_amd.Context(l0, 0, np_, nx, 1, nb_algorithms=4, precision=prec, **kw)
+        ctx.upload(f, fld)
+        ctx.step(h, steps)
+        out[prec] = ctx.download() + (ctx.moments(),)
+    fg, fldg, mom = out[oshun_amd.FP64]
+    assert rel(fg, fo) < 1e-12
+    assert rel(fldg[0].real, fldo[0].real) < 1e-12
+    qx = mom[6, 0]
+    assert np.abs(qx).max() > 0 and np.all(np.isfinite(qx))  # a heat flux has developed
+    f32, fld32, _ = out[oshun_amd.FP32_SPLIT]
+    e = {"f00": l2rel(f32[0], fg[0]), "f10": l2rel(f32[1], fg[1]), "f20": l2rel(f32[2], fg[2]),
+         "Ex": l2rel(fld32[0].real, fldg[0].real)}
+    print("C1 FP32-split relative L2 error vs FP64 after 100 steps:", {k: f"{v:.2e}" for k, v in e.items()})
+    assert e["f00"] < 1e-6 and e["f10"] < 1e-4 and e["Ex"] < 1e-4
+
+
+def test_c5_shape_vs_oracle():
+    import oshun_amd
+    print("case C5 shape, Nx=512 Np=192 l0=3, 5 steps", flush=True)
+    nx, np_, l0, pmax, h, steps = 512, 192, 3, 0.25, 0.1, 5
+    L = 100.0 * lambda_ei(1.0)
+    dx = L / nx
+    x = np.arange(nx) * dx
+    rng = np.random.default_rng(1234)
+    pert = np.zeros(nx)
+    for m in range(1, 9):
+        pert += rng.uniform(-1, 1) * np.cos(2 * np.pi * m * x / L + rng.uniform(0, 2 * np.pi))
+    T = 1.0 * (1.0 + 0.05 * pert / np.abs(pert).max())
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f[0, 0] = maxwellian(np_, pmax, T)
+    fld = np.zeros((6, 1, nx), np.complex128)
+    kw = dict(pmax=pmax, dx=dx, density_np=DENSITY, small_dt=h)
+    o = Oracle(Config(l0=l0, m0=0, np=np_, nx=nx, ny=1, nb=4, **kw))
+    o.set_state(f, fld)
+    o.step(h, steps)
+    fo, fldo = o.get_state()
+    ctx = oshun_amd.Context(l0, 0, np_, nx, 1, nb_algorithms=4, **kw)
+    ctx.upload(f, fld)
+    ctx.step(h, steps)
+    fg, fldg = ctx.download()
+    assert rel(fg, fo) < 1e-12
+    assert rel(fldg[0].real, fldo[0].real) < 1e-12
+
+
+def test_c3_hot_spot_full_size():
+    import oshun_amd
+    print("case C3 hot spot, 256x256 Np=128 l0=m0=4, 2 steps", flush=True)
+    nx = ny = 256
+    np_, l0, m0, pmax, h, steps = 128, 4, 4, 0.25, 0.5, 2
+    lam = lambda_ei(0.5)
+    dx = dy = 20.0 * lam / 16.0
+    x = (np.arange(nx) - nx / 2) * dx
+    y = (np.arange(ny) - ny / 2) * dy
+    r2 = x[None, :] ** 2 + y[:, None] ** 2
+    T = 0.5 + 1.5 * np.exp(-r2 / (2 * (20.0 * lam) ** 2))
+    nh = num_harmonics(l0, m0)
+    f = np.zeros((nh, ny, nx, np_), np.complex128)
+    f[0] = maxwellian(np_, pmax, T)
+    fld = np.zeros((6, ny, nx), np.complex128)
+    fld[5] = 0.5 * nu_ei_over_wp(0.5)  # Bz: omega_c / nu_ei = 0.5 at the background temperature
+    kw = dict(pmax=pmax, dx=dx, dy=dy, density_np=DENSITY, small_dt=h, nb_algorithms=4)
+    res = {}
+    for prec in (oshun_amd.FP64, oshun_amd.FP32_SPLIT):
+        ctx = oshun_amd.Context(l0, m0, np_, nx, ny, precision=prec, **kw)
+        ctx.upload(f, fld)
+        m_0 = ctx.moments()
+        ctx.step(h, steps)
+        m_1 = ctx.moments()
+        res[prec] = (ctx.download(), m_0, m_1)
+        del ctx
+    (fg, fldg), m_0, m_1 = res[oshun_amd.FP64]
+    assert np.all(np.isfinite(fg.real)) and np.all(np.isfinite(fg.imag))
+    N0, N1 = m_0[0].sum(), m_1[0].sum()
+    E0 = (m_0[0] * (m_0[3] + m_0[4] + m_0[5])).sum()
+    E1 = (m_1[0] * (m_1[3] + m_1[4] + m_1[5])).sum()
+    print(f"C3 dN/N {(N1 - N0) / N0:.2e} dE/E {(E1 - E0) / E0:.2e}")
+    assert abs(N1 - N0) < 1e-9 * N0
+    assert abs(E1 - E0) < 1e-6 * E0
+    (f32, fld32), _, _ = res[oshun_amd.FP32_SPLIT]
+    e = {"f00": l2rel(f32[0], fg[0]), "f11": l2rel(f32[2], fg[2]), "Bz": l2rel(fld32[5].real, fldg[5].real)}
+    print("C3 FP32-split relative L2 error vs FP64 after 2 steps:", {k: f"{v:.2e}" for k, v in e.items()})
+    assert e["f00"] < 1e-6 and e["Bz"] < 1e-6
