@@ -105,7 +105,7 @@ int record_step(oshun_batch* b, int q) {
   BLaunch<oshun::CurArgs<T>> cur;
   BLaunch<oshun::FieldArgs<T>> f0, f1a, f1b, f2;
   std::map<int, BLaunch<oshun::Stage1DArgs<T>>> p0, p1;   // by l0
-  std::map<int, BLaunch<oshun::CollArgs<T>>> c1, c2;      // by collision layout
+  std::map<int, BLaunch<oshun::CollArgs<T>>> c1, c2, c3;  // by collision layout (c3: both in one launch)
   for (int i = 0; i < nr; ++i) {
     oshun_ctx* c = b->runs[i];
     const double h = b->h[i];
@@ -131,11 +131,15 @@ int record_step(oshun_batch* b, int q) {
       a.f = static_cast<T*>(B1);  // the state after RK4
       a.tclk = b->d_t + i;
       a.hstep = h;
-      if (coll & OSHUN_COLL_EE) {
+      const bool same = c->rows_segw == c->rows_segw2 && c->rows_npl == c->rows_npl2;
+      if ((coll & OSHUN_COLL_EE) && (coll & OSHUN_COLL_FLM) && same) {
+        a.rt = c->d_rt;
+        c3[c->rows_segw * 100 + c->rows_npl].add(i, a, ((c->nx + 256 / c->rows_segw - 1) / (256 / c->rows_segw)));
+      } else if (coll & OSHUN_COLL_EE) {
         a.rt = c->d_rt;
         c1[c->rows_segw * 100 + c->rows_npl].add(i, a, ((c->nx + 256 / c->rows_segw - 1) / (256 / c->rows_segw)));
       }
-      if (coll & OSHUN_COLL_FLM) {
+      if ((coll & OSHUN_COLL_FLM) && !((coll & OSHUN_COLL_EE) && same)) {
         a.rt = c->d_rt2;
         c2[c->rows_segw2 * 100 + c->rows_npl2].add(i, a, ((c->nx + 256 / c->rows_segw2 - 1) / (256 / c->rows_segw2)));
       }
@@ -148,7 +152,7 @@ int record_step(oshun_batch* b, int q) {
   for (auto* m : {&p0, &p1})
     for (auto& kv : *m)
       if ((rc = kv.second.upload(b))) return rc;
-  for (auto* m : {&c1, &c2})
+  for (auto* m : {&c1, &c2, &c3})
     for (auto& kv : *m)
       if ((rc = kv.second.upload(b))) return rc;
   // capture the step
@@ -164,6 +168,7 @@ int record_step(oshun_batch* b, int q) {
   hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f2.total()), fblk, 0, s, f2.d_args, f2.d_ts, nr);
   for (auto& kv : c1) launch_coll_batch<T, 1>(s, kv.first, kv.second);
   for (auto& kv : c2) launch_coll_batch<T, 2>(s, kv.first, kv.second);
+  for (auto& kv : c3) launch_coll_batch<T, 3>(s, kv.first, kv.second);
   hipLaunchKernelGGL(oshun::batch_clock, dim3((nr + 255) / 256), dim3(256), 0, s, b->d_t, b->d_h, nr);
   hipGraph_t g = nullptr;
   BCHK(b, hipStreamEndCapture(s, &g));
@@ -186,7 +191,9 @@ int build_graphs(oshun_batch* b) {
   drop_graphs(b);
   for (int q = 0; q < 2; ++q) {
     int rc;
-    if (b->fp32) rc = b->pair_r == 2 ? record_step<float, 2>(b, q) : record_step<float, 3>(b, q);
+    if (b->fp32)
+      rc = b->pair_r == 2 ? record_step<float, 2>(b, q)
+                          : (b->pair_r == 3 ? record_step<float, 3>(b, q) : record_step<float, 4>(b, q));
     else rc = b->pair_r == 2 ? record_step<double, 2>(b, q) : record_step<double, 3>(b, q);
     if (rc) return rc;
   }
@@ -250,6 +257,7 @@ int oshun_batch_create(const oshun_grid* grids, const oshun_phys* phys, const do
       hipMalloc((void**)&b->d_h, nruns * sizeof(double)) != hipSuccess ||
       hipMemcpy(b->d_h, h, nruns * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return bail(OSHUN_ERR_HIP);
+  b->pair_r = 3;  // the single-run tile height (oshun_step): a run is bitwise the same alone
   const int rc = build_graphs(b);
   if (rc) return bail(rc);
   *out = b;
@@ -322,7 +330,8 @@ int oshun_batch_autotune(oshun_batch* b, int steps, int* chosen_rows, double* ms
     }
     hipStreamSynchronize(b->stream);
   };
-  const int cand[2] = {2, 3};
+  std::vector<int> cand = {2, 3};  // rows per thread: 6 or 10 p rows per tile
+  if (b->fp32) cand.push_back(4);   // 14 rows (float only: double would exceed 256 VGPRs)
   double best = 1e300;
   int best_r = b->pair_r;
   hipEvent_t e0, e1;

@@ -1464,7 +1464,8 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 }
 
 // MODE 1: e-e subcycles on f_00 (Explicit_EC_FP::loop); MODE 2: implicit
-// l >= 1 step (Euler_Backward::advance_1/advance_lm).  Separate launches.
+// l >= 1 step (Euler_Backward::advance_1/advance_lm); MODE 3: both in one
+// launch (the implicit step continues from the collided f00 in registers).
 template <typename T, int NPL, int SEGW, int MODE>
 __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int tile) {
   constexpr int kCols = 256 / SEGW;  // columns per workgroup
@@ -1494,7 +1495,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     const int p = p0 + i;
     f[i] = (p < np) ? static_cast<double>(a.f[base + (long long)p * nx]) + (a.bg ? a.bg[p] : 0.0) : 0.0;
   }
-  if (MODE == 1) {
+  if (MODE == 1 || MODE == 3) {
     // RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553, numh times (:590-625)
     const double h = h_ee;
     for (int s = 0; s < numh; ++s) {
@@ -1520,7 +1521,16 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
       const int p = p0 + i;
       if (colv && p < np) a.f[base + (long long)p * nx] = static_cast<T>(f[i] - (a.bg ? a.bg[p] : 0.0));
     }
-    return;
+    if (MODE == 1) return;
+    // MODE 3 goes on with the implicit step, which reads f00 as stored
+    if (sizeof(T) < sizeof(double)) {
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        const int p = p0 + i;
+        const double b = (a.bg && p < np) ? a.bg[p] : 0.0;
+        f[i] = (p < np) ? static_cast<double>(static_cast<T>(f[i] - b)) + b : 0.0;
+      }
+    }
   }
   // Implicit_FLM_step::reset_coeff, def-FokkerPlanck.cpp:808-939
   double Asub[NPL], Bdia[NPL], Csup[NPL], Sct[NPL];

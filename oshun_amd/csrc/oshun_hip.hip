@@ -305,16 +305,22 @@ int rk4(oshun_ctx* c, double h) {
   return OSHUN_OK;
 }
 
-constexpr int kPairR = 3;  // rows per thread of the pair kernels: 4*3-2 = 10 p rows per tile
-constexpr int kPairTP = 4 * kPairR - 2;
+// rows per thread of the pair kernels (tile height 4R-2 p rows): 3 (10 rows);
+// measured at the target shape, 14-row float tiles were 15% slower (less halo
+// re-read but fewer wavefronts), 6-row tiles re-read more halo
+template <typename T>
+constexpr int pair_rows() { return 3; }
+constexpr int kPairTP = 4 * 3 - 2;
+template <typename T>
+int pair_tiles(const oshun_ctx* c) { return (c->np + 4 * pair_rows<T>() - 3) / (4 * pair_rows<T>() - 2); }
 constexpr int kPairTPMin = 4 * 2 - 2;  // smallest tile height the batched path can choose
 
 template <typename T, int L0>
 void launch_pair_l0(oshun_ctx* c, int pair, const oshun::Stage1DArgs<T>& a, dim3 grid) {
   if (pair == 0)
-    hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, kPairR, 0>), grid, dim3(64, 4), 0, c->stream, a);
+    hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, pair_rows<T>(), 0>), grid, dim3(64, 4), 0, c->stream, a);
   else
-    hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, kPairR, 1>), grid, dim3(64, 4), 0, c->stream, a);
+    hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, pair_rows<T>(), 1>), grid, dim3(64, 4), 0, c->stream, a);
 }
 
 // RK4 stages 1+2 (pair 0) or 3+4 (pair 1) of the 1D m0 = 0 system
@@ -335,7 +341,7 @@ int launch_pair(oshun_ctx* c, int pair, const void* in, const void* y0, void* ou
                 const double* fin2, double* jpA, double* jpB, double h) {
   const oshun::Stage1DArgs<T> a = pair_args<T>(c, in, y0, out, acc, fin, fin2, jpA, jpB, h);
   const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
-  dim3 grid(ntx * c->npt);
+  dim3 grid(ntx * pair_tiles<T>(c));
   ProfRec r;
   prof_begin(c, 0, &r);
   switch (c->T.l0) {
@@ -376,10 +382,11 @@ int rk4_pair(oshun_ctx* c, double h) {
   }
   if ((rc = launch_field<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, c->S, h, c->jp0, c->npb0))) return rc;
   if ((rc = launch_pair<T>(c, 0, c->S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h))) return rc;
-  if ((rc = launch_field<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, c->npt, c->f1y))) return rc;
-  if ((rc = launch_field<T>(c, 3, c->B2f, c->Sf, c->B1f, c->Af, c->B2, h, c->jp2, c->npt))) return rc;
+  const int npt = pair_tiles<T>(c);
+  if ((rc = launch_field<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, npt, c->f1y))) return rc;
+  if ((rc = launch_field<T>(c, 3, c->B2f, c->Sf, c->B1f, c->Af, c->B2, h, c->jp2, npt))) return rc;
   if ((rc = launch_pair<T>(c, 1, c->B2, c->S, c->B1, c->A, c->B2f, c->B1f, c->jp1, nullptr, h))) return rc;
-  if ((rc = launch_field<T>(c, 4, c->B1f, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, c->npt, c->f1y))) return rc;
+  if ((rc = launch_field<T>(c, 4, c->B1f, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, npt, c->f1y))) return rc;
   std::swap(c->S, c->B1);
   c->t_rk += h;
   return OSHUN_OK;
@@ -437,6 +444,16 @@ int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_e
     ProfRec r;
     prof_begin(c, 2, &r);
     int rc = OSHUN_OK;
+    if (a.do_ee && (a.do_f1 || a.do_flm) && c->rows_segw == c->rows_segw2 && c->rows_npl == c->rows_npl2) {
+      a.rt = c->d_rt;  // one launch for both (same lane layout, same tables)
+      switch (c->rows_segw * 100 + c->rows_npl) {
+        case 1604: rc = launch_collide_rows<T, 4, 16, 3>(c, a); break;
+        case 1608: rc = launch_collide_rows<T, 8, 16, 3>(c, a); break;
+        default: rc = launch_collide_rows<T, 8, 32, 3>(c, a); break;
+      }
+      prof_end(c, &r);
+      return rc;
+    }
     if (a.do_ee) {
       a.rt = c->d_rt;
       switch (c->rows_segw * 100 + c->rows_npl) {
