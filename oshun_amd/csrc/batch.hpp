@@ -1,8 +1,8 @@
 // Batched independent runs: many 1D m0 = 0 runs of uneven Nx, Np and l0
 // advanced together, one hipGraph launch per RK4 + collision step.
 //
-// Every launch of the step (current, four field stages, two stage pairs, two
-// collision launches, clock) covers the tiles of all runs at once: the run of
+// Every launch of the step (four field stages, two stage pairs, the collision
+// launches, clock) covers the tiles of all runs at once: the run of
 // a workgroup comes from a prefix table of per-run tile counts and its
 // arguments from a per-run array in device memory.  Kernels templated on l0
 // (stage pairs) or on the collision layout get one launch per distinct value.
@@ -102,7 +102,6 @@ template <typename T, int R>
 int record_step(oshun_batch* b, int q) {
   constexpr int TP = 4 * R - 2;
   const int nr = static_cast<int>(b->runs.size());
-  BLaunch<oshun::CurArgs<T>> cur;
   BLaunch<oshun::FieldArgs<T>> f0, f1a, f1b, f2;
   std::map<int, BLaunch<oshun::Stage1DArgs<T>>> p0, p1;   // by l0
   std::map<int, BLaunch<oshun::CollArgs<T>>> c1, c2, c3;  // by collision layout (c3: both in one launch)
@@ -116,9 +115,9 @@ int record_step(oshun_batch* b, int q) {
     const int npt = (c->np + TP - 1) / TP;
     const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
     const int fblocks = (c->nxy + oshun::kFieldX - 1) / oshun::kFieldX;
-    const T* f10 = static_cast<const T*>(S) + (size_t)c->T.comp_re[oshun::hind(1, 0, 0)] * c->plane;
-    cur.add(i, oshun::CurArgs<T>{f10, c->d_p3og, c->jp0, c->np, c->nx}, ((c->nx + 63) / 64) * c->npb0);
-    f0.add(i, field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, S, h, c->jp0, c->npb0, nullptr), fblocks);
+    oshun::FieldArgs<T> fa0 = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, S, h, c->jp0, c->npb0, nullptr);
+    fa0.p3og = c->d_p3og;  // current of Y0 summed directly over p
+    f0.add(i, fa0, fblocks);
     p0[c->T.l0].add(i, pair_args<T>(c, S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h), ntx * npt);
     f1a.add(i, field_args<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, npt, c->f1y), fblocks);
     f1b.add(i, field_args<T>(c, 3, c->B2f, c->Sf, c->B1f, c->Af, c->B2, h, c->jp2, npt, nullptr), fblocks);
@@ -146,7 +145,7 @@ int record_step(oshun_batch* b, int q) {
     }
   }
   int rc;
-  if ((rc = cur.upload(b)) || (rc = f0.upload(b)) || (rc = f1a.upload(b)) || (rc = f1b.upload(b)) ||
+  if ((rc = f0.upload(b)) || (rc = f1a.upload(b)) || (rc = f1b.upload(b)) ||
       (rc = f2.upload(b)))
     return rc;
   for (auto* m : {&p0, &p1})
@@ -159,7 +158,6 @@ int record_step(oshun_batch* b, int q) {
   hipStream_t s = b->stream;
   BCHK(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const dim3 fblk(oshun::kFieldX, oshun::kFieldRows);
-  hipLaunchKernelGGL(oshun::current_1d_batch<T>, dim3(cur.total()), dim3(64, 4), 0, s, cur.d_args, cur.d_ts, nr);
   hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f0.total()), fblk, 0, s, f0.d_args, f0.d_ts, nr);
   for (auto& kv : p0) launch_pair_batch<T, R, 0>(s, kv.first, kv.second);
   hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f1a.total()), fblk, 0, s, f1a.d_args, f1a.d_ts, nr);

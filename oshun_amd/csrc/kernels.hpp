@@ -589,6 +589,7 @@ struct FieldArgs {
   const double* jpart;
   const T* fpl;  // distribution planes of the stage input
   const double* f1p1;  // if set: f10 at p1 per x of the stage input (instead of fpl)
+  const double* p3og;  // if set: the current of f10 is summed here over p (no partials)
   int npb, np, nx, ny, mode;
   int c10, c11r, c11i;
   long long plane;
@@ -636,6 +637,33 @@ __device__ __forceinline__ void sum_parts(const double* jpart, int npb, int nxy,
   __syncthreads();
 }
 
+// the current integrand of one plane summed over p by the rows of the block
+// (row r takes p = r, r + kFieldRows, ...), then the same fixed tree
+template <typename T>
+__device__ __forceinline__ void sum_direct(const T* plane, const double* p3og, int np, int nx, int nxy, int i,
+                                           int row, double (&sh)[kFieldRows][kFieldX], double* out) {
+  double s = 0.0;
+  if (i < nxy) {
+    const int x = i % nx, y = i / nx;
+    const T* col = plane + (long long)y * np * nx + x;
+#pragma unroll 4
+    for (int p = row; p < np; p += kFieldRows) {
+      const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
+      s += w * (static_cast<double>(col[(long long)p * nx]) * p3og[p]);
+    }
+  }
+  const int lx = threadIdx.x;
+  sh[row][lx] = s;
+  __syncthreads();
+#pragma unroll
+  for (int w = kFieldRows / 2; w >= 1; w /= 2) {
+    if (row < w) sh[row][lx] += sh[row + w][lx];
+    __syncthreads();
+  }
+  *out = sh[0][lx];
+  __syncthreads();
+}
+
 template <typename T>
 __device__ __forceinline__ void field_body(const FieldArgs<T>& a, const int block) {
   const int i = block * kFieldX + threadIdx.x;
@@ -643,7 +671,10 @@ __device__ __forceinline__ void field_body(const FieldArgs<T>& a, const int bloc
   const int nx = a.nx, ny = a.ny, nxy = nx * ny;
   __shared__ double sh[kFieldRows][kFieldX];
   double jx = 0.0, jr = 0.0, ji = 0.0;
-  if (a.c10 >= 0) sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &jx);
+  if (a.c10 >= 0) {
+    if (a.p3og) sum_direct<T>(a.fpl + (long long)a.c10 * a.plane, a.p3og, a.np, nx, nxy, i, row, sh, &jx);
+    else sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &jx);
+  }
   if (a.c11r >= 0) {
     sum_parts(a.jpart, a.npb, nxy, i, 1, row, sh, &jr);
     sum_parts(a.jpart, a.npb, nxy, i, 2, row, sh, &ji);

@@ -259,6 +259,7 @@ oshun::FieldArgs<T> field_args(oshun_ctx* c, int mode, const double* fin, const 
   a.jpart = jp ? jp : c->jpart;
   a.fpl = static_cast<const T*>(fpl);
   a.f1p1 = f1p1;
+  a.p3og = nullptr;
   a.npb = jp ? npb : c->npb; a.np = c->np; a.nx = c->nx; a.ny = c->ny; a.mode = mode;
   a.c10 = c->T.comp_re[oshun::hind(1, 0, c->T.m0)];
   a.c11r = c->T.m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, c->T.m0)] : -1;
@@ -371,16 +372,16 @@ int launch_pair(oshun_ctx* c, int pair, const void* in, const void* y0, void* ou
 template <typename T>
 int rk4_pair(oshun_ctx* c, double h) {
   int rc;
-  {
+  {  // first field stage: the current of Y0 summed directly over p (no partials)
+    oshun::FieldArgs<T> a = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, c->S, h, c->jp0, c->npb0, nullptr);
+    a.p3og = c->d_p3og;
     ProfRec r;
     prof_begin(c, 1, &r);
-    const T* f10 = static_cast<const T*>(c->S) + (size_t)c->T.comp_re[oshun::hind(1, 0, 0)] * c->plane;
-    oshun::CurArgs<T> ca{f10, c->d_p3og, c->jp0, c->np, c->nx};
-    hipLaunchKernelGGL(oshun::current_1d<T>, dim3(((c->nx + 63) / 64) * c->npb0), dim3(64, 4), 0, c->stream, ca);
+    hipLaunchKernelGGL(oshun::field_stage<T>, dim3((c->nxy + oshun::kFieldX - 1) / oshun::kFieldX),
+                       dim3(oshun::kFieldX, oshun::kFieldRows), 0, c->stream, a);
     prof_end(c, &r);
     HIPCHK(c, hipGetLastError());
   }
-  if ((rc = launch_field<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, c->S, h, c->jp0, c->npb0))) return rc;
   if ((rc = launch_pair<T>(c, 0, c->S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h))) return rc;
   const int npt = pair_tiles<T>(c);
   if ((rc = launch_field<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, npt, c->f1y))) return rc;
