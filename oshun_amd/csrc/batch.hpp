@@ -102,7 +102,8 @@ template <typename T, int R>
 int record_step(oshun_batch* b, int q) {
   constexpr int TP = 4 * R - 2;
   const int nr = static_cast<int>(b->runs.size());
-  BLaunch<oshun::FieldArgs<T>> f0, f1a, f1b, f2;
+  BLaunch<oshun::FieldArgs<T>> f0, f2;
+  BLaunch<oshun::Field23Args<T>> f23;
   std::map<int, BLaunch<oshun::Stage1DArgs<T>>> p0, p1;   // by l0
   std::map<int, BLaunch<oshun::CollArgs<T>>> c1, c2, c3;  // by collision layout (c3: both in one launch)
   for (int i = 0; i < nr; ++i) {
@@ -119,10 +120,12 @@ int record_step(oshun_batch* b, int q) {
     fa0.p3og = c->d_p3og;  // current of Y0 summed directly over p
     f0.add(i, fa0, fblocks);
     p0[c->T.l0].add(i, pair_args<T>(c, S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h), ntx * npt);
-    f1a.add(i, field_args<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, npt, c->f1y), fblocks);
-    f1b.add(i, field_args<T>(c, 3, c->B2f, c->Sf, c->B1f, c->Af, c->B2, h, c->jp2, npt, nullptr), fblocks);
-    p1[c->T.l0].add(i, pair_args<T>(c, c->B2, S, B1, c->A, c->B2f, c->B1f, c->jp1, nullptr, h), ntx * npt);
-    f2.add(i, field_args<T>(c, 4, c->B1f, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, npt, c->f1y), fblocks);
+    oshun::Field23Args<T> fa23;
+    fa23.a2 = field_args<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, npt, c->f1y);
+    fa23.a3 = field_args<T>(c, 3, c->B2f, c->Sf, c->Cf, c->Af, c->B2, h, c->jp2, npt, nullptr);
+    f23.add(i, fa23, fblocks);
+    p1[c->T.l0].add(i, pair_args<T>(c, c->B2, S, B1, c->A, c->B2f, c->Cf, c->jp1, nullptr, h), ntx * npt);
+    f2.add(i, field_args<T>(c, 4, c->Cf, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, npt, c->f1y), fblocks);
     const int coll = c->ph.collisions;
     if (coll) {
       oshun::CollArgs<T> a = coll_args<T>(c, (coll & OSHUN_COLL_EE) ? 1 : 0, (coll & OSHUN_COLL_FLM) ? 1 : 0,
@@ -145,7 +148,7 @@ int record_step(oshun_batch* b, int q) {
     }
   }
   int rc;
-  if ((rc = f0.upload(b)) || (rc = f1a.upload(b)) || (rc = f1b.upload(b)) ||
+  if ((rc = f0.upload(b)) || (rc = f23.upload(b)) ||
       (rc = f2.upload(b)))
     return rc;
   for (auto* m : {&p0, &p1})
@@ -160,8 +163,7 @@ int record_step(oshun_batch* b, int q) {
   const dim3 fblk(oshun::kFieldX, oshun::kFieldRows);
   hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f0.total()), fblk, 0, s, f0.d_args, f0.d_ts, nr);
   for (auto& kv : p0) launch_pair_batch<T, R, 0>(s, kv.first, kv.second);
-  hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f1a.total()), fblk, 0, s, f1a.d_args, f1a.d_ts, nr);
-  hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f1b.total()), fblk, 0, s, f1b.d_args, f1b.d_ts, nr);
+  hipLaunchKernelGGL(oshun::field_stage23_1d_batch<T>, dim3(f23.total()), fblk, 0, s, f23.d_args, f23.d_ts, nr);
   for (auto& kv : p1) launch_pair_batch<T, R, 1>(s, kv.first, kv.second);
   hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f2.total()), fblk, 0, s, f2.d_args, f2.d_ts, nr);
   for (auto& kv : c1) launch_coll_batch<T, 1>(s, kv.first, kv.second);

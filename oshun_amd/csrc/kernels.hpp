@@ -664,6 +664,20 @@ __device__ __forceinline__ void sum_direct(const T* plane, const double* p3og, i
   __syncthreads();
 }
 
+// MaxwellEq::Ampere (def-VlasovMax.cpp:1009-1035) and Faraday (:979-1005)
+// added to k; F(c, at) is field c (Ex..Bz) at x-1, x+1, y-1, y+1 (at = 0..3)
+template <typename Get>
+__device__ __forceinline__ void maxwell_terms(Get F, double mx, double my, double (&k)[6]) {
+  k[0] += F(5, 2) * my - F(5, 3) * my;
+  k[1] += F(5, 0) * (-mx) - F(5, 1) * (-mx);
+  k[2] += F(3, 2) * (-my) - F(3, 3) * (-my);
+  k[2] += F(4, 0) * mx - F(4, 1) * mx;
+  k[3] += F(2, 2) * (-my) - F(2, 3) * (-my);
+  k[4] += F(2, 0) * mx - F(2, 1) * mx;
+  k[5] += F(0, 2) * my - F(0, 3) * my;
+  k[5] += F(1, 0) * (-mx) - F(1, 1) * (-mx);
+}
+
 template <typename T>
 __device__ __forceinline__ void field_body(const FieldArgs<T>& a, const int block) {
   const int i = block * kFieldX + threadIdx.x;
@@ -710,17 +724,12 @@ __device__ __forceinline__ void field_body(const FieldArgs<T>& a, const int bloc
     k[1] += jr;
     k[2] += -ji;
   }
-  const double mx = a.midx, my = a.midy;
-  // MaxwellEq::Ampere (def-VlasovMax.cpp:1009-1035)
-  k[0] += Bz[ym] * my - Bz[yp] * my;
-  k[1] += Bz[xm] * (-mx) - Bz[xp] * (-mx);
-  k[2] += Bx[ym] * (-my) - Bx[yp] * (-my);
-  k[2] += By[xm] * mx - By[xp] * mx;
-  // MaxwellEq::Faraday (def-VlasovMax.cpp:979-1005)
-  k[3] += Ez[ym] * (-my) - Ez[yp] * (-my);
-  k[4] += Ez[xm] * mx - Ez[xp] * mx;
-  k[5] += Ex[ym] * my - Ex[yp] * my;
-  k[5] += Ey[xm] * (-mx) - Ey[xp] * (-mx);
+  maxwell_terms(
+      [&](int c, int at) {
+        const double* F = (c == 0) ? Ex : (c == 1) ? Ey : (c == 2) ? Ez : (c == 3) ? Bx : (c == 4) ? By : Bz;
+        return F[at == 0 ? xm : at == 1 ? xp : at == 2 ? ym : yp];
+      },
+      a.midx, a.midy, k);
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     const long long o = (long long)c * nxy + i;
@@ -732,6 +741,81 @@ __device__ __forceinline__ void field_body(const FieldArgs<T>& a, const int bloc
 template <typename T>
 __global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage(FieldArgs<T> a) {
   field_body<T>(a, blockIdx.x);
+}
+
+// RK4 field stages 2 and 3 of the 1D m0 = 0 system in one pass (a2: mode 2,
+// a3: mode 3, same arithmetic as two field_stage launches).  Only Ex couples
+// to f and its update is local; the stage-3 update of the other five fields
+// at x needs their stage-2 values at x +- 1, which are recomputed here from
+// the stage-1 fields at x +- 2.  a3.fin is not read (the stage-2 fields are
+// in registers) and a3.fout must not alias a2.fin.
+template <typename T>
+struct Field23Args {
+  FieldArgs<T> a2, a3;
+};
+template <typename T>
+__device__ __forceinline__ void field23_body(const Field23Args<T>& A, const int block) {
+  const FieldArgs<T>& a = A.a2;
+  const FieldArgs<T>& b = A.a3;
+  const int i = block * kFieldX + threadIdx.x;
+  const int row = threadIdx.y;
+  const int nx = a.nx, nxy = nx;  // ny = 1
+  __shared__ double sh[kFieldRows][kFieldX];
+  double j1 = 0.0, j2 = 0.0;
+  sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &j1);
+  sum_parts(b.jpart, b.npb, nxy, i, 0, row, sh, &j2);
+  if (i >= nxy || row != 0) return;
+  auto wrap = [nx](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
+  // Current::Jxyz (def-VlasovMax.cpp:680-745) of the stage inputs Y1 and Y2
+  auto current = [&](const FieldArgs<T>& f, double j) {
+    j *= f.delta_p;
+    const long long o1 = (long long)f.c10 * f.plane + nx + i;
+    j += f.small * ((f.f1p1 ? f.f1p1[i] : static_cast<double>(f.fpl[o1])) * f.p3og1);
+    j *= 4.0 * M_PI / 3.0;
+    return j;
+  };
+  // stage 2 at x-1, x, x+1 (Ex only where it is used: at x)
+  double F2[3][6];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int xx = wrap(i + d - 1);
+    const int xm = wrap(xx - 1), xp = wrap(xx + 1);
+    double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (d == 1) k[0] += current(a, j1);
+    maxwell_terms([&](int c, int at) { return a.fin[(long long)c * nxy + (at == 0 ? xm : at == 1 ? xp : xx)]; },
+                  a.midx, a.midy, k);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double kh = k[c] * a.c05h;
+      F2[d][c] = a.f0[(long long)c * nxy + xx] + kh;  // rk_combine mode 2: out = y0 + kh
+      if (d == 1) {
+        double* acc = a.facc + (long long)c * nxy + i;
+        *acc = *acc + kh * a.twothird;
+        a.fout[(long long)c * nxy + i] = F2[1][c];
+      }
+    }
+  }
+  // stage 3 at x from the stage-2 fields
+  double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  k[0] += current(b, j2);
+  maxwell_terms([&](int c, int at) { return at == 0 ? F2[0][c] : at == 1 ? F2[2][c] : F2[1][c]; }, b.midx, b.midy,
+                k);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const long long o = (long long)c * nxy + i;
+    rk_combine(3, k[c], b.f0[o], b.fout + o, b.facc + o, b.c05h, b.hh, b.h6, b.third, b.twothird);
+  }
+}
+template <typename T>
+__global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage23_1d(Field23Args<T> a) {
+  field23_body<T>(a, blockIdx.x);
+}
+template <typename T>
+__global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage23_1d_batch(const Field23Args<T>* __restrict__ args,
+                                                                              const int* __restrict__ ts, int nruns) {
+  const int g = blockIdx.x;
+  const int r = batch_run(ts, nruns, g);
+  field23_body<T>(args[r], g - ts[r]);
 }
 template <typename T>
 __global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage_batch(const FieldArgs<T>* __restrict__ args,

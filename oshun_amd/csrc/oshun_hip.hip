@@ -38,6 +38,7 @@ struct oshun_ctx {
   size_t esz = 8;
   void *S = nullptr, *A = nullptr, *B1 = nullptr, *B2 = nullptr;
   double *Sf = nullptr, *Af = nullptr, *B1f = nullptr, *B2f = nullptr;
+  double* Cf = nullptr;  // 1D pair path: fields of the fourth stage's input
   double* jpart = nullptr;
   Term* d_terms = nullptr;
   int* d_tstart = nullptr;
@@ -384,10 +385,19 @@ int rk4_pair(oshun_ctx* c, double h) {
   }
   if ((rc = launch_pair<T>(c, 0, c->S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h))) return rc;
   const int npt = pair_tiles<T>(c);
-  if ((rc = launch_field<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, npt, c->f1y))) return rc;
-  if ((rc = launch_field<T>(c, 3, c->B2f, c->Sf, c->B1f, c->Af, c->B2, h, c->jp2, npt))) return rc;
-  if ((rc = launch_pair<T>(c, 1, c->B2, c->S, c->B1, c->A, c->B2f, c->B1f, c->jp1, nullptr, h))) return rc;
-  if ((rc = launch_field<T>(c, 4, c->B1f, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, npt, c->f1y))) return rc;
+  {  // field stages 2 and 3 in one pass (E2 -> B2f, E3 -> Cf)
+    oshun::Field23Args<T> f23;
+    f23.a2 = field_args<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, npt, c->f1y);
+    f23.a3 = field_args<T>(c, 3, c->B2f, c->Sf, c->Cf, c->Af, c->B2, h, c->jp2, npt, nullptr);
+    ProfRec r;
+    prof_begin(c, 1, &r);
+    hipLaunchKernelGGL(oshun::field_stage23_1d<T>, dim3((c->nxy + oshun::kFieldX - 1) / oshun::kFieldX),
+                       dim3(oshun::kFieldX, oshun::kFieldRows), 0, c->stream, f23);
+    prof_end(c, &r);
+    HIPCHK(c, hipGetLastError());
+  }
+  if ((rc = launch_pair<T>(c, 1, c->B2, c->S, c->B1, c->A, c->B2f, c->Cf, c->jp1, nullptr, h))) return rc;
+  if ((rc = launch_field<T>(c, 4, c->Cf, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, npt, c->f1y))) return rc;
   std::swap(c->S, c->B1);
   c->t_rk += h;
   return OSHUN_OK;
@@ -622,7 +632,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 }
 
 void free_all(oshun_ctx* c) {
-  void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->jpart, c->d_terms, c->d_tstart,
+  void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
                   c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp0, c->jp1, c->jp2, c->f1y};
   for (void* p : ptrs)
     if (p) hipFree(p);
@@ -708,8 +718,10 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     if (hipMalloc((void**)&c->jp0, (size_t)c->npb0 * 3 * nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->jp1, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->jp2, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&c->f1y, nx * sizeof(double)) != hipSuccess)
+        hipMalloc((void**)&c->f1y, nx * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->Cf, fldbytes) != hipSuccess)
       return bail(OSHUN_ERR_ALLOC);
+    if (hipMemset(c->Cf, 0, fldbytes) != hipSuccess) return bail(OSHUN_ERR_HIP);
   }
   if (c->fp32) {
     c->bg.assign(c->np, 0.0);
