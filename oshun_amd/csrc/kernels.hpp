@@ -1584,18 +1584,33 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 template <typename T, int NPL, int SEGW, int MODE>
 __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int tile) {
   constexpr int kCols = 256 / SEGW;  // columns per workgroup
+  // Planes cross HBM in row order (a wavefront moves 64/kCols rows of kCols
+  // adjacent x) and are turned to the lane layout (lane seg of column w holds
+  // rows seg*NPL .. seg*NPL+NPL-1) through LDS.  Column stride kS and one pad
+  // per NPL rows keep the lane-layout reads free of bank conflicts.
+  constexpr int kS = SEGW * NPL + SEGW + 8;
   __shared__ double tb[RT_COUNT * kRowP];
+  __shared__ double tr[kCols * kS];
   for (int i = threadIdx.x; i < RT_COUNT * kRowP; i += blockDim.x) tb[i] = a.rt[i];
-  __syncthreads();
   const int nx = a.nx, np = a.np;
   const int ntx = (nx + kCols - 1) / kCols;
   const int y = tile / ntx;
-  const int xc = (tile % ntx) * kCols + threadIdx.x / SEGW;
-  const bool colv = xc < nx;
-  const int x = colv ? xc : 0;  // idle rows compute on column 0 and store nothing
+  const int x0 = (tile % ntx) * kCols;
   const int seg = threadIdx.x & (SEGW - 1);
   const int p0 = seg * NPL;
-  const long long base = (long long)y * np * nx + x;
+  const int lrow = (threadIdx.x / SEGW) * kS + p0 + seg;  // LDS index of lane-layout row i: lrow + i
+  // row-order layout: thread t moves column cc = t % kCols, rows r0 + k*SEGW (k < NPL)
+  const int cc = threadIdx.x % kCols, r0 = threadIdx.x / kCols;
+  const bool ccv = x0 + cc < nx;
+  const int crow = cc * kS + r0 + r0 / NPL;  // LDS index of row-order element k: crow + k*(SEGW + SEGW/NPL)
+  constexpr int kStepL = SEGW + SEGW / NPL;
+  // byte offsets within a plane; rows past np and idle columns read 0 / drop their stores
+  constexpr int kOut = 0x7ffffff0;
+  const int cofs = static_cast<int>(((long long)y * np * nx + (ccv ? x0 + cc : 0) + (long long)r0 * nx) * sizeof(T));
+  const int crowb = SEGW * nx * static_cast<int>(sizeof(T));
+  auto ld_ofs = [&](int k) { return (r0 + k * SEGW < np) ? cofs + k * crowb : kOut; };
+  auto st_ofs = [&](int k) { return (ccv && r0 + k * SEGW < np) ? cofs + k * crowb : kOut; };
+  const unsigned pbytes = static_cast<unsigned>(a.plane * sizeof(T));
   int numh = a.numh;
   double h_ee = a.h_ee, dt_flm = a.dt_flm;
   if (a.tclk) {  // batched runs: the sub-steps of Explicit_EC_FP::loop from the run's clock (:597-599)
@@ -1605,10 +1620,19 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     dt_flm = tnew - t;
   }
   double f[NPL];
+  {
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f, pbytes);
+    T v[NPL];
 #pragma unroll
-  for (int i = 0; i < NPL; ++i) {
-    const int p = p0 + i;
-    f[i] = (p < np) ? static_cast<double>(a.f[base + (long long)p * nx]) + (a.bg ? a.bg[p] : 0.0) : 0.0;
+    for (int k = 0; k < NPL; ++k) v[k] = bload<T>(r, ld_ofs(k), 0);
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+      const int p = r0 + k * SEGW;
+      tr[crow + k * kStepL] = (p < np) ? static_cast<double>(v[k]) + (a.bg ? a.bg[p] : 0.0) : 0.0;
+    }
+    __syncthreads();  // also covers the table fill
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) f[i] = tr[lrow + i];
   }
   if (MODE == 1 || MODE == 3) {
     // RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553, numh times (:590-625)
@@ -1631,10 +1655,18 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { fs[i] *= (h / 6.0); f[i] += fs[i]; }
     }
+    __syncthreads();  // every lane has read its rows of f00
 #pragma unroll
-    for (int i = 0; i < NPL; ++i) {
-      const int p = p0 + i;
-      if (colv && p < np) a.f[base + (long long)p * nx] = static_cast<T>(f[i] - (a.bg ? a.bg[p] : 0.0));
+    for (int i = 0; i < NPL; ++i) tr[lrow + i] = f[i];
+    __syncthreads();
+    {
+      const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f, pbytes);
+#pragma unroll
+      for (int k = 0; k < NPL; ++k) {
+        const int p = r0 + k * SEGW;
+        const double b = (a.bg && p < np) ? a.bg[p] : 0.0;
+        bstore<T>(static_cast<T>(tr[crow + k * kStepL] - b), r, st_ofs(k), 0);
+      }
     }
     if (MODE == 1) return;
     // MODE 3 goes on with the implicit step, which reads f00 as stored
@@ -1701,39 +1733,34 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   // Implicit_FLM_step::advance for every l >= 1 plane, def-FokkerPlanck.cpp:943-1035,
   // TridiagonalSolve def-Nmethods.cpp:142-167 with the lane boundaries joined by
   // scans.  Planes come in ind(l,m) order (l non-decreasing): the pivots are
-  // formed when l changes, and the next plane's loads are in flight while the
-  // current plane is solved.  Rows past np (and idle columns, for stores) use
-  // an offset outside the plane: the buffer access returns 0 / is dropped.
+  // formed when l changes, and the next plane's row-order loads are in flight
+  // while the current plane is solved.
   const int lmin = a.do_f1 ? 1 : 2;
   const int lmax = a.do_flm ? a.l0 : 1;
-  const unsigned pbytes = static_cast<unsigned>(a.plane * sizeof(T));
-  constexpr int kOut = 0x7ffffff0;
-  int lo[NPL], so[NPL];
-#pragma unroll
-  for (int i = 0; i < NPL; ++i) {
-    const int p = p0 + i;
-    lo[i] = (p < np) ? static_cast<int>((base + (long long)p * nx) * (long long)sizeof(T)) : kOut;
-    so[i] = colv ? lo[i] : kOut;
-  }
   int c = 1;
   while (c < a.nc && a.comp_l[c] < lmin) ++c;
   int cend = c;
   while (cend < a.nc && a.comp_l[cend] <= lmax) ++cend;
-  double dn[NPL], w[NPL];
+  T vn[NPL];  // next plane, row order
   if (c < cend) {
     const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)c * a.plane, pbytes);
 #pragma unroll
-    for (int i = 0; i < NPL; ++i) dn[i] = static_cast<double>(bload<T>(r, lo[i], 0));
+    for (int k = 0; k < NPL; ++k) vn[k] = bload<T>(r, ld_ofs(k), 0);
   }
+  double w[NPL];
   int lcur = -1;
   for (; c < cend; ++c) {
     double d[NPL];
+    __syncthreads();  // the previous plane's results have left LDS (and f00 in MODE 3)
 #pragma unroll
-    for (int i = 0; i < NPL; ++i) d[i] = dn[i];
-    if (c + 1 < cend) {  // next plane
+    for (int k = 0; k < NPL; ++k) tr[crow + k * kStepL] = static_cast<double>(vn[k]);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) d[i] = tr[lrow + i];
+    if (c + 1 < cend) {  // next plane in flight while this one is solved
       const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)(c + 1) * a.plane, pbytes);
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) dn[i] = static_cast<double>(bload<T>(r, lo[i], 0));
+      for (int k = 0; k < NPL; ++k) vn[k] = bload<T>(r, ld_ofs(k), 0);
     }
     const int l = a.comp_l[c];
     if (l != lcur) {
@@ -1836,10 +1863,14 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     double xv = seg_next<SEGW>(de, 0.0);
 #pragma unroll
     for (int i = NPL - 1; i >= 0; --i) { xv = d[i] - (Csup[i] * w[i]) * xv; d[i] = xv; }
+    __syncthreads();  // every lane has read its rows of this plane
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) tr[lrow + i] = d[i];
+    __syncthreads();
     {
       const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)c * a.plane, pbytes);
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) bstore<T>(static_cast<T>(d[i]), r, so[i], 0);
+      for (int k = 0; k < NPL; ++k) bstore<T>(static_cast<T>(tr[crow + k * kStepL]), r, st_ofs(k), 0);
     }
   }
 }
