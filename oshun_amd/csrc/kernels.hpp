@@ -395,6 +395,21 @@ __device__ __forceinline__ void bstore<float>(float v, __amdgpu_buffer_rsrc_t r,
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
 }
 
+// value of lane-1 (wave_shr:1) or lane+1 (wave_shl:1) across the whole
+// wavefront; the lane without a source (0, resp. 63) keeps `fill`
+constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ double wave_shift(double v, double fill) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float wave_shift(float v, float fill) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill), __builtin_bit_cast(int, v),
+                                                               CTRL, 0xF, 0xF, false));
+}
+
 template <typename T, int L0, int R, int PAIR>
 __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int tile) {
   constexpr int NL = L0 + 1;
@@ -425,28 +440,64 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
   // come from HBM: the tile's inputs are out of L2 by then)
   constexpr bool kKeepY0 = true;
   T accr[R][NL], y0r[R][NL];
-  // ---- first stage on rows p0-1 .. p0+TP, columns x0-1 .. x0+kPairX
+  // ---- first stage on rows p0-1 .. p0+TP, columns x0-1 .. x0+kPairX.  Wave ty
+  // marches over rows s = ty*R .. ty*R+R-1 with rows p-1, p, p+1 in registers;
+  // the x neighbours come from the adjacent lanes, except for the outer
+  // columns of lanes 0 and 63 (one load), and the next row is in flight while
+  // the current one is evaluated.
+  constexpr int kOut = 0x7ffffff0;  // offset outside the buffer: the load returns 0
+  const int xedge = (lane == 0) ? xm : xp;
+  auto ld_row = [&](int q, T (&v)[NL]) {
+    const int o = (q >= 0 && q < np) ? (q * nx + xe) * SZ : kOut;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) v[l] = bload<T>(rin, o, l * pb);
+  };
+  auto ld_edge = [&](int q, T (&v)[NL]) {
+    const int o = (q >= 0 && q < np && (lane == 0 || lane == 63)) ? (q * nx + xedge) * SZ : kOut;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) v[l] = bload<T>(rin, o, l * pb);
+  };
+  // stage 3 reads Y0 at every point it evaluates and the accumulator at the tile's own points
+  // (0 elsewhere)
+  auto ld_y0acc = [&](int r, T (&y0)[NL], T (&ac)[NL]) {
+    const int s = ty * R + r, q = p0 - 1 + s;
+    const bool ok = q >= 0 && q < np;
+    const int o = ok ? (q * nx + xe) * SZ : kOut;
+    const int oa = (ok && s >= 1 && s <= TP && inner) ? o : kOut;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      y0[l] = bload<T>(ry0, o, l * pb);
+      ac[l] = bload<T>(racc, oa, l * pb);
+    }
+  };
+  const double Ex = a.fin[xe];
+  const int pa = p0 - 1 + ty * R;  // first row of this wave (wave-uniform)
+  T fm[NL], fc[NL], fn[NL], fe[NL], cy0[NL], cac[NL];
+  ld_row(pa - 1, fm);
+  ld_row(pa, fc);
+  ld_row(pa + 1, fn);
+  ld_edge(pa, fe);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int s = ty + 4 * r;
-    const int p = p0 - 1 + s;  // wave-uniform
-    const bool mid = s >= 1 && s <= TP && inner;  // a point of the tile itself
+    const int s = ty * R + r;
+    const int p = pa + r;  // wave-uniform
+    // issued ahead of the next row's loads, so that waiting for them leaves those in flight
+    if (PAIR == 1) ld_y0acc(r, cy0, cac);
+    T nn[NL], ne[NL];
+    if (r + 1 < R) {
+      ld_row(p + 2, nn);
+      ld_edge(p + 1, ne);
+    }
     if (p >= 0 && p < np) {
-      const int row = p * nx;
-      const int vc = (row + xe) * SZ, vl = (row + xm) * SZ, vr = (row + xp) * SZ;
-      const int vm = (p > 0) ? vc - nx * SZ : vc;        // row p-1 (unused at p = 0)
-      const int vn = (p < np - 1) ? vc + nx * SZ : vc;   // row p+1; f(N-1) itself at pmax
-      T fm[NL], fc[NL], fn[NL], fl[NL], fr[NL];
+      T fnv[NL], fl[NL], fr[NL];
 #pragma unroll
       for (int l = 0; l < NL; ++l) {
-        fc[l] = bload<T>(rin, vc, l * pb);
-        fm[l] = bload<T>(rin, vm, l * pb);
-        fn[l] = bload<T>(rin, vn, l * pb);
-        fl[l] = bload<T>(rin, vl, l * pb);
-        fr[l] = bload<T>(rin, vr, l * pb);
+        fnv[l] = (p < np - 1) ? fn[l] : fc[l];  // f(N-1) itself at pmax; fm is unused at p = 0
+        fl[l] = wave_shift<kWaveShr1>(fc[l], fe[l]);
+        fr[l] = wave_shift<kWaveShl1>(fc[l], fe[l]);
       }
       T kv[NL];
-      rhs_point_1d<T, NL>(a, p, a.fin[xe], a.vtab[p], a.bgG ? a.bgG[p] : T(0), fm, fc, fn, fl, fr, kv);
+      rhs_point_1d<T, NL>(a, p, Ex, a.vtab[p], a.bgG ? a.bgG[p] : T(0), fm, fc, fnv, fl, fr, kv);
 #pragma unroll
       for (int l = 0; l < NL; ++l) {
         T y;
@@ -455,10 +506,10 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
           y = fc[l] + kh;
           accr[r][l] = fc[l] + kh * a.third;
           y0r[r][l] = fc[l];
-        } else {          // stage 3: Y3 = Y0 + h k3, acc += h/3 k3
+        } else {          // stage 3: Y3 = Y0 + h k3, acc += h/3 k3 (0 outside the tile)
           const T kh = kv[l] * a.hh;
-          y = bload<T>(ry0, vc, l * pb) + kh;
-          accr[r][l] = mid ? bload<T>(racc, vc, l * pb) + kh * a.third : T(0);
+          y = cy0[l] + kh;
+          accr[r][l] = cac[l] + kh * a.third;
           y0r[r][l] = T(0);
         }
         ys[s][l][lane] = y;
@@ -471,13 +522,19 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
         y0r[r][l] = T(0);
       }
     }
+    if (r + 1 < R) {
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        fm[l] = fc[l]; fc[l] = fn[l]; fn[l] = nn[l]; fe[l] = ne[l];
+      }
+    }
   }
   __syncthreads();
   // ---- second stage on the tile, stencil from LDS
   double jin = 0.0, jout = 0.0;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int s = ty + 4 * r;
+    const int s = ty * R + r;
     const int p = p0 - 1 + s;  // wave-uniform
     if (s >= 1 && s <= TP && p < np && inner) {
       // rows outside 0..np-1 hold zeros and are not read: p = 0 uses no f(p-1),
