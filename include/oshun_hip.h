@@ -115,6 +115,10 @@ int oshun_get_stream(const oshun_ctx* ctx, void** hip_stream);
  * planes.  Plane layout [y][p][x] (x fastest); see oshun_plane_info.  In
  * OSHUN_FP32_SPLIT plane 0 is f00 minus the background (see above). */
 int oshun_device_state(oshun_ctx* ctx, void** f_dev, double** fld_dev);
+/* Copies the device state to (into_ctx = 0) or from (into_ctx = 1) caller
+ * device buffers of the same layout (f in the storage precision); either
+ * pointer may be NULL.  Synchronises the context's stream. */
+int oshun_copy_device_state(oshun_ctx* ctx, void* f_dev, double* fld_dev, int into_ctx);
 /* ncomp real planes; comp_l/comp_m/comp_im describe each plane (arrays of
  * length ncomp, may be NULL); plane_elems = ny*np*nx. */
 int oshun_plane_info(const oshun_ctx* ctx, int* ncomp, long long* plane_elems, int* comp_l, int* comp_m,
@@ -152,6 +156,30 @@ int oshun_batch_synchronize(oshun_batch* batch);
  * kernels on the current state (restored afterwards) and keeps the fastest;
  * reports its p rows per tile and ms per step. */
 int oshun_batch_autotune(oshun_batch* batch, int steps, int* chosen_rows, double* ms_per_step);
+
+/* ---- Adjoint of the step -------------------------------------------------
+ * Reverse-mode derivative (vector-Jacobian product) of oshun_step: RK4 of the
+ * Vlasov-Maxwell right-hand side (Runge_Kutta_4::advance_p1), then the e-e
+ * sub-cycles (Explicit_EC_FP::loop) and the implicit l >= 1 solves
+ * (Euler_Backward::advance_1/advance_lm), each transposed by hand.  No
+ * counterpart in the reference.  1D runs with m0 = 0 and 1 <= l0 <= 8 in
+ * OSHUN_FP64 only (other contexts get OSHUN_ERR_ARG).
+ * Cotangents are DEVICE buffers in the device-state layout (oshun_device_state):
+ * lam_f = ncomp planes of np*nx doubles, lam_fld = 6*nx doubles.
+ *
+ * oshun_adjoint_step: the context holds the step's input y_n (unchanged on
+ * return); on entry lam_* is the cotangent of y_{n+1} = step(y_n, h), on
+ * return that of y_n. */
+int oshun_adjoint_step(oshun_ctx* ctx, double h, double* lam_f, double* lam_fld);
+/* Checkpointed runs: oshun_adjoint_forward advances nsteps steps of size h
+ * from the current state, keeping a copy of the state every `every` steps on
+ * the device; oshun_adjoint_backward then turns the cotangent of the final
+ * state into that of the initial state, recomputing each segment's steps from
+ * its checkpoint (the context ends at the final state again). */
+int oshun_adjoint_forward(oshun_ctx* ctx, double h, int nsteps, int every);
+int oshun_adjoint_backward(oshun_ctx* ctx, double* lam_f, double* lam_fld);
+/* frees the adjoint work buffers and checkpoints (also done by oshun_destroy) */
+int oshun_adjoint_release(oshun_ctx* ctx);
 
 #ifdef __cplusplus
 }

@@ -119,6 +119,11 @@ def _load():
     lib.oshun_batch_step.argtypes = [P, I]
     lib.oshun_batch_synchronize.argtypes = [P]
     lib.oshun_batch_autotune.argtypes = [P, I, ctypes.POINTER(I), dp]
+    lib.oshun_copy_device_state.argtypes = [P, P, P, I]
+    lib.oshun_adjoint_step.argtypes = [P, D, P, P]
+    lib.oshun_adjoint_forward.argtypes = [P, D, I, I]
+    lib.oshun_adjoint_backward.argtypes = [P, P, P]
+    lib.oshun_adjoint_release.argtypes = [P]
     _lib = lib
     return lib
 
@@ -186,6 +191,13 @@ class Context:
 
     def fld_shape(self, nbc=0):
         return (6, self.ny + 2 * nbc, self.nx + 2 * nbc)
+
+    def device_shape(self):
+        """Shape of the device-state planes: (ncomp, ny, np, nx), x fastest."""
+        return (self.plane_info()["ncomp"], self.ny, self.np, self.nx)
+
+    def fld_device_shape(self):
+        return (6, self.ny, self.nx)
 
     def upload(self, f, fld, nbc=0):
         f = np.ascontiguousarray(f, np.complex128).reshape(self.f_shape(nbc))
@@ -263,6 +275,22 @@ class Context:
         out = np.zeros((7, self.ny, self.nx))
         self._chk(self._lib.oshun_moments(self._h, _dptr(out)))
         return out
+
+    # ---- device-resident state and the adjoint (device pointers, e.g. torch data_ptr())
+    def copy_device_state(self, f_ptr, fld_ptr, into_ctx):
+        """Copy the device state to (into_ctx False) or from (True) device buffers laid
+        out as oshun_device_state: f = ncomp planes of ny*np*nx, fields 6*nx*ny."""
+        self._chk(self._lib.oshun_copy_device_state(self._h, f_ptr, fld_ptr, 1 if into_ctx else 0))
+
+    def adjoint_step(self, h, lam_f_ptr, lam_fld_ptr):
+        """Cotangent of the step output -> of its input, in place (device pointers)."""
+        self._chk(self._lib.oshun_adjoint_step(self._h, float(h), lam_f_ptr, lam_fld_ptr))
+
+    def adjoint_forward(self, h, nsteps, every):
+        self._chk(self._lib.oshun_adjoint_forward(self._h, float(h), int(nsteps), int(every)))
+
+    def adjoint_backward(self, lam_f_ptr, lam_fld_ptr):
+        self._chk(self._lib.oshun_adjoint_backward(self._h, lam_f_ptr, lam_fld_ptr))
 
     def diag_copy(self, nbytes):
         """Plain device copy of nbytes between scratch buffers (counter calibration)."""

@@ -863,6 +863,7 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
 int oshun_destroy(oshun_ctx* ctx) {
   if (!ctx) return OSHUN_ERR_ARG;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  oshun_adjoint_release(ctx);
   free_all(ctx);
   delete ctx;
   return OSHUN_OK;
@@ -1007,6 +1008,21 @@ int oshun_device_state(oshun_ctx* c, void** f_dev, double** fld_dev) {
   return OSHUN_OK;
 }
 
+int oshun_copy_device_state(oshun_ctx* c, void* f_dev, double* fld_dev, int into_ctx) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  const size_t fb = (size_t)c->nc * c->plane * c->esz, gb = (size_t)6 * c->nxy * sizeof(double);
+  if (f_dev) {
+    if (into_ctx) HIPCHK(c, hipMemcpyAsync(c->S, f_dev, fb, hipMemcpyDeviceToDevice, c->stream));
+    else HIPCHK(c, hipMemcpyAsync(f_dev, c->S, fb, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (fld_dev) {
+    if (into_ctx) HIPCHK(c, hipMemcpyAsync(c->Sf, fld_dev, gb, hipMemcpyDeviceToDevice, c->stream));
+    else HIPCHK(c, hipMemcpyAsync(fld_dev, c->Sf, gb, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return OSHUN_OK;
+}
+
 int oshun_plane_info(const oshun_ctx* c, int* ncomp, long long* plane_elems, int* comp_l, int* comp_m,
                      int* comp_im) {
   if (!sane(c)) return OSHUN_ERR_ARG;
@@ -1075,3 +1091,5 @@ int oshun_diag_copy(oshun_ctx* c, long long nbytes) {
 
 // batched runs (oshun_batch_*)
 #include "batch.hpp"
+// adjoint of the step (oshun_adjoint_*)
+#include "adjoint.hpp"
