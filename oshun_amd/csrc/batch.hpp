@@ -116,7 +116,7 @@ int record_step(oshun_batch* b, int q) {
     const int npt = (c->np + TP - 1) / TP;
     const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
     const int fblocks = (c->nxy + oshun::kFieldX - 1) / oshun::kFieldX;
-    oshun::FieldArgs<T> fa0 = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, S, h, c->jp0, c->npb0, nullptr);
+    oshun::FieldArgs<T> fa0 = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, S, h, nullptr, 0, nullptr);
     fa0.p3og = c->d_p3og;  // current of Y0 summed directly over p
     f0.add(i, fa0, fblocks);
     p0[c->T.l0].add(i, pair_args<T>(c, S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h), ntx * npt);

@@ -593,50 +593,6 @@ __global__ __launch_bounds__(256) void vlasov_pair_1d_batch(const Stage1DArgs<T>
   pair_body<T, L0, R, PAIR>(args[r], g - ts[r]);
 }
 
-// Current::Jxyz integrand of f10 summed over 64-row blocks of p: [pb][3][nx], component 0
-template <typename T>
-struct CurArgs {
-  const T* f10;
-  const double* p3og;
-  double* jp;
-  int np, nx;
-};
-template <typename T>
-__device__ __forceinline__ void current_body(const CurArgs<T>& a, const int tile) {
-  __shared__ double sh[4][64];
-  const int lane = threadIdx.x, ty = threadIdx.y;
-  const int nbx = (a.nx + 63) / 64;
-  const int bx = tile % nbx, by = tile / nbx;
-  const int np = a.np, nx = a.nx;
-  const int x = bx * 64 + lane;
-  double s = 0.0;
-  if (x < nx) {
-#pragma unroll 4
-    for (int i = 0; i < 16; ++i) {
-      const int p = by * 64 + ty + 4 * i;
-      if (p < np) {
-        const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
-        s += w * (static_cast<double>(a.f10[(long long)p * nx + x]) * a.p3og[p]);
-      }
-    }
-  }
-  sh[ty][lane] = s;
-  __syncthreads();
-  if (ty == 0 && x < nx)
-    a.jp[((long long)by * 3) * nx + x] = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
-}
-template <typename T>
-__global__ __launch_bounds__(256) void current_1d(CurArgs<T> a) {
-  current_body<T>(a, blockIdx.x);
-}
-template <typename T>
-__global__ __launch_bounds__(256) void current_1d_batch(const CurArgs<T>* __restrict__ args,
-                                                        const int* __restrict__ ts, int nruns) {
-  const int g = blockIdx.x;
-  const int r = batch_run(ts, nruns, g);
-  current_body<T>(args[r], g - ts[r]);
-}
-
 template <typename T>
 struct FieldArgs {
   const double* fin;

@@ -61,8 +61,6 @@ struct oshun_ctx {
   // 1D m0 = 0: RK4 as two fused stage pairs (vlasov_pair_1d)
   bool use_pair = false;
   int npt = 0;                 // p tiles of the pair kernels
-  int npb0 = 0;                // 64-row blocks of current_1d
-  double* jp0 = nullptr;       // current partials of Y0 [npb0][3][nx]
   double* jp1 = nullptr;       // of the second stage input of a pair [npt][3][nx]
   double* jp2 = nullptr;       // of Y2 [npt][3][nx]
   double* f1y = nullptr;       // f10(p1) of the second stage input, per x
@@ -374,7 +372,7 @@ template <typename T>
 int rk4_pair(oshun_ctx* c, double h) {
   int rc;
   {  // first field stage: the current of Y0 summed directly over p (no partials)
-    oshun::FieldArgs<T> a = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, c->S, h, c->jp0, c->npb0, nullptr);
+    oshun::FieldArgs<T> a = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, c->S, h, nullptr, 0, nullptr);
     a.p3og = c->d_p3og;
     ProfRec r;
     prof_begin(c, 1, &r);
@@ -633,7 +631,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp0, c->jp1, c->jp2, c->f1y};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -713,10 +711,8 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
   if (c->use_pair) {
     c->npt = (c->np + kPairTP - 1) / kPairTP;
     const int npt_max = (c->np + kPairTPMin - 1) / kPairTPMin;  // partial buffers fit every tile height
-    c->npb0 = (c->np + 63) / 64;
     const size_t nx = c->nx;
-    if (hipMalloc((void**)&c->jp0, (size_t)c->npb0 * 3 * nx * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&c->jp1, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
+    if (hipMalloc((void**)&c->jp1, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->jp2, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->f1y, nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->Cf, fldbytes) != hipSuccess)
