@@ -120,6 +120,8 @@ def _load():
         lib.oo_tridiag.argtypes = [I, dp, dp, dp, dp, dp]
         lib.oo_axes.argtypes = [P, dp, dp, dp]
         lib.oo_moments.argtypes = [P, dp]
+        lib.oo_tlm_1d.argtypes = [P, dp, dp, dp, dp, D, I, dp, dp, dp, dp]
+        lib.oo_tlm_1d.restype = I
         _lib = lib
     return _lib
 
@@ -214,6 +216,31 @@ class Oracle:
         pd, pc, vr = np.zeros(n), np.zeros(n), np.zeros(n)
         self._lib.oo_axes(self._h, _dptr(pd), _dptr(pc), _dptr(vr))
         return pd, pc, vr
+
+
+    def tlm_1d(self, f, fld, df=None, dfld=None, h=0.1, nsteps=1):
+        """Tangent-linear model of nsteps steps from t = 0 of a 1D m0 = 0 run
+        (Step1D in oshun_oracle.cpp).  f, df: real [l0+1, nx, np]; fld, dfld: real
+        [6, nx].  Returns (f_N, fld_N) and, when df is given, (df_N, dfld_N) too."""
+        c = self.cfg
+        nf, ng = (c.l0 + 1, c.nx, c.np), (6, c.nx)
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(nf)
+        fld = np.ascontiguousarray(fld, dtype=np.float64).reshape(ng)
+        fo, go = np.zeros(nf), np.zeros(ng)
+        if df is None:
+            rc = self._lib.oo_tlm_1d(self._h, _dptr(f), _dptr(fld), None, None, float(h), int(nsteps),
+                                     _dptr(fo), _dptr(go), None, None)
+            if rc:
+                raise ValueError("tlm_1d: 1D runs with m0 = 0 only")
+            return fo, go
+        df = np.ascontiguousarray(df, dtype=np.float64).reshape(nf)
+        dfld = np.ascontiguousarray(dfld, dtype=np.float64).reshape(ng)
+        dfo, dgo = np.zeros(nf), np.zeros(ng)
+        rc = self._lib.oo_tlm_1d(self._h, _dptr(f), _dptr(fld), _dptr(df), _dptr(dfld), float(h), int(nsteps),
+                                 _dptr(fo), _dptr(go), _dptr(dfo), _dptr(dgo))
+        if rc:
+            raise ValueError("tlm_1d: 1D runs with m0 = 0 only")
+        return fo, go, dfo, dgo
 
 
 def tridiag(a, b, c, d):

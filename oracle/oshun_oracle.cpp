@@ -22,6 +22,9 @@
 //       Anisotropic_Collisions/Euler_Backward def-FokkerPlanck.cpp:1049-1304
 //       TridiagonalSolve/Thomas_Tridiagonal def-Nmethods.cpp:142-207
 //   * per-step ordering                     main.cpp:79-107
+//   * a tangent-linear model of that step for 1D runs with m0 = 0
+//     (Step1D<R> below, R = double or a forward-mode dual number): the same
+//     formulas written on real scalars, used to check the GPU adjoint
 //
 // Differences from the reference that are deliberate:
 //   * periodic indexing on the interior instead of guard cells (the reference
@@ -885,6 +888,305 @@ struct Oracle {
   }
 };
 
+
+// ===================================================================== TLM ==
+// Forward-mode dual number: value and directional derivative.
+struct Dual {
+  double v, d;
+  Dual(double a = 0.0, double b = 0.0) : v(a), d(b) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) { return {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)}; }
+inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+inline Dual& operator-=(Dual& a, Dual b) { return a = a - b; }
+inline Dual& operator*=(Dual& a, Dual b) { return a = a * b; }
+inline Dual& operator/=(Dual& a, Dual b) { return a = a / b; }
+inline Dual lg(Dual a) { return {std::log(a.v), a.d / a.v}; }
+inline Dual sq(Dual a) { double s = std::sqrt(a.v); return {s, a.d / (2.0 * s)}; }
+inline double lg(double a) { return std::log(a); }
+inline double sq(double a) { return std::sqrt(a); }
+inline double val(double a) { return a; }
+inline double val(Dual a) { return a.v; }
+
+// One oshun step (main.cpp:79-107) of a 1D run with m0 = 0 on real scalars R:
+// the oracle's formulas above restricted to m = 0 (every coupling to m >= 1 and
+// every y term vanishes).  State: f[l][x][p], fields[6][x].
+template <class R>
+struct Step1D {
+  const Oracle& o;
+  int L, np, nx;
+  explicit Step1D(const Oracle& q) : o(q), L(q.c.l0), np(q.np), nx(q.nx) {}
+  size_t ix(int l, int x, int p) const { return ((size_t)l * nx + x) * np + p; }
+
+  // Actions::Adv (def-Actions.cpp:73-94) for m0 = 0, 1D
+  void adv(const std::vector<R>& f, const std::vector<R>& fld, std::vector<R>& kf, std::vector<R>& kfld) const {
+    const int N = np;
+    kf.assign(f.size(), R(0.0));
+    kfld.assign(fld.size(), R(0.0));
+    std::vector<R> G(N), Hh(N), D(N);
+    const double dpc = o.dpc.real();
+    for (int x = 0; x < nx; ++x) {
+      const R Ex = fld[x];
+      for (int l = 0; l <= L; ++l) {  // ElectricField::Exyz for m = 0 (def-VlasovMax.cpp:346-491)
+        const R* fc = &f[ix(l, x, 0)];
+        D[0] = R(0.0);
+        for (int p = 1; p < N - 1; ++p) D[p] = fc[p - 1] - fc[p + 1];
+        D[N - 1] = fc[N - 2] - fc[N - 1];
+        if (l == 0) {  // G00 (:519-535)
+          const double p0p1_sq = (o.pc[0] * o.pc[0] / (o.pc[1] * o.pc[1])).real();
+          const double inv = 1.0 / (1.0 - p0p1_sq);
+          const double g_r = (-4.0 * o.dpc * o.pc[0] / (o.pc[1] * o.pc[1])).real();
+          for (int p = 0; p < N; ++p) G[p] = D[p];
+          const R f00 = (fc[0] - fc[1] * p0p1_sq) * inv;
+          G[0] = (fc[1] - f00) * g_r;
+        } else {  // MakeGH (:496-515)
+          const double sg = (-2.0 * (l + 1.0)) * dpc, cg = -(2.0 * l + 1.0) / l;
+          for (int p = 0; p < N; ++p) {
+            const double invpax = o.invpr[p].real() * sg;
+            Hh[p] = fc[p] * invpax + D[p];
+            G[p] = D[p] * cg + Hh[p];
+          }
+          G[0] = R(0.0);
+          Hh[0] = fc[1] * o.Hp0[l].real();
+        }
+        if (l >= 1) {  // H of l feeds l-1 with A2(l,0)
+          const R mult = Ex * o.eA2[o.l2i(l, 0)].real();
+          for (int p = 0; p < N; ++p) kf[ix(l - 1, x, p)] += Hh[p] * mult;
+        }
+        if (l + 1 <= L) {  // G of l feeds l+1 with A1(l,0)
+          const R mult = Ex * o.eA1[o.l2i(l, 0)].real();
+          for (int p = 0; p < N; ++p) kf[ix(l + 1, x, p)] += G[p] * mult;
+        }
+      }
+      {  // Current::Jxyz into Ex (:680-745)
+        const R* f1 = &f[ix(1, x, 0)];
+        R j = f1[0] * o.p3og[0].real();
+        j += f1[N - 1] * o.p3og[N - 1].real();
+        j *= 0.5;
+        for (int p = 1; p < N - 1; ++p) j += f1[p] * o.p3og[p].real();
+        j *= o.Delta_p.real();
+        j += o.small.real() * (f1[1] * o.p3og[1].real());
+        j *= 4.0 * M_PI / 3.0;
+        kfld[x] += j;
+      }
+      // Ampere and Faraday, x terms (:979-1035)
+      const double mx = o.midx.real();
+      const int xm = o.xm(x), xp = o.xp(x);
+      auto F = [&](int c, int xx) { return fld[(size_t)c * nx + xx]; };
+      kfld[(size_t)nx + x] += F(5, xm) * (-mx) - F(5, xp) * (-mx);
+      kfld[(size_t)2 * nx + x] += F(4, xm) * mx - F(4, xp) * mx;
+      kfld[(size_t)4 * nx + x] += F(2, xm) * mx - F(2, xp) * mx;
+      kfld[(size_t)5 * nx + x] += F(1, xm) * (-mx) - F(1, xp) * (-mx);
+    }
+    // Spatial_Advection::Axy, x terms (:842-853)
+    for (int l = 0; l <= L; ++l)
+      for (int x = 0; x < nx; ++x) {
+        const int xm = o.xm(x), xp = o.xp(x);
+        for (int p = 0; p < N; ++p) {
+          const R df = f[ix(l, xm, p)] - f[ix(l, xp, p)];
+          if (l >= 1) kf[ix(l - 1, x, p)] += df * (o.vr[p] * o.aA2[o.l2i(l, 0)]).real();
+          if (l + 1 <= L) kf[ix(l + 1, x, p)] += df * (o.vr[p] * o.aA1[o.l2i(l, 0)]).real();
+        }
+      }
+    // CLEAN_LOW_PC (def-Actions.cpp:240-253)
+    for (int l = 1; l <= L; ++l)
+      for (int x = 0; x < nx; ++x)
+        for (int p = 0; p < std::min(l - 1, 80); ++p) kf[ix(l, x, p)] = R(0.0);
+  }
+
+  // Runge_Kutta_4::advance_p1 (def-RK.cpp:110-145)
+  void rk4(std::vector<R>& f, std::vector<R>& fld, double h) const {
+    std::vector<R> f0 = f, g0 = fld, f1 = f, g1 = fld, kf, kg;
+    auto axpy = [](std::vector<R>& y, const std::vector<R>& k, double a, double b, std::vector<R>* acc) {
+      for (size_t i = 0; i < y.size(); ++i) {
+        const R kh = k[i] * a;
+        y[i] += kh;
+        if (acc) (*acc)[i] += kh * b;
+      }
+    };
+    adv(f1, g1, kf, kg);
+    axpy(f1, kf, 0.5 * h, 1.0 / 3.0, &f); axpy(g1, kg, 0.5 * h, 1.0 / 3.0, &fld);
+    adv(f1, g1, kf, kg);
+    f1 = f0; g1 = g0;
+    axpy(f1, kf, 0.5 * h, 2.0 / 3.0, &f); axpy(g1, kg, 0.5 * h, 2.0 / 3.0, &fld);
+    adv(f1, g1, kf, kg);
+    axpy(f0, kf, h, 1.0 / 3.0, &f); axpy(g0, kg, h, 1.0 / 3.0, &fld);
+    adv(f0, g0, kf, kg);
+    for (size_t i = 0; i < f.size(); ++i) f[i] += kf[i] * (h / 6.0);
+    for (size_t i = 0; i < fld.size(); ++i) fld[i] += kg[i] * (h / 6.0);
+  }
+
+  // LOGee / ZLOGei (def-FokkerPlanck.cpp:41-95)
+  R logee(R ne, R Te) const {
+    if (val(ne) > 0.000000001) {
+      Te /= (3.0 * ne);
+      Te *= 511000.0;
+      ne *= o.c.density_np;
+      Te = lg(Te);
+      ne = lg(ne);
+      R lnee = 23.5 - 0.5 * ne + 1.25 * Te - sq(0.00001 + 0.0625 * (Te - 2.0) * (Te - 2.0));
+      if (val(lnee) > 2.0) return lnee;
+    }
+    return R(2.0);
+  }
+  R zlogei(R ne, R Te) const {
+    if (val(ne) > 0.0000001) {
+      Te /= (3.0 * ne);
+      Te *= 511000.0;
+      ne *= o.c.density_np;
+      Te = lg(Te);
+      ne = lg(ne);
+      R lnei = 24.0 - 0.5 * ne + Te;
+      if (val(lnei) > 2.0) return lnei * o.c.zeta;
+    }
+    return R(2.0 * o.c.zeta);
+  }
+
+  // EE_isotropic_conserva::Getslope with G (def-FokkerPlanck.cpp:178-267)
+  void getslope(const R* fin, R* fh) const {
+    const int N = np;
+    const std::vector<double>& v = o.vr_c;
+    std::vector<R> I4(N), I2(N), J1(N);
+    I4[0] = R(0.0);
+    for (int k = 1; k < N; ++k) { I4[k] = o.U4[k] * fin[k] + o.U4m1[k] * fin[k - 1]; I4[k] += I4[k - 1]; }
+    I2[0] = R(0.0);
+    for (int k = 1; k < N; ++k) { I2[k] = o.U2[k] * fin[k] + o.U2m1[k] * fin[k - 1]; I2[k] += I2[k - 1]; }
+    J1[N - 1] = R(0.0);
+    for (int k = N - 2; k > -1; --k) { J1[k] = o.U1[k + 1] * fin[k + 1] + o.U1m1[k + 1] * fin[k]; J1[k] += J1[k + 1]; }
+    const R Ln_ee = logee(4.0 * M_PI * I2[N - 1], 4.0 * M_PI * I4[N - 1]);
+    std::vector<R> G(N);
+    for (int k = 0; k < N; ++k) G[k] = (I4[k] + o.U3[k] * J1[k]) * fin[k] + (I2[k] * J1[k]) * (-3.0);
+    const R f00 = (fin[0] - fin[1] * (v[0] * v[0]) / (v[1] * v[1])) / (1.0 - (v[0] * v[0]) / (v[1] * v[1]));
+    for (int k = 0; k < o.c.nb && k < N; ++k) {
+      const R i2s = f00 * std::pow(v[k], 3) / 3.0 + (fin[1] - f00) * (std::pow(v[k], 5) / (v[1] * v[1]) * 0.2);
+      const R i4s = f00 * std::pow(v[k], 5) * 0.2 + (fin[1] - f00) * (std::pow(v[k], 7) / (v[1] * v[1] * 7.0));
+      G[k] = fin[k] * i4s + (std::pow(v[k], 3) * fin[k] - 3.0 * i2s) * J1[k];
+    }
+    R g0 = (-1.0) * G[0];
+    for (int k = 0; k < N - 1; ++k) G[k] -= G[k + 1];
+    g0 *= 2.0 / (v[0] * v[0]);
+    for (int k = 0; k < N - 1; ++k) G[k] *= o.Pn[k];
+    g0 -= G[0];
+    for (int k = 0; k < N - 1; ++k) G[k] -= G[k + 1];
+    g0 *= o.Qn[0];
+    fh[0] = g0;
+    for (int k = 0; k < N - 1; ++k) fh[k + 1] = o.Qn[k + 1] * G[k];
+    for (int k = 0; k < N; ++k) fh[k] *= o.c_kpre * Ln_ee;
+  }
+  // RungeKutta4_ee::advance (def-FokkerPlanck.cpp:518-553)
+  void rk4_ee(R* f, double h) const {
+    const int N = np;
+    std::vector<R> f0(f, f + N), f1(f, f + N), fh(N);
+    getslope(f1.data(), fh.data());
+    for (int k = 0; k < N; ++k) { fh[k] *= (0.5 * h); f1[k] += fh[k]; }
+    for (int k = 0; k < N; ++k) { fh[k] *= (1.0 / 3.0); f[k] += fh[k]; }
+    getslope(f1.data(), fh.data());
+    f1 = f0;
+    for (int k = 0; k < N; ++k) { fh[k] *= (0.5 * h); f1[k] += fh[k]; }
+    for (int k = 0; k < N; ++k) { fh[k] *= (2.0 / 3.0); f[k] += fh[k]; }
+    getslope(f1.data(), fh.data());
+    for (int k = 0; k < N; ++k) { fh[k] *= h; f0[k] += fh[k]; }
+    for (int k = 0; k < N; ++k) { fh[k] *= (1.0 / 3.0); f[k] += fh[k]; }
+    getslope(f0.data(), fh.data());
+    for (int k = 0; k < N; ++k) { fh[k] *= (h / 6.0); f[k] += fh[k]; }
+  }
+  // reset_coeff + advance + TridiagonalSolve (def-FokkerPlanck.cpp:808-1035, def-Nmethods.cpp:142-167)
+  void implicit(const R* fin, double Dt, int lmin, int lmax, std::vector<R>& f, int x) const {
+    const int N = np;
+    const std::vector<double>& v = o.vr_c;
+    std::vector<R> I2(N), I0(N), J1m(N);
+    I2[0] = R(0.0);
+    for (int q = 1; q < N; ++q) { I2[q] = o.U4[q] * fin[q] + o.U4m1[q] * fin[q - 1]; I2[q] += I2[q - 1]; }
+    for (int q = 0; q < N; ++q) I2[q] *= 4.0 * M_PI;
+    const R I2_temperature = I2[N - 1];
+    for (int q = 0; q < N; ++q) I2[q] /= v[q] * v[q];
+    I0[0] = R(0.0);
+    for (int q = 1; q < N; ++q) { I0[q] = o.U2[q] * fin[q] + o.U2m1[q] * fin[q - 1]; I0[q] += I0[q - 1]; }
+    for (int q = 0; q < N; ++q) I0[q] *= 4.0 * M_PI;
+    const R I0_density = I0[N - 1];
+    J1m[N - 1] = R(0.0);
+    for (int q = N - 2; q > -1; --q) { J1m[q] = o.U1[q + 1] * fin[q + 1] + o.U1m1[q + 1] * fin[q]; J1m[q] += J1m[q + 1]; }
+    for (int q = 0; q < N; ++q) J1m[q] *= 4.0 * M_PI * v[q];
+    const R lnee = logee(I0_density, I2_temperature);
+    const R zlnei = zlogei(I0_density, I2_temperature);
+    std::vector<R> T1(N), T2(N), sc(N), sub(N), dia(N), sup(N);
+    for (int i = 0; i < N; ++i) {
+      T1[i] = I0[i] + (2.0 * J1m[i] - I2[i]) / 3.0;
+      T2[i] = (I2[i] + J1m[i]) / 3.0;
+    }
+    const R zi = zlnei * I0_density;
+    const double kd = o.kpre * Dt;
+    for (int i = 0; i < N; ++i) {
+      R s_ = (i == 0) ? R(0.0) : T1[i] * lnee;
+      s_ += zi;
+      s_ /= std::pow(v[i], 3);
+      sc[i] = s_ * kd;
+    }
+    dia[0] = 8.0 * M_PI * fin[0];
+    for (int i = 1; i < N - 1; ++i) {
+      const double IvDnDm1 = 1.0 / (v[i] * 0.5 * (v[i + 1] - v[i - 1]) * (v[i] - v[i - 1]));
+      const double IvDnDp1 = 1.0 / (v[i] * 0.5 * (v[i + 1] - v[i - 1]) * (v[i + 1] - v[i]));
+      const double Ivsq2Dn = 1.0 / (v[i] * v[i] * (v[i + 1] - v[i - 1]));
+      dia[i] = 8.0 * M_PI * fin[i] - T2[i] * (IvDnDm1 + IvDnDp1);
+      sub[i] = T2[i] * IvDnDm1 - T1[i] * Ivsq2Dn;
+      sup[i] = T2[i] * IvDnDp1 + T1[i] * Ivsq2Dn;
+    }
+    const R s = (-1.0) * lnee * o.kpre * Dt;
+    for (int i = 0; i < N; ++i) { sub[i] *= s; dia[i] *= s; sup[i] *= s; }
+    for (int l = lmin; l <= lmax; ++l) {
+      std::vector<R> a(N, R(0.0)), b(N), cc(N, R(0.0)), d(N), xx(N);
+      for (int i = 1; i < N; ++i) a[i] = sub[i];
+      for (int i = 0; i < N; ++i) b[i] = dia[i];
+      for (int i = 0; i < N - 1; ++i) cc[i] = sup[i];
+      if (l > 1) b[0] = R(0.0);
+      double ll1 = static_cast<double>(l);
+      ll1 *= (-0.5) * (ll1 + 1.0);
+      for (int i = 0; i < N; ++i) b[i] += 1.0 - ll1 * sc[i];
+      for (int i = 0; i < N; ++i) d[i] = f[ix(l, x, i)];
+      cc[0] /= b[0];
+      d[0] /= b[0];
+      for (int i = 1; i < N; ++i) {
+        const R id = 1.0 / (b[i] - cc[i - 1] * a[i]);
+        cc[i] *= id;
+        d[i] -= d[i - 1] * a[i];
+        d[i] *= id;
+      }
+      xx[N - 1] = d[N - 1];
+      for (int i = N - 2; i > -1; --i) { xx[i] = d[i]; xx[i] -= cc[i] * xx[i + 1]; }
+      for (int i = 0; i < N; ++i) f[ix(l, x, i)] = xx[i];
+    }
+  }
+  // main.cpp:79-107 with the clocks of Oracle::step
+  void step(std::vector<R>& f, std::vector<R>& fld, double h, double clk[3]) const {
+    rk4(f, fld, h);
+    clk[0] += h;
+    const double t = clk[0];
+    if (o.c.coll_ee) {
+      int numh = static_cast<int>(static_cast<size_t>(static_cast<int>((t - clk[1]) / o.c.small_dt)) + 1);
+      double hh = (t - clk[1]) / static_cast<double>(static_cast<size_t>(numh));
+      std::vector<R> fc(np);
+      for (int x = 0; x < nx; ++x) {
+        for (int p = 0; p < np; ++p) fc[p] = f[ix(0, x, p)];
+        for (int s = 0; s < numh; ++s) rk4_ee(fc.data(), hh);
+        for (int p = 0; p < np; ++p) f[ix(0, x, p)] = fc[p];
+      }
+    }
+    clk[1] = t;
+    if (o.c.coll_flm) {
+      const double Dt = t - clk[2];
+      for (int x = 0; x < nx; ++x) {
+        std::vector<R> f00(&f[ix(0, x, 0)], &f[ix(0, x, 0)] + np);
+        implicit(f00.data(), Dt, 1, 1, f, x);         // Euler_Backward::advance_1
+        implicit(f00.data(), Dt, 2, L, f, x);         // Euler_Backward::advance_lm
+      }
+    }
+    clk[2] = t;
+  }
+};
+
 void put_state(State& s, const double* f, const double* fld) {
   if (f) std::memcpy((void*)s.f.data(), f, s.f.size() * sizeof(cd));
   if (fld) std::memcpy((void*)s.fld.data(), fld, s.fld.size() * sizeof(cd));
@@ -938,6 +1240,32 @@ void oo_step(void* o, double h, int nsteps) {
   for (int s = 0; s < nsteps; ++s) q->step(h);
 }
 void oo_moments(void* o, double* out) { static_cast<Oracle*>(o)->moments(out); }
+// Tangent-linear model of nsteps steps of a 1D m0 = 0 run from t = 0.
+// f, df: real [l][x][p]; fld, dfld: real [6][x]; outputs the final state and
+// its tangent (df_out/dfld_out may be null: the double instantiation only).
+int oo_tlm_1d(void* o, const double* f, const double* fld, const double* df, const double* dfld, double h,
+              int nsteps, double* f_out, double* fld_out, double* df_out, double* dfld_out) {
+  const Oracle* q = static_cast<Oracle*>(o);
+  if (q->c.m0 != 0 || q->ny != 1 || q->c.l0 < 1) return -1;
+  const size_t nf = (size_t)(q->c.l0 + 1) * q->nx * q->np, ng = (size_t)6 * q->nx;
+  double clk[3] = {0.0, 0.0, 0.0};
+  if (!df_out) {
+    Step1D<double> S(*q);
+    std::vector<double> F(f, f + nf), G(fld, fld + ng);
+    for (int s = 0; s < nsteps; ++s) S.step(F, G, h, clk);
+    std::copy(F.begin(), F.end(), f_out);
+    std::copy(G.begin(), G.end(), fld_out);
+    return 0;
+  }
+  Step1D<Dual> S(*q);
+  std::vector<Dual> F(nf), G(ng);
+  for (size_t i = 0; i < nf; ++i) F[i] = Dual(f[i], df[i]);
+  for (size_t i = 0; i < ng; ++i) G[i] = Dual(fld[i], dfld[i]);
+  for (int s = 0; s < nsteps; ++s) S.step(F, G, h, clk);
+  for (size_t i = 0; i < nf; ++i) { f_out[i] = F[i].v; df_out[i] = F[i].d; }
+  for (size_t i = 0; i < ng; ++i) { fld_out[i] = G[i].v; dfld_out[i] = G[i].d; }
+  return 0;
+}
 // Single-column pieces for closed-form tests.
 void oo_ee_getslope(void* o, const double* fin, double* fh) { static_cast<Oracle*>(o)->getslope(fin, fh); }
 void oo_flm_solve(void* o, const double* f00, double Dt, int el, double* col_re_im) {

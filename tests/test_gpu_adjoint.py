@@ -4,6 +4,9 @@
   only, all on): <adjoint(lam), d> against the central difference
   <lam, (step(y + eps d) - step(y - eps d)) / (2 eps)> for seeded random lam, d,
   <= 1e-8 relative.
+* Against the FP64 tangent-linear model of the oracle (forward-mode dual
+  numbers, tests/test_oracle_tlm.py pins it): <adjoint(lam_N), d_0> =
+  <lam_N, TLM(d_0)> over six checkpointed steps, <= 1e-10 relative.
 * C5, the adjoint sensitivity configuration: Nx = 512, Np = 192, l0 = 3,
   200 RK4 + FP steps checkpointed every 20; objective J = sum_x |q_x| dx at
   the last step; gradient with respect to the 512 initial temperatures T(x)
@@ -215,3 +218,51 @@ def test_adjoint_c5_gradient_vs_finite_differences():
         worst = max(worst, rel)
         print(f"direction {k:2d}: adjoint {ad:+.10e} central difference {fd:+.10e} rel {rel:.2e}", flush=True)
     assert worst <= 1e-6
+
+
+def test_adjoint_matches_the_tangent_linear_oracle():
+    """<adjoint(lam_N), d_0> = <lam_N, TLM(d_0)> over 6 steps (checkpoints every 2),
+    the tangent from the oracle's forward-mode model (Step1D<Dual>), all FP64."""
+    import oshun_amd
+    from oracle.oracle import Config, Oracle
+    torch = _torch()
+    print("case adjoint vs tangent-linear oracle: l0=3 np=48 nx=24, 6 steps", flush=True)
+    l0, np_, nx, h, steps, every = 3, 48, 24, 0.2, 6, 2
+    kw = dict(pmax=0.3, dx=0.4, density_np=1e22, small_dt=0.1)
+    ctx = oshun_amd.Context(l0, 0, np_, nx, 1, nb_algorithms=4, **kw)
+    f, fld = _maxwellian_state(l0, 0, np_, nx, 1, 0.3, seed=91, amp=1e-2)
+    x = np.arange(nx) / nx
+    fld[0] = 2e-3 * np.sin(2 * np.pi * x)[None, :]
+    fld[1] = 1e-3 * np.cos(2 * np.pi * x)[None, :]
+    fld[5] = 5e-4 * np.sin(4 * np.pi * x)[None, :]
+    F, G = _to_dev(ctx, f, fld)
+    ctx.set_time(0.0)
+    ctx.adjoint_forward(h, steps, every)
+    FN = torch.empty_like(F)
+    GN = torch.empty_like(G)
+    ctx.copy_device_state(FN.data_ptr(), GN.data_ptr(), False)
+    gen = np.random.default_rng(2024)
+    f0 = F.cpu().numpy()[:, 0].transpose(0, 2, 1).copy()  # [l][x][p]
+    g0 = G.cpu().numpy()[:, 0].copy()
+    rms = np.sqrt((f0 ** 2).mean(axis=(1, 2)))[:, None, None]
+    lam = gen.standard_normal(f0.shape) / rms
+    lamf = gen.standard_normal(g0.shape) * 1e3
+    d = gen.standard_normal(f0.shape) * np.abs(f0[0])[None] * np.array([1.0] + [1e-2] * l0)[:, None, None]
+    dg = gen.standard_normal(g0.shape) * 1e-3
+    # GPU adjoint of the 6 steps
+    L = torch.as_tensor(lam.transpose(0, 2, 1)[:, None].copy(), device="cuda")
+    Lf = torch.as_tensor(lamf[:, None].copy(), device="cuda")
+    torch.cuda.synchronize()
+    ctx.adjoint_backward(L.data_ptr(), Lf.data_ptr())
+    torch.cuda.synchronize()
+    lam0 = L.cpu().numpy()[:, 0].transpose(0, 2, 1)
+    lamf0 = Lf.cpu().numpy()[:, 0]
+    # oracle tangent-linear model of the same 6 steps
+    o = Oracle(Config(l0=l0, m0=0, np=np_, nx=nx, nb=4, **kw))
+    fo, go, dfo, dgo = o.tlm_1d(f0, g0, d, dg, h=h, nsteps=steps)
+    fgpu = FN.cpu().numpy()[:, 0].transpose(0, 2, 1)
+    assert np.abs(fgpu - fo).max() <= 1e-12 * np.abs(fo).max()
+    lhs = float((lam0 * d).sum() + (lamf0 * dg).sum())
+    rhs = float((lam * dfo).sum() + (lamf * dgo).sum())
+    print(f"<adjoint, d> {lhs:.14e}  <lam, TLM d> {rhs:.14e}  rel {abs(lhs - rhs) / abs(rhs):.2e}")
+    assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
