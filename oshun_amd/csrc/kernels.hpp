@@ -216,12 +216,34 @@ struct Stage1DArgs {
   double* f1A;         // f10(p1) of the second stage's input, per x
 };
 
+// Read-only table access through the constant address space: with a
+// wave-uniform index the load becomes a scalar load even after the kernel's
+// own global stores (which a generic pointer could alias).
+template <typename T>
+__device__ __forceinline__ T ldc(const T* ptr, long long i) {
+  return ((const __attribute__((address_space(4))) T*)ptr)[i];
+}
+
 // Vlasov RHS of every l at one (x,p) of the 1D m0 = 0 system from the values
 // of all planes at (p-1, p, p+1) and (x-1, x+1):
 // Actions::Adv (def-Actions.cpp:73-94) = ElectricField::Exyz + Spatial_Advection::Axy
 // restricted to m = 0, with CLEAN_LOW_PC (def-Actions.cpp:240-253).
 template <typename T, int NL>
-__device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, double Ex, T v, T bgg,
+struct EMul {  // E-field multipliers of one x: T(Ex * eA1[l]), T(Ex * eA2[l]) per source l
+  T e1[NL], e2[NL];
+};
+template <typename T, int NL>
+__device__ __forceinline__ EMul<T, NL> emul(const Stage1DArgs<T>& a, double Ex) {
+  EMul<T, NL> m;
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    m.e1[l] = static_cast<T>(Ex * ldc(a.eA1, l));
+    m.e2[l] = static_cast<T>(Ex * ldc(a.eA2, l));
+  }
+  return m;
+}
+template <typename T, int NL>
+__device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, const EMul<T, NL>& em, T v, T bgg,
                                              const T (&fm)[NL], const T (&fc)[NL], const T (&fn)[NL],
                                              const T (&fl)[NL], const T (&fr)[NL], T (&k)[NL]) {
   const int np = a.np;
@@ -236,7 +258,7 @@ __device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, dou
         Hh[0] = T(0);
       } else {
         G[l] = T(0);
-        Hh[l] = fn[l] * a.hp0[l];                                    // H(p0), :513-514
+        Hh[l] = fn[l] * ldc(a.hp0, l);                                    // H(p0), :513-514
       }
     } else {
       // SHarmonic::Dp, def-State.cpp:359-380: f(p-1) - f(p+1), and f(N-2) - f(N-1)
@@ -247,20 +269,20 @@ __device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, dou
         if (a.bgG) G[0] += bgg;
         Hh[0] = T(0);
       } else {
-        T hv = fc[l] * a.invpax[l * np + p];                           // MakeGH, :504-509
+        T hv = fc[l] * ldc(a.invpax, l * np + p);                           // MakeGH, :504-509
         hv = hv + dpf;
         Hh[l] = hv;
-        G[l] = dpf * a.cg[l] + hv;
+        G[l] = dpf * ldc(a.cg, l) + hv;
       }
     }
   }
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
     T acc = T(0);
-    if (l >= 1) acc += G[l - 1] * static_cast<T>(Ex * a.eA1[l - 1]);
-    if (l + 1 < NL) acc += Hh[l + 1] * static_cast<T>(Ex * a.eA2[l + 1]);
-    if (l >= 1) acc += (fl[l - 1] - fr[l - 1]) * a.xv1[(l - 1) * np + p];  // v * xA1, wave-uniform
-    if (l + 1 < NL) acc += (fl[l + 1] - fr[l + 1]) * a.xv2[(l + 1) * np + p];
+    if (l >= 1) acc += G[l - 1] * em.e1[l - 1];
+    if (l + 1 < NL) acc += Hh[l + 1] * em.e2[l + 1];
+    if (l >= 1) acc += (fl[l - 1] - fr[l - 1]) * ldc(a.xv1, (l - 1) * np + p);  // v * xA1, wave-uniform
+    if (l + 1 < NL) acc += (fl[l + 1] - fr[l + 1]) * ldc(a.xv2, (l + 1) * np + p);
     k[l] = acc;
   }
   if (p < NL - 2) {  // CLEAN_LOW_PC, def-Actions.cpp:240-253 (rows p < min(l-1, 80) of l >= 1)
@@ -309,7 +331,7 @@ __global__ __launch_bounds__(64 * kRows1D) void vlasov_stage_1d(Stage1DArgs<T> a
     }
     const T bgg = a.bgG ? a.bgG[p] : T(0);
     T kv[NL];
-    rhs_point_1d<T, NL>(a, p, a.fin[x], a.vtab[p], bgg, fm, fc, fn, fl, fr, kv);
+    rhs_point_1d<T, NL>(a, p, emul<T, NL>(a, a.fin[x]), a.vtab[p], bgg, fm, fc, fn, fl, fr, kv);
 #pragma unroll
     for (int l = 0; l < NL; ++l) {
       const T acc = kv[l];
@@ -470,7 +492,8 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
       ac[l] = bload<T>(racc, oa, l * pb);
     }
   };
-  const double Ex = a.fin[xe];
+  // the E multipliers are re-formed per row: holding them for the whole tile
+  // costs a wavefront of occupancy (measured slower in FP32 and FP64)
   const int pa = p0 - 1 + ty * R;  // first row of this wave (wave-uniform)
   T fm[NL], fc[NL], fn[NL], fe[NL], cy0[NL], cac[NL];
   ld_row(pa - 1, fm);
@@ -497,7 +520,8 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
         fr[l] = wave_shift<kWaveShl1>(fc[l], fe[l]);
       }
       T kv[NL];
-      rhs_point_1d<T, NL>(a, p, Ex, a.vtab[p], a.bgG ? a.bgG[p] : T(0), fm, fc, fnv, fl, fr, kv);
+      rhs_point_1d<T, NL>(a, p, emul<T, NL>(a, a.fin[xe]), ldc(a.vtab, p), a.bgG ? ldc(a.bgG, p) : T(0), fm, fc,
+                          fnv, fl, fr, kv);
 #pragma unroll
       for (int l = 0; l < NL; ++l) {
         T y;
@@ -549,7 +573,8 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
         fr[l] = ys[s][l][lane + 1];
       }
       T kv[NL];
-      rhs_point_1d<T, NL>(a, p, a.fin2[xe], a.vtab[p], a.bgG ? a.bgG[p] : T(0), fm, fc, fn, fl, fr, kv);
+      rhs_point_1d<T, NL>(a, p, emul<T, NL>(a, a.fin2[xe]), ldc(a.vtab, p), a.bgG ? ldc(a.bgG, p) : T(0), fm, fc,
+                          fn, fl, fr, kv);
       const int vc = (p * nx + xe) * SZ;
       const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
 #pragma unroll
@@ -559,14 +584,14 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
           const T y2 = (kKeepY0 ? y0r[r][l] : bload<T>(rin, vc, l * pb)) + kh;
           bstore<T>(y2, rout, vc, l * pb);
           bstore<T>(accr[r][l] + kh * a.twothird, racc, vc, l * pb);
-          if (l == 1) jout += w * (static_cast<double>(y2) * a.p3og[p]);
+          if (l == 1) jout += w * (static_cast<double>(y2) * ldc(a.p3og, p));
         } else {          // stage 4: Y = acc + h/6 k4
           const T kh = kv[l] * a.h6;
           bstore<T>(accr[r][l] + kh, rout, vc, l * pb);
         }
       }
       if (NL > 1) {
-        jin += w * (static_cast<double>(fc[1]) * a.p3og[p]);
+        jin += w * (static_cast<double>(fc[1]) * ldc(a.p3og, p));
         if (p == 1) a.f1A[xe] = static_cast<double>(fc[1]);
       }
     }
