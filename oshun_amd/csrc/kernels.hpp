@@ -1419,6 +1419,15 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
 // or uniform quantities are multiplications by reciprocals.
 constexpr int kRowP = 256;    // LDS table length: SEGW lanes x NPL rows
 
+// DPP move with bound_ctrl: lanes without a source lane read 0 (full row and
+// bank masks, so every lane is written and no old value is needed)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov0(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+
 template <int N>
 __device__ __forceinline__ double row_bcast(double v) {  // lane N of the 16-lane row, to the whole row
   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + N, 0xF, 0xF, false);
@@ -1426,77 +1435,106 @@ __device__ __forceinline__ double row_bcast(double v) {  // lane N of the 16-lan
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double row_scan(double v) {  // inclusive prefix over the row
-  v += dpp_mov<row_shr<1>()>(v, 0.0);
-  v += dpp_mov<row_shr<2>()>(v, 0.0);
-  v += dpp_mov<row_shr<4>()>(v, 0.0);
-  v += dpp_mov<row_shr<8>()>(v, 0.0);
+  v += dpp_mov0<row_shr<1>()>(v);
+  v += dpp_mov0<row_shr<2>()>(v);
+  v += dpp_mov0<row_shr<4>()>(v);
+  v += dpp_mov0<row_shr<8>()>(v);
   return v;
 }
 __device__ __forceinline__ double row_scan_rev(double v) {  // inclusive suffix over the row
-  v += dpp_mov<row_shl<1>()>(v, 0.0);
-  v += dpp_mov<row_shl<2>()>(v, 0.0);
-  v += dpp_mov<row_shl<4>()>(v, 0.0);
-  v += dpp_mov<row_shl<8>()>(v, 0.0);
+  v += dpp_mov0<row_shl<1>()>(v);
+  v += dpp_mov0<row_shl<2>()>(v);
+  v += dpp_mov0<row_shl<4>()>(v);
+  v += dpp_mov0<row_shl<8>()>(v);
   return v;
 }
-// the value of lane ^ 16 (the other row of a 32-lane column); all lanes active
-__device__ __forceinline__ double xor16(double v) {
-  const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), 0x401F);  // bit mode: and 0x1f, xor 0x10
-  const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), 0x401F);
+// Joining the two 16-lane rows of a 32-lane column (rows 0+1 and 2+3 of the
+// wavefront) without LDS: v_permlane16_swap_b32 exchanges rows 1 and 3 of x
+// with rows 0 and 2 of y (x' = x with rows 1,3 taken from y's rows 0,2;
+// y' = y with rows 0,2 taken from x's rows 1,3), and the DPP row_bcast:15 with
+// row mask 0xA moves lane 15 (47) to every lane of row 1 (3).
+__device__ __forceinline__ void pl16_swap(double& x, double& y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+  x = __hiloint2double(static_cast<int>(hi[0]), static_cast<int>(lo[0]));
+  y = __hiloint2double(static_cast<int>(hi[1]), static_cast<int>(lo[1]));
+}
+// lower row of each column: `fill`; upper row: lane 15 of the lower row
+__device__ __forceinline__ double bcast15_up(double v, double fill) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x142, 0xA, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x142, 0xA, 0xF, false);
   return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ bool upper_row() { return (threadIdx.x & 16) != 0; }
+// lower row of each column: lane 0 of the upper row; upper row: `fill`
+__device__ __forceinline__ double bcast0_down(double v, double fill) {
+  double x = row_bcast<0>(v), y = fill;
+  pl16_swap(x, y);
+  return y;
+}
+// whole-wavefront shifts by one lane (lane 0, resp. 63, keeps `fill`)
+__device__ __forceinline__ double wave_shr1(double v, double fill) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x138, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_shl1(double v, double fill) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
 
 // column-wide (SEGW lanes) versions of the row primitives
 template <int SEGW>
 __device__ __forceinline__ double seg_scan(double v) {  // inclusive prefix
   v = row_scan(v);
-  if (SEGW == 32) {
-    const double t = xor16(row_bcast<15>(v));
-    v += upper_row() ? t : 0.0;
-  }
+  if (SEGW == 32) v += bcast15_up(v, 0.0);
   return v;
 }
 template <int SEGW>
 __device__ __forceinline__ double seg_scan_rev(double v) {  // inclusive suffix
   v = row_scan_rev(v);
-  if (SEGW == 32) {
-    const double t = xor16(row_bcast<0>(v));
-    v += upper_row() ? 0.0 : t;
-  }
+  if (SEGW == 32) v += bcast0_down(v, 0.0);
   return v;
 }
 template <int SEGW>
 __device__ __forceinline__ double seg_last(double v) {  // the column's last lane, to all its lanes
-  const double t = row_bcast<15>(v);
+  double t = row_bcast<15>(v);
   if (SEGW == 16) return t;
-  const double o = xor16(t);
-  return upper_row() ? t : o;
+  double u = t;
+  pl16_swap(u, t);  // t: rows 0,2 from rows 1,3; rows 1,3 kept
+  return t;
 }
 template <int SEGW>
 __device__ __forceinline__ double seg_first(double v) {  // the column's lane 0, to all its lanes
-  const double b = row_bcast<0>(v);
+  double b = row_bcast<0>(v);
   if (SEGW == 16) return b;
-  const double o = xor16(b);
-  return upper_row() ? o : b;
+  double u = b;
+  pl16_swap(b, u);  // b: rows 1,3 from rows 0,2; rows 0,2 kept
+  return b;
 }
 template <int SEGW>
 __device__ __forceinline__ double seg_prev(double v, double fill) {  // lane - 1 (fill at lane 0)
-  double r = dpp_mov<row_shr<1>()>(v, fill);
-  if (SEGW == 32) {
-    const double o = xor16(row_bcast<15>(v));
-    r = ((threadIdx.x & 31) == 16) ? o : r;
-  }
-  return r;
+  if (SEGW == 16) return dpp_mov<row_shr<1>()>(v, fill);
+  const double r = wave_shr1(v, fill);
+  return ((threadIdx.x & 31) == 0) ? fill : r;
+}
+template <int SEGW>
+__device__ __forceinline__ double seg_prev0(double v) {  // lane - 1 (0 at lane 0)
+  if (SEGW == 16) return dpp_mov0<row_shr<1>()>(v);
+  const double r = dpp_mov0<0x138>(v);
+  return ((threadIdx.x & 31) == 0) ? 0.0 : r;
+}
+template <int SEGW>
+__device__ __forceinline__ double seg_next0(double v) {  // lane + 1 (0 at the last lane)
+  if (SEGW == 16) return dpp_mov0<row_shl<1>()>(v);
+  const double r = dpp_mov0<0x130>(v);
+  return ((threadIdx.x & 31) == 31) ? 0.0 : r;
 }
 template <int SEGW>
 __device__ __forceinline__ double seg_next(double v, double fill) {  // lane + 1 (fill at the last lane)
-  double r = dpp_mov<row_shl<1>()>(v, fill);
-  if (SEGW == 32) {
-    const double o = xor16(row_bcast<0>(v));
-    r = ((threadIdx.x & 31) == 15) ? o : r;
-  }
-  return r;
+  if (SEGW == 16) return dpp_mov<row_shl<1>()>(v, fill);
+  const double r = wave_shl1(v, fill);
+  return ((threadIdx.x & 31) == 31) ? fill : r;
 }
 // 1/d by v_rcp_f64 and two Newton steps (within an ulp or so of the correctly rounded quotient)
 __device__ __forceinline__ double rcp_nr(double d) {
@@ -1543,11 +1581,14 @@ __device__ __forceinline__ double rt(const double* tb, int k, int i, int seg) {
 // rows p = seg*NPL + i of one lane; f in, slope out (same array)
 template <typename T, int NPL, int SEGW>
 __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double* tb, int seg, double (&f)[NPL]) {
+  // contracting a*b+c into one rounding is allowed in the collision kernels:
+  // their scans already reorder the reference's sums (results agree to ~1e-15)
+#pragma clang fp contract(fast)
   // re-read the tables from LDS on every call instead of letting the
   // compiler keep all 9*NPL of them in registers across the subcycles
   __asm__ volatile("" ::: "memory");
-  const double fprev = seg_prev<SEGW>(f[NPL - 1], 0.0);
-  const double fnext = seg_next<SEGW>(f[0], 0.0);
+  const double fprev = seg_prev0<SEGW>(f[NPL - 1]);
+  const double fnext = seg_next0<SEGW>(f[0]);
   // trapezoid prefix sums I4, I2 (from p = 1) and suffix J1, lane-local first
   double I4[NPL], I2[NPL], J1[NPL];
   double s4 = 0.0, s2 = 0.0, s1 = 0.0;
@@ -1566,8 +1607,8 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
     J1[i] = s1;
   }
   const double inc4 = seg_scan<SEGW>(s4), inc2 = seg_scan<SEGW>(s2), inc1 = seg_scan_rev<SEGW>(s1);
-  const double ex4 = seg_prev<SEGW>(inc4, 0.0), ex2 = seg_prev<SEGW>(inc2, 0.0);
-  const double ex1 = seg_next<SEGW>(inc1, 0.0);
+  const double ex4 = seg_prev0<SEGW>(inc4), ex2 = seg_prev0<SEGW>(inc2);
+  const double ex1 = seg_next0<SEGW>(inc1);
   const double lnee = log_ee(4.0 * M_PI * seg_last<SEGW>(inc2), 4.0 * M_PI * seg_last<SEGW>(inc4), a.density_np);
   const double f0 = seg_first<SEGW>(f[0]);
   const double f1 = seg_first<SEGW>(f[1]);
@@ -1591,7 +1632,7 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
     G[i] = g;
   }
   // -DG, -DG/(v Dv), DDG/(v Dv), DDG/(v^3 DDv)   (:247-260)
-  const double Gnext = seg_next<SEGW>(G[0], 0.0);
+  const double Gnext = seg_next0<SEGW>(G[0]);
   double D2[NPL];
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
@@ -1602,11 +1643,11 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
   fh0 *= a.c2v0;
   fh0 -= seg_first<SEGW>(D2[0]);
   fh0 *= tb[RT_QN * kRowP];
-  const double D2next = seg_next<SEGW>(D2[0], 0.0);
+  const double D2next = seg_next0<SEGW>(D2[0]);
   double D3[NPL];
 #pragma unroll
   for (int i = 0; i < NPL; ++i) D3[i] = D2[i] - ((i < NPL - 1) ? D2[i + 1] : D2next);
-  const double D3prev = seg_prev<SEGW>(D3[NPL - 1], 0.0);
+  const double D3prev = seg_prev0<SEGW>(D3[NPL - 1]);
   const double sc = a.c_kpre * lnee;
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
@@ -1621,6 +1662,7 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 // launch (the implicit step continues from the collided f00 in registers).
 template <typename T, int NPL, int SEGW, int MODE>
 __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int tile) {
+#pragma clang fp contract(fast)
   constexpr int kCols = 256 / SEGW;  // columns per workgroup
   // Planes cross HBM in row order (a wavefront moves 64/kCols rows of kCols
   // adjacent x) and are turned to the lane layout (lane seg of column w holds
@@ -1720,8 +1762,8 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   // Implicit_FLM_step::reset_coeff, def-FokkerPlanck.cpp:808-939
   double Asub[NPL], Bdia[NPL], Csup[NPL], Sct[NPL];
   {
-    const double fprev = seg_prev<SEGW>(f[NPL - 1], 0.0);
-    const double fnext = seg_next<SEGW>(f[0], 0.0);
+    const double fprev = seg_prev0<SEGW>(f[NPL - 1]);
+    const double fnext = seg_next0<SEGW>(f[0]);
     double I2[NPL], I0[NPL], J1[NPL];
     double s4 = 0.0, s2 = 0.0, s1 = 0.0;
 #pragma unroll
@@ -1739,8 +1781,8 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
       J1[i] = s1;
     }
     const double inc4 = seg_scan<SEGW>(s4), inc2 = seg_scan<SEGW>(s2), inc1 = seg_scan_rev<SEGW>(s1);
-    const double ex4 = seg_prev<SEGW>(inc4, 0.0), ex2 = seg_prev<SEGW>(inc2, 0.0);
-    const double ex1 = seg_next<SEGW>(inc1, 0.0);
+    const double ex4 = seg_prev0<SEGW>(inc4), ex2 = seg_prev0<SEGW>(inc2);
+    const double ex1 = seg_next0<SEGW>(inc1);
     const double I2_temperature = seg_last<SEGW>(inc4) * (4.0 * M_PI);
     const double I0_density = seg_last<SEGW>(inc2) * (4.0 * M_PI);
     const double lnee = log_ee(I0_density, I2_temperature, a.density_np);
@@ -1823,8 +1865,8 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
 #define OSHUN_RMOB(N)                                                                    \
       {                                                                                    \
         Mob q;                                                                             \
-        q.a = dpp_mov<row_shr<N>()>(P.a, 1.0); q.b = dpp_mov<row_shr<N>()>(P.b, 0.0);      \
-        q.c = dpp_mov<row_shr<N>()>(P.c, 0.0); q.d = dpp_mov<row_shr<N>()>(P.d, 1.0);      \
+        q.a = dpp_mov<row_shr<N>()>(P.a, 1.0); q.b = dpp_mov0<row_shr<N>()>(P.b);      \
+        q.c = dpp_mov0<row_shr<N>()>(P.c); q.d = dpp_mov<row_shr<N>()>(P.d, 1.0);      \
         P = mob_mul(P, q);  /* normalised lane products: no range issue over 5 steps */    \
       }
       OSHUN_RMOB(1)
@@ -1832,15 +1874,14 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
       OSHUN_RMOB(4)
       OSHUN_RMOB(8)
 #undef OSHUN_RMOB
-      if (SEGW == 32) {  // the upper row continues from the lower row's product
+      if (SEGW == 32) {  // the upper row continues from the lower row's product (identity below)
         Mob q;
-        q.a = xor16(row_bcast<15>(P.a)); q.b = xor16(row_bcast<15>(P.b));
-        q.c = xor16(row_bcast<15>(P.c)); q.d = xor16(row_bcast<15>(P.d));
-        const Mob Pq = mob_mul(P, q);
-        if (upper_row()) P = Pq;
+        q.a = bcast15_up(P.a, 1.0); q.b = bcast15_up(P.b, 0.0);
+        q.c = bcast15_up(P.c, 0.0); q.d = bcast15_up(P.d, 1.0);
+        P = mob_mul(P, q);
       }
       // c' entering the lane from the exclusive prefix, then the reference's recurrence
-      double cprev = seg_prev<SEGW>(P.b, 0.0) * rcp_nr(seg_prev<SEGW>(P.d, 1.0));
+      double cprev = seg_prev0<SEGW>(P.b) * rcp_nr(seg_prev<SEGW>(P.d, 1.0));
 #pragma unroll
       for (int i = 0; i < NPL; ++i) {
         w[i] = rcp_nr(bl[i] - cprev * Asub[i]);
@@ -1857,7 +1898,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     }
 #define OSHUN_RAFF(N)                                                                              \
     {                                                                                            \
-      const double qa = dpp_mov<row_shr<N>()>(al, 1.0), qb = dpp_mov<row_shr<N>()>(be, 0.0);     \
+      const double qa = dpp_mov<row_shr<N>()>(al, 1.0), qb = dpp_mov0<row_shr<N>()>(be);     \
       be = al * qb + be;                                                                         \
       al = al * qa;                                                                              \
     }
@@ -1866,12 +1907,12 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     OSHUN_RAFF(4)
     OSHUN_RAFF(8)
 #undef OSHUN_RAFF
-    if (SEGW == 32) {
-      const double qa = xor16(row_bcast<15>(al)), qb = xor16(row_bcast<15>(be));
-      const double be2 = al * qb + be, al2 = al * qa;
-      if (upper_row()) { be = be2; al = al2; }
+    if (SEGW == 32) {  // identity map on the lower row
+      const double qa = bcast15_up(al, 1.0), qb = bcast15_up(be, 0.0);
+      be = al * qb + be;
+      al = al * qa;
     }
-    double yv = seg_prev<SEGW>(be, 0.0);
+    double yv = seg_prev0<SEGW>(be);
 #pragma unroll
     for (int i = 0; i < NPL; ++i) { yv = (d[i] - yv * Asub[i]) * w[i]; d[i] = yv; }
     // back substitution x_i = d'_i - c'_i x_{i+1}: affine map per lane, reverse scan
@@ -1884,7 +1925,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     }
 #define OSHUN_RAFB(N)                                                                              \
     {                                                                                            \
-      const double qa = dpp_mov<row_shl<N>()>(ga, 1.0), qb = dpp_mov<row_shl<N>()>(de, 0.0);     \
+      const double qa = dpp_mov<row_shl<N>()>(ga, 1.0), qb = dpp_mov0<row_shl<N>()>(de);     \
       de = ga * qb + de;                                                                         \
       ga = ga * qa;                                                                              \
     }
@@ -1893,12 +1934,12 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     OSHUN_RAFB(4)
     OSHUN_RAFB(8)
 #undef OSHUN_RAFB
-    if (SEGW == 32) {
-      const double qa = xor16(row_bcast<0>(ga)), qb = xor16(row_bcast<0>(de));
-      const double de2 = ga * qb + de, ga2 = ga * qa;
-      if (!upper_row()) { de = de2; ga = ga2; }
+    if (SEGW == 32) {  // identity map on the upper row
+      const double qa = bcast0_down(ga, 1.0), qb = bcast0_down(de, 0.0);
+      de = ga * qb + de;
+      ga = ga * qa;
     }
-    double xv = seg_next<SEGW>(de, 0.0);
+    double xv = seg_next0<SEGW>(de);
 #pragma unroll
     for (int i = NPL - 1; i >= 0; --i) { xv = d[i] - (Csup[i] * w[i]) * xv; d[i] = xv; }
     __syncthreads();  // every lane has read its rows of this plane
