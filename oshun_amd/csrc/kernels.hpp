@@ -1430,8 +1430,8 @@ __device__ __forceinline__ double dpp_mov0(double v) {
 
 template <int N>
 __device__ __forceinline__ double row_bcast(double v) {  // lane N of the 16-lane row, to the whole row
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + N, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + N, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x150 + N, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x150 + N, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double row_scan(double v) {  // inclusive prefix over the row
@@ -1536,6 +1536,32 @@ __device__ __forceinline__ double seg_next(double v, double fill) {  // lane + 1
   const double r = wave_shl1(v, fill);
   return ((threadIdx.x & 31) == 31) ? fill : r;
 }
+// LOGee and ZLOGei (def-FokkerPlanck.cpp:41-95) of one column: the two
+// 16-lane rows of a 32-lane column evaluate one logarithm each (lower row
+// ln T_e, upper row ln n_e) and swap the results, so a wavefront issues one
+// vector log instead of two; the clamps are applied by selection.
+template <int SEGW>
+__device__ __forceinline__ void coulomb_logs(double ne, double Te, double density_np, double zeta, double* lnee,
+                                             double* zlnei) {
+  if (SEGW != 32) {
+    *lnee = log_ee(ne, Te, density_np);
+    if (zlnei) *zlnei = zlog_ei(ne, Te, density_np, zeta);
+    return;
+  }
+  double te = Te / (3.0 * ne);
+  te *= 511000;
+  const double nn = ne * density_np;
+  double lt = log(((threadIdx.x & 16) != 0) ? nn : te);
+  double ln = lt;
+  pl16_swap(lt, ln);  // lt: ln T_e on every lane; ln: ln n_e on every lane
+  const double ee = 23.5 - 0.5 * ln + 1.25 * lt - sqrt(0.00001 + 0.0625 * (lt - 2.0) * (lt - 2.0));
+  *lnee = (ne > 0.000000001 && ee > 2.0) ? ee : 2.0;
+  if (zlnei) {
+    const double ei = 24.0 - 0.5 * ln + lt;
+    *zlnei = ((ne > 0.0000001 && ei > 2.0) ? ei : 2.0) * zeta;
+  }
+}
+
 // 1/d by v_rcp_f64 and two Newton steps (within an ulp or so of the correctly rounded quotient)
 __device__ __forceinline__ double rcp_nr(double d) {
   double r = __builtin_amdgcn_rcp(d);
@@ -1595,21 +1621,23 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const double fm = (i > 0) ? f[i - 1] : fprev;
-    s4 += rt<SEGW>(tb, RT_U4, i, seg) * f[i] + rt<SEGW>(tb, RT_U4M1, i, seg) * fm;
-    s2 += rt<SEGW>(tb, RT_U2, i, seg) * f[i] + rt<SEGW>(tb, RT_U2M1, i, seg) * fm;
+    s4 = fma(rt<SEGW>(tb, RT_U4M1, i, seg), fm, fma(rt<SEGW>(tb, RT_U4, i, seg), f[i], s4));
+    s2 = fma(rt<SEGW>(tb, RT_U2M1, i, seg), fm, fma(rt<SEGW>(tb, RT_U2, i, seg), f[i], s2));
     I4[i] = s4;
     I2[i] = s2;
   }
 #pragma unroll
   for (int i = NPL - 1; i >= 0; --i) {
     const double fn = (i < NPL - 1) ? f[i + 1] : fnext;
-    s1 += rt<SEGW>(tb, RT_U1S, i, seg) * fn + rt<SEGW>(tb, RT_U1M1S, i, seg) * f[i];
+    s1 = fma(rt<SEGW>(tb, RT_U1M1S, i, seg), f[i], fma(rt<SEGW>(tb, RT_U1S, i, seg), fn, s1));
     J1[i] = s1;
   }
   const double inc4 = seg_scan<SEGW>(s4), inc2 = seg_scan<SEGW>(s2), inc1 = seg_scan_rev<SEGW>(s1);
   const double ex4 = seg_prev0<SEGW>(inc4), ex2 = seg_prev0<SEGW>(inc2);
   const double ex1 = seg_next0<SEGW>(inc1);
-  const double lnee = log_ee(4.0 * M_PI * seg_last<SEGW>(inc2), 4.0 * M_PI * seg_last<SEGW>(inc4), a.density_np);
+  double lnee;
+  coulomb_logs<SEGW>(4.0 * M_PI * seg_last<SEGW>(inc2), 4.0 * M_PI * seg_last<SEGW>(inc4), a.density_np, 0.0, &lnee,
+                     nullptr);
   const double f0 = seg_first<SEGW>(f[0]);
   const double f1 = seg_first<SEGW>(f[1]);
   const double f00 = (f0 - f1 * a.r01) * a.inv_gden;
@@ -1671,7 +1699,9 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   constexpr int kS = SEGW * NPL + SEGW + 8;
   __shared__ double tb[RT_COUNT * kRowP];
   __shared__ double tr[kCols * kS];
+  __shared__ double sbg[kRowP];  // f00 background per p (0 beyond np or without one), read in the lane layout
   for (int i = threadIdx.x; i < RT_COUNT * kRowP; i += blockDim.x) tb[i] = a.rt[i];
+  for (int i = threadIdx.x; i < kRowP; i += blockDim.x) sbg[i] = (a.bg && i < a.np) ? a.bg[i] : 0.0;
   const int nx = a.nx, np = a.np;
   const int ntx = (nx + kCols - 1) / kCols;
   const int y = tile / ntx;
@@ -1706,13 +1736,10 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
 #pragma unroll
     for (int k = 0; k < NPL; ++k) v[k] = bload<T>(r, ld_ofs(k), 0);
 #pragma unroll
-    for (int k = 0; k < NPL; ++k) {
-      const int p = r0 + k * SEGW;
-      tr[crow + k * kStepL] = (p < np) ? static_cast<double>(v[k]) + (a.bg ? a.bg[p] : 0.0) : 0.0;
-    }
+    for (int k = 0; k < NPL; ++k) tr[crow + k * kStepL] = static_cast<double>(v[k]);  // rows >= np load 0
     __syncthreads();  // also covers the table fill
 #pragma unroll
-    for (int i = 0; i < NPL; ++i) f[i] = tr[lrow + i];
+    for (int i = 0; i < NPL; ++i) f[i] = tr[lrow + i] + sbg[p0 + i];
   }
   if (MODE == 1 || MODE == 3) {
     // RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553, numh times (:590-625)
@@ -1737,25 +1764,20 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     }
     __syncthreads();  // every lane has read its rows of f00
 #pragma unroll
-    for (int i = 0; i < NPL; ++i) tr[lrow + i] = f[i];
+    for (int i = 0; i < NPL; ++i) tr[lrow + i] = f[i] - sbg[p0 + i];
     __syncthreads();
     {
       const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f, pbytes);
 #pragma unroll
-      for (int k = 0; k < NPL; ++k) {
-        const int p = r0 + k * SEGW;
-        const double b = (a.bg && p < np) ? a.bg[p] : 0.0;
-        bstore<T>(static_cast<T>(tr[crow + k * kStepL] - b), r, st_ofs(k), 0);
-      }
+      for (int k = 0; k < NPL; ++k) bstore<T>(static_cast<T>(tr[crow + k * kStepL]), r, st_ofs(k), 0);
     }
     if (MODE == 1) return;
     // MODE 3 goes on with the implicit step, which reads f00 as stored
     if (sizeof(T) < sizeof(double)) {
 #pragma unroll
       for (int i = 0; i < NPL; ++i) {
-        const int p = p0 + i;
-        const double b = (a.bg && p < np) ? a.bg[p] : 0.0;
-        f[i] = (p < np) ? static_cast<double>(static_cast<T>(f[i] - b)) + b : 0.0;
+        const double b = sbg[p0 + i];
+        f[i] = (p0 + i < np) ? static_cast<double>(static_cast<T>(f[i] - b)) + b : 0.0;
       }
     }
   }
@@ -1769,15 +1791,15 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
       const double fm = (i > 0) ? f[i - 1] : fprev;
-      s4 += rt<SEGW>(tb, RT_U4, i, seg) * f[i] + rt<SEGW>(tb, RT_U4M1, i, seg) * fm;
-      s2 += rt<SEGW>(tb, RT_U2, i, seg) * f[i] + rt<SEGW>(tb, RT_U2M1, i, seg) * fm;
+      s4 = fma(rt<SEGW>(tb, RT_U4M1, i, seg), fm, fma(rt<SEGW>(tb, RT_U4, i, seg), f[i], s4));
+      s2 = fma(rt<SEGW>(tb, RT_U2M1, i, seg), fm, fma(rt<SEGW>(tb, RT_U2, i, seg), f[i], s2));
       I2[i] = s4;
       I0[i] = s2;
     }
 #pragma unroll
     for (int i = NPL - 1; i >= 0; --i) {
       const double fn = (i < NPL - 1) ? f[i + 1] : fnext;
-      s1 += rt<SEGW>(tb, RT_U1S, i, seg) * fn + rt<SEGW>(tb, RT_U1M1S, i, seg) * f[i];
+      s1 = fma(rt<SEGW>(tb, RT_U1M1S, i, seg), f[i], fma(rt<SEGW>(tb, RT_U1S, i, seg), fn, s1));
       J1[i] = s1;
     }
     const double inc4 = seg_scan<SEGW>(s4), inc2 = seg_scan<SEGW>(s2), inc1 = seg_scan_rev<SEGW>(s1);
@@ -1785,8 +1807,9 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     const double ex1 = seg_next0<SEGW>(inc1);
     const double I2_temperature = seg_last<SEGW>(inc4) * (4.0 * M_PI);
     const double I0_density = seg_last<SEGW>(inc2) * (4.0 * M_PI);
-    const double lnee = log_ee(I0_density, I2_temperature, a.density_np);
-    const double zi = zlog_ei(I0_density, I2_temperature, a.density_np, a.zeta) * I0_density;
+    double lnee, zlnei;
+    coulomb_logs<SEGW>(I0_density, I2_temperature, a.density_np, a.zeta, &lnee, &zlnei);
+    const double zi = zlnei * I0_density;
     const double kd = a.kpre * dt_flm;
     const double sc = (-1.0) * lnee * a.kpre * dt_flm;
 #pragma unroll
@@ -1818,9 +1841,9 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   const int lmin = a.do_f1 ? 1 : 2;
   const int lmax = a.do_flm ? a.l0 : 1;
   int c = 1;
-  while (c < a.nc && a.comp_l[c] < lmin) ++c;
+  while (c < a.nc && ldc(a.comp_l, c) < lmin) ++c;
   int cend = c;
-  while (cend < a.nc && a.comp_l[cend] <= lmax) ++cend;
+  while (cend < a.nc && ldc(a.comp_l, cend) <= lmax) ++cend;
   T vn[NPL];  // next plane, row order
   if (c < cend) {
     const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)c * a.plane, pbytes);
@@ -1842,7 +1865,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
 #pragma unroll
       for (int k = 0; k < NPL; ++k) vn[k] = bload<T>(r, ld_ofs(k), 0);
     }
-    const int l = a.comp_l[c];
+    const int l = ldc(a.comp_l, c);  // scalar load: a vector load's wait would also drain the prefetch
     if (l != lcur) {
       lcur = l;
       double ll1 = static_cast<double>(l);
