@@ -883,6 +883,20 @@ struct CollArgs {
   const double* tclk;           // batched runs: the run's clock before the step (else null)
   double hstep, small_dt;       // batched runs: RK4 step and e-e sub-step target
   double r01, inv_gden, c2v0;   // v0^2/v1^2, 1/g_den, 2/v0^2
+  // single 1D m0 = 0 runs: the step's last RK4 field stage and, with next1,
+  // the next step's first field stage done by this launch (fuse = 0: off)
+  int fuse;
+  struct {
+    const double* cf;   // stage-3 fields (input of the last field stage)
+    const double* af;   // RK4 field accumulator
+    double* sf;         // out: fields after the step
+    double* e1;         // out (next1): stage-1 fields of the next step
+    double* af2;        // out (next1): the next step's accumulator
+    const double* jp;   // current partials of the stage-4 input, [tile][3][nx]
+    const double* f1y;  // f10(p1) of the stage-4 input per x
+    int npt, next1, c10;
+    double p3og1, small, delta_p, midx, midy, c05h, h6, third;
+  } fz;
 };
 
 __device__ __forceinline__ double log_ee(double ne, double Te, double density_np) {
@@ -1595,6 +1609,7 @@ enum {
   RT_B2,      // 0.2 v^5/v1^2
   RT_A4,      // 0.2 v^5
   RT_B4,      // v^7/(7 v1^2)
+  RT_JW,      // w p^3/gamma with the trapezoid weights w of Current::Jxyz (0 beyond np)
   RT_COUNT
 };
 
@@ -1685,6 +1700,89 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
   }
 }
 
+// Field stage 4 of the step and field stage 1 of the next one for column x of
+// a 1D m0 = 0 run (FieldArgs modes 4 and 1, def-RK.cpp:116-138 with
+// Current::Jxyz and MaxwellEq of def-VlasovMax.cpp:680-745,979-1035), the
+// same arithmetic as field_body.  In 1D only Ex couples to f and its update is
+// local; the stage-1 curl at x needs the stage-4 fields at x +- 1, which are
+// recomputed from the stage-3 fields.  pre: the column's prefetched inputs
+// (stage-3 fields at x-2..x+2, accumulator at x-1..x+1); j3 and j0: the p sums
+// of the stage-4 input's and of the collided f10's current integrand.
+struct FusePre {
+  double cf[6][5], af[6][3];
+};
+// row r of sum_direct (p = r, r + kFieldRows, ...) over a plane held in the
+// row-order LDS layout of collide_rows (row p at p + p / NPL), as stored in T;
+// jw = w p3og with w = 1 or 1/2, so f (w p3og) = w (f p3og) exactly
+template <typename T, int NPL, int SEGW>
+__device__ __forceinline__ double current_row(const double* trc, const double* jw, int r, int np) {
+  double js = 0.0;
+  for (int p = r; p < np; p += kFieldRows)
+    js += static_cast<double>(static_cast<T>(trc[p + p / NPL])) * jw[(p % NPL) * SEGW + p / NPL];
+  return js;
+}
+// the fixed tree of sum_parts / sum_direct over kFieldRows row sums
+__device__ __forceinline__ double field_tree(double (&sh)[kFieldRows]) {
+#pragma unroll
+  for (int w = kFieldRows / 2; w >= 1; w /= 2)
+#pragma unroll
+    for (int r = 0; r < w; ++r) sh[r] += sh[r + w];
+  return sh[0];
+}
+template <typename T>
+__device__ void fused_fields_1d(const CollArgs<T>& a, int x, const FusePre& q, const double* jp3, double f1y3,
+                                const double* jrows0, double f10p1) {
+  const auto& z = a.fz;
+  const int nx = a.nx;
+  double sh[kFieldRows];
+#pragma unroll
+  for (int r = 0; r < kFieldRows; ++r) {  // sum_parts with npb <= 4 kFieldRows partials
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = (r + kFieldRows * k < z.npt) ? jp3[r + kFieldRows * k] : 0.0;
+    double s = 0.0;
+    if (r < z.npt) s += (v[0] + v[1]) + (v[2] + v[3]);
+    sh[r] = s;
+  }
+  const double j3 = field_tree(sh);
+  double j0 = 0.0;
+  if (z.next1) {
+#pragma unroll
+    for (int r = 0; r < kFieldRows; ++r) sh[r] = jrows0[r];
+    j0 = field_tree(sh);
+  }
+  auto current = [&](double j, double f1) {
+    j *= z.delta_p;
+    j += z.small * (f1 * z.p3og1);
+    j *= 4.0 * M_PI / 3.0;
+    return j;
+  };
+  double F4[3][6];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {  // stage 4 at x-1, x, x+1 (Ex only at x)
+    double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (d == 1) k[0] += current(j3, f1y3);
+    maxwell_terms([&](int c, int at) { return q.cf[c][at == 0 ? d : at == 1 ? d + 2 : d + 1]; }, z.midx, z.midy, k);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double kh = k[c] * z.h6;
+      F4[d][c] = q.af[c][d] + kh;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) z.sf[(long long)c * nx + x] = F4[1][c];
+  if (!z.next1) return;
+  double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  k[0] += current(j0, f10p1);
+  maxwell_terms([&](int c, int at) { return F4[at == 0 ? 0 : at == 1 ? 2 : 1][c]; }, z.midx, z.midy, k);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double kh = k[c] * z.c05h;
+    z.e1[(long long)c * nx + x] = F4[1][c] + kh;
+    z.af2[(long long)c * nx + x] = F4[1][c] + kh * z.third;
+  }
+}
+
 // MODE 1: e-e subcycles on f_00 (Explicit_EC_FP::loop); MODE 2: implicit
 // l >= 1 step (Euler_Backward::advance_1/advance_lm); MODE 3: both in one
 // launch (the implicit step continues from the collided f00 in registers).
@@ -1706,6 +1804,29 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   const int ntx = (nx + kCols - 1) / kCols;
   const int y = tile / ntx;
   const int x0 = (tile % ntx) * kCols;
+  // fused field stages: their inputs are fetched now, while the collisions run
+  // (stage-3 fields at x0-2 .. x0+kCols+1, accumulator at x0-1 .. x0+kCols,
+  // f10(p1) and the current partials of the stage-4 input at the columns)
+  constexpr int kZcf = kCols + 4, kZaf = kCols + 2, kZpt = 32;
+  constexpr int kZoAf = 6 * kZcf, kZoF1 = kZoAf + 6 * kZaf, kZoJp = kZoF1 + kCols, kZn = kZoJp + kCols * kZpt;
+  __shared__ double zpre[kZn];
+  if (a.fuse) {
+    const int npt = a.fz.npt;
+    const int n = kZoJp + kCols * npt;
+    auto wrap = [nx](int xx) { return ((xx % nx) + nx) % nx; };
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      double v;
+      if (i < kZoAf) v = a.fz.cf[(long long)(i / kZcf) * nx + wrap(x0 - 2 + i % kZcf)];
+      else if (i < kZoF1) v = a.fz.af[(long long)((i - kZoAf) / kZaf) * nx + wrap(x0 - 1 + (i - kZoAf) % kZaf)];
+      else if (i < kZoJp) v = a.fz.f1y[wrap(x0 + i - kZoF1)];
+      else {
+        const int k = i - kZoJp, col = k / npt, b = k % npt;
+        v = a.fz.jp[((long long)b * 3) * nx + wrap(x0 + col)];
+      }
+      zpre[i < kZoJp ? i : kZoJp + ((i - kZoJp) / npt) * kZpt + (i - kZoJp) % npt] = v;
+    }
+  }
+  __shared__ double zj[kCols][kFieldRows], zf1[kCols];  // the collided f10's current rows and f10(p1)
   const int seg = threadIdx.x & (SEGW - 1);
   const int p0 = seg * NPL;
   const int lrow = (threadIdx.x / SEGW) * kS + p0 + seg;  // LDS index of lane-layout row i: lrow + i
@@ -1974,6 +2095,25 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
 #pragma unroll
       for (int k = 0; k < NPL; ++k) bstore<T>(static_cast<T>(tr[crow + k * kStepL]), r, st_ofs(k), 0);
     }
+    if (a.fuse && a.fz.next1 && c == a.fz.c10 && seg < kFieldRows) {
+      // the next step's Y0 current from f10 as stored, summed like sum_direct:
+      // row r of the field block takes p = r, r + kFieldRows, ... (then the tree)
+      const int col = threadIdx.x / SEGW;
+      zj[col][seg] = current_row<T, NPL, SEGW>(&tr[col * kS], &tb[RT_JW * kRowP], seg, np);
+      if (seg == 0) zf1[col] = static_cast<double>(static_cast<T>(tr[col * kS + 1]));
+    }
+  }
+  if (a.fuse && seg == 0 && x0 + threadIdx.x / SEGW < nx) {
+    const int col = threadIdx.x / SEGW;
+    FusePre q;
+#pragma unroll
+    for (int c6 = 0; c6 < 6; ++c6) {
+#pragma unroll
+      for (int d = 0; d < 5; ++d) q.cf[c6][d] = zpre[c6 * kZcf + col + d];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q.af[c6][d] = zpre[kZoAf + c6 * kZaf + col + d];
+    }
+    fused_fields_1d<T>(a, x0 + col, q, &zpre[kZoJp + col * kZpt], zpre[kZoF1 + col], zj[col], zf1[col]);
   }
 }
 

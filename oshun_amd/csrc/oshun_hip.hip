@@ -39,6 +39,14 @@ struct oshun_ctx {
   void *S = nullptr, *A = nullptr, *B1 = nullptr, *B2 = nullptr;
   double *Sf = nullptr, *Af = nullptr, *B1f = nullptr, *B2f = nullptr;
   double* Cf = nullptr;  // 1D pair path: fields of the fourth stage's input
+  // 1D pair path with fused collisions: the collision launch does the last
+  // field stage of its step (fuse_fields >= 1) and the first field stage of
+  // the next step (fuse_fields = 2, its accumulator into Af2, then Af and Af2
+  // swap); next1_ready says that the next rk4 finds its stage-1 fields there
+  double* Af2 = nullptr;
+  int fuse_fields = 0;
+  double fuse_h = 0.0;
+  bool next1_ready = false;
   double* jpart = nullptr;
   Term* d_terms = nullptr;
   int* d_tstart = nullptr;
@@ -371,7 +379,9 @@ int launch_pair(oshun_ctx* c, int pair, const void* in, const void* y0, void* ou
 template <typename T>
 int rk4_pair(oshun_ctx* c, double h) {
   int rc;
-  {  // first field stage: the current of Y0 summed directly over p (no partials)
+  const bool have1 = c->next1_ready;
+  c->next1_ready = false;
+  if (!have1) {  // first field stage: the current of Y0 summed directly over p (no partials)
     oshun::FieldArgs<T> a = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, c->S, h, nullptr, 0, nullptr);
     a.p3og = c->d_p3og;
     ProfRec r;
@@ -395,7 +405,9 @@ int rk4_pair(oshun_ctx* c, double h) {
     HIPCHK(c, hipGetLastError());
   }
   if ((rc = launch_pair<T>(c, 1, c->B2, c->S, c->B1, c->A, c->B2f, c->Cf, c->jp1, nullptr, h))) return rc;
-  if ((rc = launch_field<T>(c, 4, c->Cf, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, npt, c->f1y))) return rc;
+  // the last field stage: here, or in the collision launch that follows
+  if (!c->fuse_fields && (rc = launch_field<T>(c, 4, c->Cf, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, npt, c->f1y)))
+    return rc;
   std::swap(c->S, c->B1);
   c->t_rk += h;
   return OSHUN_OK;
@@ -443,7 +455,23 @@ oshun::CollArgs<T> coll_args(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int
   a.bg = c->d_bg;
   a.tclk = nullptr; a.hstep = 0.0; a.small_dt = c->ph.small_dt;
   a.r01 = c->T.v0sq / c->T.v1sq; a.inv_gden = 1.0 / c->T.g_den; a.c2v0 = 2.0 / c->T.v0sq;
+  a.fuse = 0;
+  a.fz = {};
   return a;
+}
+
+// the field stages a fused collision launch takes over (see oshun_ctx::fuse_fields)
+template <typename T>
+void set_fused_fields(oshun_ctx* c, oshun::CollArgs<T>& a, double h) {
+  a.fuse = 1;
+  a.fz.cf = c->Cf; a.fz.af = c->Af; a.fz.sf = c->Sf; a.fz.e1 = c->B1f; a.fz.af2 = c->Af2;
+  a.fz.jp = c->jp1; a.fz.f1y = c->f1y;
+  a.fz.npt = pair_tiles<T>(c);
+  a.fz.next1 = c->fuse_fields == 2;
+  a.fz.c10 = c->T.comp_re[oshun::hind(1, 0, c->T.m0)];
+  a.fz.p3og1 = c->T.p3og[1]; a.fz.small = c->T.small; a.fz.delta_p = c->T.delta_p;
+  a.fz.midx = c->T.midx; a.fz.midy = c->T.midy;
+  a.fz.c05h = 0.5 * h; a.fz.h6 = h / 6.0; a.fz.third = 1.0 / 3.0;
 }
 
 template <typename T>
@@ -455,6 +483,7 @@ int collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double h_e
     int rc = OSHUN_OK;
     if (a.do_ee && (a.do_f1 || a.do_flm) && c->rows_segw == c->rows_segw2 && c->rows_npl == c->rows_npl2) {
       a.rt = c->d_rt;  // one launch for both (same lane layout, same tables)
+      if (c->fuse_fields) set_fused_fields<T>(c, a, c->fuse_h);
       switch (c->rows_segw * 100 + c->rows_npl) {
         case 1604: rc = launch_collide_rows<T, 4, 16, 3>(c, a); break;
         case 1608: rc = launch_collide_rows<T, 8, 16, 3>(c, a); break;
@@ -500,24 +529,37 @@ void ee_substeps(const oshun_ctx* c, double tnew, int* numh, double* h) {
   *h = (tnew - c->t_ee) / static_cast<double>(n);
 }
 
-int full_step(oshun_ctx* c, double h) {
-  int rc = do_rk4(c, h);
-  if (rc) return rc;
-  const double t = c->t_rk;
+// One step of main.cpp:79-107.  On the 1D pair path with both collision
+// operators in one launch, that launch also does the step's last RK4 field
+// stage and, if next1 (another step with the same h follows), the next step's
+// first field stage, from the collided f10.
+int full_step(oshun_ctx* c, double h, bool next1) {
   const int coll = c->ph.collisions;
   const int do_ee = (coll & OSHUN_COLL_EE) ? 1 : 0;
   const int do_flm = (coll & OSHUN_COLL_FLM) ? 1 : 0;
+  const bool fuse = c->use_pair && c->ny == 1 && do_ee && do_flm && c->rows_npl && c->rows_segw == c->rows_segw2 &&
+                    c->rows_npl == c->rows_npl2 && c->npt <= 32;
+  c->fuse_fields = fuse ? (next1 ? 2 : 1) : 0;
+  c->fuse_h = h;
+  int rc = do_rk4(c, h);
+  if (rc) { c->fuse_fields = 0; return rc; }
+  const double t = c->t_rk;
   int numh = 0;
   double hee = 0.0;
   if (do_ee) ee_substeps(c, t, &numh, &hee);
-  if (do_ee || do_flm) {
-    rc = do_collide(c, do_ee, do_flm, do_flm, numh, hee, t - c->t_eb);
-    if (rc) return rc;
+  if (do_ee || do_flm) rc = do_collide(c, do_ee, do_flm, do_flm, numh, hee, t - c->t_eb);
+  const int ff = c->fuse_fields;
+  c->fuse_fields = 0;
+  if (rc) return rc;
+  if (ff == 2) {
+    std::swap(c->Af, c->Af2);
+    c->next1_ready = true;
   }
   c->t_ee = t;
   c->t_eb = t;
   return OSHUN_OK;
 }
+int full_step(oshun_ctx* c, double h) { return full_step(c, h, false); }
 
 // FP32 split background -------------------------------------------------------
 // bg(p) = mean over the interior x,y of Re f00 of the harmonic array sh0
@@ -631,7 +673,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -715,9 +757,10 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     if (hipMalloc((void**)&c->jp1, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->jp2, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->f1y, nx * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&c->Cf, fldbytes) != hipSuccess)
+        hipMalloc((void**)&c->Cf, fldbytes) != hipSuccess || hipMalloc((void**)&c->Af2, fldbytes) != hipSuccess)
       return bail(OSHUN_ERR_ALLOC);
-    if (hipMemset(c->Cf, 0, fldbytes) != hipSuccess) return bail(OSHUN_ERR_HIP);
+    if (hipMemset(c->Cf, 0, fldbytes) != hipSuccess || hipMemset(c->Af2, 0, fldbytes) != hipSuccess)
+      return bail(OSHUN_ERR_HIP);
   }
   if (c->fp32) {
     c->bg.assign(c->np, 0.0);
@@ -843,6 +886,8 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
         put(oshun::RT_A4, p, T.pw5[p] * 0.2);
         put(oshun::RT_B4, p, T.pw7[p] / (v1sq * 7.0));
       }
+      // Current::Jxyz integrand weight: trapezoid w (1/2 at both ends, exact) times p^3/gamma
+      put(oshun::RT_JW, p, in ? ((p == 0 || p == np - 1) ? 0.5 : 1.0) * T.p3og[p] : 0.0);
     }
   };
   if (c->rows_npl) {
@@ -978,9 +1023,10 @@ int oshun_collide_flm(oshun_ctx* c, double tnew, int which) {
 
 int oshun_step(oshun_ctx* c, double h, int nsteps) {
   if (!sane(c) || nsteps < 0) return OSHUN_ERR_ARG;
+  c->next1_ready = false;
   for (int s = 0; s < nsteps; ++s) {
-    int rc = full_step(c, h);
-    if (rc) return rc;
+    int rc = full_step(c, h, s + 1 < nsteps);
+    if (rc) { c->next1_ready = false; return rc; }
   }
   return OSHUN_OK;
 }
