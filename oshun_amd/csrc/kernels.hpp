@@ -1798,8 +1798,16 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   __shared__ double tb[RT_COUNT * kRowP];
   __shared__ double tr[kCols * kS];
   __shared__ double sbg[kRowP];  // f00 background per p (0 beyond np or without one), read in the lane layout
-  for (int i = threadIdx.x; i < RT_COUNT * kRowP; i += blockDim.x) tb[i] = a.rt[i];
-  for (int i = threadIdx.x; i < kRowP; i += blockDim.x) sbg[i] = (a.bg && i < a.np) ? a.bg[i] : 0.0;
+  // Every global load of the prologue (tables, background, fused-field
+  // inputs, f00) is issued before the first LDS write: the workgroup then
+  // waits for one memory latency instead of one per fill-loop iteration.
+  // (blockDim.x == 256 == kRowP)
+  constexpr int kTbPer = RT_COUNT * kRowP / 256;
+  static_assert(RT_COUNT * kRowP % 256 == 0 && kRowP == 256, "one table element per thread and pass");
+  double tbv[kTbPer];
+#pragma unroll
+  for (int k = 0; k < kTbPer; ++k) tbv[k] = a.rt[threadIdx.x + 256 * k];
+  const double bgv = (a.bg && static_cast<int>(threadIdx.x) < a.np) ? a.bg[threadIdx.x] : 0.0;
   const int nx = a.nx, np = a.np;
   const int ntx = (nx + kCols - 1) / kCols;
   const int y = tile / ntx;
@@ -1810,20 +1818,27 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   constexpr int kZcf = kCols + 4, kZaf = kCols + 2, kZpt = 32;
   constexpr int kZoAf = 6 * kZcf, kZoF1 = kZoAf + 6 * kZaf, kZoJp = kZoF1 + kCols, kZn = kZoJp + kCols * kZpt;
   __shared__ double zpre[kZn];
+  constexpr int kZper = (kZn + 255) / 256;
+  double zv[kZper];
+  int zi[kZper];
   if (a.fuse) {
     const int npt = a.fz.npt;
-    const int n = kZoJp + kCols * npt;
     auto wrap = [nx](int xx) { return ((xx % nx) + nx) % nx; };
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      double v;
-      if (i < kZoAf) v = a.fz.cf[(long long)(i / kZcf) * nx + wrap(x0 - 2 + i % kZcf)];
-      else if (i < kZoF1) v = a.fz.af[(long long)((i - kZoAf) / kZaf) * nx + wrap(x0 - 1 + (i - kZoAf) % kZaf)];
-      else if (i < kZoJp) v = a.fz.f1y[wrap(x0 + i - kZoF1)];
-      else {
-        const int k = i - kZoJp, col = k / npt, b = k % npt;
-        v = a.fz.jp[((long long)b * 3) * nx + wrap(x0 + col)];
+#pragma unroll
+    for (int k = 0; k < kZper; ++k) {
+      const int i = threadIdx.x + 256 * k;
+      zi[k] = -1;
+      zv[k] = 0.0;
+      if (i < kZoJp) {
+        zi[k] = i;
+        if (i < kZoAf) zv[k] = a.fz.cf[(long long)(i / kZcf) * nx + wrap(x0 - 2 + i % kZcf)];
+        else if (i < kZoF1) zv[k] = a.fz.af[(long long)((i - kZoAf) / kZaf) * nx + wrap(x0 - 1 + (i - kZoAf) % kZaf)];
+        else zv[k] = a.fz.f1y[wrap(x0 + i - kZoF1)];
+      } else if (i < kZoJp + kCols * npt) {
+        const int j = i - kZoJp, col = j / npt, b = j % npt;
+        zi[k] = kZoJp + col * kZpt + b;
+        zv[k] = a.fz.jp[((long long)b * 3) * nx + wrap(x0 + col)];
       }
-      zpre[i < kZoJp ? i : kZoJp + ((i - kZoJp) / npt) * kZpt + (i - kZoJp) % npt] = v;
     }
   }
   __shared__ double zj[kCols][kFieldRows], zf1[kCols];  // the collided f10's current rows and f10(p1)
@@ -1856,6 +1871,14 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     T v[NPL];
 #pragma unroll
     for (int k = 0; k < NPL; ++k) v[k] = bload<T>(r, ld_ofs(k), 0);
+#pragma unroll
+    for (int k = 0; k < kTbPer; ++k) tb[threadIdx.x + 256 * k] = tbv[k];
+    sbg[threadIdx.x] = bgv;
+    if (a.fuse) {
+#pragma unroll
+      for (int k = 0; k < kZper; ++k)
+        if (zi[k] >= 0) zpre[zi[k]] = zv[k];
+    }
 #pragma unroll
     for (int k = 0; k < NPL; ++k) tr[crow + k * kStepL] = static_cast<double>(v[k]);  // rows >= np load 0
     __syncthreads();  // also covers the table fill
