@@ -446,6 +446,7 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
   // p-tiles of one x strip are consecutive and each XCD gets a contiguous run
   // of tiles, so the halo rows shared by neighbouring p-tiles are L2 hits
   const int npt = (np + TP - 1) / TP;
+  if (tile >= ((nx + kPairX - 1) / kPairX) * npt) return;  // grid padding (whole workgroup)
   const int x0 = (tile / npt) * kPairX;
   const int pt = tile % npt;
   const int p0 = pt * TP;

@@ -349,7 +349,9 @@ int launch_pair(oshun_ctx* c, int pair, const void* in, const void* y0, void* ou
                 const double* fin2, double* jpA, double* jpB, double h) {
   const oshun::Stage1DArgs<T> a = pair_args<T>(c, in, y0, out, acc, fin, fin2, jpA, jpB, h);
   const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
-  dim3 grid(ntx * pair_tiles<T>(c));
+  // padded to a multiple of the 8 XCDs so that xcd_tile gives each XCD one
+  // contiguous run of tiles (p-tiles of a strip share their halo rows in L2)
+  dim3 grid((ntx * pair_tiles<T>(c) + 7) / 8 * 8);
   ProfRec r;
   prof_begin(c, 0, &r);
   switch (c->T.l0) {
