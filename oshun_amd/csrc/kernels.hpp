@@ -1709,6 +1709,7 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 // recomputed from the stage-3 fields.  pre: the column's prefetched inputs
 // (stage-3 fields at x-2..x+2, accumulator at x-1..x+1); j3 and j0: the p sums
 // of the stage-4 input's and of the collided f10's current integrand.
+constexpr int kFuseTiles = 32;  // most stage-pair p tiles a fused collision launch takes (LDS slots per column)
 struct FusePre {
   double cf[6][5], af[6][3];
 };
@@ -1816,7 +1817,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   // fused field stages: their inputs are fetched now, while the collisions run
   // (stage-3 fields at x0-2 .. x0+kCols+1, accumulator at x0-1 .. x0+kCols,
   // f10(p1) and the current partials of the stage-4 input at the columns)
-  constexpr int kZcf = kCols + 4, kZaf = kCols + 2, kZpt = 32;
+  constexpr int kZcf = kCols + 4, kZaf = kCols + 2, kZpt = kFuseTiles;  // the host fuses only npt <= kFuseTiles
   constexpr int kZoAf = 6 * kZcf, kZoF1 = kZoAf + 6 * kZaf, kZoJp = kZoF1 + kCols, kZn = kZoJp + kCols * kZpt;
   __shared__ double zpre[kZn];
   constexpr int kZper = (kZn + 255) / 256;

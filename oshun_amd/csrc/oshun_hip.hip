@@ -539,8 +539,10 @@ int full_step(oshun_ctx* c, double h, bool next1) {
   const int coll = c->ph.collisions;
   const int do_ee = (coll & OSHUN_COLL_EE) ? 1 : 0;
   const int do_flm = (coll & OSHUN_COLL_FLM) ? 1 : 0;
+  // (the launch keeps up to oshun::kFuseTiles current partials per column in LDS)
+  const int npt = c->fp32 ? pair_tiles<float>(c) : pair_tiles<double>(c);
   const bool fuse = c->use_pair && c->ny == 1 && do_ee && do_flm && c->rows_npl && c->rows_segw == c->rows_segw2 &&
-                    c->rows_npl == c->rows_npl2 && c->npt <= 32;
+                    c->rows_npl == c->rows_npl2 && npt <= oshun::kFuseTiles;
   c->fuse_fields = fuse ? (next1 ? 2 : 1) : 0;
   c->fuse_h = h;
   int rc = do_rk4(c, h);
