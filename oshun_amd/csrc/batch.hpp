@@ -71,9 +71,14 @@ struct BLaunch {
   }
 };
 
+// grids of the remapped batched kernels are padded to a multiple of the 8 XCDs
+// (xcd_tile then gives each XCD one contiguous run of tiles; the padding
+// workgroups return at once)
+inline int pad8(int n) { return (n + 7) / 8 * 8; }
+
 template <typename T, int R, int PAIR>
 void launch_pair_batch(hipStream_t s, int l0, const BLaunch<oshun::Stage1DArgs<T>>& L) {
-  const dim3 g(L.total()), blk(64, 4);
+  const dim3 g(pad8(L.total())), blk(64, 4);
   const int n = static_cast<int>(L.sel.size());
   switch (l0) {
     case 1: hipLaunchKernelGGL((oshun::vlasov_pair_1d_batch<T, 1, R, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
@@ -88,7 +93,7 @@ void launch_pair_batch(hipStream_t s, int l0, const BLaunch<oshun::Stage1DArgs<T
 }
 template <typename T, int MODE>
 void launch_coll_batch(hipStream_t s, int key, const BLaunch<oshun::CollArgs<T>>& L) {
-  const dim3 g(L.total()), blk(256);
+  const dim3 g(pad8(L.total())), blk(256);
   const int n = static_cast<int>(L.sel.size());
   switch (key) {
     case 1604: hipLaunchKernelGGL((oshun::collide_rows_batch<T, 4, 16, MODE>), g, blk, 0, s, L.d_args, L.d_ts, n); break;

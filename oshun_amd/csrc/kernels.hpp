@@ -615,6 +615,7 @@ template <typename T, int L0, int R, int PAIR>
 __global__ __launch_bounds__(256) void vlasov_pair_1d_batch(const Stage1DArgs<T>* __restrict__ args,
                                                             const int* __restrict__ ts, int nruns) {
   const int g = xcd_tile(blockIdx.x, gridDim.x);
+  if (g >= ts[nruns]) return;  // grid padding
   const int r = batch_run(ts, nruns, g);
   pair_body<T, L0, R, PAIR>(args[r], g - ts[r]);
 }
@@ -2150,6 +2151,7 @@ template <typename T, int NPL, int SEGW, int MODE>
 __global__ __launch_bounds__(64 * 4) void collide_rows_batch(const CollArgs<T>* __restrict__ args,
                                                              const int* __restrict__ ts, int nruns) {
   const int g = xcd_tile(blockIdx.x, gridDim.x);
+  if (g >= ts[nruns]) return;  // grid padding
   const int r = batch_run(ts, nruns, g);
   collide_body<T, NPL, SEGW, MODE>(args[r], g - ts[r]);
 }
