@@ -1621,9 +1621,10 @@ __device__ __forceinline__ double rt(const double* tb, int k, int i, int seg) {
 }
 
 // EE_isotropic_conserva::Getslope (def-FokkerPlanck.cpp:192-267) for the NPL
-// rows p = seg*NPL + i of one lane; f in, slope out (same array)
+// rows p = seg*NPL + i of one lane; f in, slope times `scale` out (same array)
 template <typename T, int NPL, int SEGW>
-__device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double* tb, int seg, double (&f)[NPL]) {
+__device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double* tb, int seg, double (&f)[NPL],
+                                              double scale) {
   // contracting a*b+c into one rounding is allowed in the collision kernels:
   // their scans already reorder the reference's sums (results agree to ~1e-15)
 #pragma clang fp contract(fast)
@@ -1650,8 +1651,8 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
     J1[i] = s1;
   }
   const double inc4 = seg_scan<SEGW>(s4), inc2 = seg_scan<SEGW>(s2), inc1 = seg_scan_rev<SEGW>(s1);
-  const double ex4 = seg_prev0<SEGW>(inc4), ex2 = seg_prev0<SEGW>(inc2);
-  const double ex1 = seg_next0<SEGW>(inc1);
+  // the lanes before (after) this one: inclusive scan minus the lane's own sum
+  const double ex4 = inc4 - s4, ex2 = inc2 - s2, ex1 = inc1 - s1;
   double lnee;
   coulomb_logs<SEGW>(4.0 * M_PI * seg_last<SEGW>(inc2), 4.0 * M_PI * seg_last<SEGW>(inc4), a.density_np, 0.0, &lnee,
                      nullptr);
@@ -1693,7 +1694,7 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 #pragma unroll
   for (int i = 0; i < NPL; ++i) D3[i] = D2[i] - ((i < NPL - 1) ? D2[i + 1] : D2next);
   const double D3prev = seg_prev0<SEGW>(D3[NPL - 1]);
-  const double sc = a.c_kpre * lnee;
+  const double sc = (a.c_kpre * lnee) * scale;
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     double v = rt<SEGW>(tb, RT_QN, i, seg) * ((i > 0) ? D3[i - 1] : D3prev);
@@ -1896,18 +1897,19 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
       double f0[NPL], fs[NPL];
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { f0[i] = f[i]; fs[i] = f[i]; }
-      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs);
+      // each slope comes back multiplied by its RK4 factor h/2, h/2, h, h/6
+      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs, 0.5 * h);
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) { fs[i] *= (0.5 * h); const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (1.0 / 3.0); }
-      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs);
+      for (int i = 0; i < NPL; ++i) { const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (1.0 / 3.0); }
+      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs, 0.5 * h);
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) { fs[i] *= (0.5 * h); const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (2.0 / 3.0); }
-      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs);
+      for (int i = 0; i < NPL; ++i) { const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (2.0 / 3.0); }
+      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs, h);
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) { fs[i] *= h; const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (1.0 / 3.0); }
-      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs);
+      for (int i = 0; i < NPL; ++i) { const double hs = fs[i]; fs[i] += f0[i]; f[i] += hs * (1.0 / 3.0); }
+      ee_slope_rows<T, NPL, SEGW>(a, tb, seg, fs, h / 6.0);
 #pragma unroll
-      for (int i = 0; i < NPL; ++i) { fs[i] *= (h / 6.0); f[i] += fs[i]; }
+      for (int i = 0; i < NPL; ++i) f[i] += fs[i];
     }
     __syncthreads();  // every lane has read its rows of f00
 #pragma unroll
@@ -1950,8 +1952,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
       J1[i] = s1;
     }
     const double inc4 = seg_scan<SEGW>(s4), inc2 = seg_scan<SEGW>(s2), inc1 = seg_scan_rev<SEGW>(s1);
-    const double ex4 = seg_prev0<SEGW>(inc4), ex2 = seg_prev0<SEGW>(inc2);
-    const double ex1 = seg_next0<SEGW>(inc1);
+    const double ex4 = inc4 - s4, ex2 = inc2 - s2, ex1 = inc1 - s1;
     const double I2_temperature = seg_last<SEGW>(inc4) * (4.0 * M_PI);
     const double I0_density = seg_last<SEGW>(inc2) * (4.0 * M_PI);
     double lnee, zlnei;
@@ -2030,7 +2031,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
       for (int i = 0; i < NPL; ++i) {  // P <- [[0, c], [-a, b]] P
         const Mob Q = {Csup[i] * P.c, Csup[i] * P.d, fma(bl[i], P.c, -Asub[i] * P.a), fma(bl[i], P.d, -Asub[i] * P.b)};
         P = Q;
-        if ((i & 3) == 3) P = mob_norm(P);  // exact power-of-two rescaling against overflow
+        if (i == NPL - 1) P = mob_norm(P);  // exact power-of-two rescaling against overflow (NPL <= 8 rows)
       }
 #define OSHUN_RMOB(N)                                                                    \
       {                                                                                    \
