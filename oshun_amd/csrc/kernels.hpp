@@ -801,49 +801,62 @@ __device__ __forceinline__ void field23_body(const Field23Args<T>& A, const int 
   const int row = threadIdx.y;
   const int nx = a.nx, nxy = nx;  // ny = 1
   __shared__ double sh[kFieldRows][kFieldX];
+  auto wrap = [nx](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
+  // the field inputs of row 0 are loaded before the current sums, so that
+  // their latency overlaps the partial-sum loads and the reduction barriers
+  double fin[6][5], f0v[6][3], accv[6], f1a = 0.0, f2b = 0.0;
+  if (i < nxy && row == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+#pragma unroll
+      for (int d = 0; d < 5; ++d) fin[c][d] = a.fin[(long long)c * nxy + wrap(i + d - 2)];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f0v[c][d] = a.f0[(long long)c * nxy + wrap(i + d - 1)];
+      accv[c] = a.facc[(long long)c * nxy + i];
+    }
+    f1a = a.f1p1 ? a.f1p1[i] : static_cast<double>(a.fpl[(long long)a.c10 * a.plane + nx + i]);
+    f2b = b.f1p1 ? b.f1p1[i] : static_cast<double>(b.fpl[(long long)b.c10 * b.plane + nx + i]);
+  }
   double j1 = 0.0, j2 = 0.0;
   sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &j1);
   sum_parts(b.jpart, b.npb, nxy, i, 0, row, sh, &j2);
   if (i >= nxy || row != 0) return;
-  auto wrap = [nx](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
   // Current::Jxyz (def-VlasovMax.cpp:680-745) of the stage inputs Y1 and Y2
-  auto current = [&](const FieldArgs<T>& f, double j) {
+  auto current = [&](const FieldArgs<T>& f, double j, double f1) {
     j *= f.delta_p;
-    const long long o1 = (long long)f.c10 * f.plane + nx + i;
-    j += f.small * ((f.f1p1 ? f.f1p1[i] : static_cast<double>(f.fpl[o1])) * f.p3og1);
+    j += f.small * (f1 * f.p3og1);
     j *= 4.0 * M_PI / 3.0;
     return j;
   };
   // stage 2 at x-1, x, x+1 (Ex only where it is used: at x)
-  double F2[3][6];
+  double F2[3][6], acc2[6];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int xx = wrap(i + d - 1);
-    const int xm = wrap(xx - 1), xp = wrap(xx + 1);
     double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (d == 1) k[0] += current(a, j1);
-    maxwell_terms([&](int c, int at) { return a.fin[(long long)c * nxy + (at == 0 ? xm : at == 1 ? xp : xx)]; },
-                  a.midx, a.midy, k);
+    if (d == 1) k[0] += current(a, j1, f1a);
+    maxwell_terms([&](int c, int at) { return fin[c][at == 0 ? d : at == 1 ? d + 2 : d + 1]; }, a.midx, a.midy, k);
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const double kh = k[c] * a.c05h;
-      F2[d][c] = a.f0[(long long)c * nxy + xx] + kh;  // rk_combine mode 2: out = y0 + kh
+      F2[d][c] = f0v[c][d] + kh;  // rk_combine mode 2: out = y0 + kh
       if (d == 1) {
-        double* acc = a.facc + (long long)c * nxy + i;
-        *acc = *acc + kh * a.twothird;
+        acc2[c] = accv[c] + kh * a.twothird;
         a.fout[(long long)c * nxy + i] = F2[1][c];
       }
     }
   }
-  // stage 3 at x from the stage-2 fields
+  // stage 3 at x from the stage-2 fields (rk_combine mode 3, with the
+  // accumulator of stage 2 kept in registers)
   double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  k[0] += current(b, j2);
+  k[0] += current(b, j2, f2b);
   maxwell_terms([&](int c, int at) { return at == 0 ? F2[0][c] : at == 1 ? F2[2][c] : F2[1][c]; }, b.midx, b.midy,
                 k);
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     const long long o = (long long)c * nxy + i;
-    rk_combine(3, k[c], b.f0[o], b.fout + o, b.facc + o, b.c05h, b.hh, b.h6, b.third, b.twothird);
+    const double kh = k[c] * b.hh;
+    b.fout[o] = f0v[c][1] + kh;
+    b.facc[o] = acc2[c] + kh * b.third;
   }
 }
 template <typename T>
