@@ -677,6 +677,40 @@ __device__ __forceinline__ void sum_parts(const double* jpart, int npb, int nxy,
   __syncthreads();
 }
 
+// two sum_parts in one pass: each sum keeps its own order, the tree levels
+// share their barriers
+__device__ __forceinline__ void sum_parts2(const double* ja, const double* jb, int npb, int nxy, int i, int row,
+                                           double (&sh)[2][kFieldRows][kFieldX], double* oa, double* ob) {
+  double sa = 0.0, sb = 0.0;
+  if (i < nxy) {
+    for (int b0 = row; b0 < npb; b0 += 4 * kFieldRows) {
+      double va[4], vb[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int b = b0 + kFieldRows * k;
+        va[k] = (b < npb) ? ja[((long long)b * 3) * nxy + i] : 0.0;
+        vb[k] = (b < npb) ? jb[((long long)b * 3) * nxy + i] : 0.0;
+      }
+      sa += (va[0] + va[1]) + (va[2] + va[3]);
+      sb += (vb[0] + vb[1]) + (vb[2] + vb[3]);
+    }
+  }
+  const int lx = threadIdx.x;
+  sh[0][row][lx] = sa;
+  sh[1][row][lx] = sb;
+  __syncthreads();
+#pragma unroll
+  for (int w = kFieldRows / 2; w >= 1; w /= 2) {
+    if (row < w) {
+      sh[0][row][lx] += sh[0][row + w][lx];
+      sh[1][row][lx] += sh[1][row + w][lx];
+    }
+    __syncthreads();
+  }
+  *oa = sh[0][0][lx];
+  *ob = sh[1][0][lx];
+}
+
 // the current integrand of one plane summed over p by the rows of the block
 // (row r takes p = r, r + kFieldRows, ...), then the same fixed tree
 template <typename T>
@@ -800,7 +834,7 @@ __device__ __forceinline__ void field23_body(const Field23Args<T>& A, const int 
   const int i = block * kFieldX + threadIdx.x;
   const int row = threadIdx.y;
   const int nx = a.nx, nxy = nx;  // ny = 1
-  __shared__ double sh[kFieldRows][kFieldX];
+  __shared__ double sh[2][kFieldRows][kFieldX];
   auto wrap = [nx](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
   // the field inputs of row 0 are loaded before the current sums, so that
   // their latency overlaps the partial-sum loads and the reduction barriers
@@ -818,8 +852,7 @@ __device__ __forceinline__ void field23_body(const Field23Args<T>& A, const int 
     f2b = b.f1p1 ? b.f1p1[i] : static_cast<double>(b.fpl[(long long)b.c10 * b.plane + nx + i]);
   }
   double j1 = 0.0, j2 = 0.0;
-  sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &j1);
-  sum_parts(b.jpart, b.npb, nxy, i, 0, row, sh, &j2);
+  sum_parts2(a.jpart, b.jpart, a.npb, nxy, i, row, sh, &j1, &j2);  // a.npb == b.npb: the same pair tiles
   if (i >= nxy || row != 0) return;
   // Current::Jxyz (def-VlasovMax.cpp:680-745) of the stage inputs Y1 and Y2
   auto current = [&](const FieldArgs<T>& f, double j, double f1) {
