@@ -114,6 +114,8 @@ def _load():
         lib.oo_collide_ee.argtypes = [P, D]
         lib.oo_collide_flm.argtypes = [P, D, I]
         lib.oo_step.argtypes = [P, D, I]
+        lib.oo_step_implicit_1d.argtypes = [P, D, I, I]
+        lib.oo_step_implicit_1d.restype = I
         lib.oo_ee_getslope.argtypes = [P, dp, dp]
         lib.oo_flm_solve.argtypes = [P, dp, D, I, dp]
         lib.oo_flm_coeffs.argtypes = [P, dp, D, dp, dp, dp, dp]
@@ -185,6 +187,11 @@ class Oracle:
 
     def step(self, h, nsteps=1):
         self._lib.oo_step(self._h, float(h), int(nsteps))
+
+    def step_implicit_1d(self, h, nsteps=1, es=0):
+        """main.cpp:79-107 with implicit_E and if_implicit1D (RK2, implicit Ex); l0 >= 3."""
+        if self._lib.oo_step_implicit_1d(self._h, float(h), int(nsteps), int(es)) != 0:
+            raise ValueError("the 1D implicit-E step needs l0 >= 3")
 
     def ee_getslope(self, f00):
         f00 = np.ascontiguousarray(f00, np.float64)
@@ -258,8 +265,9 @@ def have_ref():
 
 
 def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
-            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0):
-    """Run the reference driver; returns dict(dx, dy, h, t, p, f, fld)."""
+            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0):
+    """Run the reference driver; returns dict(dx, dy, h, t, p, f, fld).
+    op 5 (implicit-E step) takes es = if_implicitES."""
     if not have_ref():
         raise RuntimeError("reference driver not built (make -C oracle ref)")
     nh = num_harmonics(l0, m0)
@@ -269,7 +277,7 @@ def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
         with open(fin, "wb") as fh:
-            np.array([l0, m0, np_, nx, ny, op, nsteps, nb, 0, 0], np.int32).tofile(fh)
+            np.array([l0, m0, np_, nx, ny, op, nsteps, nb, es, 0], np.int32).tofile(fh)
             np.array([pmax, xmin, xmax, ymin, ymax, h, small_dt, density_np, zeta],
                      np.float64).tofile(fh)
             f.tofile(fh)

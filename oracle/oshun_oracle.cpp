@@ -44,9 +44,11 @@
 //   f   : complex<double>[nh][ny][nx][np], p fastest, harmonic order ind(l,m)
 //   fld : complex<double>[6][ny][nx]  (Ex,Ey,Ez,Bx,By,Bz)
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 typedef std::complex<double> cd;
@@ -886,6 +888,147 @@ struct Oracle {
     }
     t_eb = t;
   }
+
+  // ------------------------------------------- implicit E, 1D variant ----
+  // The step of main.cpp:79-107 with implicit_E and if_implicit1D:
+  // Runge_Kutta_2_Imp (def-RK.cpp:400-588), Implicit_Actions (def-Actions.cpp:101-225),
+  // Implicit_E_Field::advance (def-ImplicitE.cpp:472-606).  Only the m = 0
+  // harmonics and Ex take part in the 1D variant's field solve.
+  static void zero_state(State& y) {
+    std::fill(y.f.begin(), y.f.end(), cd(0.0, 0.0));
+    std::fill(y.fld.begin(), y.fld.end(), cd(0.0, 0.0));
+  }
+  // Runge_Kutta_2_Imp::advance_p1/p2/IEx (def-RK.cpp:470-498, 503-524, 529-550)
+  template <class F>
+  void rk2(F&& fn, double h) {
+    State Y0 = Y, Yh;
+    Yh.init(nh, n, nxy);
+    fn(Y0, Yh);
+    scale_state(Yh, h); axpy_state(Y0, Yh);
+    scale_state(Yh, 0.5); axpy_state(Y, Yh);
+    fn(Y0, Yh);
+    scale_state(Yh, 0.5 * h); axpy_state(Y, Yh);
+  }
+  // The m = 0 E-field couplings with Ex of ElectricField::Exyz1D
+  // (def-VlasovMax.cpp:291-341; sources l <= lsrc, targets l <= ltgt): per
+  // source, H to l-1 then G to l+1.  Implicit_Ex1D (:200-226) is the same
+  // with sources l <= 3 and targets l <= 2.
+  void exyz_m0(const State& Yin, State& Yh, int lsrc, int ltgt) const {
+    std::vector<cd> G(np), Hh(np);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const cd Ex = Yin.F(0)[(size_t)iy * nx + ix];
+        for (int l = 0; l <= std::min(lsrc, c.l0); ++l) {
+          const cd* fcol = Yin.h(H(l, 0)) + at(0, ix, iy);
+          if (l == 0) g00_col(fcol, G.data());
+          else make_gh_col(l, fcol, G.data(), Hh.data());
+          if (l >= 1 && l - 1 <= ltgt) {
+            const cd mult = Ex * eA2[l2i(l, 0)];
+            cd* dst = Yh.h(H(l - 1, 0)) + at(0, ix, iy);
+            for (int ip = 0; ip < np; ++ip) dst[ip] += Hh[ip] * mult;
+          }
+          if (l + 1 <= c.l0 && l + 1 <= ltgt) {
+            const cd mult = Ex * eA1[l2i(l, 0)];
+            cd* dst = Yh.h(H(l + 1, 0)) + at(0, ix, iy);
+            for (int ip = 0; ip < np; ++ip) dst[ip] += G[ip] * mult;
+          }
+        }
+      }
+  }
+  // Implicit_Actions::Adv1 (def-Actions.cpp:118-132)
+  void adv1(const State& Yin, State& Yh, bool es) const {
+    zero_state(Yh);
+    axy(Yin, Yh);
+    if (!es) {
+      bxyz(Yin, Yh);
+      faraday(Yin, Yh);
+    }
+  }
+  // Implicit_Actions::Adv2 with if_implicit1D (def-Actions.cpp:138-160)
+  void adv2_1d(const State& Yin, State& Yh) const {
+    zero_state(Yh);
+    exyz_m0(Yin, Yh, c.l0, c.l0);
+    clean_low_pc(Yh);
+  }
+  // Electric_Field_Methods::Current_xyz::calculate_J, x component
+  // (def-ImplicitE.cpp:125-156).  Its trapezoid end weight is applied as
+  // jayx(ix) *= 0.5, a 1D-style subscript of the x-fastest field, i.e. to
+  // element (ix, 0) -- a guard row -- so interior cells keep weight 1 at p0
+  // and pmax.
+  void current_imp_x(const State& Yin, std::vector<cd>& jx) const {
+    jx.assign(nxy, cd(0.0, 0.0));
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const cd* f = Yin.h(H(1, 0)) + at(0, ix, iy);
+        cd j = f[0] * p3og[0];
+        j += f[np - 1] * p3og[np - 1];
+        for (int ip = 1; ip < np - 1; ++ip) j += f[ip] * p3og[ip];
+        j *= Delta_p;
+        j += small * (f[1] * p3og[1]);
+        j *= (-4.0) * M_PI / 3.0;
+        jx[(size_t)iy * nx + ix] = j;
+      }
+  }
+  // Implicit_E_Field::advance with if_implicit1D (def-ImplicitE.cpp:472-606):
+  // the current without E (after the l = 1 collisions) and with the probe
+  // field DE (FindDE, :628-652) after an RK2 of Implicit_Ex1D and the l = 1
+  // collisions; Ex = DE J0 / (J0 - J_DE), Ey = Ez = 0.  The harmonics the
+  // probes touch are restored after each.
+  void implicit_e_1d(double h) {
+    const double Eps = 16.0 * std::numeric_limits<double>::epsilon();
+    const double LargeEps = std::sqrt(Eps);
+    std::vector<cd> DE(nxy);
+    for (int k = 0; k < nxy; ++k) {
+      cd ex = Y.F(0)[k], ey = Y.F(1)[k], ez = Y.F(2)[k];
+      ex *= ex; ey *= ey; ez *= ez;
+      ex += ey; ex += ez;
+      ex = std::sqrt(ex);
+      ex *= LargeEps;
+      ex += Eps;
+      DE[k] = ex;
+    }
+    std::vector<int> saved;
+    for (int l = 0; l <= std::min(2, c.l0); ++l)
+      for (int m = 0; m <= std::min(l, c.m0); ++m) saved.push_back(H(l, m));
+    std::vector<std::vector<cd>> keep;
+    for (int i : saved) keep.emplace_back(Y.h(i), Y.h(i) + n);
+    auto restore = [&](int lmax) {
+      for (size_t k = 0; k < saved.size(); ++k) {
+        bool in = false;
+        for (int l = 0; l <= lmax; ++l)
+          for (int m = 0; m <= std::min(l, c.m0); ++m) in = in || (saved[k] == H(l, m));
+        if (in) std::copy(keep[k].begin(), keep[k].end(), Y.h(saved[k]));
+      }
+    };
+    const double Dt = t_rk - t_eb;
+    std::vector<cd> J0, JE;
+    if (c.coll_flm) collide_flm(Dt, 1);
+    current_imp_x(Y, J0);
+    restore(1);
+    for (int k = 0; k < nxy; ++k) Y.F(0)[k] = DE[k];
+    rk2([&](const State& a, State& b) { zero_state(b); exyz_m0(a, b, 3, 2); }, h);
+    if (c.coll_flm) collide_flm(Dt, 1);
+    current_imp_x(Y, JE);
+    restore(2);
+    for (int k = 0; k < nxy; ++k) {
+      Y.F(0)[k] = DE[k] * J0[k] / (J0[k] - JE[k]);
+      Y.F(1)[k] = 0.0;
+      Y.F(2)[k] = 0.0;
+    }
+  }
+  void step_implicit_1d(double h, bool es) {
+    rk2([&](const State& a, State& b) { adv1(a, b, es); }, h);
+    t_rk += h;
+    double t = t_rk;
+    if (c.coll_ee) collide_ee(t); else t_ee = t;
+    implicit_e_1d(h);
+    rk2([&](const State& a, State& b) { adv2_1d(a, b); }, h);
+    if (c.coll_flm) {
+      collide_flm(t - t_eb, 1);
+      collide_flm(t - t_eb, 2);
+    }
+    t_eb = t;
+  }
 };
 
 
@@ -1240,6 +1383,13 @@ void oo_step(void* o, double h, int nsteps) {
   for (int s = 0; s < nsteps; ++s) q->step(h);
 }
 void oo_moments(void* o, double* out) { static_cast<Oracle*>(o)->moments(out); }
+// nsteps implicit-E steps of the 1D variant (l0 >= 3); es = if_implicitES
+int oo_step_implicit_1d(void* o, double h, int nsteps, int es) {
+  Oracle* q = static_cast<Oracle*>(o);
+  if (q->c.l0 < 3) return -1;
+  for (int s = 0; s < nsteps; ++s) q->step_implicit_1d(h, es != 0);
+  return 0;
+}
 // Tangent-linear model of nsteps steps of a 1D m0 = 0 run from t = 0.
 // f, df: real [l][x][p]; fld, dfld: real [6][x]; outputs the final state and
 // its tangent (df_out/dfld_out may be null: the double instantiation only).

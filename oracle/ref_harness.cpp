@@ -1,9 +1,9 @@
 // Test driver for the UNMODIFIED reference classes of OSHUN-1.
 //
 // TEST INFRASTRUCTURE ONLY (oracle side).  It is compiled by oracle/Makefile
-// together with seven reference translation units taken as they lie under
+// together with eight reference translation units taken as they lie under
 // the reference tree (def-Input, def-Nmethods, def-State, def-VlasovMax,
-// def-Actions, def-RK, def-FokkerPlanck); none of them uses MPI.  Nothing of
+// def-Actions, def-RK, def-FokkerPlanck, def-ImplicitE); none of them uses MPI.  Nothing of
 // the reference is copied into this repository; the binary lands in
 // oracle/_ref/, which git ignores.
 //
@@ -18,7 +18,7 @@
 // and after each step, which is what the single-node periodic exchange does.
 //
 // Usage:  oshun_ref <in.bin> <out.bin>
-//   in.bin : int32[10] {l0,m0,np,nx,ny,op,nsteps,nb,pad,pad}
+//   in.bin : int32[10] {l0,m0,np,nx,ny,op,nsteps,nb,es,pad}  (es: if_implicitES, op 5)
 //            double[9] {pmax,xmin,xmax,ymin,ymax,h,small_dt,density_np,zeta}
 //            complex<double>[nh][ny][nx][np]  (interior f, p fastest)
 //            complex<double>[6][ny][nx]       (interior Ex..Bz)
@@ -27,6 +27,9 @@
 //            2 = nsteps x {RK4, halo}
 //            3 = nsteps x {e-e}       (tnew advanced by h each time)
 //            4 = nsteps x {f1, flm}   (tnew advanced by h each time)
+//            5 = nsteps x {RK2 advance_p1, e-e, implicit E, RK2 advance_p2,
+//                f1, flm, halo} with implicit_E and if_implicit1D on
+//                (main.cpp:79-107 with Runge_Kutta_2_Imp and Implicit_E_Field)
 //   out.bin: double[4] {dx_used, dy_used, h_used, t_final},
 //            double[np] p axis, then f and fields as in in.bin.
 
@@ -46,6 +49,7 @@
 #include "decl-actions.h"
 #include "decl-RK.h"
 #include "decl-fokkerplanck.h"
+#include "decl-implicitE.h"
 
 typedef std::complex<double> cd;
 
@@ -105,6 +109,7 @@ int main(int argc, char** argv) {
   if (std::fread(iv, sizeof(int), 10, fi) != 10) die("short header");
   if (std::fread(dv, sizeof(double), 9, fi) != 9) die("short header");
   const int l0 = iv[0], m0 = iv[1], np = iv[2], nx = iv[3], ny = iv[4], op = iv[5], nsteps = iv[6], nb = iv[7];
+  const int es = iv[8];
   const int nh = ((m0 + 1) * (2 * l0 - m0 + 2)) / 2;
   const int nbc = 4;
 
@@ -120,7 +125,7 @@ int main(int argc, char** argv) {
   L.n_outsteps = 1; L.t_stop = dv[5];  // dt_out = h
   L.small_dt = dv[6];
   L.NB_algorithms = nb;
-  L.if_tridiagonal = true; L.implicit_E = false; L.if_implicit1D = false; L.if_implicitES = false;
+  L.if_tridiagonal = true; L.implicit_E = (op == 5); L.if_implicit1D = (op == 5); L.if_implicitES = (op == 5 && es);
   L.density_np = dv[7]; L.Zeta = dv[8];
   L.bndX = 0; L.bndY = 0;
   Inputdata::IN().inp() = Inputdata::decl_input(L);
@@ -186,6 +191,30 @@ int main(int argc, char** argv) {
       fill_guards_periodic(Y, nbc);
     }
     t_final = t;
+    Out = Y;
+  } else if (op == 5) {
+    Runge_Kutta_2_Imp rk(Y, 0);
+    Explicit_EC_FP ee(Y, 0);
+    Euler_Backward eb(Y, 0);
+    Electric_Field_Methods::Implicit_E_Field eim(Y, 0);
+    for (int s = 0; s < nsteps; ++s) {
+      rk.advance_p1();
+      // Neighbor_ImplicitE_Communications moves the B guards only
+      // (def-Parallel.cpp:2945-3033); f guards are not read before the
+      // full exchange below (the operators in between are x-local)
+      {
+        Stat G(Y);
+        fill_guards_periodic(G, nbc);
+        for (int k = 3; k < 6; ++k) Y.FLD(k) = G.FLD(k);
+      }
+      ee.loop(rk.time());
+      eim.advance(&rk, &eb);
+      rk.advance_p2();
+      eb.advance_1(rk.time());
+      eb.advance_lm(rk.time());
+      fill_guards_periodic(Y, nbc);
+    }
+    t_final = rk.time();
     Out = Y;
   } else {
     die("bad op");

@@ -20,6 +20,11 @@ Cases (the reference needs l0 >= 3 and m0 >= 2, def-Input.cpp:73):
   rk4_1d_nocoll  4 RK4 steps, collisions off (density 1e-30), l0=3 m0=2
   ee_only        3 e-e calls (Explicit_EC_FP::loop)
   flm_only       3 implicit f_lm calls (Euler_Backward::advance_1/advance_lm)
+  imp_1d_m0      3 implicit-E steps (Runge_Kutta_2_Imp, Implicit_E_Field with
+                 if_implicit1D, e-e, f1, flm), l0=5 m0=0 physics through the
+                 m0=2 y-uniform stand-in, if_implicitES on
+  imp_1d_m0_l8   the same at l0=8, Np=64, if_implicitES off (Faraday in
+                 Adv1; the transverse fields are zero)
 """
 import os
 import sys
@@ -158,6 +163,24 @@ def main():
                     ymax=c["ymax"], h=c["h"], small_dt=c["small_dt"],
                     density_np=c["density_np"], nb=c["nb"])
         save(name, c, f, fld, r)
+
+    # 8), 9) implicit-E steps, 1D variant, m0=0 physics through the m0=2 stand-in
+    for name, l0, np_, nx, es, seed in (("imp_1d_m0", 5, 32, 12, 1, 21), ("imp_1d_m0_l8", 8, 64, 10, 0, 22)):
+        c = dict(l0=l0, m0=2, np=np_, nx=nx, ny=1, pmax=0.25, xmin=0.0, xmax=100.0, ymin=0.0, ymax=1.0,
+                 op=5, nsteps=3, nb=4, h=0.5, small_dt=0.2, density_np=1e21, zeta=1.0, es=es)
+        f, fld = maxwellian_state(c["l0"], c["m0"], c["np"], c["nx"], 1, c["pmax"], seed, eyz=False)
+        for l in range(c["l0"] + 1):
+            for m in range(1, min(l, c["m0"]) + 1):
+                f[hind(l, m, c["m0"])] = 0.0
+        fld[1:] = 0.0
+        r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], 1, f, fld, op=5, nsteps=c["nsteps"],
+                    pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"],
+                    h=c["h"], small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"], es=es)
+        for l in range(c["l0"] + 1):
+            for m in range(1, min(l, c["m0"]) + 1):
+                assert not np.any(r["f"][hind(l, m, c["m0"])]), (l, m)
+        assert not np.any(r["fld"][1:])
+        save(name, c, f, fld, r, keep_m0_only=True, m0_phys=0)
 
 
 if __name__ == "__main__":

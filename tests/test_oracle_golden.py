@@ -91,3 +91,51 @@ def test_m0_zero_native_matches_reference_standin(golden):
     assert rel(fo.real, g["f_out"]) < 1e-13
     assert np.abs(fo.imag).max() == 0.0
     assert rel(fldo[0].real, g["fld_out"][0]) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["imp_1d_m0", "imp_1d_m0_l8"])
+def test_implicit_1d_matches_reference(golden, name):
+    """Implicit-E steps (Runge_Kutta_2_Imp + Implicit_E_Field, if_implicit1D), m0=0 run natively
+    by the oracle vs the reference's m0=2 y-uniform stand-in."""
+    g = golden(name)
+    c = g["cfg"]
+    l0, nx, np_ = int(c["l0"]), int(c["nx"]), int(c["np"])
+    o = make(g, m0=0)
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f[:] = g["f_in"]
+    fld = np.zeros((6, 1, nx), np.complex128)
+    fld[0] = g["fld_in"][0]
+    o.set_state(f, fld)
+    o.step_implicit_1d(float(c["h"]), int(c["nsteps"]), int(c["es"]))
+    fo, fldo = o.get_state()
+    assert rel(fo.real, g["f_out"]) < 1e-13
+    assert np.abs(fo.imag).max() == 0.0
+    assert rel(fldo[0].real, g["fld_out"][0]) < 1e-13
+    assert not np.any(fldo[1:3])                         # the 1D solve sets Ey = Ez = 0
+    assert rel(g["fld_out"][0], g["fld_in"][0]) > 1e-3   # the field solve changed Ex
+    assert rel(g["f_out"], g["f_in"]) > 1e-6
+
+
+def test_implicit_1d_field_is_the_zero_current_field(golden):
+    """The 1D implicit field is the root of the linear current response: with
+    E = DE J0/(J0 - J_DE), the current of the probe step with E scales linearly,
+    so a second solve from the solved state changes Ex only at second order."""
+    g = golden("imp_1d_m0")
+    c = g["cfg"]
+    l0, nx, np_ = int(c["l0"]), int(c["nx"]), int(c["np"])
+    o = make(g, m0=0)
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f[:] = g["f_in"]
+    fld = np.zeros((6, 1, nx), np.complex128)
+    fld[0] = g["fld_in"][0]
+    o.set_state(f, fld)
+    o.step_implicit_1d(0.5, 1, 1)
+    _, fld1 = o.get_state()
+    # the solved field does not depend on the incoming Ex beyond the probe size DE ~ 1e-7 |E|
+    o2 = make(g, m0=0)
+    fld2 = fld.copy()
+    fld2[0] *= 3.0
+    o2.set_state(f, fld2)
+    o2.step_implicit_1d(0.5, 1, 1)
+    _, fld2o = o2.get_state()
+    assert rel(fld2o[0], fld1[0]) < 1e-6
