@@ -100,6 +100,14 @@ int oshun_collide_ee(oshun_ctx* ctx, double tnew);
 int oshun_collide_flm(oshun_ctx* ctx, double tnew, int which);
 /* nsteps x main.cpp:79-107 (RK4, e-e, f1, flm; guards are implicit). */
 int oshun_step(oshun_ctx* ctx, double h, int nsteps);
+/* nsteps x main.cpp:79-107 with implicit_E and if_implicit1D (the RKLevel = 2
+ * loop): Runge_Kutta_2_Imp::advance_p1 (def-RK.cpp:470-498), e-e,
+ * Implicit_E_Field::advance (def-ImplicitE.cpp:472-606, 1D variant),
+ * Runge_Kutta_2_Imp::advance_p2 (def-RK.cpp:503-524), Euler_Backward::
+ * advance_1/advance_lm.  if_implicitES: Adv1 without Faraday
+ * (def-Actions.cpp:118-132).  1D runs with m0 = 0, 3 <= l0 <= 12, FP64;
+ * otherwise OSHUN_ERR_ARG. */
+int oshun_step_implicit_1d(oshun_ctx* ctx, double h, int nsteps, int if_implicitES);
 
 /* Nonrelativistic moments of the current state in double
  * (Parallel_Output::nonrelativistic_moments, def-Parallel.cpp:1677-2134;

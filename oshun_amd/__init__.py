@@ -100,6 +100,7 @@ def _load():
     lib.oshun_collide_ee.argtypes = [P, D]
     lib.oshun_collide_flm.argtypes = [P, D, I]
     lib.oshun_step.argtypes = [P, D, I]
+    lib.oshun_step_implicit_1d.argtypes = [P, D, I, I]
     lib.oshun_synchronize.argtypes = [P]
     lib.oshun_get_stream.argtypes = [P, ctypes.POINTER(P)]
     lib.oshun_device_state.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
@@ -245,6 +246,11 @@ class Context:
 
     def step(self, h, nsteps=1):
         self._chk(self._lib.oshun_step(self._h, float(h), int(nsteps)))
+
+    def step_implicit_1d(self, h, nsteps=1, es=0):
+        """main.cpp:79-107 with implicit_E and if_implicit1D (Runge_Kutta_2_Imp +
+        Implicit_E_Field); 1D, m0 = 0, l0 >= 3, FP64."""
+        self._chk(self._lib.oshun_step_implicit_1d(self._h, float(h), int(nsteps), int(es)))
 
     def synchronize(self):
         self._chk(self._lib.oshun_synchronize(self._h))

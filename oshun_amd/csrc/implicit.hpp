@@ -1,0 +1,291 @@
+// implicit.hpp -- the implicit-E step of OSHUN-1 in its 1D variant
+// (implicit_E and if_implicit1D, main.cpp:79-107 with Runge_Kutta_2_Imp and
+// Implicit_E_Field) for 1D runs with m0 = 0 in FP64.  Included at the end of
+// oshun_hip.hip.
+//
+// One step:
+//   RK2 of Implicit_Actions::Adv1 (x-advection; with Faraday on the fields
+//     unless if_implicitES)                       def-RK.cpp:470-498, def-Actions.cpp:118-132
+//   e-e sub-cycles                                 Explicit_EC_FP::loop
+//   Implicit_E_Field::advance, 1D variant          def-ImplicitE.cpp:472-606
+//     probe field DE = sqrt(|E|^2) sqrt(16 eps) + 16 eps (FindDE, :628-652);
+//     J0 = current after Euler_Backward::advance_1 with E = 0;
+//     J_DE = current after an RK2 of Implicit_Ex1D with Ex = DE and advance_1;
+//     (the harmonics the probes touch are restored after each)
+//     Ex = DE J0 / (J0 - J_DE), Ey = Ez = 0
+//   RK2 of Implicit_Actions::Adv2 (Exyz1D + CLEAN_LOW_PC)   def-RK.cpp:503-524
+//   Euler_Backward::advance_1, advance_lm
+// The current of the field solve is Current_xyz::calculate_J
+// (def-ImplicitE.cpp:125-156), whose trapezoid end weight lands on a guard row
+// (a 1D-style subscript), so interior cells sum p0 and pmax with weight 1.
+//
+// Buffers: S holds the state; an RK2 stage reads S (or B1) and writes B1
+// (Y + h k) and A (Y + h k / 2), the second stage writes S; B2 keeps the
+// probed planes l <= 2, B2f the per-x probe field and currents.
+
+namespace oshun {
+
+struct Imp1DArgs {
+  Stage1DArgs<double> s;  // in: stage input planes; fin: Ex; tables
+  const double* y;        // mode 0: the state Y (= in); mode 1: Y + h k1 / 2
+  double* out0;           // mode 0: Y + h k1
+  double* outy;           // mode 0: Y + h k1 / 2; mode 1: final Y
+  int mode, use_e, use_adv, clean, tmax;
+  double h, h2;           // h and 0.5 h
+};
+
+// one RK2 stage of Runge_Kutta_2_Imp, thread per (x, p); the slope is the 1D
+// right-hand side of the E couplings (Ex of s.fin) and/or the x-advection
+template <int L0, bool CLEAN>
+__device__ __forceinline__ void imp_stage_body(const Imp1DArgs& A) {
+  constexpr int NL = L0 + 1;
+  const Stage1DArgs<double>& a = A.s;
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  const int p = blockIdx.y * kRows1D + threadIdx.y;
+  const int np = a.np, nx = a.nx;
+  if (x >= nx || p >= np) return;
+  const long long idx = (long long)p * nx + x;
+  const long long xm = (x == 0) ? (nx - 1) : -1;
+  const long long xp = (x == nx - 1) ? -(long long)(nx - 1) : 1;
+  const long long up = (p > 0) ? -(long long)nx : 0;
+  const long long dn = (p < np - 1) ? (long long)nx : 0;
+  double fm[NL], fc[NL], fn[NL], fl[NL], fr[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    const double* s = a.in + (long long)l * a.plane + idx;
+    fc[l] = s[0];
+    fm[l] = s[up];
+    fn[l] = s[dn];
+    fl[l] = A.use_adv ? s[xm] : 0.0;  // without advection fl - fr = 0 adds +-0
+    fr[l] = A.use_adv ? s[xp] : 0.0;
+  }
+  double k[NL];
+  rhs_point_1d<double, NL, CLEAN>(a, p, emul<double, NL>(a, A.use_e ? a.fin[x] : 0.0), a.vtab[p], 0.0, fm, fc, fn,
+                                  fl, fr, k);
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    const double kl = (l <= A.tmax) ? k[l] : 0.0;
+    const long long o = (long long)l * a.plane + idx;
+    if (A.mode == 0) {  // Yh *= h; Y0 += Yh; Yh *= 0.5; Y += Yh
+      const double kh = kl * A.h;
+      A.out0[o] = fc[l] + kh;
+      A.outy[o] = fc[l] + kh * 0.5;
+    } else {            // Yh *= 0.5 h; Y += Yh
+      A.outy[o] = A.y[o] + kl * A.h2;
+    }
+  }
+}
+template <int L0, bool CLEAN>
+__global__ __launch_bounds__(64 * kRows1D) void imp_stage_1d(Imp1DArgs A) {
+  imp_stage_body<L0, CLEAN>(A);
+}
+
+struct ImpFieldArgs {
+  double* fld;                  // [6][nx] state fields
+  double* de;                   // [nx] probe field
+  double* j0;                   // [nx] current without E
+  double* jde;                  // [nx] current with the probe field
+  const double* f10;            // l = 1 plane [p][x]
+  const double* p3og;
+  int np, nx;
+  double delta_p, small, midx, midy, h, h2;
+};
+
+// both RK2 stages of Adv1's Faraday term on the fields (MaxwellEq::Faraday,
+// def-VlasovMax.cpp:979-1005): E does not change during Adv1, so the two
+// slopes are equal; B = (B + (h k) / 2) + (h / 2) k
+__global__ __launch_bounds__(256) void imp_faraday_1d(ImpFieldArgs a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = a.nx;
+  if (x >= nx) return;
+  const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+  const double* Ex = a.fld;
+  const double* Ey = a.fld + nx;
+  const double* Ez = a.fld + 2 * nx;
+  double k3 = 0.0, k4 = 0.0, k5 = 0.0;
+  k3 += Ez[x] * (-a.midy) - Ez[x] * (-a.midy);   // y differences of a y-uniform field
+  k4 += Ez[xm] * a.midx - Ez[xp] * a.midx;
+  k5 += Ex[x] * a.midy - Ex[x] * a.midy;
+  k5 += Ey[xm] * (-a.midx) - Ey[xp] * (-a.midx);
+  const double k[3] = {k3, k4, k5};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double* B = a.fld + (3 + c) * nx + x;
+    const double kh = k[c] * a.h;
+    const double y1 = *B + kh * 0.5;
+    *B = y1 + k[c] * a.h2;
+  }
+}
+
+// Current_xyz::calculate_J, x component (def-ImplicitE.cpp:125-156): p0 and
+// pmax with weight 1 on interior cells (see above)
+__device__ __forceinline__ double current_imp(const ImpFieldArgs& a, int x) {
+  const int np = a.np, nx = a.nx;
+  const double* f = a.f10 + x;
+  double j = f[0] * a.p3og[0];
+  j += f[(long long)(np - 1) * nx] * a.p3og[np - 1];
+  for (int ip = 1; ip < np - 1; ++ip) j += f[(long long)ip * nx] * a.p3og[ip];
+  j *= a.delta_p;
+  j += a.small * (f[nx] * a.p3og[1]);
+  j *= (-4.0) * M_PI / 3.0;
+  return j;
+}
+// stage 0: probe field from the state E, J0, and Ex = DE; stage 1: J_DE
+// and the solved field
+__global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int stage) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = a.nx;
+  if (x >= nx) return;
+  if (stage == 0) {
+    const double Eps = 16.0 * 2.220446049250313e-16;  // 16 numeric_limits<double>::epsilon()
+    const double LargeEps = sqrt(Eps);
+    double ex = a.fld[x], ey = a.fld[nx + x], ez = a.fld[2 * nx + x];
+    ex *= ex; ey *= ey; ez *= ez;
+    ex += ey; ex += ez;
+    ex = sqrt(ex);
+    ex *= LargeEps;
+    ex += Eps;
+    a.de[x] = ex;
+    a.j0[x] = current_imp(a, x);
+    a.fld[x] = ex;
+  } else {
+    const double jde = current_imp(a, x);
+    a.jde[x] = jde;
+    a.fld[x] = a.de[x] * a.j0[x] / (a.j0[x] - jde);
+    a.fld[nx + x] = 0.0;
+    a.fld[2 * nx + x] = 0.0;
+  }
+}
+
+}  // namespace oshun
+
+namespace {
+
+// one RK2 (two stages) of a 1D slope on the state S (def-RK.cpp:470-550)
+template <int L0>
+int imp_rk2(oshun_ctx* c, double h, int use_e, int use_adv, int clean, int tmax) {
+  oshun::Imp1DArgs A;
+  fill_1d_args<double>(c, A.s, 0, c->S, nullptr, nullptr, nullptr, c->Sf, h);
+  A.use_e = use_e; A.use_adv = use_adv; A.clean = clean; A.tmax = tmax;
+  A.h = h; A.h2 = 0.5 * h;
+  const dim3 block(64, oshun::kRows1D, 1), grid((c->nx + 63) / 64, c->npb, 1);
+  for (int st = 0; st < 2; ++st) {
+    A.mode = st;
+    if (st == 0) { A.s.in = static_cast<const double*>(c->S); A.y = nullptr;
+                   A.out0 = static_cast<double*>(c->B1); A.outy = static_cast<double*>(c->A); }
+    else         { A.s.in = static_cast<const double*>(c->B1); A.y = static_cast<const double*>(c->A);
+                   A.out0 = nullptr; A.outy = static_cast<double*>(c->S); }
+    if (clean) hipLaunchKernelGGL((oshun::imp_stage_1d<L0, true>), grid, block, 0, c->stream, A);
+    else hipLaunchKernelGGL((oshun::imp_stage_1d<L0, false>), grid, block, 0, c->stream, A);
+    HIPCHK(c, hipGetLastError());
+  }
+  return OSHUN_OK;
+}
+int imp_rk2_l0(oshun_ctx* c, double h, int use_e, int use_adv, int clean, int tmax) {
+  switch (c->T.l0) {
+    case 3: return imp_rk2<3>(c, h, use_e, use_adv, clean, tmax);
+    case 4: return imp_rk2<4>(c, h, use_e, use_adv, clean, tmax);
+    case 5: return imp_rk2<5>(c, h, use_e, use_adv, clean, tmax);
+    case 6: return imp_rk2<6>(c, h, use_e, use_adv, clean, tmax);
+    case 7: return imp_rk2<7>(c, h, use_e, use_adv, clean, tmax);
+    case 8: return imp_rk2<8>(c, h, use_e, use_adv, clean, tmax);
+    case 9: return imp_rk2<9>(c, h, use_e, use_adv, clean, tmax);
+    case 10: return imp_rk2<10>(c, h, use_e, use_adv, clean, tmax);
+    case 11: return imp_rk2<11>(c, h, use_e, use_adv, clean, tmax);
+    default: return imp_rk2<12>(c, h, use_e, use_adv, clean, tmax);
+  }
+}
+
+oshun::ImpFieldArgs imp_field_args(oshun_ctx* c, double h) {
+  oshun::ImpFieldArgs a;
+  a.fld = c->Sf;
+  a.de = c->B2f;
+  a.j0 = c->B2f + c->nx;
+  a.jde = c->B2f + 2 * c->nx;
+  a.f10 = static_cast<const double*>(c->S) + c->plane;  // plane 1 = f10 (m0 = 0)
+  a.p3og = c->d_p3og;
+  a.np = c->np; a.nx = c->nx;
+  a.delta_p = c->T.delta_p; a.small = c->T.small; a.midx = c->T.midx; a.midy = c->T.midy;
+  a.h = h; a.h2 = 0.5 * h;
+  return a;
+}
+
+// copies of the planes l <= lmax of S into / out of B2
+int imp_planes(oshun_ctx* c, int lmax, bool save) {
+  const size_t bytes = (size_t)(lmax + 1) * c->plane * sizeof(double);
+  if (save) HIPCHK(c, hipMemcpyAsync(c->B2, c->S, bytes, hipMemcpyDeviceToDevice, c->stream));
+  else HIPCHK(c, hipMemcpyAsync(c->S, c->B2, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return OSHUN_OK;
+}
+
+int implicit_step_1d(oshun_ctx* c, double h, int es) {
+  int rc;
+  const dim3 fb(256), fg((c->nx + 255) / 256);
+  ProfRec r;
+  // RK2 of Adv1: x-advection, and Faraday on the fields unless electrostatic
+  prof_begin(c, 0, &r);
+  rc = imp_rk2_l0(c, h, 0, 1, 0, c->T.l0);
+  if (!rc && !es) {
+    hipLaunchKernelGGL(oshun::imp_faraday_1d, fg, fb, 0, c->stream, imp_field_args(c, h));
+    HIPCHK(c, hipGetLastError());
+  }
+  prof_end(c, &r);
+  if (rc) return rc;
+  c->t_rk += h;
+  const double t = c->t_rk;
+  const int coll = c->ph.collisions;
+  const int do_ee = (coll & OSHUN_COLL_EE) ? 1 : 0;
+  const int do_flm = (coll & OSHUN_COLL_FLM) ? 1 : 0;
+  if (do_ee) {
+    int numh;
+    double hee;
+    ee_substeps(c, t, &numh, &hee);
+    if ((rc = do_collide(c, 1, 0, 0, numh, hee, 0.0))) return rc;
+  }
+  c->t_ee = t;
+  // Implicit_E_Field::advance, 1D variant
+  const double dt_flm = t - c->t_eb;
+  if ((rc = imp_planes(c, 2, true))) return rc;
+  if (do_flm && (rc = do_collide(c, 0, 1, 0, 0, 0.0, dt_flm))) return rc;
+  prof_begin(c, 1, &r);
+  hipLaunchKernelGGL(oshun::imp_field_solve_1d, fg, fb, 0, c->stream, imp_field_args(c, h), 0);
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = imp_planes(c, 1, false))) return rc;
+  prof_begin(c, 0, &r);
+  rc = imp_rk2_l0(c, h, 1, 0, 0, 2);  // Implicit_Ex1D: sources l <= 3, targets l <= 2
+  prof_end(c, &r);
+  if (rc) return rc;
+  if (do_flm && (rc = do_collide(c, 0, 1, 0, 0, 0.0, dt_flm))) return rc;
+  prof_begin(c, 1, &r);
+  hipLaunchKernelGGL(oshun::imp_field_solve_1d, fg, fb, 0, c->stream, imp_field_args(c, h), 1);
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = imp_planes(c, 2, false))) return rc;
+  // RK2 of Adv2 (Exyz1D + CLEAN_LOW_PC) with the solved field
+  prof_begin(c, 0, &r);
+  rc = imp_rk2_l0(c, h, 1, 0, 1, c->T.l0);
+  prof_end(c, &r);
+  if (rc) return rc;
+  if (do_flm && (rc = do_collide(c, 0, 1, 1, 0, 0.0, dt_flm))) return rc;
+  c->t_eb = t;
+  return OSHUN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oshun_step_implicit_1d(oshun_ctx* c, double h, int nsteps, int if_implicitES) {
+  if (!sane(c) || nsteps < 0 || !(h > 0.0)) return OSHUN_ERR_ARG;
+  if (!c->use1d || c->fp32 || c->T.l0 < 3 || c->T.l0 > 12)
+    return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_1d: needs a 1D run with m0 = 0, 3 <= l0 <= 12, FP64");
+  c->next1_ready = false;
+  for (int s = 0; s < nsteps; ++s) {
+    const int rc = implicit_step_1d(c, h, if_implicitES);
+    if (rc) return rc;
+  }
+  return OSHUN_OK;
+}
+
+}  // extern "C"

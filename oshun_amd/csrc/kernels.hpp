@@ -242,7 +242,7 @@ __device__ __forceinline__ EMul<T, NL> emul(const Stage1DArgs<T>& a, double Ex) 
   }
   return m;
 }
-template <typename T, int NL>
+template <typename T, int NL, bool CLEAN = true>
 __device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, const EMul<T, NL>& em, T v, T bgg,
                                              const T (&fm)[NL], const T (&fc)[NL], const T (&fn)[NL],
                                              const T (&fl)[NL], const T (&fr)[NL], T (&k)[NL]) {
@@ -285,7 +285,7 @@ __device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, con
     if (l + 1 < NL) acc += (fl[l + 1] - fr[l + 1]) * ldc(a.xv2, (l + 1) * np + p);
     k[l] = acc;
   }
-  if (p < NL - 2) {  // CLEAN_LOW_PC, def-Actions.cpp:240-253 (rows p < min(l-1, 80) of l >= 1)
+  if (CLEAN && p < NL - 2) {  // CLEAN_LOW_PC, def-Actions.cpp:240-253 (rows p < min(l-1, 80) of l >= 1)
 #pragma unroll
     for (int l = 2; l < NL; ++l)
       if (p < min(l - 1, 80)) k[l] = T(0);

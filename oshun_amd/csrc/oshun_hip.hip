@@ -1139,3 +1139,5 @@ int oshun_diag_copy(oshun_ctx* c, long long nbytes) {
 #include "batch.hpp"
 // adjoint of the step (oshun_adjoint_*)
 #include "adjoint.hpp"
+
+#include "implicit.hpp"
