@@ -201,6 +201,61 @@ def run_ensemble(steps, warmup, nruns=512):
             "cell_updates_per_s": cells * steps / dt}
 
 
+def run_implicit(steps, warmup, no_cpu=False):
+    """The implicit-E step of the 1D variant (oshun_step_implicit_1d: Runge_Kutta_2_Imp +
+    Implicit_E_Field with if_implicit1D, e-e, implicit f_lm) on the single-run workload, FP64,
+    electrostatic (if_implicitES), with the reference's op-5 driver timed on a sample."""
+    import oshun_amd
+    f, fld, dx = target_state()
+    ctx = make_ctx(oshun_amd.FP64, dx)
+    ctx.upload(f, fld)
+    ctx.step_implicit_1d(H, warmup, 1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step_implicit_1d(H, steps, 1)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.profile(True)
+    ctx.step_implicit_1d(H, steps, 1)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    fo, fldo = ctx.download()
+    if not (np.all(np.isfinite(fo.real)) and np.all(np.isfinite(fldo.real))):
+        raise RuntimeError("non-finite state after the timed implicit steps")
+    cells = (L0 + 1) * NP * NX
+    cpu = None
+    if not no_cpu:
+        from oracle import oracle as orc
+        if orc.have_ref():
+            nx = 64
+            fs, flds, dxs = target_state(nx=nx)
+            nh2 = orc.num_harmonics(L0, 2)
+            f2 = np.zeros((nh2, 1, nx, NP), np.complex128)
+            for l in range(L0 + 1):
+                f2[orc.hind(l, 0, 2)] = fs[l]
+            kw = dict(nb=NB, pmax=PMAX, xmin=0.0, xmax=dxs * (nx - 1), h=H, small_dt=H, density_np=DENSITY, es=1)
+            t0 = time.perf_counter()
+            orc.run_ref(L0, 2, NP, nx, 1, f2, flds, op=5, nsteps=1, **kw)
+            setup = time.perf_counter() - t0
+            n = 4
+            t0 = time.perf_counter()
+            orc.run_ref(L0, 2, NP, nx, 1, f2, flds, op=5, nsteps=n + 1, **kw)
+            per_step = max(time.perf_counter() - t0 - setup, 1e-9) / n
+            cpu = {"value": (L0 + 1) * NP * nx / per_step, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                   "sample": f"OSHUN-1 classes (oracle/_ref, op 5), l0=8 as m0=2 y-uniform stand-in, Np=256, Nx=64, "
+                             f"{n} implicit-E steps minus a 1-step run; value counts the 9 useful harmonics",
+                   "s_per_step": per_step}
+    return {"metric": "phase-space cell-updates/s per implicit-E RK2+FP step (1D variant, FP64)",
+            "value": cells / (ms * 1e-3), "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (the single-run workload's initial state)",
+            "config": {"workload": "1D periodic, l0=8 m0=0, Np=256, Nx=4096, h=0.01, implicit_E with if_implicit1D "
+                                   "and if_implicitES, e-e+e-i collisions on", "cells": cells},
+            "kernels": {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps}
+                        for k, v in prof.items() if v[1]},
+            "cpu_baseline": cpu}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +268,8 @@ def main():
                     help="profiling aid: also launch one plain copy of S_f bytes (counter calibration)")
     ap.add_argument("--batch", action="store_true",
                     help="the batched ensemble (512 independent 1D runs) instead of the single-run workload")
+    ap.add_argument("--implicit", action="store_true",
+                    help="the implicit-E step of the 1D variant instead of the RK4 step (FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -220,6 +277,9 @@ def main():
         sys.exit(2)
     import oshun_amd
 
+    if args.implicit:
+        print(json.dumps(run_implicit(args.steps, args.warmup, args.no_cpu)))
+        return
     if args.batch:
         print(json.dumps(run_ensemble(args.steps, args.warmup)))
         return

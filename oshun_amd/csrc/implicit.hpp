@@ -117,25 +117,39 @@ __global__ __launch_bounds__(256) void imp_faraday_1d(ImpFieldArgs a) {
   }
 }
 
-// Current_xyz::calculate_J, x component (def-ImplicitE.cpp:125-156): p0 and
-// pmax with weight 1 on interior cells (see above)
-__device__ __forceinline__ double current_imp(const ImpFieldArgs& a, int x) {
+// Current_xyz::calculate_J, x component (def-ImplicitE.cpp:125-156), with p0
+// and pmax at weight 1 on interior cells (see above), in the reference's
+// sequential order: a block of 16 columns x 16 rows stages 64-row chunks of
+// f10 p3og in LDS and the first row of threads adds them up in p order.
+// stage 0: probe field from the state E, J0, and Ex = DE; stage 1: J_DE and
+// the solved field.
+constexpr int kImpCols = 16, kImpRows = 64;
+__global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int stage) {
+  __shared__ double chunk[kImpRows][kImpCols];
+  const int tx = threadIdx.x % kImpCols, ty = threadIdx.x / kImpCols;
+  const int x = blockIdx.x * kImpCols + tx;
   const int np = a.np, nx = a.nx;
-  const double* f = a.f10 + x;
-  double j = f[0] * a.p3og[0];
-  j += f[(long long)(np - 1) * nx] * a.p3og[np - 1];
-  for (int ip = 1; ip < np - 1; ++ip) j += f[(long long)ip * nx] * a.p3og[ip];
+  const bool ok = x < nx;
+  const double* f = a.f10 + (ok ? x : 0);
+  double j = 0.0;
+  if (ok && ty == 0) {
+    j = f[0] * a.p3og[0];
+    j += f[(long long)(np - 1) * nx] * a.p3og[np - 1];
+  }
+  for (int ip0 = 1; ip0 < np - 1; ip0 += kImpRows) {
+    __syncthreads();
+    for (int r = ty; r < kImpRows; r += 256 / kImpCols) {
+      const int ip = ip0 + r;
+      chunk[r][tx] = (ok && ip < np - 1) ? f[(long long)ip * nx] * a.p3og[ip] : 0.0;
+    }
+    __syncthreads();
+    if (ok && ty == 0)
+      for (int r = 0; r < kImpRows && ip0 + r < np - 1; ++r) j += chunk[r][tx];
+  }
+  if (!ok || ty != 0) return;
   j *= a.delta_p;
   j += a.small * (f[nx] * a.p3og[1]);
   j *= (-4.0) * M_PI / 3.0;
-  return j;
-}
-// stage 0: probe field from the state E, J0, and Ex = DE; stage 1: J_DE
-// and the solved field
-__global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int stage) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nx = a.nx;
-  if (x >= nx) return;
   if (stage == 0) {
     const double Eps = 16.0 * 2.220446049250313e-16;  // 16 numeric_limits<double>::epsilon()
     const double LargeEps = sqrt(Eps);
@@ -146,12 +160,11 @@ __global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int st
     ex *= LargeEps;
     ex += Eps;
     a.de[x] = ex;
-    a.j0[x] = current_imp(a, x);
+    a.j0[x] = j;
     a.fld[x] = ex;
   } else {
-    const double jde = current_imp(a, x);
-    a.jde[x] = jde;
-    a.fld[x] = a.de[x] * a.j0[x] / (a.j0[x] - jde);
+    a.jde[x] = j;
+    a.fld[x] = a.de[x] * a.j0[x] / (a.j0[x] - j);
     a.fld[nx + x] = 0.0;
     a.fld[2 * nx + x] = 0.0;
   }
@@ -247,18 +260,21 @@ int implicit_step_1d(oshun_ctx* c, double h, int es) {
   const double dt_flm = t - c->t_eb;
   if ((rc = imp_planes(c, 2, true))) return rc;
   if (do_flm && (rc = do_collide(c, 0, 1, 0, 0, 0.0, dt_flm))) return rc;
+  const dim3 sg((c->nx + oshun::kImpCols - 1) / oshun::kImpCols);
   prof_begin(c, 1, &r);
-  hipLaunchKernelGGL(oshun::imp_field_solve_1d, fg, fb, 0, c->stream, imp_field_args(c, h), 0);
+  hipLaunchKernelGGL(oshun::imp_field_solve_1d, sg, fb, 0, c->stream, imp_field_args(c, h), 0);
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   if ((rc = imp_planes(c, 1, false))) return rc;
   prof_begin(c, 0, &r);
-  rc = imp_rk2_l0(c, h, 1, 0, 0, 2);  // Implicit_Ex1D: sources l <= 3, targets l <= 2
+  // Implicit_Ex1D: sources l <= 3, targets l <= 2; the planes above l = 3
+  // only receive +0, so the stages run on planes 0..3 alone
+  rc = imp_rk2<3>(c, h, 1, 0, 0, 2);
   prof_end(c, &r);
   if (rc) return rc;
   if (do_flm && (rc = do_collide(c, 0, 1, 0, 0, 0.0, dt_flm))) return rc;
   prof_begin(c, 1, &r);
-  hipLaunchKernelGGL(oshun::imp_field_solve_1d, fg, fb, 0, c->stream, imp_field_args(c, h), 1);
+  hipLaunchKernelGGL(oshun::imp_field_solve_1d, sg, fb, 0, c->stream, imp_field_args(c, h), 1);
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   if ((rc = imp_planes(c, 2, false))) return rc;
