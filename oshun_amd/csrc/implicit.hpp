@@ -19,9 +19,11 @@
 // (def-ImplicitE.cpp:125-156), whose trapezoid end weight lands on a guard row
 // (a 1D-style subscript), so interior cells sum p0 and pmax with weight 1.
 //
-// Buffers: S holds the state; an RK2 stage reads S (or B1) and writes B1
-// (Y + h k) and A (Y + h k / 2), the second stage writes S; B2 keeps the
-// probed planes l <= 2, B2f the per-x probe field and currents.
+// Buffers: S holds the state.  The RK2 of Adv1 and of Adv2 are one fused
+// stage-pair launch each (vlasov_pair_1d modes 2 and 3: S -> B1, then S and
+// B1 swap); the RK2 of Implicit_Ex1D touches planes 0..3 only and runs as two
+// stage launches on them (S -> B1 = Y + h k and A = Y + h k / 2 -> S); B2
+// keeps the probed planes l <= 2, B2f the per-x probe field and currents.
 
 namespace oshun {
 
@@ -194,19 +196,38 @@ int imp_rk2(oshun_ctx* c, double h, int use_e, int use_adv, int clean, int tmax)
   }
   return OSHUN_OK;
 }
-int imp_rk2_l0(oshun_ctx* c, double h, int use_e, int use_adv, int clean, int tmax) {
+
+// one RK2 of Adv1 (pair = 2: x-advection) or Adv2 (pair = 3: Exyz1D +
+// CLEAN_LOW_PC) on every plane as one fused stage-pair launch: S -> B1, then
+// S and B1 swap roles (neighbouring tiles still read S while one writes)
+template <int L0>
+void launch_imp_pair(oshun_ctx* c, int pair, const oshun::Stage1DArgs<double>& a, dim3 grid) {
+  if (pair == 2)
+    hipLaunchKernelGGL((oshun::vlasov_pair_1d<double, L0, pair_rows<double>(), 2>), grid, dim3(64, 4), 0, c->stream,
+                       a);
+  else
+    hipLaunchKernelGGL((oshun::vlasov_pair_1d<double, L0, pair_rows<double>(), 3>), grid, dim3(64, 4), 0, c->stream,
+                       a);
+}
+int imp_rk2_pair(oshun_ctx* c, double h, int pair) {
+  const oshun::Stage1DArgs<double> a = pair_args<double>(c, c->S, c->S, c->B1, c->A, c->Sf, c->Sf, c->jp1, c->jp2, h);
+  const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
+  const dim3 grid((ntx * pair_tiles<double>(c) + 7) / 8 * 8);
   switch (c->T.l0) {
-    case 3: return imp_rk2<3>(c, h, use_e, use_adv, clean, tmax);
-    case 4: return imp_rk2<4>(c, h, use_e, use_adv, clean, tmax);
-    case 5: return imp_rk2<5>(c, h, use_e, use_adv, clean, tmax);
-    case 6: return imp_rk2<6>(c, h, use_e, use_adv, clean, tmax);
-    case 7: return imp_rk2<7>(c, h, use_e, use_adv, clean, tmax);
-    case 8: return imp_rk2<8>(c, h, use_e, use_adv, clean, tmax);
-    case 9: return imp_rk2<9>(c, h, use_e, use_adv, clean, tmax);
-    case 10: return imp_rk2<10>(c, h, use_e, use_adv, clean, tmax);
-    case 11: return imp_rk2<11>(c, h, use_e, use_adv, clean, tmax);
-    default: return imp_rk2<12>(c, h, use_e, use_adv, clean, tmax);
+    case 3: launch_imp_pair<3>(c, pair, a, grid); break;
+    case 4: launch_imp_pair<4>(c, pair, a, grid); break;
+    case 5: launch_imp_pair<5>(c, pair, a, grid); break;
+    case 6: launch_imp_pair<6>(c, pair, a, grid); break;
+    case 7: launch_imp_pair<7>(c, pair, a, grid); break;
+    case 8: launch_imp_pair<8>(c, pair, a, grid); break;
+    case 9: launch_imp_pair<9>(c, pair, a, grid); break;
+    case 10: launch_imp_pair<10>(c, pair, a, grid); break;
+    case 11: launch_imp_pair<11>(c, pair, a, grid); break;
+    default: launch_imp_pair<12>(c, pair, a, grid); break;
   }
+  HIPCHK(c, hipGetLastError());
+  std::swap(c->S, c->B1);
+  return OSHUN_OK;
 }
 
 oshun::ImpFieldArgs imp_field_args(oshun_ctx* c, double h) {
@@ -237,7 +258,7 @@ int implicit_step_1d(oshun_ctx* c, double h, int es) {
   ProfRec r;
   // RK2 of Adv1: x-advection, and Faraday on the fields unless electrostatic
   prof_begin(c, 0, &r);
-  rc = imp_rk2_l0(c, h, 0, 1, 0, c->T.l0);
+  rc = imp_rk2_pair(c, h, 2);
   if (!rc && !es) {
     hipLaunchKernelGGL(oshun::imp_faraday_1d, fg, fb, 0, c->stream, imp_field_args(c, h));
     HIPCHK(c, hipGetLastError());
@@ -280,7 +301,7 @@ int implicit_step_1d(oshun_ctx* c, double h, int es) {
   if ((rc = imp_planes(c, 2, false))) return rc;
   // RK2 of Adv2 (Exyz1D + CLEAN_LOW_PC) with the solved field
   prof_begin(c, 0, &r);
-  rc = imp_rk2_l0(c, h, 1, 0, 1, c->T.l0);
+  rc = imp_rk2_pair(c, h, 3);
   prof_end(c, &r);
   if (rc) return rc;
   if (do_flm && (rc = do_collide(c, 0, 1, 1, 0, 0.0, dt_flm))) return rc;

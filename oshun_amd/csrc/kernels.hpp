@@ -388,6 +388,10 @@ __device__ __forceinline__ int batch_run(const int* __restrict__ ts, int nruns, 
 // the accumulator; read Y2, Y0 and the accumulator; write Y) instead of 16
 // for four separate stages.  The stage arithmetic is the same as in
 // vlasov_stage_1d (def-RK.cpp:116-138): PAIR 0 = stages 1+2, PAIR 1 = 3+4.
+// PAIR 2 and 3 are the two stages of one RK2 of Runge_Kutta_2_Imp
+// (def-RK.cpp:470-524: Y0 = Y + h k1, Y += (h k1)/2, Y += (h/2) k2): PAIR 2 on
+// the x-advection alone (Adv1), PAIR 3 on the E couplings alone with
+// CLEAN_LOW_PC (Adv2, Exyz1D); no current partials.
 constexpr int kPairX = 62;  // interior columns; lanes 0 and 63 are the halo columns
 
 // raw buffer access: 32-bit byte offsets, plane offset in an SGPR, and
@@ -458,7 +462,7 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc(a.in, nbytes);
   const __amdgpu_buffer_rsrc_t rout = buf_rsrc(a.out, nbytes);
   const __amdgpu_buffer_rsrc_t racc = buf_rsrc(a.acc, nbytes);
-  const __amdgpu_buffer_rsrc_t ry0 = buf_rsrc(PAIR == 0 ? a.in : a.y0, nbytes);
+  const __amdgpu_buffer_rsrc_t ry0 = buf_rsrc(PAIR == 1 ? a.y0 : a.in, nbytes);
   // Y0 of the tile points stays in registers for stage 2 (a re-read would
   // come from HBM: the tile's inputs are out of L2 by then)
   constexpr bool kKeepY0 = true;
@@ -520,13 +524,22 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
         fl[l] = wave_shift<kWaveShr1>(fc[l], fe[l]);
         fr[l] = wave_shift<kWaveShl1>(fc[l], fe[l]);
       }
+      if (PAIR == 3) {  // E couplings only: fl - fr = 0 adds +-0
+#pragma unroll
+        for (int l = 0; l < NL; ++l) { fl[l] = T(0); fr[l] = T(0); }
+      }
       T kv[NL];
-      rhs_point_1d<T, NL>(a, p, emul<T, NL>(a, a.fin[xe]), ldc(a.vtab, p), a.bgG ? ldc(a.bgG, p) : T(0), fm, fc,
-                          fnv, fl, fr, kv);
+      rhs_point_1d<T, NL, PAIR != 2>(a, p, emul<T, NL>(a, PAIR == 2 ? 0.0 : a.fin[xe]), ldc(a.vtab, p),
+                                     a.bgG ? ldc(a.bgG, p) : T(0), fm, fc, fnv, fl, fr, kv);
 #pragma unroll
       for (int l = 0; l < NL; ++l) {
         T y;
-        if (PAIR == 0) {  // stage 1: Y1 = Y0 + h/2 k1, acc = Y0 + h/6 k1
+        if (PAIR >= 2) {  // RK2 stage 1: Y0 = Y + h k1, acc = Y + (h k1)/2
+          const T kh = kv[l] * a.hh;
+          y = fc[l] + kh;
+          accr[r][l] = fc[l] + kh * T(0.5);
+          y0r[r][l] = T(0);
+        } else if (PAIR == 0) {  // stage 1: Y1 = Y0 + h/2 k1, acc = Y0 + h/6 k1
           const T kh = kv[l] * a.c05h;
           y = fc[l] + kh;
           accr[r][l] = fc[l] + kh * a.third;
@@ -570,13 +583,18 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
         fc[l] = ys[s][l][lane];
         fm[l] = ys[s - 1][l][lane];
         fn[l] = (p < np - 1) ? ys[s + 1][l][lane] : fc[l];
-        fl[l] = ys[s][l][lane - 1];
-        fr[l] = ys[s][l][lane + 1];
+        fl[l] = (PAIR == 3) ? T(0) : ys[s][l][lane - 1];
+        fr[l] = (PAIR == 3) ? T(0) : ys[s][l][lane + 1];
       }
       T kv[NL];
-      rhs_point_1d<T, NL>(a, p, emul<T, NL>(a, a.fin2[xe]), ldc(a.vtab, p), a.bgG ? ldc(a.bgG, p) : T(0), fm, fc,
-                          fn, fl, fr, kv);
+      rhs_point_1d<T, NL, PAIR != 2>(a, p, emul<T, NL>(a, PAIR == 2 ? 0.0 : a.fin2[xe]), ldc(a.vtab, p),
+                                     a.bgG ? ldc(a.bgG, p) : T(0), fm, fc, fn, fl, fr, kv);
       const int vc = (p * nx + xe) * SZ;
+      if (PAIR >= 2) {  // RK2 stage 2: Y = acc + (h/2) k2
+#pragma unroll
+        for (int l = 0; l < NL; ++l) bstore<T>(accr[r][l] + kv[l] * a.c05h, rout, vc, l * pb);
+        continue;
+      }
       const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
 #pragma unroll
       for (int l = 0; l < NL; ++l) {
@@ -597,6 +615,7 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
       }
     }
   }
+  if (PAIR >= 2) return;  // (uniform) no current partials for the RK2 stages
   jsh[0][ty][lane] = jin;
   jsh[1][ty][lane] = jout;
   __syncthreads();
