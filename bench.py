@@ -201,6 +201,99 @@ def run_ensemble(steps, warmup, nruns=512):
             "cell_updates_per_s": cells * steps / dt}
 
 
+def _config_cases():
+    """The reference's synthetic configurations as throughput cases (shapes, collisions and step
+    sizes of each; states are their initial conditions): name, (l0, m0, Np, Nx, Ny), Context
+    keywords, h, f, fld."""
+    from oshun_amd import hind, num_harmonics
+
+    def nu_ei_over_wp(t_kev, n=1e21, z=1.0, lnl=10.0):  # NRL formulary
+        return 2.91e-6 * z * n * lnl * (t_kev * 1e3) ** -1.5 / (5.64e4 * np.sqrt(n))
+
+    def lam_ei(t_kev):
+        return np.sqrt(t_kev / 511.0) / nu_ei_over_wp(t_kev)
+
+    def f00(np_, pmax, t_kev, n=1.0):
+        pmin = pmax / (2.0 * np_)
+        p = np.arange(np_) * ((pmax - pmin) / (np_ - 1)) + pmin
+        pt2 = np.asarray(t_kev)[..., None] / 511.0
+        return np.asarray(n)[..., None] * np.exp(-p ** 2 / (2 * pt2)) / (np.sqrt(2 * np.pi) * 2 * np.pi * pt2 ** 1.5)
+
+    cases = []
+    # C1: heat-flow relaxation, Nx = 64 on L = 100 lambda_ei, Np = 64, l0 = 2, collisions (Z = 1), h = 0.1
+    nx, np_, l0 = 64, 64, 2
+    x = np.arange(nx) / nx
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f[0, 0] = f00(np_, 0.25, 1.0 + 0.1 * np.cos(2 * np.pi * x))
+    cases.append(("C1 heat flow, l0=2 Np=64 Nx=64, collisions", (l0, 0, np_, nx, 1),
+                  dict(pmax=0.25, dx=100 * lam_ei(1.0) / nx, density_np=1e21, small_dt=0.1, nb_algorithms=4),
+                  0.1, f, np.zeros((6, 1, nx), np.complex128)))
+    # C2: collisionless Langmuir wave, Nx = 256, Np = 256, l0 = 8, k lambda_D = 0.3, h = 0.01
+    nx, np_, l0 = 256, 256, 8
+    k = 0.3 / np.sqrt(1.0 / 511.0)
+    xs = np.arange(nx) * (2 * np.pi / k / nx)
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f[0, 0] = f00(np_, 0.3, np.ones(nx), 1.0 + 1e-3 * np.sin(k * xs))
+    cases.append(("C2 Langmuir, l0=8 Np=256 Nx=256, collisions off", (l0, 0, np_, nx, 1),
+                  dict(pmax=0.3, dx=2 * np.pi / k / nx, density_np=1e21, small_dt=0.01, nb_algorithms=6,
+                       collisions=0), 0.01, f, np.zeros((6, 1, nx), np.complex128)))
+    # C3: 2D magnetised hot spot, 256 x 256, Np = 128, l0 = m0 = 4, e-e + e-i, uniform Bz
+    nx = ny = 256
+    np_, l0, m0 = 128, 4, 4
+    lam = lam_ei(0.5)
+    d = 20.0 * lam / 16.0
+    xx = (np.arange(nx) - nx / 2) * d
+    r2 = xx[None, :] ** 2 + xx[:, None] ** 2
+    f = np.zeros((num_harmonics(l0, m0), ny, nx, np_), np.complex128)
+    f[hind(0, 0, m0)] = f00(np_, 0.25, 0.5 + 1.5 * np.exp(-r2 / (2 * (20.0 * lam) ** 2)))
+    fld = np.zeros((6, ny, nx), np.complex128)
+    fld[5] = 0.5 * nu_ei_over_wp(0.5)
+    cases.append(("C3 2D hot spot, l0=m0=4 Np=128 256x256, collisions, Bz", (l0, m0, np_, nx, ny),
+                  dict(pmax=0.25, dx=d, dy=d, density_np=1e21, small_dt=0.5, nb_algorithms=4), 0.5, f, fld))
+    # C5: adjoint configuration's forward step, Nx = 512, Np = 192, l0 = 3, h = 0.1
+    nx, np_, l0 = 512, 192, 3
+    x = np.arange(nx) / nx
+    rng = np.random.default_rng(1234)
+    T = 1.0 + sum(0.05 / 8 * rng.uniform(-1, 1) * np.cos(2 * np.pi * (m + 1) * x + rng.uniform(0, 2 * np.pi))
+                  for m in range(8))
+    f = np.zeros((l0 + 1, 1, nx, np_), np.complex128)
+    f[0, 0] = f00(np_, 0.25, T)
+    cases.append(("C5 heat flow (adjoint config forward), l0=3 Np=192 Nx=512, collisions", (l0, 0, np_, nx, 1),
+                  dict(pmax=0.25, dx=100 * lam_ei(1.0) / nx, density_np=1e21, small_dt=0.1, nb_algorithms=4),
+                  0.1, f, np.zeros((6, 1, nx), np.complex128)))
+    return cases
+
+
+def run_configs(steps, warmup):
+    """cell-updates/s of the RK4 + FP step on each synthetic configuration, FP64 and FP32 split, with the
+    step's algorithmic HBM bytes (16 S_f) over its time as the roofline fraction."""
+    import oshun_amd
+    out = []
+    for name, (l0, m0, np_, nx, ny), kw, h, f, fld in _config_cases():
+        cells = oshun_amd.num_harmonics(l0, m0) * np_ * nx * ny
+        row = {"config": name, "cells": cells}
+        for prec, tag in ((oshun_amd.FP64, "fp64"), (oshun_amd.FP32_SPLIT, "fp32_split")):
+            ctx = oshun_amd.Context(l0, m0, np_, nx, ny, precision=prec, **kw)
+            ctx.upload(f, fld)
+            ctx.step(h, warmup)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.step(h, steps)
+            ctx.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
+            fo, _ = ctx.download()
+            if not np.all(np.isfinite(fo.real)):
+                raise RuntimeError(f"non-finite state in {name} ({tag})")
+            S = ctx.state_bytes()
+            row[tag] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
+                        "step_roofline_frac": 16.0 * S / (ms * 1e-3) / HBM_PEAK}
+            ctx.close()
+        out.append(row)
+    return {"metric": "phase-space cell-updates/s per RK4+FP step on the synthetic configurations",
+            "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "data": "synthetic",
+            "configs": out}
+
+
 def run_implicit(steps, warmup, no_cpu=False):
     """The implicit-E step of the 1D variant (oshun_step_implicit_1d: Runge_Kutta_2_Imp +
     Implicit_E_Field with if_implicit1D, e-e, implicit f_lm) on the single-run workload, FP64,
@@ -268,6 +361,8 @@ def main():
                     help="profiling aid: also launch one plain copy of S_f bytes (counter calibration)")
     ap.add_argument("--batch", action="store_true",
                     help="the batched ensemble (512 independent 1D runs) instead of the single-run workload")
+    ap.add_argument("--configs", action="store_true",
+                    help="throughput of the RK4 + FP step on the synthetic configurations C1, C2, C3, C5")
     ap.add_argument("--implicit", action="store_true",
                     help="the implicit-E step of the 1D variant instead of the RK4 step (FP64)")
     args = ap.parse_args()
@@ -277,6 +372,9 @@ def main():
         sys.exit(2)
     import oshun_amd
 
+    if args.configs:
+        print(json.dumps(run_configs(args.steps, args.warmup)))
+        return
     if args.implicit:
         print(json.dumps(run_implicit(args.steps, args.warmup, args.no_cpu)))
         return
