@@ -141,6 +141,12 @@ int oshun_profile_read(oshun_ctx* ctx, double* ms_total, long long* launches, in
  * two scratch buffers of the context, used to calibrate hardware byte
  * counters (FETCH_SIZE / WRITE_SIZE) against a kernel of known traffic. */
 int oshun_diag_copy(oshun_ctx* ctx, long long nbytes);
+/* Diagnostics: which kernel runs the Vlasov stage of this context
+ * (*which = 0 the 1D m0 = 0 kernels, 1 the compile-time specialised general
+ * kernel, 2 the term-list interpreter).  set = 1 forces the interpreter,
+ * 0 restores the default, -1 only queries.  Both general kernels evaluate
+ * Actions::Adv (def-Actions.cpp:73-94) with the same per-target sums. */
+int oshun_stage_kernel(oshun_ctx* ctx, int set, int* which);
 
 /* ---- Batched independent runs ------------------------------------------
  * Many 1D runs (m0 = 0, ny = 1, l0 <= 8, one precision for all) of uneven

@@ -109,6 +109,7 @@ def _load():
     lib.oshun_profile_enable.argtypes = [P, I]
     lib.oshun_profile_read.argtypes = [P, dp, ctypes.POINTER(ctypes.c_longlong), I]
     lib.oshun_diag_copy.argtypes = [P, ctypes.c_longlong]
+    lib.oshun_stage_kernel.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.oshun_moments.argtypes = [P, dp]
     lib.oshun_batch_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Phys), dp, I, ctypes.POINTER(P)]
     lib.oshun_batch_destroy.argtypes = [P]
@@ -301,6 +302,14 @@ class Context:
     def diag_copy(self, nbytes):
         """Plain device copy of nbytes between scratch buffers (counter calibration)."""
         self._chk(self._lib.oshun_diag_copy(self._h, int(nbytes)))
+
+    def stage_kernel(self, set=-1):
+        """Which kernel runs the Vlasov stage: "1d" (m0 = 0, one y row), "spec" (compile-time
+        specialised general kernel) or "general" (term-list interpreter).  set=1 forces the
+        interpreter, set=0 restores the default, set=-1 only queries."""
+        w = ctypes.c_int(-1)
+        self._chk(self._lib.oshun_stage_kernel(self._h, int(set), ctypes.byref(w)))
+        return ("1d", "spec", "general")[w.value]
 
     def profile(self, on=True):
         self._chk(self._lib.oshun_profile_enable(self._h, 1 if on else 0))

@@ -44,6 +44,7 @@ struct StageArgs {
   const T* hp0;
   const double* p3og;
   const T* bgG;  // G00 of the f00 background (plane 0 stores f00 minus it), or null
+  const double* coef;  // term coefficients in visit order (vlasov_stage_spec only)
   T p0p1_sq, inv_mp0p1_sq, g_r;
   T c05h, hh, h6, third, twothird;
   int np, nx, ny, nc;

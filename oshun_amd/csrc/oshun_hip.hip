@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "stage_spec.hpp"
 #include "oshun_hip.h"
 
 using oshun::HostTables;
@@ -58,6 +59,11 @@ struct oshun_ctx {
   double* d_mw = nullptr;   // moment weights w2 | w3 | w4 | w5
   double* d_mom = nullptr;  // moments output, 7 planes of nxy
   bool use1d = false;       // specialised 1D m0 = 0 Vlasov kernel
+  // general system: index into OSHUN_STAGE_SPECS of the compile-time stage
+  // kernel for this (l0, m0, 2D), -1 for the term-list interpreter
+  int spec = -1;
+  bool force_general = false;
+  double* d_coef = nullptr;  // term coefficients in the spec kernel's visit order
   double* d_coll = nullptr;
   double* d_rt = nullptr;   // collide_rows tables of the e-e launch (interleaved for rows_npl/rows_segw)
   double* d_rt2 = nullptr;  // of the implicit l >= 1 launch (rows_npl2/rows_segw2)
@@ -247,11 +253,25 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
   a.c10 = c->T.comp_re[oshun::hind(1, 0, c->T.m0)];
   a.c11r = c->T.m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, c->T.m0)] : -1;
   a.c11i = c->T.m0 >= 1 ? c->T.comp_im[oshun::hind(1, 1, c->T.m0)] : -1;
+  a.coef = c->d_coef;
   dim3 block(oshun::kTileX, oshun::kTileP, 1);
   dim3 grid((c->nx + oshun::kTileX - 1) / oshun::kTileX, c->npb, c->ny);
   ProfRec r;
   prof_begin(c, 0, &r);
-  hipLaunchKernelGGL(oshun::vlasov_stage<T>, grid, block, 0, c->stream, a);
+  if (c->spec >= 0 && !c->force_general) {
+    switch (c->spec) {
+#define OSHUN_SPEC_LAUNCH(ID, L, M, TD)                                                        \
+      case ID:                                                                                 \
+        hipLaunchKernelGGL((oshun::vlasov_stage_spec<T, L, M, TD>), grid, block, 0, c->stream, a); \
+        break;
+      OSHUN_STAGE_SPECS(OSHUN_SPEC_LAUNCH)
+#undef OSHUN_SPEC_LAUNCH
+      default:
+        return fail(c, OSHUN_ERR_ARG, "no specialised stage kernel for this shape");
+    }
+  } else {
+    hipLaunchKernelGGL(oshun::vlasov_stage<T>, grid, block, 0, c->stream, a);
+  }
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
@@ -677,7 +697,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -775,6 +795,14 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
       upload_vec(c, &c->d_clean, c->T.clean) || upload_vec(c, &c->d_comp_l, c->T.comp_l) ||
       upload_vec(c, &c->d_p3og, c->T.p3og))
     return bail(OSHUN_ERR_HIP);
+  if (!c->use1d) {
+    std::vector<double> coef;
+#define OSHUN_SPEC_FIND(ID, L, M, TD) \
+    if (c->spec < 0 && oshun::spec_coefficients<L, M, TD>(c->T, coef)) c->spec = ID;
+    OSHUN_STAGE_SPECS(OSHUN_SPEC_FIND)
+#undef OSHUN_SPEC_FIND
+    if (c->spec >= 0 && upload_vec(c, &c->d_coef, coef)) return bail(OSHUN_ERR_HIP);
+  }
   {
     std::vector<double> e;
     e.insert(e.end(), c->T.e1d_A1.begin(), c->T.e1d_A1.end());
@@ -1116,6 +1144,14 @@ int oshun_moments(oshun_ctx* c, double* out) {
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(out, c->d_mom, (size_t)7 * c->nxy * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return OSHUN_OK;
+}
+
+int oshun_stage_kernel(oshun_ctx* c, int set, int* which) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  if (set == 0 || set == 1) c->force_general = (set == 1);
+  else if (set != -1) return fail(c, OSHUN_ERR_ARG, "set must be -1, 0 or 1");
+  if (which) *which = c->use1d ? 0 : (c->spec >= 0 && !c->force_general ? 1 : 2);
   return OSHUN_OK;
 }
 
