@@ -262,7 +262,7 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
     switch (c->spec) {
 #define OSHUN_SPEC_LAUNCH(ID, L, M, TD)                                                        \
       case ID:                                                                                 \
-        hipLaunchKernelGGL((oshun::vlasov_stage_spec<T, L, M, TD>), grid, block, 0, c->stream, a); \
+        hipLaunchKernelGGL((oshun::vlasov_stage_spec<T, L, M, TD, oshun::kSpecDepth>), grid, block, 0, c->stream, a); \
         break;
       OSHUN_STAGE_SPECS(OSHUN_SPEC_LAUNCH)
 #undef OSHUN_SPEC_LAUNCH

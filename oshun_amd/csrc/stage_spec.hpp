@@ -106,7 +106,7 @@ template <int L0, int M0, int PH, int J, class S>
 __host__ __device__ constexpr __forceinline__ void spec_source(S& s) {
   constexpr int m = src_m<L0, M0>(J), l = src_l<L0, M0>(J);
   if constexpr (PH == PH_E) {  // ElectricField::Exyz, def-VlasovMax.cpp:346-491
-    s.source(PH_E, l, m);
+    s.template source<PH_E, J>(l, m);
     if constexpr (l == 0) {  // G00 feeds (1,0) and (1,1)
       if (L0 >= 1) {
         spec_e<L0, M0>(s, 1, 0, 0, OP_G00, 0, false);
@@ -125,11 +125,11 @@ __host__ __device__ constexpr __forceinline__ void spec_source(S& s) {
       }
     }
   } else if constexpr (PH == PH_X) {  // Spatial_Advection::Axy, x part, def-VlasovMax.cpp:842-853
-    s.source(PH_X, l, m);
+    s.template source<PH_X, J>(l, m);
     spec_a<L0, M0>(s, l - 1, m, m, OP_DX, false);
     if (l + 1 <= L0) spec_a<L0, M0>(s, l + 1, m, m, OP_DX, false);
   } else if constexpr (PH == PH_Y) {  // y part, def-VlasovMax.cpp:857-947
-    s.source(PH_Y, l, m);
+    s.template source<PH_Y, J>(l, m);
     if (m + 1 <= M0) {
       spec_a<L0, M0>(s, l - 1, m + 1, m, OP_DY, false);
       if (l + 1 <= L0) spec_a<L0, M0>(s, l + 1, m + 1, m, OP_DY, false);
@@ -141,10 +141,10 @@ __host__ __device__ constexpr __forceinline__ void spec_source(S& s) {
       spec_a<L0, M0>(s, l - 1, m - 1, m, OP_DY, false);
       if (l + 1 <= L0) spec_a<L0, M0>(s, l + 1, m - 1, m, OP_DY, false);
     }
-  } else if constexpr (l >= 1) {  // MagneticField::Bxyz, def-VlasovMax.cpp:597-644
-    s.source(PH_B, l, m);
-    if (m + 1 <= M0 && m + 1 <= l) spec_cplx<L0, M0>(s, l, m + 1, m, OP_ID, MUL_BZ, MUL_BY, false);
-    if (m >= 1) {
+  } else {  // MagneticField::Bxyz, def-VlasovMax.cpp:597-644 (no terms from l = 0)
+    s.template source<PH_B, J>(l, m);
+    if (l >= 1 && m + 1 <= M0 && m + 1 <= l) spec_cplx<L0, M0>(s, l, m + 1, m, OP_ID, MUL_BZ, MUL_BY, false);
+    if (l >= 1 && m >= 1) {
       spec_cplx<L0, M0>(s, l, m, m, OP_ID, -1, MUL_BX, false);
       if (m == 1) spec_cplx<L0, M0>(s, l, 0, m, OP_ID, MUL_BZ, MUL_BY, true);
       else spec_cplx<L0, M0>(s, l, m - 1, m, OP_ID, MUL_BZ, MUL_BY, false);
@@ -159,7 +159,7 @@ __host__ __device__ constexpr __forceinline__ void spec_phase(S& s, std::integer
 
 // Every term of Actions::Adv in emission order, as straight-line code (no
 // loops, so every target index is a constant).  S provides
-//   source(phase, l, m)               before the terms of source harmonic (l, m)
+//   source<phase, j>(l, m)            before the terms of source harmonic j = (l, m)
 //   emit(target, src_im, op, mul)     one term (src_im: the source's imaginary plane)
 template <int L0, int M0, bool TWO_D, class S>
 __host__ __device__ constexpr __forceinline__ void spec_terms(S& s) {
@@ -172,7 +172,8 @@ __host__ __device__ constexpr __forceinline__ void spec_terms(S& s) {
 
 struct SpecCount {
   int n = 0;
-  __host__ __device__ constexpr void source(int, int, int) {}
+  template <int PH, int J>
+  __host__ __device__ constexpr void source(int, int) {}
   __host__ __device__ constexpr void emit(int, int, int, int) { ++n; }
 };
 template <int L0, int M0, bool TWO_D>
@@ -190,7 +191,8 @@ template <int L0, int M0>
 struct SpecRecord {
   int cl = 0, cm = 0;
   std::vector<std::array<int, 5>> e;  // target, source plane, op, mul, source l
-  void source(int, int l, int m) { cl = l; cm = m; }
+  template <int PH, int J>
+  void source(int l, int m) { cl = l; cm = m; }
   void emit(int t, int sim, int op, int mul) { e.push_back({t, SpecShape<L0, M0>::re(cl, cm) + sim, op, mul, cl}); }
 };
 
@@ -217,65 +219,149 @@ bool spec_coefficients(const HostTables& T, std::vector<double>& coef) {
 }
 
 // ---------------------------------------------------------------------------
-// Device: operator values of the current source harmonic, accumulators of
-// every target plane.
+// Device.  A wavefront is 64 x at one (p, y).  The work is a fixed sequence of
+// steps: every source harmonic of every phase, then the RK4 combination of
+// the planes in chunks of kSpecChunk.  The loads of step g + D - 1 are issued
+// at step g (a ring of D register slots), so each wavefront has D steps of
+// loads in flight while it adds the terms of the current one.
 
-template <typename T, int L0, int M0>
-struct SpecEval {
+template <int L0, int M0>
+__host__ __device__ constexpr int plane_l(int c) {
+  for (int l = 0; l <= L0; ++l)
+    for (int m = 0; m <= (l < M0 ? l : M0); ++m) {
+      const int n = m == 0 ? 1 : 2;
+      if (c < n) return l;
+      c -= n;
+    }
+  return -1;
+}
+
+constexpr int kSpecDepth = 3;  // steps of loads in flight per wavefront
+constexpr int kSpecChunk = 3;  // planes per RK4 step (y0 and acc of each: 6 values, the size of a slot)
+
+template <int L0, int M0, bool TWO_D>
+struct SpecSeq {
+  static constexpr int NS = num_sources<L0, M0>();
+  static constexpr int NPH = 2 + ((TWO_D && M0 >= 1) ? 1 : 0) + (M0 >= 1 ? 1 : 0);
+  static constexpr int NSRC = NPH * NS;
   static constexpr int NC = SpecShape<L0, M0>::NC;
+  static constexpr int NCH = (NC + kSpecChunk - 1) / kSpecChunk;
+  static constexpr int NG = NSRC + NCH;
+  // k-th phase present: E, X, then Y (2D with m0 >= 1), then B (m0 >= 1)
+  __host__ __device__ static constexpr int phase(int k) {
+    return k == 0 ? PH_E : k == 1 ? PH_X : (k == 2 && TWO_D && M0 >= 1) ? PH_Y : PH_B;
+  }
+  __host__ __device__ static constexpr int base(int ph) {
+    for (int k = 0; k < NPH; ++k)
+      if (phase(k) == ph) return k * NS;
+    return -1;
+  }
+};
+
+template <typename T, int L0, int M0, bool TWO_D, int D>
+struct SpecEval {
+  using Sq = SpecSeq<L0, M0, TWO_D>;
+  using Sh = SpecShape<L0, M0>;
+  static constexpr int NC = Sh::NC;
   const StageArgs<T>& a;
-  long long idx, xm, xp, ym, yp;
+  long long idx, xm, xp, ym, yp, pm, pn;
   int p;
   double F[6];
   T v, bgg;
   T acc[NC];
-  T val[2][2];  // [re | im][G, Dx, Dy or f | H]
+  T raw[D][6];  // loads in flight, one slot per step
+  T val[2][2];  // [re | im][G, Dx, Dy or f | H] of the current source
   const double* coef;  // the coefficients in LDS
   int k;               // next coefficient
 
   __device__ __forceinline__ SpecEval(const StageArgs<T>& args, const double* c) : a(args), coef(c), k(0) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) acc[c] = T(0);
+    for (int q = 0; q < NC; ++q) acc[q] = T(0);
   }
 
-  __device__ __forceinline__ void source(int ph, int l, int m) {
-    // loads of one source harmonic at a time: without this fence the compiler
-    // issues every load of the kernel up front (and keeps f(p) of every
-    // plane from the E field for the B rotation), which needs more registers
-    // than a SIMD has for two wavefronts
-    asm volatile("" ::: "memory");
-    // and the accumulators as they stand: otherwise every sum is sunk to the
-    // stores at the end, keeping all operator values and coefficients live
+  // loads of step G into slot G % D
+  template <int G>
+  __device__ __forceinline__ void issue() {
+    if constexpr (G < Sq::NG) {
+      T* r = raw[G % D];
+      if constexpr (G < Sq::NSRC) {
+        constexpr int ph = Sq::phase(G / Sq::NS), J = G % Sq::NS;
+        constexpr int l = src_l<L0, M0>(J), m = src_m<L0, M0>(J);
+        if constexpr (!(ph == PH_B && l == 0)) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) asm volatile("" : "+v"(acc[c]));
-    plane_ops(ph, l, SpecShape<L0, M0>::re(l, m), 0);
-    if (m > 0) plane_ops(ph, l, SpecShape<L0, M0>::im(l, m), 1);
+          for (int s = 0; s < (m > 0 ? 2 : 1); ++s) {
+            const T* __restrict__ f = a.in + (long long)(Sh::re(l, m) + s) * a.plane;
+            if constexpr (ph == PH_E) {
+              r[3 * s] = f[idx + pm];
+              r[3 * s + 1] = f[idx];
+              r[3 * s + 2] = f[idx + pn];
+            } else if constexpr (ph == PH_X) {
+              r[3 * s] = f[idx + xm];
+              r[3 * s + 2] = f[idx + xp];
+            } else if constexpr (ph == PH_Y) {
+              r[3 * s] = f[idx + ym];
+              r[3 * s + 2] = f[idx + yp];
+            } else {
+              r[3 * s + 1] = f[idx];
+            }
+          }
+        }
+      } else {
+        constexpr int c0 = (G - Sq::NSRC) * kSpecChunk;
+#pragma unroll
+        for (int q = 0; q < kSpecChunk; ++q) {
+          if (c0 + q < NC) {
+            const long long o = (long long)(c0 + q) * a.plane + idx;
+            if (a.mode >= 1 && a.mode <= 3) r[2 * q] = a.y0[o];
+            if (a.mode >= 2) r[2 * q + 1] = a.acc[o];
+          }
+        }
+      }
+    }
+  }
+  template <int... I>
+  __device__ __forceinline__ void prologue(std::integer_sequence<int, I...>) {
+    (issue<I>(), ...);
   }
 
-  __device__ __forceinline__ void plane_ops(int ph, int l, int c, int s) {
-    {
-      const T* __restrict__ f = a.in + (long long)c * a.plane;
-      if (ph == PH_X) {
-        val[s][0] = f[idx + xm] - f[idx + xp];
-      } else if (ph == PH_Y) {
-        val[s][0] = f[idx + ym] - f[idx + yp];
-      } else if (ph == PH_B) {
-        val[s][0] = f[idx];
+  // step G: fence, the loads of step G + D - 1, then wait for this step's own
+  template <int G>
+  __device__ __forceinline__ void advance() {
+    // the fence keeps the loads in step order (the compiler would otherwise
+    // issue all of them up front); the accumulators are pinned as they stand,
+    // or every sum is sunk to the stores at the end with all its operands live
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int q = 0; q < NC; ++q) asm volatile("" : "+v"(acc[q]));
+    if constexpr (G == 0) prologue(std::make_integer_sequence<int, D - 1>{});
+    issue<G + D - 1>();
+  }
+
+  template <int PH, int J>
+  __device__ __forceinline__ void source(int l, int m) {
+    constexpr int G = Sq::base(PH) + J;
+    advance<G>();
+    const T* r = raw[G % D];
+#pragma unroll
+    for (int s = 0; s < (m > 0 ? 2 : 1); ++s) {
+      if (PH == PH_X || PH == PH_Y) {
+        val[s][0] = r[3 * s] - r[3 * s + 2];
+      } else if (PH == PH_B) {
+        val[s][0] = r[3 * s + 1];
       } else {  // G and H of MakeGH / G00, def-VlasovMax.cpp:496-535, as op_value
-        const T f0 = f[idx];
+        const T fm = r[3 * s], f0 = r[3 * s + 1], fn = r[3 * s + 2];
         if (p == 0) {
-          const T f1 = f[idx + a.nx];
           if (l == 0) {
-            const T fe = (f0 - f1 * a.p0p1_sq) * a.inv_mp0p1_sq;
-            val[s][0] = (f1 - fe) * a.g_r;
+            const T fe = (f0 - fn * a.p0p1_sq) * a.inv_mp0p1_sq;
+            val[s][0] = (fn - fe) * a.g_r;
             if (a.bgG) val[s][0] += bgg;
           } else {
             val[s][0] = T(0);
-            val[s][1] = f1 * ldc(a.hp0, l);
+            val[s][1] = fn * ldc(a.hp0, l);
           }
         } else {
-          const T fm = f[idx - a.nx];
-          const T dpf = (p == a.np - 1) ? (fm - f0) : (fm - f[idx + a.nx]);
+          // SHarmonic::Dp, def-State.cpp:359-380: f(N-2) - f(N-1) at pmax
+          const T dpf = (p == a.np - 1) ? (fm - f0) : (fm - fn);
           if (l == 0) {
             val[s][0] = dpf;
             if (a.bgG) val[s][0] += bgg;
@@ -299,61 +385,60 @@ struct SpecEval {
     else mult = static_cast<T>(F[mul] * c);
     acc[t] += x * mult;
   }
+
+  // CLEAN_LOW_PC (def-Actions.cpp:240-253) and the RK4 combination
+  // (def-RK.cpp:116-138) of the planes of chunk CH
+  template <int CH>
+  __device__ __forceinline__ void rk_chunk() {
+    constexpr int G = Sq::NSRC + CH;
+    advance<G>();
+    const T* r = raw[G % D];
+#pragma unroll
+    for (int q = 0; q < kSpecChunk; ++q) {
+      const int c = CH * kSpecChunk + q;
+      if (c < NC) {
+        const int l = plane_l<L0, M0>(c);
+        T sl = acc[c < NC ? c : 0];
+        if (l >= 1 && p < (l - 1 < 80 ? l - 1 : 80)) sl = T(0);
+        const long long o = (long long)c * a.plane + idx;
+        const T y0 = r[2 * q], ac = r[2 * q + 1];
+        switch (a.mode) {
+          case 0:
+            a.out[o] = sl;
+            break;
+          case 1: {
+            const T kh = sl * a.c05h;
+            a.out[o] = y0 + kh;
+            a.acc[o] = y0 + kh * a.third;
+          } break;
+          case 2: {
+            const T kh = sl * a.c05h;
+            a.out[o] = y0 + kh;
+            a.acc[o] = ac + kh * a.twothird;
+          } break;
+          case 3: {
+            const T kh = sl * a.hh;
+            a.out[o] = y0 + kh;
+            a.acc[o] = ac + kh * a.third;
+          } break;
+          default: {
+            const T kh = sl * a.h6;
+            a.out[o] = ac + kh;
+          } break;
+        }
+      }
+    }
+  }
+  template <int... CH>
+  __device__ __forceinline__ void rk_all(std::integer_sequence<int, CH...>) {
+    (rk_chunk<CH>(), ...);
+  }
 };
 
-template <int L0, int M0>
-__host__ __device__ constexpr int plane_l(int c) {
-  for (int l = 0; l <= L0; ++l)
-    for (int m = 0; m <= (l < M0 ? l : M0); ++m) {
-      const int n = m == 0 ? 1 : 2;
-      if (c < n) return l;
-      c -= n;
-    }
-  return -1;
-}
-
-// CLEAN_LOW_PC (def-Actions.cpp:240-253) and the RK4 combination of one
-// plane (def-RK.cpp:116-138)
-template <typename T, int L0, int M0, int C>
-__device__ __forceinline__ void spec_rk_plane(const StageArgs<T>& a, T acc, long long idx, int p) {
-  constexpr int l = plane_l<L0, M0>(C);
-  if (l >= 1 && p < (l - 1 < 80 ? l - 1 : 80)) acc = T(0);
-  const long long o = (long long)C * a.plane + idx;
-  switch (a.mode) {
-    case 0:
-      a.out[o] = acc;
-      break;
-    case 1: {
-      const T y0 = a.y0[o];
-      const T kh = acc * a.c05h;
-      a.out[o] = y0 + kh;
-      a.acc[o] = y0 + kh * a.third;
-    } break;
-    case 2: {
-      const T kh = acc * a.c05h;
-      a.out[o] = a.y0[o] + kh;
-      a.acc[o] = a.acc[o] + kh * a.twothird;
-    } break;
-    case 3: {
-      const T kh = acc * a.hh;
-      a.out[o] = a.y0[o] + kh;
-      a.acc[o] = a.acc[o] + kh * a.third;
-    } break;
-    default: {
-      const T kh = acc * a.h6;
-      a.out[o] = a.acc[o] + kh;
-    } break;
-  }
-}
-template <typename T, int L0, int M0, int NC, int... C>
-__device__ __forceinline__ void spec_rk(const StageArgs<T>& a, const T (&acc)[NC], long long idx, int p,
-                                        std::integer_sequence<int, C...>) {
-  (spec_rk_plane<T, L0, M0, C>(a, acc[C], idx, p), ...);
-}
-
-template <typename T, int L0, int M0, bool TWO_D>
+template <typename T, int L0, int M0, bool TWO_D, int D>
 __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T> a) {
   using Sh = SpecShape<L0, M0>;
+  using Sq = SpecSeq<L0, M0, TWO_D>;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int x = blockIdx.x * kTileX + tx;
   const int p = __builtin_amdgcn_readfirstlane(blockIdx.y * kTileP + ty);  // one p per wavefront
@@ -369,7 +454,7 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T>
   __syncthreads();
   double j10 = 0.0, j11r = 0.0, j11i = 0.0;
   if (x < nx && p < np) {
-    SpecEval<T, L0, M0> e(a, csh);
+    SpecEval<T, L0, M0, TWO_D, D> e(a, csh);
     e.p = p;
     e.idx = ((long long)y * np + p) * nx + x;
     e.xm = (x == 0) ? (nx - 1) : -1;
@@ -377,15 +462,16 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T>
     const long long ystride = (long long)np * nx;
     e.ym = (y == 0) ? (long long)(ny - 1) * ystride : -ystride;
     e.yp = (y == ny - 1) ? -(long long)(ny - 1) * ystride : ystride;
+    e.pm = p > 0 ? -(long long)nx : 0;       // p-1 and p+1 rows, clamped at the ends where
+    e.pn = p < np - 1 ? (long long)nx : 0;   // MakeGH / G00 do not use them
     const int ixy = y * nx + x;
 #pragma unroll
     for (int c = 0; c < 6; ++c) e.F[c] = a.fin[(long long)c * nxy + ixy];
     e.v = ldc(a.vtab, p);
     e.bgg = a.bgG ? ldc(a.bgG, p) : T(0);
     spec_terms<L0, M0, TWO_D>(e);
-    asm volatile("" ::: "memory");
+    e.rk_all(std::make_integer_sequence<int, Sq::NCH>{});
     const long long idx = e.idx;
-    spec_rk<T, L0, M0>(a, e.acc, idx, p, std::make_integer_sequence<int, Sh::NC>{});
     // trapezoid weights of Current::Jxyz: 1/2 at both ends of the p axis
     const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
     const double g = a.p3og[p];
