@@ -795,7 +795,7 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
       upload_vec(c, &c->d_clean, c->T.clean) || upload_vec(c, &c->d_comp_l, c->T.comp_l) ||
       upload_vec(c, &c->d_p3og, c->T.p3og))
     return bail(OSHUN_ERR_HIP);
-  if (!c->use1d) {
+  if (!c->use1d && (unsigned long long)c->plane * sizeof(double) < (1ull << 32)) {  // 32-bit plane offsets
     std::vector<double> coef;
 #define OSHUN_SPEC_FIND(ID, L, M, TD) \
     if (c->spec < 0 && oshun::spec_coefficients<L, M, TD>(c->T, coef)) c->spec = ID;

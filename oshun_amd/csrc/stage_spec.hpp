@@ -236,7 +236,7 @@ __host__ __device__ constexpr int plane_l(int c) {
   return -1;
 }
 
-constexpr int kSpecDepth = 3;  // steps of loads in flight per wavefront
+constexpr int kSpecDepth = 4;  // steps of loads in flight per wavefront
 constexpr int kSpecChunk = 3;  // planes per RK4 step (y0 and acc of each: 6 values, the size of a slot)
 
 template <int L0, int M0, bool TWO_D>
@@ -264,7 +264,9 @@ struct SpecEval {
   using Sh = SpecShape<L0, M0>;
   static constexpr int NC = Sh::NC;
   const StageArgs<T>& a;
-  long long idx, xm, xp, ym, yp, pm, pn;
+  // byte offsets within a plane of (x,p,y) and of its stencil neighbours;
+  // with a wave-uniform plane base the loads take the SGPR-base form
+  unsigned o0, oxm, oxp, oym, oyp, opm, opn;
   int p;
   double F[6];
   T v, bgg;
@@ -273,6 +275,13 @@ struct SpecEval {
   T val[2][2];  // [re | im][G, Dx, Dy or f | H] of the current source
   const double* coef;  // the coefficients in LDS
   int k;               // next coefficient
+
+  static __device__ __forceinline__ T at(const T* base, unsigned off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+  }
+  static __device__ __forceinline__ void put(T* base, unsigned off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+  }
 
   __device__ __forceinline__ SpecEval(const StageArgs<T>& args, const double* c) : a(args), coef(c), k(0) {
 #pragma unroll
@@ -292,17 +301,17 @@ struct SpecEval {
           for (int s = 0; s < (m > 0 ? 2 : 1); ++s) {
             const T* __restrict__ f = a.in + (long long)(Sh::re(l, m) + s) * a.plane;
             if constexpr (ph == PH_E) {
-              r[3 * s] = f[idx + pm];
-              r[3 * s + 1] = f[idx];
-              r[3 * s + 2] = f[idx + pn];
+              r[3 * s] = at(f, opm);
+              r[3 * s + 1] = at(f, o0);
+              r[3 * s + 2] = at(f, opn);
             } else if constexpr (ph == PH_X) {
-              r[3 * s] = f[idx + xm];
-              r[3 * s + 2] = f[idx + xp];
+              r[3 * s] = at(f, oxm);
+              r[3 * s + 2] = at(f, oxp);
             } else if constexpr (ph == PH_Y) {
-              r[3 * s] = f[idx + ym];
-              r[3 * s + 2] = f[idx + yp];
+              r[3 * s] = at(f, oym);
+              r[3 * s + 2] = at(f, oyp);
             } else {
-              r[3 * s + 1] = f[idx];
+              r[3 * s + 1] = at(f, o0);
             }
           }
         }
@@ -311,9 +320,9 @@ struct SpecEval {
 #pragma unroll
         for (int q = 0; q < kSpecChunk; ++q) {
           if (c0 + q < NC) {
-            const long long o = (long long)(c0 + q) * a.plane + idx;
-            if (a.mode >= 1 && a.mode <= 3) r[2 * q] = a.y0[o];
-            if (a.mode >= 2) r[2 * q + 1] = a.acc[o];
+            const long long pl = (long long)(c0 + q) * a.plane;
+            if (a.mode >= 1 && a.mode <= 3) r[2 * q] = at(a.y0 + pl, o0);
+            if (a.mode >= 2) r[2 * q + 1] = at(a.acc + pl, o0);
           }
         }
       }
@@ -400,30 +409,32 @@ struct SpecEval {
         const int l = plane_l<L0, M0>(c);
         T sl = acc[c < NC ? c : 0];
         if (l >= 1 && p < (l - 1 < 80 ? l - 1 : 80)) sl = T(0);
-        const long long o = (long long)c * a.plane + idx;
+        const long long pl = (long long)c * a.plane;
+        T* out = a.out + pl;
+        T* acw = a.acc + pl;
         const T y0 = r[2 * q], ac = r[2 * q + 1];
         switch (a.mode) {
           case 0:
-            a.out[o] = sl;
+            put(out, o0, sl);
             break;
           case 1: {
             const T kh = sl * a.c05h;
-            a.out[o] = y0 + kh;
-            a.acc[o] = y0 + kh * a.third;
+            put(out, o0, y0 + kh);
+            put(acw, o0, y0 + kh * a.third);
           } break;
           case 2: {
             const T kh = sl * a.c05h;
-            a.out[o] = y0 + kh;
-            a.acc[o] = ac + kh * a.twothird;
+            put(out, o0, y0 + kh);
+            put(acw, o0, ac + kh * a.twothird);
           } break;
           case 3: {
             const T kh = sl * a.hh;
-            a.out[o] = y0 + kh;
-            a.acc[o] = ac + kh * a.third;
+            put(out, o0, y0 + kh);
+            put(acw, o0, ac + kh * a.third);
           } break;
           default: {
             const T kh = sl * a.h6;
-            a.out[o] = ac + kh;
+            put(out, o0, ac + kh);
           } break;
         }
       }
@@ -456,14 +467,17 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T>
   if (x < nx && p < np) {
     SpecEval<T, L0, M0, TWO_D, D> e(a, csh);
     e.p = p;
-    e.idx = ((long long)y * np + p) * nx + x;
-    e.xm = (x == 0) ? (nx - 1) : -1;
-    e.xp = (x == nx - 1) ? -(long long)(nx - 1) : 1;
-    const long long ystride = (long long)np * nx;
-    e.ym = (y == 0) ? (long long)(ny - 1) * ystride : -ystride;
-    e.yp = (y == ny - 1) ? -(long long)(ny - 1) * ystride : ystride;
-    e.pm = p > 0 ? -(long long)nx : 0;       // p-1 and p+1 rows, clamped at the ends where
-    e.pn = p < np - 1 ? (long long)nx : 0;   // MakeGH / G00 do not use them
+    // element offsets within a plane (the host requires plane * 8 bytes < 4 GiB)
+    const unsigned idx = ((unsigned)y * np + p) * nx + x;
+    const unsigned ystride = (unsigned)np * nx;
+    const unsigned es = sizeof(T);
+    e.o0 = idx * es;
+    e.oxm = (x == 0 ? idx + (nx - 1) : idx - 1) * es;
+    e.oxp = (x == nx - 1 ? idx - (nx - 1) : idx + 1) * es;
+    e.oym = (y == 0 ? idx + (ny - 1) * ystride : idx - ystride) * es;
+    e.oyp = (y == ny - 1 ? idx - (ny - 1) * ystride : idx + ystride) * es;
+    e.opm = (p > 0 ? idx - nx : idx) * es;        // p-1 and p+1 rows, clamped at the ends where
+    e.opn = (p < np - 1 ? idx + nx : idx) * es;   // MakeGH / G00 do not use them
     const int ixy = y * nx + x;
 #pragma unroll
     for (int c = 0; c < 6; ++c) e.F[c] = a.fin[(long long)c * nxy + ixy];
@@ -471,7 +485,6 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T>
     e.bgg = a.bgG ? ldc(a.bgG, p) : T(0);
     spec_terms<L0, M0, TWO_D>(e);
     e.rk_all(std::make_integer_sequence<int, Sq::NCH>{});
-    const long long idx = e.idx;
     // trapezoid weights of Current::Jxyz: 1/2 at both ends of the p axis
     const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
     const double g = a.p3og[p];
