@@ -258,7 +258,14 @@ struct SpecSeq {
   }
 };
 
-template <typename T, int L0, int M0, bool TWO_D, int D>
+// the LDS row cache of a workgroup (4 wavefronts x every plane x 64 lanes),
+// used where it leaves room for three workgroups per CU
+template <typename T, int L0, int M0>
+__host__ __device__ constexpr bool spec_lds() {
+  return SpecShape<L0, M0>::NC * kTileX * kTileP * sizeof(T) <= 52 * 1024;
+}
+
+template <typename T, int L0, int M0, bool TWO_D, int D, bool LDS = spec_lds<T, L0, M0>()>
 struct SpecEval {
   using Sq = SpecSeq<L0, M0, TWO_D>;
   using Sh = SpecShape<L0, M0>;
@@ -272,6 +279,12 @@ struct SpecEval {
   T v, bgg;
   T acc[NC];
   T raw[D][6];  // loads in flight, one slot per step
+  // LDS row cache (when it fits, see spec_lds): the E field stores f(p) of
+  // every plane in the wavefront's own region, the x advection and the B
+  // rotation read it back (no re-fetch of rows the L2 has already evicted);
+  // each wavefront reads only what it wrote, so no barrier is needed
+  T* lrow;     // this lane's slot of plane 0
+  bool xl, xr;  // the x-1 / x+1 neighbour is in the cached row (else from memory)
   T val[2][2];  // [re | im][G, Dx, Dy or f | H] of the current source
   const double* coef;  // the coefficients in LDS
   int k;               // next coefficient
@@ -300,18 +313,25 @@ struct SpecEval {
 #pragma unroll
           for (int s = 0; s < (m > 0 ? 2 : 1); ++s) {
             const T* __restrict__ f = a.in + (long long)(Sh::re(l, m) + s) * a.plane;
+            const int c = Sh::re(l, m) + s;
             if constexpr (ph == PH_E) {
               r[3 * s] = at(f, opm);
               r[3 * s + 1] = at(f, o0);
               r[3 * s + 2] = at(f, opn);
             } else if constexpr (ph == PH_X) {
-              r[3 * s] = at(f, oxm);
-              r[3 * s + 2] = at(f, oxp);
+              if constexpr (LDS) {
+                r[3 * s] = xl ? lrow[c * kTileX - 1] : at(f, oxm);
+                r[3 * s + 2] = xr ? lrow[c * kTileX + 1] : at(f, oxp);
+              } else {
+                r[3 * s] = at(f, oxm);
+                r[3 * s + 2] = at(f, oxp);
+              }
             } else if constexpr (ph == PH_Y) {
               r[3 * s] = at(f, oym);
               r[3 * s + 2] = at(f, oyp);
             } else {
-              r[3 * s + 1] = at(f, o0);
+              if constexpr (LDS) r[3 * s + 1] = lrow[c * kTileX];
+              else r[3 * s + 1] = at(f, o0);
             }
           }
         }
@@ -359,6 +379,7 @@ struct SpecEval {
         val[s][0] = r[3 * s + 1];
       } else {  // G and H of MakeGH / G00, def-VlasovMax.cpp:496-535, as op_value
         const T fm = r[3 * s], f0 = r[3 * s + 1], fn = r[3 * s + 2];
+        if constexpr (LDS) lrow[(Sh::re(l, m) + s) * kTileX] = f0;
         if (p == 0) {
           if (l == 0) {
             const T fe = (f0 - fn * a.p0p1_sq) * a.inv_mp0p1_sq;
@@ -456,11 +477,16 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T>
   const int y = blockIdx.z;
   const int nx = a.nx, np = a.np, ny = a.ny;
   const int nxy = nx * ny;
-  __shared__ double jsh[3][kTileP][kTileX];
   // the coefficients, read by every lane at the same address (LDS broadcast):
   // as scalar loads the compiler hoists all of them and spills SGPRs
   constexpr int NT = spec_nterms<L0, M0, TWO_D>();
   __shared__ double csh[NT];
+  constexpr bool LDS = spec_lds<T, L0, M0>();
+  constexpr int kJsh = 3 * kTileP * kTileX * sizeof(double) / sizeof(T);
+  constexpr int kRows = LDS ? Sh::NC * kTileX * kTileP : 1;
+  // the row cache, and after the barrier below the current partial sums
+  __shared__ __align__(16) T lds_rows[kRows > kJsh ? kRows : kJsh];
+  double(*jsh)[kTileP][kTileX] = reinterpret_cast<double(*)[kTileP][kTileX]>(lds_rows);
   for (int i = ty * kTileX + tx; i < NT; i += kTileX * kTileP) csh[i] = a.coef[i];
   __syncthreads();
   double j10 = 0.0, j11r = 0.0, j11i = 0.0;
@@ -478,6 +504,9 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T>
     e.oyp = (y == ny - 1 ? idx - (ny - 1) * ystride : idx + ystride) * es;
     e.opm = (p > 0 ? idx - nx : idx) * es;        // p-1 and p+1 rows, clamped at the ends where
     e.opn = (p < np - 1 ? idx + nx : idx) * es;   // MakeGH / G00 do not use them
+    e.lrow = lds_rows + (LDS ? ty * Sh::NC * kTileX + tx : 0);
+    e.xl = tx > 0;                                // lane x-1 of this row (x = 0 wraps: memory)
+    e.xr = tx < kTileX - 1 && x < nx - 1;         // lane x+1 (tile edge and x = nx-1: memory)
     const int ixy = y * nx + x;
 #pragma unroll
     for (int c = 0; c < 6; ++c) e.F[c] = a.fin[(long long)c * nxy + ixy];
@@ -494,6 +523,7 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T>
       j11i = w * (static_cast<double>(a.in[(long long)Sh::im(1, 1) * a.plane + idx]) * g);
     }
   }
+  __syncthreads();  // every wavefront is done with its cached rows
   jsh[0][ty][tx] = j10;
   jsh[1][ty][tx] = j11r;
   jsh[2][ty][tx] = j11i;
