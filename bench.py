@@ -264,15 +264,20 @@ def _config_cases():
     return cases
 
 
-def run_configs(steps, warmup):
+def run_configs(steps, warmup, only=None, prec_only=None):
     """cell-updates/s of the RK4 + FP step on each synthetic configuration, FP64 and FP32 split, with the
-    step's algorithmic HBM bytes (16 S_f) over its time as the roofline fraction."""
+    step's algorithmic HBM bytes (16 S_f) over its time as the roofline fraction.  only: a configuration
+    name prefix ("C3"); prec_only: "fp64" or "fp32"."""
     import oshun_amd
     out = []
     for name, (l0, m0, np_, nx, ny), kw, h, f, fld in _config_cases():
+        if only and not name.startswith(only):
+            continue
         cells = oshun_amd.num_harmonics(l0, m0) * np_ * nx * ny
         row = {"config": name, "cells": cells}
         for prec, tag in ((oshun_amd.FP64, "fp64"), (oshun_amd.FP32_SPLIT, "fp32_split")):
+            if prec_only and not tag.startswith(prec_only):
+                continue
             ctx = oshun_amd.Context(l0, m0, np_, nx, ny, precision=prec, **kw)
             ctx.upload(f, fld)
             ctx.step(h, warmup)
@@ -286,7 +291,8 @@ def run_configs(steps, warmup):
                 raise RuntimeError(f"non-finite state in {name} ({tag})")
             S = ctx.state_bytes()
             row[tag] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
-                        "step_roofline_frac": 16.0 * S / (ms * 1e-3) / HBM_PEAK}
+                        "step_roofline_frac": 16.0 * S / (ms * 1e-3) / HBM_PEAK,
+                        "stage_kernel": ctx.stage_kernel()}
             ctx.close()
         out.append(row)
     return {"metric": "phase-space cell-updates/s per RK4+FP step on the synthetic configurations",
@@ -363,6 +369,7 @@ def main():
                     help="the batched ensemble (512 independent 1D runs) instead of the single-run workload")
     ap.add_argument("--configs", action="store_true",
                     help="throughput of the RK4 + FP step on the synthetic configurations C1, C2, C3, C5")
+    ap.add_argument("--config", default=None, help="with --configs: only the configuration of this name (C3)")
     ap.add_argument("--implicit", action="store_true",
                     help="the implicit-E step of the 1D variant instead of the RK4 step (FP64)")
     args = ap.parse_args()
@@ -373,7 +380,7 @@ def main():
     import oshun_amd
 
     if args.configs:
-        print(json.dumps(run_configs(args.steps, args.warmup)))
+        print(json.dumps(run_configs(args.steps, args.warmup, args.config, args.only)))
         return
     if args.implicit:
         print(json.dumps(run_implicit(args.steps, args.warmup, args.no_cpu)))
