@@ -147,6 +147,14 @@ int oshun_diag_copy(oshun_ctx* ctx, long long nbytes);
  * 0 restores the default, -1 only queries.  Both general kernels evaluate
  * Actions::Adv (def-Actions.cpp:73-94) with the same per-target sums. */
 int oshun_stage_kernel(oshun_ctx* ctx, int set, int* which);
+/* Tile height (p rows) of the 1D stage-pair kernels: 10 by default, 6 where
+ * the grid at 10 would give fewer than two workgroups per CU (latency-bound
+ * small runs, l0 <= 8).  set = 6 or 10 chooses it, 0 only queries (*rows = 0
+ * on contexts without stage pairs).  The tile height groups the p sum of the
+ * current differently (results agree to rounding, not bitwise); a batch
+ * member equals its run alone bitwise at the same height.  No counterpart in
+ * the reference. */
+int oshun_pair_rows(oshun_ctx* ctx, int set, int* rows);
 
 /* ---- Batched independent runs ------------------------------------------
  * Many 1D runs (m0 = 0, ny = 1, l0 <= 8, one precision for all) of uneven

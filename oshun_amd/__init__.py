@@ -110,6 +110,7 @@ def _load():
     lib.oshun_profile_read.argtypes = [P, dp, ctypes.POINTER(ctypes.c_longlong), I]
     lib.oshun_diag_copy.argtypes = [P, ctypes.c_longlong]
     lib.oshun_stage_kernel.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.oshun_pair_rows.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.oshun_moments.argtypes = [P, dp]
     lib.oshun_batch_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Phys), dp, I, ctypes.POINTER(P)]
     lib.oshun_batch_destroy.argtypes = [P]
@@ -310,6 +311,12 @@ class Context:
         w = ctypes.c_int(-1)
         self._chk(self._lib.oshun_stage_kernel(self._h, int(set), ctypes.byref(w)))
         return ("1d", "spec", "general")[w.value]
+
+    def pair_rows(self, set=0):
+        """Tile height (p rows) of the 1D stage-pair kernels; set=6 or 10 chooses it, 0 only queries."""
+        r = ctypes.c_int(0)
+        self._chk(self._lib.oshun_pair_rows(self._h, int(set), ctypes.byref(r)))
+        return r.value
 
     def profile(self, on=True):
         self._chk(self._lib.oshun_profile_enable(self._h, 1 if on else 0))

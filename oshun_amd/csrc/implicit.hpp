@@ -212,7 +212,7 @@ void launch_imp_pair(oshun_ctx* c, int pair, const oshun::Stage1DArgs<double>& a
 int imp_rk2_pair(oshun_ctx* c, double h, int pair) {
   const oshun::Stage1DArgs<double> a = pair_args<double>(c, c->S, c->S, c->B1, c->A, c->Sf, c->Sf, c->jp1, c->jp2, h);
   const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
-  const dim3 grid((ntx * pair_tiles<double>(c) + 7) / 8 * 8);
+  const dim3 grid((ntx * pair_tiles_r(c->np, pair_rows<double>()) + 7) / 8 * 8);  // the RK2 kernels keep R = 3
   switch (c->T.l0) {
     case 3: launch_imp_pair<3>(c, pair, a, grid); break;
     case 4: launch_imp_pair<4>(c, pair, a, grid); break;

@@ -74,6 +74,7 @@ struct oshun_ctx {
   float* d_bgG = nullptr;
   // 1D m0 = 0: RK4 as two fused stage pairs (vlasov_pair_1d)
   bool use_pair = false;
+  int pair_r = 3;              // rows per thread of the RK4 pair kernels (tile height 4 pair_r - 2)
   int npt = 0;                 // p tiles of the pair kernels
   double* jp1 = nullptr;       // of the second stage input of a pair [npt][3][nx]
   double* jp2 = nullptr;       // of Y2 [npt][3][nx]
@@ -335,16 +336,29 @@ int rk4(oshun_ctx* c, double h) {
 
 // rows per thread of the pair kernels (tile height 4R-2 p rows): 3 (10 rows);
 // measured at the target shape, 14-row float tiles were 15% slower (less halo
-// re-read but fewer wavefronts), 6-row tiles re-read more halo
+// re-read but fewer wavefronts), 6-row tiles re-read more halo.  Grids of
+// fewer than two workgroups per CU at R = 3 take R = 2 (c->pair_r): there the
+// launch is latency bound and more, shorter row marches finish sooner.
 template <typename T>
 constexpr int pair_rows() { return 3; }
 constexpr int kPairTP = 4 * 3 - 2;
+constexpr int kPairSmallL0 = 8;  // R = 2 single-run kernels exist for l0 <= 8
+inline int pair_tiles_r(int np, int r) { return (np + 4 * r - 3) / (4 * r - 2); }
 template <typename T>
-int pair_tiles(const oshun_ctx* c) { return (c->np + 4 * pair_rows<T>() - 3) / (4 * pair_rows<T>() - 2); }
+int pair_tiles(const oshun_ctx* c) { return pair_tiles_r(c->np, c->pair_r); }
 constexpr int kPairTPMin = 4 * 2 - 2;  // smallest tile height the batched path can choose
 
 template <typename T, int L0>
 void launch_pair_l0(oshun_ctx* c, int pair, const oshun::Stage1DArgs<T>& a, dim3 grid) {
+  if constexpr (L0 <= kPairSmallL0) {
+    if (c->pair_r == 2) {
+      if (pair == 0)
+        hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, 2, 0>), grid, dim3(64, 4), 0, c->stream, a);
+      else
+        hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, 2, 1>), grid, dim3(64, 4), 0, c->stream, a);
+      return;
+    }
+  }
   if (pair == 0)
     hipLaunchKernelGGL((oshun::vlasov_pair_1d<T, L0, pair_rows<T>(), 0>), grid, dim3(64, 4), 0, c->stream, a);
   else
@@ -775,7 +789,12 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     return bail(OSHUN_ERR_ALLOC);
   c->use_pair = c->use1d;
   if (c->use_pair) {
-    c->npt = (c->np + kPairTP - 1) / kPairTP;
+    const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
+    const bool fusable = (phys->collisions & OSHUN_COLL_EE) && (phys->collisions & OSHUN_COLL_FLM);
+    if (g.l0 <= kPairSmallL0 && ntx * pair_tiles_r(c->np, 3) < 2 * 256 &&
+        (!fusable || pair_tiles_r(c->np, 3) > oshun::kFuseTiles || pair_tiles_r(c->np, 2) <= oshun::kFuseTiles))
+      c->pair_r = 2;  // small grid (keeping the fused field stages where R = 3 has them)
+    c->npt = pair_tiles_r(c->np, c->pair_r);
     const int npt_max = (c->np + kPairTPMin - 1) / kPairTPMin;  // partial buffers fit every tile height
     const size_t nx = c->nx;
     if (hipMalloc((void**)&c->jp1, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
@@ -1144,6 +1163,20 @@ int oshun_moments(oshun_ctx* c, double* out) {
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(out, c->d_mom, (size_t)7 * c->nxy * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return OSHUN_OK;
+}
+
+int oshun_pair_rows(oshun_ctx* c, int set, int* rows) {
+  if (!sane(c)) return OSHUN_ERR_ARG;
+  if (set != 0) {
+    if (!c->use_pair) return fail(c, OSHUN_ERR_ARG, "no stage-pair kernels on this context (1D, m0 = 0 only)");
+    if (set == 10) c->pair_r = 3;
+    else if (set == 6 && c->T.l0 <= kPairSmallL0) c->pair_r = 2;
+    else return fail(c, OSHUN_ERR_ARG, "tile height must be 10, or 6 for l0 <= 8");
+    c->npt = pair_tiles_r(c->np, c->pair_r);
+    c->next1_ready = false;
+  }
+  if (rows) *rows = c->use_pair ? 4 * c->pair_r - 2 : 0;
   return OSHUN_OK;
 }
 

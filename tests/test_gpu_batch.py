@@ -34,6 +34,7 @@ def _alone(r, f, fld, steps):
     import oshun_amd
     kw = {k: v for k, v in r.items() if k not in ("l0", "nump", "numx")}
     ctx = oshun_amd.Context(r["l0"], 0, r["nump"], r["numx"], 1, **kw)
+    ctx.pair_rows(10)  # the batch's default tile height (a small run alone would take 6)
     ctx.upload(f, fld)
     ctx.step(H, steps)
     fo, fldo = ctx.download()
