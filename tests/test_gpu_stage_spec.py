@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 # (l0, m0, Np, Nx, Ny): every shape with a specialised kernel, ragged Np and Nx
 SHAPES = [(4, 4, 40, 70, 5), (4, 3, 16, 9, 4), (6, 3, 24, 12, 3), (7, 3, 10, 5, 4), (8, 2, 36, 20, 1),
-          (3, 2, 64, 10, 1)]
+          (3, 2, 64, 10, 1), (4, 4, 10, 9, 3)]  # the last: p rows past Np in the last workgroup
 
 
 def _ctx(l0, m0, np_, nx, ny, prec):
@@ -42,7 +42,7 @@ def test_spec_slope_equals_interpreter(l0, m0, np_, nx, ny):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("l0,m0,np_,nx,ny", [SHAPES[0], SHAPES[2], SHAPES[4]])
+@pytest.mark.parametrize("l0,m0,np_,nx,ny", [SHAPES[0], SHAPES[2], SHAPES[4], SHAPES[6]])
 def test_spec_steps_equal_interpreter(l0, m0, np_, nx, ny, prec):
     import oshun_amd
     print(f"case stage spec steps l0={l0} m0={m0} np={np_} nx={nx} ny={ny} {prec}", flush=True)
