@@ -264,7 +264,56 @@ def _config_cases():
     return cases
 
 
-def run_configs(steps, warmup, only=None, prec_only=None):
+def _config_cpu(name, shape, kw, h, f, fld, budget_s=6.0):
+    """The reference classes (oracle/_ref, one core) on a bounded sample of one configuration: the same
+    l0, m0, Np, h and collisions on fewer x (and y) cells; m0 < 2 runs as a y-uniform m0 = 2 stand-in
+    (the reference needs l0 >= 3, m0 >= 2), so C1 (l0 = 2) is timed on the oracle's restatement instead.
+    Returns the rate in useful cell-updates/s (harmonics of the configuration, not of the stand-in)."""
+    from oracle import oracle as orc
+    l0, m0, np_, nx, ny = shape
+    sx, sy = min(nx, 16 if ny == 1 else 8), min(ny, 8)
+    fs = np.ascontiguousarray(f[:, :sy, :sx, :])
+    flds = np.ascontiguousarray(fld[:, :sy, :sx])
+    coll = kw.get("collisions", 3)
+    cells = orc.num_harmonics(l0, m0) * np_ * sx * sy
+    if l0 >= 3 and orc.have_ref():
+        m2 = max(m0, 2)
+        f2 = np.zeros((orc.num_harmonics(l0, m2), sy, sx, np_), np.complex128)
+        for l in range(l0 + 1):
+            for m in range(min(l, m0) + 1):
+                f2[orc.hind(l, m, m2)] = fs[orc.hind(l, m, m0)]
+        dx, dy = kw["dx"], kw.get("dy", 1.0)
+        rk = dict(nb=kw["nb_algorithms"], pmax=kw["pmax"], xmin=0.0, xmax=dx * (sx - 1), ymin=0.0,
+                  ymax=dy * max(sy - 1, 1), h=h, small_dt=kw["small_dt"], density_np=kw["density_np"])
+        op = 1 if coll else 2
+        t0 = time.perf_counter()
+        orc.run_ref(l0, m2, np_, sx, sy, f2, flds, op=op, nsteps=1, **rk)
+        setup = time.perf_counter() - t0
+        n = max(1, min(8, int(budget_s / max(setup, 1e-3))))
+        t0 = time.perf_counter()
+        orc.run_ref(l0, m2, np_, sx, sy, f2, flds, op=op, nsteps=n + 1, **rk)
+        per_step = max(time.perf_counter() - t0 - setup, 1e-9) / n
+        stand = " as an m0 = 2 stand-in" if m0 < 2 else ""
+        return {"value": cells / per_step, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"OSHUN-1 classes (oracle/_ref){stand}, Nx={sx} Ny={sy}, {n} steps minus a 1-step run"}
+    # the restatement is fast enough for the whole (small) configuration
+    cells = orc.num_harmonics(l0, m0) * np_ * nx * ny
+    o = orc.Oracle(orc.Config(l0=l0, m0=m0, np=np_, nx=nx, ny=ny, pmax=kw["pmax"], dx=kw["dx"],
+                              density_np=kw["density_np"], small_dt=kw["small_dt"], nb=kw["nb_algorithms"],
+                              coll_ee=1 if coll else 0, coll_flm=1 if coll else 0))
+    o.set_state(f, fld)
+    t0 = time.perf_counter()
+    o.step(h, 1)
+    d1 = time.perf_counter() - t0
+    n = max(1, min(2000, int(budget_s / max(d1, 1e-4))))
+    t0 = time.perf_counter()
+    o.step(h, n)
+    per_step = (time.perf_counter() - t0) / n
+    return {"value": cells / per_step, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"oracle CPU restatement (the reference needs l0 >= 3), Nx={nx} Ny={ny}, {n} steps"}
+
+
+def run_configs(steps, warmup, only=None, prec_only=None, no_cpu=False):
     """cell-updates/s of the RK4 + FP step on each synthetic configuration, FP64 and FP32 split, with the
     step's algorithmic HBM bytes (16 S_f) over its time as the roofline fraction.  only: a configuration
     name prefix ("C3"); prec_only: "fp64" or "fp32"."""
@@ -294,6 +343,8 @@ def run_configs(steps, warmup, only=None, prec_only=None):
                         "step_roofline_frac": 16.0 * S / (ms * 1e-3) / HBM_PEAK,
                         "stage_kernel": ctx.stage_kernel()}
             ctx.close()
+        if not no_cpu:
+            row["cpu_baseline"] = _config_cpu(name, (l0, m0, np_, nx, ny), kw, h, f, fld)
         out.append(row)
     return {"metric": "phase-space cell-updates/s per RK4+FP step on the synthetic configurations",
             "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "data": "synthetic",
@@ -380,7 +431,7 @@ def main():
     import oshun_amd
 
     if args.configs:
-        print(json.dumps(run_configs(args.steps, args.warmup, args.config, args.only)))
+        print(json.dumps(run_configs(args.steps, args.warmup, args.config, args.only, args.no_cpu)))
         return
     if args.implicit:
         print(json.dumps(run_implicit(args.steps, args.warmup, args.no_cpu)))
