@@ -1,6 +1,6 @@
 """Parity of the HIP path (through the C ABI) with the reference and the oracle.
 
-Tolerances (north star: FP64 path <= 1e-12 relative in f_lm, E and B):
+Tolerances (the FP64 path is held to 1e-12 relative in f_lm, E and B):
   * against fixtures written by the compiled reference: <= 1e-12 normwise
     (max|gpu - ref| / max|ref| over the whole array);
   * against the oracle at larger sizes: <= 1e-12 normwise.
