@@ -70,3 +70,26 @@ def test_shapes_without_spec_use_the_interpreter():
         ctx = _ctx(*shape, oshun_amd.FP64)
         assert ctx.stage_kernel() == want, shape
         ctx.close()
+
+
+def test_kernel_knobs_reject_bad_values():
+    import oshun_amd
+    print("case stage knobs", flush=True)
+    ctx = _ctx(4, 4, 16, 8, 2, oshun_amd.FP64)
+    with pytest.raises(oshun_amd.OshunError):
+        ctx.stage_kernel(5)
+    with pytest.raises(oshun_amd.OshunError):
+        ctx.pair_rows(10)  # no stage-pair kernels on a 2D context
+    assert ctx.pair_rows() == 0
+    ctx.close()
+    ctx = _ctx(3, 0, 64, 64, 1, oshun_amd.FP64)
+    assert ctx.pair_rows() == 6  # a small grid
+    assert ctx.pair_rows(10) == 10
+    with pytest.raises(oshun_amd.OshunError):
+        ctx.pair_rows(7)
+    ctx.close()
+    ctx = _ctx(10, 0, 64, 64, 1, oshun_amd.FP64)
+    assert ctx.pair_rows() == 10  # l0 > 8 keeps 10 rows
+    with pytest.raises(oshun_amd.OshunError):
+        ctx.pair_rows(6)
+    ctx.close()
