@@ -517,10 +517,15 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_spec(StageArgs<T>
     // trapezoid weights of Current::Jxyz: 1/2 at both ends of the p axis
     const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
     const double g = a.p3og[p];
-    j10 = w * (static_cast<double>(a.in[(long long)Sh::re(1, 0) * a.plane + idx]) * g);
+    // f10, f11 at the point: from the row cache where there is one
+    auto fpt = [&](int c) {
+      if constexpr (LDS) return e.lrow[c * kTileX];
+      else return a.in[(long long)c * a.plane + idx];
+    };
+    j10 = w * (static_cast<double>(fpt(Sh::re(1, 0))) * g);
     if (M0 >= 1) {
-      j11r = w * (static_cast<double>(a.in[(long long)Sh::re(1, 1) * a.plane + idx]) * g);
-      j11i = w * (static_cast<double>(a.in[(long long)Sh::im(1, 1) * a.plane + idx]) * g);
+      j11r = w * (static_cast<double>(fpt(Sh::re(1, 1))) * g);
+      j11i = w * (static_cast<double>(fpt(Sh::im(1, 1))) * g);
     }
   }
   __syncthreads();  // every wavefront is done with its cached rows
