@@ -912,6 +912,74 @@ __device__ __forceinline__ void field23_body(const Field23Args<T>& A, const int 
     b.facc[o] = acc2[c] + kh * b.third;
   }
 }
+// The last RK4 field stage of a step (a4, mode 4) and the first of the next
+// (a1, mode 1) of the 1D m0 = 0 system in one pass, for steps that end
+// without a collision launch to carry them (collisions off).  The sums and
+// operations are those of the two field_stage launches: the stage-4 current
+// from the pair partials, the stage-1 current summed directly over p of the
+// new state; the stage-4 fields at x +- 1 are recomputed here (Ex there is not
+// needed: the stage-1 update at x reads Ex only at x), and the new
+// accumulator goes to af2 because neighbours still read a4.facc.
+template <typename T>
+struct Field41Args {
+  FieldArgs<T> a4, a1;
+  double* af2;
+};
+template <typename T>
+__global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage41_1d(Field41Args<T> A) {
+  const FieldArgs<T>& a = A.a4;
+  const FieldArgs<T>& b = A.a1;
+  const int i = blockIdx.x * kFieldX + threadIdx.x;
+  const int row = threadIdx.y;
+  const int nx = a.nx, nxy = nx;  // ny = 1
+  __shared__ double sh[kFieldRows][kFieldX];
+  auto wrap = [nx](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
+  double j4 = 0.0, j1 = 0.0;
+  sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &j4);
+  sum_direct<T>(b.fpl + (long long)b.c10 * b.plane, b.p3og, b.np, nx, nxy, i, row, sh, &j1);
+  if (i >= nxy || row != 0) return;
+  // stage 4 at x-1, x, x+1: out = acc + (h/6) k
+  double F4[3][6];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int xx = wrap(i + d - 1);
+    double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (d == 1) {  // Current::Jxyz (def-VlasovMax.cpp:680-745), as field_body
+      double j = j4;
+      j *= a.delta_p;
+      j += a.small * (a.f1p1[i] * a.p3og1);
+      j *= 4.0 * M_PI / 3.0;
+      k[0] += j;
+    }
+    maxwell_terms([&](int c, int at) { return a.fin[(long long)c * nxy + (at == 0 ? wrap(xx - 1) : at == 1 ? wrap(xx + 1) : xx)]; },
+                  a.midx, a.midy, k);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double kh = k[c] * a.h6;
+      F4[d][c] = a.facc[(long long)c * nxy + xx] + kh;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) a.fout[(long long)c * nxy + i] = F4[1][c];
+  // stage 1 of the next step at x: out = y0 + (h/2) k, acc = y0 + (h/2) k / 3
+  double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  {
+    double j = j1;
+    j *= b.delta_p;
+    j += b.small * (static_cast<double>(b.fpl[(long long)b.c10 * b.plane + nx + i]) * b.p3og1);
+    j *= 4.0 * M_PI / 3.0;
+    k[0] += j;
+  }
+  maxwell_terms([&](int c, int at) { return at == 0 ? F4[0][c] : at == 1 ? F4[2][c] : F4[1][c]; }, b.midx, b.midy, k);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const long long o = (long long)c * nxy + i;
+    const double kh = k[c] * b.c05h;
+    b.fout[o] = F4[1][c] + kh;
+    A.af2[o] = F4[1][c] + kh * b.third;
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage23_1d(Field23Args<T> a) {
   field23_body<T>(a, blockIdx.x);

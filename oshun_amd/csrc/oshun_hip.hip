@@ -449,6 +449,24 @@ int rk4_pair(oshun_ctx* c, double h) {
   return OSHUN_OK;
 }
 
+// stage 4 of this step and stage 1 of the next in one field launch (steps
+// without collisions; the caller swaps Af/Af2 and marks next1_ready)
+template <typename T>
+int launch_field41(oshun_ctx* c, double h) {
+  oshun::Field41Args<T> A;
+  A.a4 = field_args<T>(c, 4, c->Cf, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, pair_tiles<T>(c), c->f1y);
+  A.a1 = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af2, c->S, h, nullptr, 0, nullptr);
+  A.a1.p3og = c->d_p3og;
+  A.af2 = c->Af2;
+  ProfRec r;
+  prof_begin(c, 1, &r);
+  hipLaunchKernelGGL(oshun::field_stage41_1d<T>, dim3((c->nxy + oshun::kFieldX - 1) / oshun::kFieldX),
+                     dim3(oshun::kFieldX, oshun::kFieldRows), 0, c->stream, A);
+  prof_end(c, &r);
+  HIPCHK(c, hipGetLastError());
+  return OSHUN_OK;
+}
+
 int do_rk4(oshun_ctx* c, double h) {
   if (c->use_pair) return c->fp32 ? rk4_pair<float>(c, h) : rk4_pair<double>(c, h);
   return c->fp32 ? rk4<float>(c, h) : rk4<double>(c, h);
@@ -577,9 +595,19 @@ int full_step(oshun_ctx* c, double h, bool next1) {
   const int npt = c->fp32 ? pair_tiles<float>(c) : pair_tiles<double>(c);
   const bool fuse = c->use_pair && c->ny == 1 && do_ee && do_flm && c->rows_npl && c->rows_segw == c->rows_segw2 &&
                     c->rows_npl == c->rows_npl2 && npt <= oshun::kFuseTiles;
-  c->fuse_fields = fuse ? (next1 ? 2 : 1) : 0;
+  // without collisions the step's last field stage and the next step's first
+  // run as one field launch after the RK4 (fuse_fields = 3)
+  const bool f41 = !fuse && next1 && c->use_pair && c->ny == 1 && !do_ee && !do_flm;
+  c->fuse_fields = fuse ? (next1 ? 2 : 1) : (f41 ? 3 : 0);
   c->fuse_h = h;
   int rc = do_rk4(c, h);
+  if (!rc && f41) {
+    rc = c->fp32 ? launch_field41<float>(c, h) : launch_field41<double>(c, h);
+    if (!rc) {
+      std::swap(c->Af, c->Af2);
+      c->next1_ready = true;
+    }
+  }
   if (rc) { c->fuse_fields = 0; return rc; }
   const double t = c->t_rk;
   int numh = 0;
