@@ -176,9 +176,10 @@ __global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int st
 
 namespace {
 
-// one RK2 (two stages) of a 1D slope on the state S (def-RK.cpp:470-550)
+// one RK2 (two stages) of a 1D slope on the state S (def-RK.cpp:470-550), the
+// result into dst (S when null)
 template <int L0>
-int imp_rk2(oshun_ctx* c, double h, int use_e, int use_adv, int clean, int tmax) {
+int imp_rk2(oshun_ctx* c, double h, int use_e, int use_adv, int clean, int tmax, void* dst = nullptr) {
   oshun::Imp1DArgs A;
   fill_1d_args<double>(c, A.s, 0, c->S, nullptr, nullptr, nullptr, c->Sf, h);
   A.use_e = use_e; A.use_adv = use_adv; A.clean = clean; A.tmax = tmax;
@@ -189,7 +190,7 @@ int imp_rk2(oshun_ctx* c, double h, int use_e, int use_adv, int clean, int tmax)
     if (st == 0) { A.s.in = static_cast<const double*>(c->S); A.y = nullptr;
                    A.out0 = static_cast<double*>(c->B1); A.outy = static_cast<double*>(c->A); }
     else         { A.s.in = static_cast<const double*>(c->B1); A.y = static_cast<const double*>(c->A);
-                   A.out0 = nullptr; A.outy = static_cast<double*>(c->S); }
+                   A.out0 = nullptr; A.outy = static_cast<double*>(dst ? dst : c->S); }
     if (clean) hipLaunchKernelGGL((oshun::imp_stage_1d<L0, true>), grid, block, 0, c->stream, A);
     else hipLaunchKernelGGL((oshun::imp_stage_1d<L0, false>), grid, block, 0, c->stream, A);
     HIPCHK(c, hipGetLastError());
@@ -277,28 +278,40 @@ int implicit_step_1d(oshun_ctx* c, double h, int es) {
     if ((rc = do_collide(c, 1, 0, 0, numh, hee, 0.0))) return rc;
   }
   c->t_ee = t;
-  // Implicit_E_Field::advance, 1D variant
+  // Implicit_E_Field::advance, 1D variant.  The reference runs its two probes
+  // on the state and restores f00, f1m, f2m afterwards (def-ImplicitE.cpp:
+  // 491-520); here they run on a copy in B2 (S and B2 swap around them), so S
+  // stays the state: probe 1 needs f00 and f10 (advance_1 reads f00, writes
+  // f10), probe 2 writes its RK2 straight into B2.
   const double dt_flm = t - c->t_eb;
-  if ((rc = imp_planes(c, 2, true))) return rc;
-  if (do_flm && (rc = do_collide(c, 0, 1, 0, 0, 0.0, dt_flm))) return rc;
   const dim3 sg((c->nx + oshun::kImpCols - 1) / oshun::kImpCols);
-  prof_begin(c, 1, &r);
-  hipLaunchKernelGGL(oshun::imp_field_solve_1d, sg, fb, 0, c->stream, imp_field_args(c, h), 0);
-  prof_end(c, &r);
+  if ((rc = imp_planes(c, 1, true))) return rc;
+  std::swap(c->S, c->B2);
+  rc = do_flm ? do_collide(c, 0, 1, 0, 0, 0.0, dt_flm) : OSHUN_OK;
+  if (!rc) {
+    prof_begin(c, 1, &r);
+    hipLaunchKernelGGL(oshun::imp_field_solve_1d, sg, fb, 0, c->stream, imp_field_args(c, h), 0);
+    prof_end(c, &r);
+  }
+  std::swap(c->S, c->B2);
+  if (rc) return rc;
   HIPCHK(c, hipGetLastError());
-  if ((rc = imp_planes(c, 1, false))) return rc;
   prof_begin(c, 0, &r);
   // Implicit_Ex1D: sources l <= 3, targets l <= 2; the planes above l = 3
   // only receive +0, so the stages run on planes 0..3 alone
-  rc = imp_rk2<3>(c, h, 1, 0, 0, 2);
+  rc = imp_rk2<3>(c, h, 1, 0, 0, 2, c->B2);
   prof_end(c, &r);
   if (rc) return rc;
-  if (do_flm && (rc = do_collide(c, 0, 1, 0, 0, 0.0, dt_flm))) return rc;
-  prof_begin(c, 1, &r);
-  hipLaunchKernelGGL(oshun::imp_field_solve_1d, sg, fb, 0, c->stream, imp_field_args(c, h), 1);
-  prof_end(c, &r);
+  std::swap(c->S, c->B2);
+  rc = do_flm ? do_collide(c, 0, 1, 0, 0, 0.0, dt_flm) : OSHUN_OK;
+  if (!rc) {
+    prof_begin(c, 1, &r);
+    hipLaunchKernelGGL(oshun::imp_field_solve_1d, sg, fb, 0, c->stream, imp_field_args(c, h), 1);
+    prof_end(c, &r);
+  }
+  std::swap(c->S, c->B2);
+  if (rc) return rc;
   HIPCHK(c, hipGetLastError());
-  if ((rc = imp_planes(c, 2, false))) return rc;
   // RK2 of Adv2 (Exyz1D + CLEAN_LOW_PC) with the solved field
   prof_begin(c, 0, &r);
   rc = imp_rk2_pair(c, h, 3);
