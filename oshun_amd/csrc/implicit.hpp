@@ -21,66 +21,11 @@
 //
 // Buffers: S holds the state.  The RK2 of Adv1 and of Adv2 are one fused
 // stage-pair launch each (vlasov_pair_1d modes 2 and 3: S -> B1, then S and
-// B1 swap); the RK2 of Implicit_Ex1D touches planes 0..3 only and runs as two
-// stage launches on them (S -> B1 = Y + h k and A = Y + h k / 2 -> S); B2
-// keeps the probed planes l <= 2, B2f the per-x probe field and currents.
+// B1 swap); the field solve's probes run on a copy in B2 (mode 4, the RK2 of
+// Implicit_Ex1D on planes 0..3, S -> B2); B2f holds the per-x probe field
+// and currents.
 
 namespace oshun {
-
-struct Imp1DArgs {
-  Stage1DArgs<double> s;  // in: stage input planes; fin: Ex; tables
-  const double* y;        // mode 0: the state Y (= in); mode 1: Y + h k1 / 2
-  double* out0;           // mode 0: Y + h k1
-  double* outy;           // mode 0: Y + h k1 / 2; mode 1: final Y
-  int mode, use_e, use_adv, clean, tmax;
-  double h, h2;           // h and 0.5 h
-};
-
-// one RK2 stage of Runge_Kutta_2_Imp, thread per (x, p); the slope is the 1D
-// right-hand side of the E couplings (Ex of s.fin) and/or the x-advection
-template <int L0, bool CLEAN>
-__device__ __forceinline__ void imp_stage_body(const Imp1DArgs& A) {
-  constexpr int NL = L0 + 1;
-  const Stage1DArgs<double>& a = A.s;
-  const int x = blockIdx.x * 64 + threadIdx.x;
-  const int p = blockIdx.y * kRows1D + threadIdx.y;
-  const int np = a.np, nx = a.nx;
-  if (x >= nx || p >= np) return;
-  const long long idx = (long long)p * nx + x;
-  const long long xm = (x == 0) ? (nx - 1) : -1;
-  const long long xp = (x == nx - 1) ? -(long long)(nx - 1) : 1;
-  const long long up = (p > 0) ? -(long long)nx : 0;
-  const long long dn = (p < np - 1) ? (long long)nx : 0;
-  double fm[NL], fc[NL], fn[NL], fl[NL], fr[NL];
-#pragma unroll
-  for (int l = 0; l < NL; ++l) {
-    const double* s = a.in + (long long)l * a.plane + idx;
-    fc[l] = s[0];
-    fm[l] = s[up];
-    fn[l] = s[dn];
-    fl[l] = A.use_adv ? s[xm] : 0.0;  // without advection fl - fr = 0 adds +-0
-    fr[l] = A.use_adv ? s[xp] : 0.0;
-  }
-  double k[NL];
-  rhs_point_1d<double, NL, CLEAN>(a, p, emul<double, NL>(a, A.use_e ? a.fin[x] : 0.0), a.vtab[p], 0.0, fm, fc, fn,
-                                  fl, fr, k);
-#pragma unroll
-  for (int l = 0; l < NL; ++l) {
-    const double kl = (l <= A.tmax) ? k[l] : 0.0;
-    const long long o = (long long)l * a.plane + idx;
-    if (A.mode == 0) {  // Yh *= h; Y0 += Yh; Yh *= 0.5; Y += Yh
-      const double kh = kl * A.h;
-      A.out0[o] = fc[l] + kh;
-      A.outy[o] = fc[l] + kh * 0.5;
-    } else {            // Yh *= 0.5 h; Y += Yh
-      A.outy[o] = A.y[o] + kl * A.h2;
-    }
-  }
-}
-template <int L0, bool CLEAN>
-__global__ __launch_bounds__(64 * kRows1D) void imp_stage_1d(Imp1DArgs A) {
-  imp_stage_body<L0, CLEAN>(A);
-}
 
 struct ImpFieldArgs {
   double* fld;                  // [6][nx] state fields
@@ -176,27 +121,6 @@ __global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int st
 
 namespace {
 
-// one RK2 (two stages) of a 1D slope on the state S (def-RK.cpp:470-550), the
-// result into dst (S when null)
-template <int L0>
-int imp_rk2(oshun_ctx* c, double h, int use_e, int use_adv, int clean, int tmax, void* dst = nullptr) {
-  oshun::Imp1DArgs A;
-  fill_1d_args<double>(c, A.s, 0, c->S, nullptr, nullptr, nullptr, c->Sf, h);
-  A.use_e = use_e; A.use_adv = use_adv; A.clean = clean; A.tmax = tmax;
-  A.h = h; A.h2 = 0.5 * h;
-  const dim3 block(64, oshun::kRows1D, 1), grid((c->nx + 63) / 64, c->npb, 1);
-  for (int st = 0; st < 2; ++st) {
-    A.mode = st;
-    if (st == 0) { A.s.in = static_cast<const double*>(c->S); A.y = nullptr;
-                   A.out0 = static_cast<double*>(c->B1); A.outy = static_cast<double*>(c->A); }
-    else         { A.s.in = static_cast<const double*>(c->B1); A.y = static_cast<const double*>(c->A);
-                   A.out0 = nullptr; A.outy = static_cast<double*>(dst ? dst : c->S); }
-    if (clean) hipLaunchKernelGGL((oshun::imp_stage_1d<L0, true>), grid, block, 0, c->stream, A);
-    else hipLaunchKernelGGL((oshun::imp_stage_1d<L0, false>), grid, block, 0, c->stream, A);
-    HIPCHK(c, hipGetLastError());
-  }
-  return OSHUN_OK;
-}
 
 // one RK2 of Adv1 (pair = 2: x-advection) or Adv2 (pair = 3: Exyz1D +
 // CLEAN_LOW_PC) on every plane as one fused stage-pair launch: S -> B1, then
@@ -297,11 +221,18 @@ int implicit_step_1d(oshun_ctx* c, double h, int es) {
   if (rc) return rc;
   HIPCHK(c, hipGetLastError());
   prof_begin(c, 0, &r);
-  // Implicit_Ex1D: sources l <= 3, targets l <= 2; the planes above l = 3
-  // only receive +0, so the stages run on planes 0..3 alone
-  rc = imp_rk2<3>(c, h, 1, 0, 0, 2, c->B2);
+  // Implicit_Ex1D with E = DE: sources l <= 3, targets l <= 2; the planes
+  // above l = 3 only receive +0, so both RK2 stages run on planes 0..3 alone,
+  // as one stage-pair launch S -> B2
+  {
+    const oshun::Stage1DArgs<double> a = pair_args<double>(c, c->S, c->S, c->B2, c->A, c->B2f, c->B2f, c->jp1,
+                                                           c->jp2, h);
+    const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
+    const dim3 grid((ntx * pair_tiles_r(c->np, pair_rows<double>()) + 7) / 8 * 8);
+    hipLaunchKernelGGL((oshun::vlasov_pair_1d<double, 3, pair_rows<double>(), 4>), grid, dim3(64, 4), 0, c->stream, a);
+  }
   prof_end(c, &r);
-  if (rc) return rc;
+  HIPCHK(c, hipGetLastError());
   std::swap(c->S, c->B2);
   rc = do_flm ? do_collide(c, 0, 1, 0, 0, 0.0, dt_flm) : OSHUN_OK;
   if (!rc) {

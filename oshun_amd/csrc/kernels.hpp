@@ -392,7 +392,9 @@ __device__ __forceinline__ int batch_run(const int* __restrict__ ts, int nruns, 
 // PAIR 2 and 3 are the two stages of one RK2 of Runge_Kutta_2_Imp
 // (def-RK.cpp:470-524: Y0 = Y + h k1, Y += (h k1)/2, Y += (h/2) k2): PAIR 2 on
 // the x-advection alone (Adv1), PAIR 3 on the E couplings alone with
-// CLEAN_LOW_PC (Adv2, Exyz1D); no current partials.
+// CLEAN_LOW_PC (Adv2, Exyz1D), PAIR 4 (L0 = 3) on the E couplings of planes
+// 0..3 into targets l <= 2 without CLEAN_LOW_PC (Implicit_Ex1D, the field
+// solve's probe); no current partials.
 constexpr int kPairX = 62;  // interior columns; lanes 0 and 63 are the halo columns
 
 // raw buffer access: 32-bit byte offsets, plane offset in an SGPR, and
@@ -525,13 +527,14 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
         fl[l] = wave_shift<kWaveShr1>(fc[l], fe[l]);
         fr[l] = wave_shift<kWaveShl1>(fc[l], fe[l]);
       }
-      if (PAIR == 3) {  // E couplings only: fl - fr = 0 adds +-0
+      if (PAIR >= 3) {  // E couplings only: fl - fr = 0 adds +-0
 #pragma unroll
         for (int l = 0; l < NL; ++l) { fl[l] = T(0); fr[l] = T(0); }
       }
       T kv[NL];
-      rhs_point_1d<T, NL, PAIR != 2>(a, p, emul<T, NL>(a, PAIR == 2 ? 0.0 : a.fin[xe]), ldc(a.vtab, p),
+      rhs_point_1d<T, NL, PAIR != 2 && PAIR != 4>(a, p, emul<T, NL>(a, PAIR == 2 ? 0.0 : a.fin[xe]), ldc(a.vtab, p),
                                      a.bgG ? ldc(a.bgG, p) : T(0), fm, fc, fnv, fl, fr, kv);
+      if (PAIR == 4) kv[NL - 1] = T(0);  // Implicit_Ex1D: targets l <= 2 (NL = 4)
 #pragma unroll
       for (int l = 0; l < NL; ++l) {
         T y;
@@ -584,12 +587,13 @@ __device__ __forceinline__ void pair_body(const Stage1DArgs<T>& a, const int til
         fc[l] = ys[s][l][lane];
         fm[l] = ys[s - 1][l][lane];
         fn[l] = (p < np - 1) ? ys[s + 1][l][lane] : fc[l];
-        fl[l] = (PAIR == 3) ? T(0) : ys[s][l][lane - 1];
-        fr[l] = (PAIR == 3) ? T(0) : ys[s][l][lane + 1];
+        fl[l] = (PAIR >= 3) ? T(0) : ys[s][l][lane - 1];
+        fr[l] = (PAIR >= 3) ? T(0) : ys[s][l][lane + 1];
       }
       T kv[NL];
-      rhs_point_1d<T, NL, PAIR != 2>(a, p, emul<T, NL>(a, PAIR == 2 ? 0.0 : a.fin2[xe]), ldc(a.vtab, p),
+      rhs_point_1d<T, NL, PAIR != 2 && PAIR != 4>(a, p, emul<T, NL>(a, PAIR == 2 ? 0.0 : a.fin2[xe]), ldc(a.vtab, p),
                                      a.bgG ? ldc(a.bgG, p) : T(0), fm, fc, fn, fl, fr, kv);
+      if (PAIR == 4) kv[NL - 1] = T(0);
       const int vc = (p * nx + xe) * SZ;
       if (PAIR >= 2) {  // RK2 stage 2: Y = acc + (h/2) k2
 #pragma unroll
