@@ -66,11 +66,12 @@ __global__ __launch_bounds__(256) void imp_faraday_1d(ImpFieldArgs a) {
 
 // Current_xyz::calculate_J, x component (def-ImplicitE.cpp:125-156), with p0
 // and pmax at weight 1 on interior cells (see above), in the reference's
-// sequential order: a block of 16 columns x 16 rows stages 64-row chunks of
-// f10 p3og in LDS and the first row of threads adds them up in p order.
+// sequential order: a block of 16 columns x 16 rows stages f10 p3og of all
+// p rows in LDS (every load of the block in flight at once) and the first row
+// of threads adds them up in p order.
 // stage 0: probe field from the state E, J0, and Ex = DE; stage 1: J_DE and
 // the solved field.
-constexpr int kImpCols = 16, kImpRows = 64;
+constexpr int kImpCols = 16, kImpRows = 512;  // rows: the largest Np oshun_create accepts
 __global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int stage) {
   __shared__ double chunk[kImpRows][kImpCols];
   const int tx = threadIdx.x % kImpCols, ty = threadIdx.x / kImpCols;
@@ -83,16 +84,11 @@ __global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int st
     j = f[0] * a.p3og[0];
     j += f[(long long)(np - 1) * nx] * a.p3og[np - 1];
   }
-  for (int ip0 = 1; ip0 < np - 1; ip0 += kImpRows) {
-    __syncthreads();
-    for (int r = ty; r < kImpRows; r += 256 / kImpCols) {
-      const int ip = ip0 + r;
-      chunk[r][tx] = (ok && ip < np - 1) ? f[(long long)ip * nx] * a.p3og[ip] : 0.0;
-    }
-    __syncthreads();
-    if (ok && ty == 0)
-      for (int r = 0; r < kImpRows && ip0 + r < np - 1; ++r) j += chunk[r][tx];
-  }
+#pragma unroll 8
+  for (int ip = 1 + ty; ip < np - 1; ip += 256 / kImpCols) chunk[ip][tx] = ok ? f[(long long)ip * nx] * a.p3og[ip] : 0.0;
+  __syncthreads();
+  if (ok && ty == 0)
+    for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip][tx];
   if (!ok || ty != 0) return;
   j *= a.delta_p;
   j += a.small * (f[nx] * a.p3og[1]);
