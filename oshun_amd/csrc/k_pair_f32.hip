@@ -1,0 +1,6 @@
+// Single-run 1D stage and stage-pair kernels, FP32 storage.
+// Explicit instantiations of the kernel templates listed in instances.hpp.
+#define OSHUN_FAMILY_TU
+#include "instances.hpp"
+
+OSHUN_FAMILY_PAIR(, float)

@@ -1,0 +1,8 @@
+// General-system stage kernels (interpreter and specialised shapes), FP32 storage.
+// Explicit instantiations of the kernel templates listed in instances.hpp.
+#define OSHUN_FAMILY_TU
+#include "instances.hpp"
+
+#define OSHUN_SPEC_EXT
+OSHUN_STAGE_SPECS(OSHUN_SPEC_F32)
+OSHUN_I_GEN(, float)

@@ -2295,11 +2295,13 @@ __global__ __launch_bounds__(64 * 4) void collide_rows_batch(const CollArgs<T>* 
   collide_body<T, NPL, SEGW, MODE>(args[r], g - ts[r]);
 }
 
+#ifndef OSHUN_FAMILY_TU  // non-template kernels: defined once, in the library's main unit
 // batched runs: every run's clock advances by its step after the collisions
 __global__ __launch_bounds__(256) void batch_clock(double* __restrict__ t, const double* __restrict__ h, int n) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) t[i] += h[i];
 }
+#endif
 
 // Nonrelativistic moments, Parallel_Output::nonrelativistic_moments
 // (def-Parallel.cpp:1677-2134) in double: one thread per (x,y), trapezoid
@@ -2384,10 +2386,12 @@ __global__ __launch_bounds__(256) void moments_kernel(MomArgs<T> a) {
   for (int k = 0; k < 7; ++k) a.out[(long long)k * nxy + i] = vals[k];
 }
 
+#ifndef OSHUN_FAMILY_TU
 // plain copy, 8 B per lane per iteration like the Vlasov and collision kernels (counter calibration)
 __global__ __launch_bounds__(256) void diag_copy(const double* __restrict__ src, double* __restrict__ dst,
                                                  long long n) {
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) dst[i] = src[i];
 }
+#endif
 
 }  // namespace oshun

@@ -17,9 +17,15 @@
 #include <string>
 #include <vector>
 
-#include "kernels.hpp"
-#include "stage_spec.hpp"
+#include "instances.hpp"
 #include "oshun_hip.h"
+
+// the heavy kernel templates are compiled in k_*.hip
+OSHUN_EXTERN_KERNELS
+#define OSHUN_SPEC_EXT extern
+OSHUN_STAGE_SPECS(OSHUN_SPEC_F64)
+OSHUN_STAGE_SPECS(OSHUN_SPEC_F32)
+#undef OSHUN_SPEC_EXT
 
 using oshun::HostTables;
 using oshun::Term;
