@@ -71,6 +71,15 @@ typedef struct oshun_ctx oshun_ctx;
 
 int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out);
 int oshun_destroy(oshun_ctx* ctx);
+/* The kernels a context of this grid would run, decided on the host alone
+ * (nothing is allocated; no GPU is needed): *stage = 0 the 1D stage-pair
+ * kernels, 1 the 1D single-stage kernel, 2 the compile-time specialised
+ * general kernel, 3 the term-list interpreter; *collide = 0 the lane-layout
+ * collision kernel, 1 one wavefront per column.  The stage pairs address the
+ * whole state with 32-bit byte offsets and the lane-layout kernel one plane,
+ * so states of 2 GiB or more (planes of 2 GiB) take the 64-bit kernels.  No
+ * counterpart in the reference. */
+int oshun_kernel_plan(const oshun_grid* grid, const oshun_phys* phys, int* stage, int* collide);
 const char* oshun_last_error(const oshun_ctx* ctx);
 int oshun_num_harmonics(const oshun_ctx* ctx);
 
@@ -110,10 +119,15 @@ int oshun_step(oshun_ctx* ctx, double h, int nsteps);
 int oshun_step_implicit_1d(oshun_ctx* ctx, double h, int nsteps, int if_implicitES);
 
 /* Nonrelativistic moments of the current state in double
- * (Parallel_Output::nonrelativistic_moments, def-Parallel.cpp:1677-2134;
- * the reference computes them in float).  out = double[7][ny][nx]:
- * density, Vx, Vy, VxVx, VyVy, VzVz, Qx.  Used for conservation reports and
- * the heat-flux objective. */
+ * (Parallel_Output::nonrelativistic_moments, def-Parallel.cpp:1677-2253;
+ * the reference computes them in float).  out = double[OSHUN_NMOMENTS][ny][nx]
+ * in the order: density, Vx, Vy, VxVx, VyVy, VzVz, Qx, VxVy, VxVz, VyVz,
+ * Vsq (<v^2> - Vx^2 - Vy^2), T [eV], P [Mbar], N_D, nu, Qy, Kx, Ky.
+ * Harmonics a run does not have count as zero.  Kx and Ky difference
+ * <v^2>^2 across the interior as the single-rank reference does (end cells
+ * take their neighbour's difference); Ky is 0 when ny < 3.  The first call
+ * allocates the context's moment buffers. */
+#define OSHUN_NMOMENTS 18
 int oshun_moments(oshun_ctx* ctx, double* out);
 
 /* Stream and device views, for callers that keep data on the GPU. */
@@ -197,9 +211,13 @@ int oshun_adjoint_step(oshun_ctx* ctx, double h, double* lam_f, double* lam_fld)
  * from the current state, keeping a copy of the state every `every` steps on
  * the device; oshun_adjoint_backward then turns the cotangent of the final
  * state into that of the initial state, recomputing each segment's steps from
- * its checkpoint (the context ends at the final state again). */
-int oshun_adjoint_forward(oshun_ctx* ctx, double h, int nsteps, int every);
-int oshun_adjoint_backward(oshun_ctx* ctx, double* lam_f, double* lam_fld);
+ * its checkpoint (the context ends at the final state again).  A context keeps
+ * one recorded run: every oshun_adjoint_forward replaces the previous one and
+ * returns its generation number (if `generation` is not NULL);
+ * oshun_adjoint_backward with generation > 0 fails with OSHUN_ERR_STATE unless
+ * that run is still the recorded one (generation <= 0: the recorded run). */
+int oshun_adjoint_forward(oshun_ctx* ctx, double h, int nsteps, int every, long long* generation);
+int oshun_adjoint_backward(oshun_ctx* ctx, long long generation, double* lam_f, double* lam_fld);
 /* frees the adjoint work buffers and checkpoints (also done by oshun_destroy) */
 int oshun_adjoint_release(oshun_ctx* ctx);
 

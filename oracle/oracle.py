@@ -22,6 +22,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboshun_oracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "oshun_ref")
 
+# planes of Oracle.moments(), in the order of the library's oshun_moments
+MOMENT_NAMES = ("n", "Vx", "Vy", "VxVx", "VyVy", "VzVz", "Qx", "VxVy", "VxVz", "VyVz", "Vsq", "T_eV", "P_Mbar",
+                "ND", "nu", "Qy", "Kx", "Ky")
+
 
 def num_harmonics(l0, m0):
     """N_h of the (l, m) trapezoid, def-State.cpp:417."""
@@ -213,8 +217,8 @@ class Oracle:
         return out  # sub, diag, sup, scatt
 
     def moments(self):
-        """density, Vx, Vy, VxVx, VyVy, VzVz, Qx per (y, x) (def-Parallel.cpp:1677-2134, FP64)."""
-        out = np.zeros((7, self.cfg.ny, self.cfg.nx))
+        """The 18 moments of MOMENT_NAMES per (y, x) (def-Parallel.cpp:1677-2253, FP64)."""
+        out = np.zeros((len(MOMENT_NAMES), self.cfg.ny, self.cfg.nx))
         self._lib.oo_moments(self._h, _dptr(out))
         return out
 

@@ -801,23 +801,38 @@ struct Oracle {
   }
 
   // Nonrelativistic moments of Parallel_Output::nonrelativistic_moments,
-  // def-Parallel.cpp:1677-2134, in double instead of float.  out[k][y][x]:
-  // k = 0 density, 1 Vx, 2 Vy, 3 VxVx, 4 VyVy, 5 VzVz, 6 Qx.
+  // def-Parallel.cpp:1677-2253, in double instead of the reference's float,
+  // each in the reference's operation order.  out[k][y][x], k =
+  //   0 density (:1699-1726)   1 Vx (:1731-1758)   2 Vy (:1770-1797)
+  //   3 VxVx (:1805-1860)      4 VyVy (:1872-1897) 5 VzVz (:1903-1914)
+  //   6 Qx (:2094-2134)        7 VxVy (:1920-1947) 8 VxVz (:1953-1980)
+  //   9 VyVz (:1986-2013)     10 Vsq = <v^2> - Vx^2 - Vy^2 (:2019-2030)
+  //  11 T [eV] (:2034-2040)   12 P [Mbar] (:2041-2051)  13 N_D (:2055-2065)
+  //  14 nu (:2067-2080)       15 Qy (:2173-2213)  16 Kx (:2135-2165)
+  //  17 Ky (:2214-2244)
   // VyVy takes its 16pi/5 term from f20 as the reference does (:1875-1886).
+  // Harmonics the run does not have (f11, f21 for m0 < 1; f22 for m0 < 2;
+  // l = 2 for l0 < 2) count as zero.  Kx and Ky difference <v^2>^2 over the
+  // whole x (y) extent of one rank, as Matrix2D::Dd1/Dd2 do (matrices.h:
+  // 511-533), and copy the first and last interior values to the two ends;
+  // Ky needs ny >= 3 and is 0 otherwise (the reference reads past its array).
+  static constexpr int kNMom = 18;
   void moments(double* out) const {
     const std::vector<double>& pr = pd;
     const int N = np;
     const double dp = dpd;
+    const double fmin = 1.17549435082228750797e-38, feps = 1.1920928955078125e-07;
     std::vector<double> ig(N);
     for (int i = 0; i < N; ++i) ig[i] = 1.0 / std::sqrt(1.0 + pr[i] * pr[i]);
     const double r = pr[0] * pr[0] / (pr[1] * pr[1]);
     const double inv = 1.0 / (1.0 - r);
-    auto trap = [&](const cd* f, int p0, double (*w)(double, double), bool real_part) {
-      (void)real_part;
-      double s = (p0 == 0 ? f[0].real() * w(pr[0], ig[0]) : 0.0);
-      s += f[N - 1].real() * w(pr[N - 1], ig[N - 1]);
+    // trapezoid of part(f) * w over p: ends first (f(p0) skipped when from1), halved, then the interior
+    auto trap = [&](const cd* f, bool from1, double (*w)(double, double), bool imag) {
+      auto v = [&](int i) { return imag ? f[i].imag() : f[i].real(); };
+      double s = from1 ? 0.0 : v(0) * w(pr[0], ig[0]);
+      s += v(N - 1) * w(pr[N - 1], ig[N - 1]);
       s *= 0.5;
-      for (int i = 1; i < N - 1; ++i) s += w(pr[i], ig[i]) * f[i].real();
+      for (int i = 1; i < N - 1; ++i) s += w(pr[i], ig[i]) * v(i);
       return s;
     };
     auto w2 = [](double p, double g) { (void)g; return p * p; };
@@ -825,56 +840,106 @@ struct Oracle {
     auto w4 = [](double p, double g) { return std::pow(p, 4) * std::pow(g, 2); };
     auto w5 = [](double p, double g) { return std::pow(p, 5) * std::pow(g, 3); };
     const size_t nxy_ = (size_t)nx * ny;
+    auto O = [&](int k, size_t i) -> double& { return out[k * nxy_ + i]; };
     for (int iy = 0; iy < ny; ++iy)
       for (int ix = 0; ix < nx; ++ix) {
         const size_t k = (size_t)iy * nx + ix;
-        const cd* f0 = Y.h(H(0, 0)) + at(0, ix, iy);
-        const cd* f10 = Y.h(H(1, 0)) + at(0, ix, iy);
-        const cd* f11 = c.m0 >= 1 ? Y.h(H(1, 1)) + at(0, ix, iy) : nullptr;
-        const cd* f20 = c.l0 >= 2 ? Y.h(H(2, 0)) + at(0, ix, iy) : nullptr;
-        // density, :1699-1726
-        double n = trap(f0, 0, w2, true);
+        auto harm = [&](int l, int m) -> const cd* {
+          if (l > c.l0 || m > c.m0) return nullptr;
+          return Y.h(H(l, m)) + at(0, ix, iy);
+        };
+        const cd *f0 = harm(0, 0), *f10 = harm(1, 0), *f11 = harm(1, 1), *f20 = harm(2, 0), *f21 = harm(2, 1),
+                 *f22 = harm(2, 2);
+        // density
+        double n = trap(f0, false, w2, false);
         double c00 = f0[1].real() / 5.0 * r + (f0[0].real() - f0[1].real() * r) * inv * (1.0 / 3.0 - 1.0 / 5.0 * r);
         c00 *= pr[0] * (pr[0] / dp) * pr[0];
         n += c00;
         n *= 4.0 * M_PI * dp;
-        if (n < 4.0 * 1.1920928955078125e-07) n = 1.0;
-        // Vx, Vy, :1731-1800
-        double vx = trap(f10, 0, w3, true) + f10[1].real() * (std::pow(pr[0], 5) / (dp * pr[1] * 5.0));
+        if (n < 4.0 * feps) n = 1.0;
+        // Vx, Vy
+        const double c5 = std::pow(pr[0], 5) / (dp * pr[1] * 5.0), c7 = std::pow(pr[0], 7) / (dp * pr[1] * 7.0);
+        double vx = trap(f10, false, w3, false) + f10[1].real() * c5;
         vx *= (4.0 * M_PI / 3.0) * dp;
         vx /= n;
         double vy = 0.0;
         if (f11) {
-          vy = trap(f11, 0, w3, true) + f11[1].real() * (std::pow(pr[0], 5) / (dp * pr[1] * 5.0));
+          vy = trap(f11, false, w3, false) + f11[1].real() * c5;
           vy *= (8.0 * M_PI / 3.0) * dp;
           vy /= n;
         }
-        // VxVx, VyVy, VzVz, :1805-1916
-        double m1 = trap(f0, 0, w4, true);
+        // VxVx, VyVy, VzVz
+        double m1 = trap(f0, false, w4, false);
         double c01 = f0[1].real() / 7.0 * r + (f0[0].real() - f0[1].real() * r) * inv * (1.0 / 5.0 - 1.0 / 7.0 * r);
         c01 *= std::pow(pr[0], 4) * (pr[0] / dp);
         m1 += c01;
         m1 *= 4.0 * M_PI / 3.0 * dp;
-        double m2 = f20 ? trap(f20, 1, w4, true) * (8.0 * M_PI / 15.0 * dp) : 0.0;
+        const double t20 = f20 ? trap(f20, true, w4, false) : 0.0;
+        double m2 = t20 * (8.0 * M_PI / 15.0 * dp);
         const double vxvx = (m1 + m2) / n;
         m2 *= 0.5;
         const double m1b = m1 - m2;
-        const double t22 = f20 ? trap(f20, 1, w4, true) * (16.0 * M_PI / 5.0 * dp) : 0.0;
+        const double t22 = t20 * (16.0 * M_PI / 5.0 * dp);
         const double vyvy = (t22 + m1b) / n;
         const double vzvz = (m1b - t22) / n;
-        // Qx, :2094-2134 (VxVy = 0 here: it needs f21)
+        // VxVy, VxVz (f21), VyVz (f22)
+        const double vxvy = f21 ? trap(f21, true, w4, false) * (8.0 * M_PI / 5.0 * dp) / n : 0.0;
+        const double vxvz = f21 ? trap(f21, true, w4, true) * (-(8.0 * M_PI / 5.0) * dp) / n : 0.0;
+        const double vyvz = f22 ? trap(f22, true, w4, true) * (-(16.0 * M_PI / 5.0) * dp) / n : 0.0;
+        // <v^2> about the mean, temperature, pressure, Debye number, collision frequency
+        double vsq = 0.0;
+        vsq += vxvx; vsq += vyvy; vsq += vzvz;
+        vsq -= vx * vx; vsq -= vy * vy;
+        const double teV = vsq * 170880.776;
+        const double pre = vsq * (2.73785179e-19 * c.density_np) * n;
+        const double nd = 1.72e+9 * std::sqrt(std::pow(vsq * 170880.776, 3) / (c.density_np * n));
+        double lnei = 24.0 - 0.5 * std::log(c.density_np * n) + std::log(vsq * 170880.776);
+        lnei *= c.zeta;
+        const double nu = std::sqrt(2.0 / M_PI) / 9.0 * lnei / nd;
+        // Qx, Qy
         double q = 0.0;
         q -= vx * (vxvx + vyvy + vzvz);
-        q -= 2.0 * (vxvx * vx);
+        q -= 2.0 * (vxvx * vx + vxvy * vy);
         q += 2.0 * (vx * vx + vy * vy) * vx;
         q *= n;
-        double m5 = trap(f10, 0, w5, true) + f10[1].real() * (std::pow(pr[0], 7) / (dp * pr[1] * 7.0));
+        double m5 = trap(f10, false, w5, false) + f10[1].real() * c7;
         m5 *= (4.0 * M_PI / 3.0) * dp;
         q += m5;
         q *= 0.5;
-        const double vals[7] = {n, vx, vy, vxvx, vyvy, vzvz, q};
-        for (int j = 0; j < 7; ++j) out[j * nxy_ + k] = vals[j];
+        double qy = 0.0;
+        qy -= vy * (vxvx + vyvy + vzvz);
+        qy -= 2.0 * (vxvy * vx + vyvy * vy);
+        qy += 2.0 * (vx * vx + vy * vy) * vy;
+        qy *= n;
+        double m5y = f11 ? trap(f11, false, w5, false) + f11[1].real() * c7 : 0.0;
+        m5y *= (8.0 * M_PI / 3.0) * dp;
+        qy += m5y;
+        qy *= 0.5;
+        const double vals[16] = {n, vx, vy, vxvx, vyvy, vzvz, q, vxvy, vxvz, vyvz, vsq, teV, pre, nd, nu, qy};
+        for (int j = 0; j < 16; ++j) O(j, k) = vals[j];
       }
+    // heat conductivities, def-Parallel.cpp:2135-2165 and :2214-2244
+    auto kond = [&](bool ydir, int kq, int kout) {
+      const int len = ydir ? ny : nx;
+      for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < nx; ++ix) {
+          const size_t k = (size_t)iy * nx + ix;
+          if (len < 3) { O(kout, k) = 0.0; continue; }
+          const int i = ydir ? iy : ix;
+          const int ii = i == 0 ? 1 : (i == len - 1 ? len - 2 : i);  // ends take their neighbour's value
+          const size_t km = ydir ? (size_t)(ii - 1) * nx + ix : (size_t)iy * nx + ii - 1;
+          const size_t kp = ydir ? (size_t)(ii + 1) * nx + ix : (size_t)iy * nx + ii + 1;
+          double cond = O(10, km) * O(10, km) - O(10, kp) * O(10, kp);  // Dd: minus central difference
+          cond *= 0.5 / (ydir ? -c.dy : -c.dx);                          // 0.5/(axis(0) - axis(1))
+          cond *= O(0, k);  // density and nu of the cell itself (applied after the end copies)
+          cond /= O(14, k);
+          cond *= (-1.0) / 18.0;
+          if (cond < 4.0 * fmin && cond > -4.0 * fmin) cond = 1.0;
+          O(kout, k) = O(kq, k) / cond;
+        }
+    };
+    kond(false, 6, 16);
+    kond(true, 15, 17);
   }
 
   // main_loop step body, main.cpp:79-107

@@ -31,6 +31,9 @@ FP64 = 0
 FP32_SPLIT = 1
 COLL_EE = 1
 COLL_FLM = 2
+# planes of Context.moments() (oshun_moments): def-Parallel.cpp:1677-2253 in FP64
+MOMENT_NAMES = ("n", "Vx", "Vy", "VxVx", "VyVy", "VzVz", "Qx", "VxVy", "VxVz", "VyVz", "Vsq", "T_eV", "P_Mbar",
+                "ND", "nu", "Qy", "Kx", "Ky")
 
 
 class OshunError(RuntimeError):
@@ -124,8 +127,8 @@ def _load():
     lib.oshun_batch_autotune.argtypes = [P, I, ctypes.POINTER(I), dp]
     lib.oshun_copy_device_state.argtypes = [P, P, P, I]
     lib.oshun_adjoint_step.argtypes = [P, D, P, P]
-    lib.oshun_adjoint_forward.argtypes = [P, D, I, I]
-    lib.oshun_adjoint_backward.argtypes = [P, P, P]
+    lib.oshun_adjoint_forward.argtypes = [P, D, I, I, ctypes.POINTER(ctypes.c_longlong)]
+    lib.oshun_adjoint_backward.argtypes = [P, ctypes.c_longlong, P, P]
     lib.oshun_adjoint_release.argtypes = [P]
     _lib = lib
     return lib
@@ -279,8 +282,8 @@ class Context:
         return info["ncomp"] * info["plane"] * esz
 
     def moments(self):
-        """density, Vx, Vy, VxVx, VyVy, VzVz, Qx per (y, x), FP64 (def-Parallel.cpp:1677-2134)."""
-        out = np.zeros((7, self.ny, self.nx))
+        """The moments of MOMENT_NAMES per (y, x) in FP64 (def-Parallel.cpp:1677-2253)."""
+        out = np.zeros((len(MOMENT_NAMES), self.ny, self.nx))
         self._chk(self._lib.oshun_moments(self._h, _dptr(out)))
         return out
 
@@ -295,10 +298,15 @@ class Context:
         self._chk(self._lib.oshun_adjoint_step(self._h, float(h), lam_f_ptr, lam_fld_ptr))
 
     def adjoint_forward(self, h, nsteps, every):
-        self._chk(self._lib.oshun_adjoint_forward(self._h, float(h), int(nsteps), int(every)))
+        """Record a checkpointed run; returns its generation number."""
+        gen = ctypes.c_longlong(0)
+        self._chk(self._lib.oshun_adjoint_forward(self._h, float(h), int(nsteps), int(every), ctypes.byref(gen)))
+        return gen.value
 
-    def adjoint_backward(self, lam_f_ptr, lam_fld_ptr):
-        self._chk(self._lib.oshun_adjoint_backward(self._h, lam_f_ptr, lam_fld_ptr))
+    def adjoint_backward(self, lam_f_ptr, lam_fld_ptr, generation=0):
+        """Cotangent of the recorded run's final state -> of its initial state, in place.
+        generation > 0: fail unless that run is still the recorded one."""
+        self._chk(self._lib.oshun_adjoint_backward(self._h, int(generation), lam_f_ptr, lam_fld_ptr))
 
     def diag_copy(self, nbytes):
         """Plain device copy of nbytes between scratch buffers (counter calibration)."""

@@ -604,6 +604,7 @@ struct AdjState {
   double yN_t[3] = {0, 0, 0};
   double h = 0.0;
   int nsteps = 0, every = 0;
+  long long generation = 0;  // of the recorded run: one more per oshun_adjoint_forward
 };
 
 std::map<const oshun_ctx*, AdjState>& adj_map() {
@@ -813,7 +814,7 @@ int oshun_adjoint_step(oshun_ctx* c, double h, double* lam_f, double* lam_fld) {
   return adj_step(c, adj_map()[c], h, lam_f, lam_fld);
 }
 
-int oshun_adjoint_forward(oshun_ctx* c, double h, int nsteps, int every) {
+int oshun_adjoint_forward(oshun_ctx* c, double h, int nsteps, int every, long long* generation) {
   int rc = adj_check(c);
   if (rc) return rc;
   if (nsteps < 1 || every < 1) return fail(c, OSHUN_ERR_ARG, "adjoint: nsteps and every must be >= 1");
@@ -821,6 +822,8 @@ int oshun_adjoint_forward(oshun_ctx* c, double h, int nsteps, int every) {
   for (double* p : s.ck) (void)hipFree(p);
   s.ck.clear();
   s.ck_t.clear();
+  ++s.generation;  // the previous run's checkpoints are gone
+  if (generation) *generation = s.generation;
   const long long S = adj_state_elems(c), Fn = 6LL * c->nxy;
   s.h = h; s.nsteps = nsteps; s.every = every;
   for (int i = 0; i < nsteps; ++i) {
@@ -840,11 +843,15 @@ int oshun_adjoint_forward(oshun_ctx* c, double h, int nsteps, int every) {
   return OSHUN_OK;
 }
 
-int oshun_adjoint_backward(oshun_ctx* c, double* lam_f, double* lam_fld) {
+int oshun_adjoint_backward(oshun_ctx* c, long long generation, double* lam_f, double* lam_fld) {
   int rc = adj_check(c);
   if (rc) return rc;
   AdjState& s = adj_map()[c];
   if (s.ck.empty() || !s.yN) return fail(c, OSHUN_ERR_STATE, "adjoint: no recorded run (oshun_adjoint_forward)");
+  if (generation > 0 && generation != s.generation)
+    return fail(c, OSHUN_ERR_STATE, "adjoint: run " + std::to_string(generation) +
+                                        " was replaced by a later oshun_adjoint_forward on this context (run " +
+                                        std::to_string(s.generation) + ")");
   if (!lam_f || !lam_fld) return fail(c, OSHUN_ERR_ARG, "adjoint: null cotangent");
   const long long S = adj_state_elems(c), Fn = 6LL * c->nxy;
   if (s.seg_cap < s.every) {

@@ -257,6 +257,8 @@ int oshun_step_implicit_1d(oshun_ctx* c, double h, int nsteps, int if_implicitES
   if (!sane(c) || nsteps < 0 || !(h > 0.0)) return OSHUN_ERR_ARG;
   if (!c->use1d || c->fp32 || c->T.l0 < 3 || c->T.l0 > 12)
     return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_1d: needs a 1D run with m0 = 0, 3 <= l0 <= 12, FP64");
+  if (!c->use_pair || !c->rows_npl)
+    return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_1d: the state is too large for its 32-bit-offset kernels");
   c->next1_ready = false;
   for (int s = 0; s < nsteps; ++s) {
     const int rc = implicit_step_1d(c, h, if_implicitES);

@@ -2304,86 +2304,181 @@ __global__ __launch_bounds__(256) void batch_clock(double* __restrict__ t, const
 #endif
 
 // Nonrelativistic moments, Parallel_Output::nonrelativistic_moments
-// (def-Parallel.cpp:1677-2134) in double: one thread per (x,y), trapezoid
-// sums along p with coalesced loads along x.  out[k][y][x], k = density, Vx,
-// Vy, VxVx, VyVy, VzVz, Qx.
+// (def-Parallel.cpp:1677-2253) in double, in three launches:
+//   moments_partial  the ten trapezoid sums over p, split into p chunks
+//                    (one thread per (x, chunk, y), coalesced along x) into
+//                    part[chunk][sum][y][x];
+//   moments_final    per (x,y): the chunks summed in order, the p0 terms, and
+//                    the 16 pointwise moments;
+//   moments_cond     per (x,y): Kx and Ky, which difference <v^2>^2 over x, y.
+// out[k][y][x], k = density, Vx, Vy, VxVx, VyVy, VzVz, Qx, VxVy, VxVz, VyVz,
+// Vsq, T[eV], P[Mbar], N_D, nu, Qy, Kx, Ky.
+constexpr int kMomSums = 10;
+constexpr int kMomOut = 18;
+constexpr int kMomChunkMin = 16;  // p rows per chunk at least
+constexpr int kMomChunkMax = 32;  // chunks at most
+
 template <typename T>
 struct MomArgs {
   const T* f;
-  double* out;
+  double* part;  // [nchunk][kMomSums][ny][nx]
+  double* out;   // [kMomOut][ny][nx]
   const double *w2, *w3, *w4, *w5;
   const double* bg;  // f00 background per p added to plane 0, or null
-  int np, nx, ny, c00, c10, c11r, c20;
+  int np, nx, ny, nchunk, plen;
+  // real planes of f00, f10, Re f11, f20, Re f21, Im f21, Im f22 (-1: not in the run)
+  int c00, c10, c11r, c20, c21r, c21i, c22i;
   long long plane;
-  double dp, r, inv, c5, c7, c00k, c04k;
+  double dp, r, inv, c5, c7, c00k, c04k, density_np, zeta, dx, dy;
 };
 
 template <typename T>
-__device__ __forceinline__ double trap_p(const T* f, long long base, int nx, int np, const double* w, bool skip0,
-                                         const double* bg = nullptr) {
-  auto val = [&](int p) { return static_cast<double>(f[base + (long long)p * nx]) + (bg ? bg[p] : 0.0); };
-  double s = skip0 ? 0.0 : val(0) * w[0];
-  s += val(np - 1) * w[np - 1];
-  s *= 0.5;
-  for (int p = 1; p < np - 1; ++p) s += w[p] * val(p);
-  return s;
+__global__ __launch_bounds__(64) void moments_partial(MomArgs<T> a) {
+  const int x = blockIdx.x * 64 + threadIdx.x;
+  if (x >= a.nx) return;
+  const int ch = blockIdx.y, y = blockIdx.z;
+  const int nx = a.nx, np = a.np;
+  const int p0 = ch * a.plen, p1 = min(np, p0 + a.plen);
+  const long long base = (long long)y * np * nx + x;
+  auto row = [&](int c, int p) -> double {
+    return c < 0 ? 0.0 : static_cast<double>(a.f[(long long)c * a.plane + base + (long long)p * nx]);
+  };
+  double s[kMomSums];
+#pragma unroll
+  for (int k = 0; k < kMomSums; ++k) s[k] = 0.0;
+  for (int p = p0; p < p1; ++p) {
+    const double we = (p == 0 || p == np - 1) ? 0.5 : 1.0;  // trapezoid end weights
+    const double wi = p == 0 ? 0.0 : we;                     // sums that start at p1 (f20(p0) = 0)
+    const double f00 = row(a.c00, p) + (a.bg ? a.bg[p] : 0.0), f10 = row(a.c10, p), f11 = row(a.c11r, p);
+    const double W2 = a.w2[p] * we, W3 = a.w3[p] * we, W4 = a.w4[p] * we, W4i = a.w4[p] * wi, W5 = a.w5[p] * we;
+    s[0] += W2 * f00;
+    s[1] += W3 * f10;
+    s[2] += W3 * f11;
+    s[3] += W4 * f00;
+    s[4] += W4i * row(a.c20, p);
+    s[5] += W4i * row(a.c21r, p);
+    s[6] += W4i * row(a.c21i, p);
+    s[7] += W4i * row(a.c22i, p);
+    s[8] += W5 * f10;
+    s[9] += W5 * f11;
+  }
+  const long long nxy = (long long)nx * a.ny, o = (long long)y * nx + x;
+#pragma unroll
+  for (int k = 0; k < kMomSums; ++k) a.part[((long long)ch * kMomSums + k) * nxy + o] = s[k];
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void moments_kernel(MomArgs<T> a) {
+__global__ __launch_bounds__(256) void moments_final(MomArgs<T> a) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   const int nxy = a.nx * a.ny;
   if (i >= nxy) return;
   const int x = i % a.nx, y = i / a.nx;
+  double s[kMomSums];
+#pragma unroll
+  for (int k = 0; k < kMomSums; ++k) {
+    double v = 0.0;
+    for (int ch = 0; ch < a.nchunk; ++ch) v += a.part[((long long)ch * kMomSums + k) * nxy + i];
+    s[k] = v;
+  }
   const long long base = (long long)y * a.np * a.nx + x;
-  const int nx = a.nx, np = a.np;
-  const T* f0 = a.f + (long long)a.c00 * a.plane;
-  const T* f10 = a.f + (long long)a.c10 * a.plane;
-  const double f0_0 = static_cast<double>(f0[base]) + (a.bg ? a.bg[0] : 0.0);
-  const double f0_1 = static_cast<double>(f0[base + nx]) + (a.bg ? a.bg[1] : 0.0);
-  double n = trap_p<T>(f0, base, nx, np, a.w2, false, a.bg);
+  auto at = [&](int c, int p) -> double {
+    return c < 0 ? 0.0 : static_cast<double>(a.f[(long long)c * a.plane + base + (long long)p * a.nx]);
+  };
+  const double f0_0 = at(a.c00, 0) + (a.bg ? a.bg[0] : 0.0);
+  const double f0_1 = at(a.c00, 1) + (a.bg ? a.bg[1] : 0.0);
+  // density, with the p0 extrapolation term (:1712-1717) and the FLT_EPSILON floor (:1722-1726)
+  double n = s[0];
   double c = f0_1 / 5.0 * a.r + (f0_0 - f0_1 * a.r) * a.inv * (1.0 / 3.0 - 1.0 / 5.0 * a.r);
   c *= a.c00k;
   n += c;
   n *= 4.0 * M_PI * a.dp;
   if (n < 4.0 * 1.1920928955078125e-07) n = 1.0;
-  const double f10_1 = static_cast<double>(f10[base + nx]);
-  double vx = trap_p<T>(f10, base, nx, np, a.w3, false) + f10_1 * a.c5;
+  double vx = s[1] + at(a.c10, 1) * a.c5;
   vx *= (4.0 * M_PI / 3.0) * a.dp;
   vx /= n;
   double vy = 0.0;
   if (a.c11r >= 0) {
-    const T* f11 = a.f + (long long)a.c11r * a.plane;
-    vy = trap_p<T>(f11, base, nx, np, a.w3, false) + static_cast<double>(f11[base + nx]) * a.c5;
+    vy = s[2] + at(a.c11r, 1) * a.c5;
     vy *= (8.0 * M_PI / 3.0) * a.dp;
     vy /= n;
   }
-  double m1 = trap_p<T>(f0, base, nx, np, a.w4, false, a.bg);
+  double m1 = s[3];
   double c1 = f0_1 / 7.0 * a.r + (f0_0 - f0_1 * a.r) * a.inv * (1.0 / 5.0 - 1.0 / 7.0 * a.r);
   c1 *= a.c04k;
   m1 += c1;
   m1 *= 4.0 * M_PI / 3.0 * a.dp;
-  double t20 = 0.0;
-  if (a.c20 >= 0) t20 = trap_p<T>(a.f + (long long)a.c20 * a.plane, base, nx, np, a.w4, true);
-  double m2 = t20 * (8.0 * M_PI / 15.0 * a.dp);
+  double m2 = s[4] * (8.0 * M_PI / 15.0 * a.dp);
   const double vxvx = (m1 + m2) / n;
   m2 *= 0.5;
   const double m1b = m1 - m2;
-  const double t22 = t20 * (16.0 * M_PI / 5.0 * a.dp);  // f20 as in def-Parallel.cpp:1875-1886
+  const double t22 = s[4] * (16.0 * M_PI / 5.0 * a.dp);  // f20 as in def-Parallel.cpp:1875-1886
   const double vyvy = (t22 + m1b) / n;
   const double vzvz = (m1b - t22) / n;
+  const double vxvy = s[5] * (8.0 * M_PI / 5.0 * a.dp) / n;
+  const double vxvz = s[6] * (-(8.0 * M_PI / 5.0) * a.dp) / n;
+  const double vyvz = s[7] * (-(16.0 * M_PI / 5.0) * a.dp) / n;
+  double vsq = 0.0;
+  vsq += vxvx; vsq += vyvy; vsq += vzvz;
+  vsq -= vx * vx; vsq -= vy * vy;
+  const double teV = vsq * 170880.776;
+  const double pre = vsq * (2.73785179e-19 * a.density_np) * n;
+  const double nd = 1.72e+9 * sqrt((teV * teV * teV) / (a.density_np * n));
+  double lnei = 24.0 - 0.5 * log(a.density_np * n) + log(teV);
+  lnei *= a.zeta;
+  const double nu = sqrt(2.0 / M_PI) / 9.0 * lnei / nd;
   double q = 0.0;
   q -= vx * (vxvx + vyvy + vzvz);
-  q -= 2.0 * (vxvx * vx);
+  q -= 2.0 * (vxvx * vx + vxvy * vy);
   q += 2.0 * (vx * vx + vy * vy) * vx;
   q *= n;
-  double m5 = trap_p<T>(f10, base, nx, np, a.w5, false) + f10_1 * a.c7;
+  double m5 = s[8] + at(a.c10, 1) * a.c7;
   m5 *= (4.0 * M_PI / 3.0) * a.dp;
   q += m5;
   q *= 0.5;
-  const double vals[7] = {n, vx, vy, vxvx, vyvy, vzvz, q};
+  double qy = 0.0;
+  qy -= vy * (vxvx + vyvy + vzvz);
+  qy -= 2.0 * (vxvy * vx + vyvy * vy);
+  qy += 2.0 * (vx * vx + vy * vy) * vy;
+  qy *= n;
+  double m5y = a.c11r >= 0 ? s[9] + at(a.c11r, 1) * a.c7 : 0.0;
+  m5y *= (8.0 * M_PI / 3.0) * a.dp;
+  qy += m5y;
+  qy *= 0.5;
+  const double vals[16] = {n, vx, vy, vxvx, vyvy, vzvz, q, vxvy, vxvz, vyvz, vsq, teV, pre, nd, nu, qy};
 #pragma unroll
-  for (int k = 0; k < 7; ++k) a.out[(long long)k * nxy + i] = vals[k];
+  for (int k = 0; k < 16; ++k) a.out[(long long)k * nxy + i] = vals[k];
+}
+
+// Kx, Ky (def-Parallel.cpp:2135-2165, :2214-2244): d<v^2>^2 as Matrix2D::Dd1/Dd2
+// (matrices.h:511-533) over the rank's interior, the two end cells taking
+// their neighbour's difference; Ky is 0 below 3 rows of y
+template <typename T>
+__global__ __launch_bounds__(256) void moments_cond(MomArgs<T> a) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int nx = a.nx, ny = a.ny, nxy = nx * ny;
+  if (i >= nxy) return;
+  const int x = i % nx, y = i / nx;
+  const double* V = a.out + 10LL * nxy;
+  const double n = a.out[i], nu = a.out[14LL * nxy + i];
+  const double fmin4 = 4.0 * 1.17549435082228750797e-38;
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    const int len = d ? ny : nx, j = d ? y : x;
+    double k = 0.0;
+    if (len >= 3) {
+      const int jj = j == 0 ? 1 : (j == len - 1 ? len - 2 : j);
+      const int im = d ? (jj - 1) * nx + x : y * nx + jj - 1;
+      const int ip = d ? (jj + 1) * nx + x : y * nx + jj + 1;
+      double cond = V[im] * V[im] - V[ip] * V[ip];
+      cond *= 0.5 / (d ? -a.dy : -a.dx);
+      cond *= n;
+      cond /= nu;
+      cond *= (-1.0) / 18.0;
+      if (cond < fmin4 && cond > -fmin4) cond = 1.0;
+      k = a.out[(d ? 15LL : 6LL) * nxy + i] / cond;
+    }
+    a.out[(16LL + d) * nxy + i] = k;
+  }
 }
 
 #ifndef OSHUN_FAMILY_TU

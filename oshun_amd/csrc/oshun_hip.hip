@@ -63,7 +63,8 @@ struct oshun_ctx {
   double* d_p3og = nullptr;
   double* d_e1d = nullptr;  // 1D m0 = 0 coefficient rows: eA1 | eA2 | xA1 | xA2
   double* d_mw = nullptr;   // moment weights w2 | w3 | w4 | w5
-  double* d_mom = nullptr;  // moments output, 7 planes of nxy
+  double* d_mom = nullptr;    // moments output, kMomOut planes of nxy (first oshun_moments call)
+  double* d_mpart = nullptr;  // partial p sums of the moments, moment_chunks x kMomSums planes
   bool use1d = false;       // specialised 1D m0 = 0 Vlasov kernel
   // general system: index into OSHUN_STAGE_SPECS of the compile-time stage
   // kernel for this (l0, m0, 2D), -1 for the term-list interpreter
@@ -745,7 +746,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -757,28 +758,95 @@ bool sane(const oshun_ctx* c) { return c && c->S; }
 
 }  // namespace
 
+// Kernel choice of a context, on the host only (oshun_create and
+// oshun_kernel_plan): c->g, c->ph, c->T, nc, np, nx, ny, plane and esz set.
+// The stage-pair kernels address the whole state with 32-bit byte offsets and
+// collide_rows a plane; larger states take the 64-bit kernels (the 1D
+// single-stage kernel, one wavefront per collision column).
+static void plan_kernels(oshun_ctx* c, std::vector<double>* spec_coef) {
+  const oshun_grid& g = c->g;
+  const unsigned long long state_bytes = (unsigned long long)c->nc * c->plane * c->esz;
+  c->use1d = (g.m0 == 0 && g.numy == 1 && g.l0 <= kMaxL0Special);
+  c->use_pair = c->use1d && state_bytes < (1ull << 31);
+  c->pair_r = 3;
+  if (c->use_pair) {
+    const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
+    const bool fusable = (c->ph.collisions & OSHUN_COLL_EE) && (c->ph.collisions & OSHUN_COLL_FLM);
+    if (g.l0 <= kPairSmallL0 && ntx * pair_tiles_r(c->np, 3) < 2 * 256 &&
+        (!fusable || pair_tiles_r(c->np, 3) > oshun::kFuseTiles || pair_tiles_r(c->np, 2) <= oshun::kFuseTiles))
+      c->pair_r = 2;  // small grid (keeping the fused field stages where R = 3 has them)
+  }
+  c->npt = c->use_pair ? pair_tiles_r(c->np, c->pair_r) : 0;
+  c->spec = -1;
+  if (!c->use1d && (unsigned long long)c->plane * sizeof(double) < (1ull << 32)) {  // 32-bit plane offsets
+    std::vector<double> coef;
+#define OSHUN_SPEC_FIND(ID, L, M, TD) \
+    if (c->spec < 0 && oshun::spec_coefficients<L, M, TD>(c->T, coef)) c->spec = ID;
+    OSHUN_STAGE_SPECS(OSHUN_SPEC_FIND)
+#undef OSHUN_SPEC_FIND
+    if (spec_coef) *spec_coef = coef;
+  }
+  // collide_rows: SEGW lanes x NPL rows per column; needs NPL >= NB (the
+  // parabolic low-p rows stay in lane 0) and SEGW*NPL >= np.  Above np = 128
+  // both launches use 32 lanes (two DPP rows) per column: at Nx = 4096 that
+  // keeps two wavefronts per SIMD in flight instead of one.  The two launches
+  // keep separate tables so that their shapes can differ.
+  c->rows_npl = c->rows_npl2 = 0;
+  c->rows_segw = c->rows_segw2 = 16;
+  const int nb = c->ph.nb_algorithms;
+  auto pick = [&](const int (*cand)[2], int n, int* segw, int* npl) {
+    for (int k = 0; k < n; ++k)
+      if (cand[k][0] * cand[k][1] >= c->np && cand[k][1] >= nb && c->np >= 3) {
+        *segw = cand[k][0];
+        *npl = cand[k][1];
+        return;
+      }
+  };
+  const int ee[3][2] = {{16, 4}, {16, 8}, {32, 8}};
+  const int im[3][2] = {{16, 4}, {16, 8}, {32, 8}};
+  pick(ee, 3, &c->rows_segw, &c->rows_npl);
+  pick(im, 3, &c->rows_segw2, &c->rows_npl2);
+  const bool plane_fits = (unsigned long long)c->plane * c->esz <= 0x7ffffff0ull;  // collide_rows' kOut
+  if (!c->rows_npl || !c->rows_npl2 || !plane_fits) c->rows_npl = c->rows_npl2 = 0;  // both or neither
+}
+
+static_assert(oshun::kMomOut == OSHUN_NMOMENTS, "moment planes of the kernels and of the C ABI");
+
+// number of p chunks of the moment sums (kMomChunkMin rows each at least)
+static int moment_chunks(const oshun_ctx* c) {
+  return std::max(1, std::min(oshun::kMomChunkMax, c->np / oshun::kMomChunkMin));
+}
+
+// the three moment launches on c's stream; part holds nchunk x kMomSums planes
+// of nxy doubles, out kMomOut planes
 template <typename T>
-static int launch_moments(oshun_ctx* c) {
+static int launch_moments(oshun_ctx* c, double* part, int nchunk, double* out) {
   oshun::MomArgs<T> a;
   a.f = static_cast<const T*>(c->S);
-  a.out = c->d_mom;
+  a.part = part;
+  a.out = out;
   const size_t n = c->np;
   a.w2 = c->d_mw; a.w3 = c->d_mw + n; a.w4 = c->d_mw + 2 * n; a.w5 = c->d_mw + 3 * n;
   a.bg = c->d_bg;
   a.np = c->np; a.nx = c->nx; a.ny = c->ny;
-  const int m0 = c->T.m0;
-  a.c00 = c->T.comp_re[0];
-  a.c10 = c->T.comp_re[oshun::hind(1, 0, m0)];
-  a.c11r = m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, m0)] : -1;
-  a.c20 = c->T.l0 >= 2 ? c->T.comp_re[oshun::hind(2, 0, m0)] : -1;
+  a.nchunk = nchunk;
+  a.plen = (c->np + nchunk - 1) / nchunk;
+  const int l0 = c->T.l0, m0 = c->T.m0;
+  auto re = [&](int l, int m) { return (l <= l0 && m <= m0) ? c->T.comp_re[oshun::hind(l, m, m0)] : -1; };
+  auto im = [&](int l, int m) { return (l <= l0 && m <= m0 && m > 0) ? c->T.comp_im[oshun::hind(l, m, m0)] : -1; };
+  a.c00 = re(0, 0); a.c10 = re(1, 0); a.c11r = re(1, 1); a.c20 = re(2, 0);
+  a.c21r = re(2, 1); a.c21i = im(2, 1); a.c22i = im(2, 2);
   a.plane = c->plane;
   a.dp = c->T.dpd; a.r = c->T.m_r; a.inv = c->T.m_inv; a.c5 = c->T.m_c5; a.c7 = c->T.m_c7;
   a.c00k = c->T.m_c00; a.c04k = c->T.m_c04;
-  hipLaunchKernelGGL(oshun::moments_kernel<T>, dim3((c->nxy + 255) / 256), dim3(256), 0, c->stream, a);
+  a.density_np = c->ph.density_np; a.zeta = c->ph.zeta; a.dx = c->g.dx; a.dy = c->g.dy;
+  const dim3 g1((c->nx + 63) / 64, nchunk, c->ny), g2((c->nxy + 255) / 256);
+  hipLaunchKernelGGL(oshun::moments_partial<T>, g1, dim3(64), 0, c->stream, a);
+  hipLaunchKernelGGL(oshun::moments_final<T>, g2, dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(oshun::moments_cond<T>, g2, dim3(256), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
-
 
 extern "C" {
 
@@ -801,10 +869,11 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
   oshun::build_tables(c->T, g.l0, g.m0, g.nump, g.numy, g.pmin, g.pmax, g.dx, g.dy, phys->density_np,
                       phys->nb_algorithms, phys->ref_quirks);
   c->nc = c->T.nc; c->np = g.nump; c->nx = g.numx; c->ny = g.numy; c->nxy = g.numx * g.numy;
-  c->use1d = (g.m0 == 0 && g.numy == 1 && g.l0 <= kMaxL0Special);
+  c->plane = (long long)g.numx * g.numy * g.nump;
+  std::vector<double> spec_coef;
+  plan_kernels(c, &spec_coef);
   const int prows = c->use1d ? oshun::kRows1D : oshun::kTileP;
   c->npb = (g.nump + prows - 1) / prows;
-  c->plane = (long long)g.numx * g.numy * g.nump;
   auto bail = [&](int rc) { free_all(c); std::string m = c->err; delete c; (void)m; return rc; };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(OSHUN_ERR_HIP);
   const size_t fbytes = (size_t)c->nc * c->plane * c->esz;
@@ -821,14 +890,7 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     if (hipMemset(*p, 0, fldbytes) != hipSuccess) return bail(OSHUN_ERR_HIP);
   if (hipMalloc((void**)&c->jpart, (size_t)c->npb * 3 * c->nxy * sizeof(double)) != hipSuccess)
     return bail(OSHUN_ERR_ALLOC);
-  c->use_pair = c->use1d;
   if (c->use_pair) {
-    const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
-    const bool fusable = (phys->collisions & OSHUN_COLL_EE) && (phys->collisions & OSHUN_COLL_FLM);
-    if (g.l0 <= kPairSmallL0 && ntx * pair_tiles_r(c->np, 3) < 2 * 256 &&
-        (!fusable || pair_tiles_r(c->np, 3) > oshun::kFuseTiles || pair_tiles_r(c->np, 2) <= oshun::kFuseTiles))
-      c->pair_r = 2;  // small grid (keeping the fused field stages where R = 3 has them)
-    c->npt = pair_tiles_r(c->np, c->pair_r);
     const int npt_max = (c->np + kPairTPMin - 1) / kPairTPMin;  // partial buffers fit every tile height
     const size_t nx = c->nx;
     if (hipMalloc((void**)&c->jp1, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
@@ -848,14 +910,7 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
       upload_vec(c, &c->d_clean, c->T.clean) || upload_vec(c, &c->d_comp_l, c->T.comp_l) ||
       upload_vec(c, &c->d_p3og, c->T.p3og))
     return bail(OSHUN_ERR_HIP);
-  if (!c->use1d && (unsigned long long)c->plane * sizeof(double) < (1ull << 32)) {  // 32-bit plane offsets
-    std::vector<double> coef;
-#define OSHUN_SPEC_FIND(ID, L, M, TD) \
-    if (c->spec < 0 && oshun::spec_coefficients<L, M, TD>(c->T, coef)) c->spec = ID;
-    OSHUN_STAGE_SPECS(OSHUN_SPEC_FIND)
-#undef OSHUN_SPEC_FIND
-    if (c->spec >= 0 && upload_vec(c, &c->d_coef, coef)) return bail(OSHUN_ERR_HIP);
-  }
+  if (c->spec >= 0 && upload_vec(c, &c->d_coef, spec_coef)) return bail(OSHUN_ERR_HIP);
   {
     std::vector<double> e;
     e.insert(e.end(), c->T.e1d_A1.begin(), c->T.e1d_A1.end());
@@ -868,7 +923,6 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     std::vector<double> w;
     for (auto* t : {&c->T.mw2, &c->T.mw3, &c->T.mw4, &c->T.mw5}) w.insert(w.end(), t->begin(), t->end());
     if (upload_vec(c, &c->d_mw, w)) return bail(OSHUN_ERR_HIP);
-    if (hipMalloc((void**)&c->d_mom, (size_t)7 * c->nxy * sizeof(double)) != hipSuccess) return bail(OSHUN_ERR_ALLOC);
   }
   {
     std::vector<double> vt;
@@ -918,27 +972,6 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     c->ct.vc = b + 9 * n; c->ct.ivdm = b + 10 * n; c->ct.ivdp = b + 11 * n; c->ct.ivs2 = b + 12 * n;
     c->ct.pw3 = b + 13 * n; c->ct.pw5 = b + 14 * n; c->ct.pw7 = b + 15 * n;
   }
-  // collide_rows: SEGW lanes x NPL rows per column; needs NPL >= NB (the
-  // parabolic low-p rows stay in lane 0) and SEGW*NPL >= np.  Above np = 128
-  // both launches use 32 lanes (two DPP rows) per column: at Nx = 4096 that
-  // keeps two wavefronts per SIMD in flight instead of one.  The two launches
-  // keep separate tables so that their shapes can differ.
-  {
-    const int nb = c->ph.nb_algorithms;
-    auto pick = [&](const int (*cand)[2], int n, int* segw, int* npl) {
-      for (int k = 0; k < n; ++k)
-        if (cand[k][0] * cand[k][1] >= c->np && cand[k][1] >= nb && c->np >= 3) {
-          *segw = cand[k][0];
-          *npl = cand[k][1];
-          return;
-        }
-    };
-    const int ee[3][2] = {{16, 4}, {16, 8}, {32, 8}};
-    const int im[3][2] = {{16, 4}, {16, 8}, {32, 8}};
-    pick(ee, 3, &c->rows_segw, &c->rows_npl);
-    pick(im, 3, &c->rows_segw2, &c->rows_npl2);
-    if (!c->rows_npl || !c->rows_npl2) c->rows_npl = c->rows_npl2 = 0;  // both or neither
-  }
   auto build_rt = [&](int segw, int npl, std::vector<double>& rt) {
     const HostTables& T = c->T;
     const int np = c->np, nb = std::min(c->ph.nb_algorithms, np);
@@ -983,6 +1016,28 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
     if (upload_vec(c, &c->d_rt2, rt)) return bail(OSHUN_ERR_HIP);
   }
   *out = c;
+  return OSHUN_OK;
+}
+
+int oshun_kernel_plan(const oshun_grid* grid, const oshun_phys* phys, int* stage, int* collide) {
+  if (!grid || !phys || !stage || !collide) return OSHUN_ERR_ARG;
+  const oshun_grid& g = *grid;
+  if (g.l0 < 1 || g.m0 < 0 || g.m0 > g.l0 || g.nump < 4 || g.nump > 512 || g.numx < 3 || g.numy < 1 ||
+      !(g.pmax > g.pmin) || !(g.pmin > 0.0) || !(g.dx > 0.0) || !(g.dy > 0.0) ||
+      (g.precision != OSHUN_FP64 && g.precision != OSHUN_FP32_SPLIT))
+    return OSHUN_ERR_ARG;
+  oshun_ctx c;  // host fields only: nothing is allocated on the device
+  c.g = g;
+  c.ph = *phys;
+  c.fp32 = g.precision == OSHUN_FP32_SPLIT;
+  c.esz = c.fp32 ? 4 : 8;
+  oshun::build_tables(c.T, g.l0, g.m0, g.nump, g.numy, g.pmin, g.pmax, g.dx, g.dy, phys->density_np,
+                      phys->nb_algorithms, phys->ref_quirks);
+  c.nc = c.T.nc; c.np = g.nump; c.nx = g.numx; c.ny = g.numy; c.nxy = g.numx * g.numy;
+  c.plane = (long long)g.numx * g.numy * g.nump;
+  plan_kernels(&c, nullptr);
+  *stage = c.use_pair ? 0 : (c.use1d ? 1 : (c.spec >= 0 ? 2 : 3));
+  *collide = c.rows_npl ? 0 : 1;
   return OSHUN_OK;
 }
 
@@ -1193,9 +1248,17 @@ int oshun_profile_read(oshun_ctx* c, double* ms_total, long long* launches, int 
 
 int oshun_moments(oshun_ctx* c, double* out) {
   if (!sane(c) || !out) return OSHUN_ERR_ARG;
-  int rc = c->fp32 ? launch_moments<float>(c) : launch_moments<double>(c);
+  const int nchunk = moment_chunks(c);
+  if (!c->d_mom) {  // output and partial-sum buffers on the first call
+    if (hipMalloc((void**)&c->d_mom, (size_t)oshun::kMomOut * c->nxy * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&c->d_mpart, (size_t)nchunk * oshun::kMomSums * c->nxy * sizeof(double)) != hipSuccess)
+      return fail(c, OSHUN_ERR_ALLOC, "moments: device buffers");
+  }
+  int rc = c->fp32 ? launch_moments<float>(c, c->d_mpart, nchunk, c->d_mom)
+                   : launch_moments<double>(c, c->d_mpart, nchunk, c->d_mom);
   if (rc) return rc;
-  HIPCHK(c, hipMemcpyAsync(out, c->d_mom, (size_t)7 * c->nxy * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_mom, (size_t)oshun::kMomOut * c->nxy * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return OSHUN_OK;
 }

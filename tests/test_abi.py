@@ -45,3 +45,33 @@ def test_null_arguments_are_rejected_without_touching_the_gpu():
     lib.oshun_create.argtypes = [ctypes.POINTER(oshun_amd.Grid), ctypes.POINTER(oshun_amd.Phys),
                                  ctypes.POINTER(ctypes.c_void_p)]
     assert lib.oshun_create(ctypes.byref(g), ctypes.byref(ph), ctypes.byref(h)) == -1
+
+
+def _plan(l0, m0, np_, nx, ny, precision=0):
+    import oshun_amd
+    lib = ctypes.CDLL(oshun_amd.library_path())
+    lib.oshun_kernel_plan.argtypes = [ctypes.POINTER(oshun_amd.Grid), ctypes.POINTER(oshun_amd.Phys),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    pmax = 0.3
+    g = oshun_amd.Grid(l0, m0, np_, nx, ny, pmax / (2 * np_), pmax, 1.0, 1.0, precision)
+    ph = oshun_amd.Phys(1e21, 1.0, 1.0, 4, 3, 1)
+    st, co = ctypes.c_int(-9), ctypes.c_int(-9)
+    assert lib.oshun_kernel_plan(ctypes.byref(g), ctypes.byref(ph), ctypes.byref(st), ctypes.byref(co)) == 0
+    return st.value, co.value
+
+
+def test_kernel_plan_keeps_32bit_offsets_in_range():
+    """The stage-pair kernels address the whole state with 32-bit byte offsets, collide_rows one plane
+    (their out-of-range sentinel is 0x7ffffff0): larger 1D states are planned on the 64-bit kernels
+    (decided on the host, before any allocation or launch)."""
+    assert _plan(8, 0, 256, 4096, 1) == (0, 0)               # the target shape: stage pairs + lane layout
+    # 9 planes x 256 p x nx x 8 B: just below and at 2 GiB
+    nx_ok = (2 ** 31 - 1) // (9 * 256 * 8)
+    assert _plan(8, 0, 256, nx_ok, 1) == (0, 0)
+    assert _plan(8, 0, 256, nx_ok + 1, 1) == (1, 0)           # 64-bit single-stage kernel
+    assert _plan(8, 0, 256, nx_ok + 1, 1, precision=1) == (0, 0)  # float state: half the bytes
+    # one plane above the collide_rows sentinel: one wavefront per column (64-bit)
+    nx_big = 0x7ffffff0 // (256 * 8) + 1
+    assert _plan(8, 0, 256, nx_big, 1) == (1, 1)
+    assert _plan(4, 4, 128, 256, 256) == (2, 0)               # C3: specialised general kernel
+    assert _plan(5, 4, 64, 16, 16)[0] == 3                    # no specialisation: interpreter

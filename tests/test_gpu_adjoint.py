@@ -266,3 +266,32 @@ def test_adjoint_matches_the_tangent_linear_oracle():
     rhs = float((lam * dfo).sum() + (lamf * dgo).sum())
     print(f"<adjoint, d> {lhs:.14e}  <lam, TLM d> {rhs:.14e}  rel {abs(lhs - rhs) / abs(rhs):.2e}")
     assert abs(lhs - rhs) <= 1e-10 * abs(rhs)
+
+
+def test_run_steps_rejects_a_replaced_recording_and_bad_tensors():
+    """A second run_steps on one context replaces the first one's checkpoints: the first segment's
+    backward then raises instead of differentiating the wrong trajectory; wrong tensors raise early."""
+    import oshun_amd
+    from oshun_amd.autograd import run_steps
+    torch = _torch()
+    print("case adjoint generation check: l0=2 np=32 nx=16", flush=True)
+    l0, np_, nx = 2, 32, 16
+    ctx = oshun_amd.Context(l0, 0, np_, nx, 1, pmax=0.3, dx=0.4, density_np=1e22, small_dt=0.1, nb_algorithms=4)
+    f, fld = _maxwellian_state(l0, 0, np_, nx, 1, 0.3, seed=5)
+    F, G = _to_dev(ctx, f, fld)
+    F.requires_grad_(True)
+    f1, g1 = run_steps(ctx, F, G, 0.1, 2, 1)
+    f2, g2 = run_steps(ctx, f1, g1, 0.1, 2, 1)   # replaces the first recording
+    with pytest.raises(oshun_amd.OshunError, match="replaced"):
+        f2.sum().backward()
+    # a single segment differentiates normally
+    F.grad = None
+    f1, _ = run_steps(ctx, F, G, 0.1, 2, 1)
+    f1.sum().backward()
+    assert torch.isfinite(F.grad).all()
+    with pytest.raises(ValueError, match="elements"):
+        run_steps(ctx, F.detach()[:, :, :-1].contiguous(), G, 0.1, 1, 1)
+    with pytest.raises(ValueError, match="float64"):
+        run_steps(ctx, F.detach(), G.float(), 0.1, 1, 1)
+    with pytest.raises(ValueError, match="GPU"):
+        run_steps(ctx, F.detach(), G.cpu(), 0.1, 1, 1)

@@ -189,18 +189,43 @@ def test_fp32_split_close_to_fp64():
     assert e_ex < 1e-4
 
 
-def test_moments_vs_oracle():
+def _drift_state(l0, m0, np_, nx, ny, pmax, precision_bg=False):
+    """Drifting Maxwellians with T(x, y) and drift u(x, y) in all three directions, projected onto the
+    harmonics: nonzero f11, f21, f22, so that every moment term (VxVy Vy in Qx among them) is exercised."""
+    from tests.sph import maxwellian3, project
+    nh = num_harmonics(l0, m0)
+    pmin = pmax / (2.0 * np_)
+    p = np.arange(np_) * ((pmax - pmin) / (np_ - 1)) + pmin
+    f = np.zeros((nh, ny, nx, np_), np.complex128)
+    for iy in range(ny):
+        for ix in range(nx):
+            ph = 2 * np.pi * (ix / nx + 0.13)
+            tt = 1.0 + 0.2 * np.sin(ph) + 0.1 * np.cos(2 * np.pi * (iy + 0.3) / ny)
+            pt = np.sqrt(tt / 511.0)
+            u = (0.4 * pt * np.cos(ph), 0.5 * pt * np.sin(ph + iy), -0.3 * pt)
+            f[:, iy, ix, :] = project(maxwellian3(1.0 + 0.1 * np.cos(ph), pt, u), p, l0, m0, nmu=24, nphi=24)
+    return f, np.zeros((6, ny, nx), np.complex128)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_moments_vs_oracle(prec):
+    """All 18 moments (def-Parallel.cpp:1677-2253) of the GPU against the oracle at <= 1e-12 normwise
+    (FP32 split: against the oracle run on the GPU's own float-rounded state)."""
     import oshun_amd
-    l0, m0, np_, nx, ny = 3, 1, 128, 16, 3
-    f, fld = _maxwellian_state(l0, m0, np_, nx, ny, 0.3, seed=9)
-    o = Oracle(Config(l0=l0, m0=m0, np=np_, nx=nx, ny=ny, pmax=0.3))
-    o.set_state(f, fld)
-    mo = o.moments()
-    ctx = oshun_amd.Context(l0, m0, np_, nx, ny, pmax=0.3)
+    l0, m0, np_, nx, ny = 3, 2, 96, 12, 5
+    f, fld = _drift_state(l0, m0, np_, nx, ny, 0.3)
+    precision = oshun_amd.FP64 if prec == "fp64" else oshun_amd.FP32_SPLIT
+    ctx = oshun_amd.Context(l0, m0, np_, nx, ny, pmax=0.3, dx=0.7, dy=1.1, density_np=3e21, zeta=2.0,
+                            precision=precision)
     ctx.upload(f, fld)
     mg = ctx.moments()
-    for k in range(7):
-        assert rel(mg[k], mo[k]) < TOL, k
+    fs, _ = ctx.download()  # the state the GPU holds (float-rounded in the split)
+    o = Oracle(Config(l0=l0, m0=m0, np=np_, nx=nx, ny=ny, pmax=0.3, dx=0.7, dy=1.1, density_np=3e21, zeta=2.0))
+    o.set_state(fs, fld)
+    mo = o.moments()
+    assert np.abs(mo[7]).max() > 0 and np.abs(mo[9]).max() > 0  # VxVy, VyVz seen
+    for k in range(len(oshun_amd.MOMENT_NAMES)):
+        assert rel(mg[k], mo[k]) < TOL, (k, oshun_amd.MOMENT_NAMES[k], rel(mg[k], mo[k]))
 
 
 def test_conservation_at_target_size():
