@@ -82,6 +82,13 @@ def _load():
     path = library_path()
     if not os.path.exists(path):
         raise OshunError(f"{path} is missing: build it with `python -m oshun_amd._build` (hipcc, gfx950)")
+    # torch ships a HIP runtime of its own; imported after this library's runtime
+    # (the system one) has initialised the GPU, it finds no GPU at all.  Imported
+    # first, both runtimes work in one process, in either order of first use.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     P = ctypes.c_void_p
     D = ctypes.c_double
