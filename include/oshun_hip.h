@@ -129,6 +129,19 @@ int oshun_step_implicit_1d(oshun_ctx* ctx, double h, int nsteps, int if_implicit
  * allocates the context's moment buffers. */
 #define OSHUN_NMOMENTS 18
 int oshun_moments(oshun_ctx* ctx, double* out);
+/* The same moments into a DEVICE buffer out_dev (double[OSHUN_NMOMENTS][ny][nx]),
+ * enqueued on the context's stream (oshun_synchronize / oshun_get_stream to
+ * wait), without allocating: no hipMalloc, hipFree or other allocation in
+ * this call, ever, also on its first call.  No counterpart in the reference;
+ * it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  The partial p sums go to one static device array per GPU
+ * (8 MiB, kept until the process ends), so nx*ny is limited to 104857
+ * (OSHUN_ERR_ARG beyond).  Calls that overlap on one GPU (contexts on
+ * different streams of the same device) share that array and race; calls on
+ * one context are ordered by its stream.  Both entry points split the p sums
+ * into the same chunks (as many as the array holds, at most 32), so their
+ * results are bitwise equal. */
+int oshun_moments_noalloc(oshun_ctx* ctx, double* out_dev);
 
 /* Stream and device views, for callers that keep data on the GPU. */
 int oshun_synchronize(oshun_ctx* ctx);

@@ -122,6 +122,7 @@ def _load():
     lib.oshun_stage_kernel.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.oshun_pair_rows.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.oshun_moments.argtypes = [P, dp]
+    lib.oshun_moments_noalloc.argtypes = [P, P]
     lib.oshun_batch_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Phys), dp, I, ctypes.POINTER(P)]
     lib.oshun_batch_destroy.argtypes = [P]
     lib.oshun_batch_last_error.argtypes = [P]
@@ -293,6 +294,11 @@ class Context:
         out = np.zeros((len(MOMENT_NAMES), self.ny, self.nx))
         self._chk(self._lib.oshun_moments(self._h, _dptr(out)))
         return out
+
+    def moments_noalloc(self, out_ptr):
+        """Moments into the device buffer out_ptr (len(MOMENT_NAMES) x ny x nx doubles) on the
+        context's stream, allocating nothing (static scratch; nx*ny <= 104857)."""
+        self._chk(self._lib.oshun_moments_noalloc(self._h, out_ptr))
 
     # ---- device-resident state and the adjoint (device pointers, e.g. torch data_ptr())
     def copy_device_state(self, f_ptr, fld_ptr, into_ctx):

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -92,6 +93,7 @@ struct oshun_ctx {
   int rows_npl2 = 0, rows_segw2 = 16;
   oshun::CollTables ct;
   hipStream_t stream = nullptr;
+  int device = 0;  // the HIP device current at oshun_create
   double t_rk = 0.0, t_ee = 0.0, t_eb = 0.0;
   std::string err;
   bool prof = false;
@@ -810,12 +812,48 @@ static void plan_kernels(oshun_ctx* c, std::vector<double>* spec_coef) {
   if (!c->rows_npl || !c->rows_npl2 || !plane_fits) c->rows_npl = c->rows_npl2 = 0;  // both or neither
 }
 
+// ---- scratch of the allocation-free moments (oshun_moments_noalloc) ---------
+// The partial p sums live in this file-scope device array, one copy per GPU,
+// held until the process ends.  Its name is a host-side stand-in: host code
+// uses the device address that hipGetSymbolAddress returns for the device
+// current at the time (kept per device below), never the name itself.
+constexpr size_t kNoAllocScratchDoubles = (size_t)1 << 20;  // 8 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+// p chunks of the moment sums: kMomChunkMin rows each at least, and no more
+// partial sums than the static scratch holds, so that oshun_moments and
+// oshun_moments_noalloc always sum in the same order (bitwise equal results)
+static int moment_chunks(const oshun_ctx* c) {
+  const size_t per_chunk = (size_t)oshun::kMomSums * c->nxy;
+  const int cap = static_cast<int>(std::min<size_t>(kNoAllocScratchDoubles / per_chunk, oshun::kMomChunkMax));
+  return std::max(1, std::min(cap, c->np / oshun::kMomChunkMin));
+}
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on device dev (resolved once per device, with dev current)
+static int scratch_address(oshun_ctx* c, double** out) {
+  if (c->device < 0 || c->device >= kMaxDevices) return fail(c, OSHUN_ERR_ARG, "moments_noalloc: device id");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[c->device]) {
+    int prev = 0;
+    HIPCHK(c, hipGetDevice(&prev));
+    if (prev != c->device) HIPCHK(c, hipSetDevice(c->device));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (prev != c->device) HIPCHK(c, hipSetDevice(prev));
+    if (e != hipSuccess)
+      return fail(c, OSHUN_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[c->device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[c->device];
+  return OSHUN_OK;
+}
+
 static_assert(oshun::kMomOut == OSHUN_NMOMENTS, "moment planes of the kernels and of the C ABI");
 
-// number of p chunks of the moment sums (kMomChunkMin rows each at least)
-static int moment_chunks(const oshun_ctx* c) {
-  return std::max(1, std::min(oshun::kMomChunkMax, c->np / oshun::kMomChunkMin));
-}
 
 // the three moment launches on c's stream; part holds nchunk x kMomSums planes
 // of nxy doubles, out kMomOut planes
@@ -875,6 +913,7 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
   const int prows = c->use1d ? oshun::kRows1D : oshun::kTileP;
   c->npb = (g.nump + prows - 1) / prows;
   auto bail = [&](int rc) { free_all(c); std::string m = c->err; delete c; (void)m; return rc; };
+  if (hipGetDevice(&c->device) != hipSuccess) return bail(OSHUN_ERR_HIP);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(OSHUN_ERR_HIP);
   const size_t fbytes = (size_t)c->nc * c->plane * c->esz;
   const size_t fldbytes = (size_t)6 * c->nxy * sizeof(double);
@@ -1261,6 +1300,23 @@ int oshun_moments(oshun_ctx* c, double* out) {
                            c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return OSHUN_OK;
+}
+
+int oshun_moments_noalloc(oshun_ctx* c, double* out_dev) {
+  if (!sane(c) || !out_dev) return OSHUN_ERR_ARG;
+  const size_t per_chunk = (size_t)oshun::kMomSums * c->nxy;  // doubles of one chunk's partial sums
+  if (per_chunk > kNoAllocScratchDoubles)
+    return fail(c, OSHUN_ERR_ARG, "oshun_moments_noalloc: nx*ny = " + std::to_string(c->nxy) + " exceeds " +
+                                      std::to_string(kNoAllocScratchDoubles / oshun::kMomSums) +
+                                      " (the static scratch holds " + std::to_string(kNoAllocScratchDoubles) +
+                                      " doubles of partial sums)");
+  double* part = nullptr;
+  int rc = scratch_address(c, &part);
+  if (rc) return rc;
+  // the chunking of oshun_moments (it fits the scratch by construction); moments_partial
+  // writes every partial sum it forms, so the scratch needs no clearing
+  const int nchunk = moment_chunks(c);
+  return c->fp32 ? launch_moments<float>(c, part, nchunk, out_dev) : launch_moments<double>(c, part, nchunk, out_dev);
 }
 
 int oshun_pair_rows(oshun_ctx* c, int set, int* rows) {
