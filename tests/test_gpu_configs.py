@@ -5,14 +5,15 @@ C1  plain heat-flow relaxation: Nx = 64 on L = 100 lambda_ei, Np = 64 to
     pmax = 0.25, l0 = 2, m0 = 0, T(x) = 1 keV (1 + 0.1 cos(2 pi x/L)), n = 1,
     collisions on (Z = 1), 100 RK4 + FP steps of 0.1/omega_p: GPU vs the
     oracle at full size (FP64 <= 1e-12) and the FP32-split error per field.
-C5  shape of the adjoint configuration: Nx = 512, Np = 192, l0 = 3, m0 = 0,
-    T0 = 1 keV with a seeded 8-mode Fourier perturbation (<= 5%): 5 steps vs
+C5  the adjoint configuration's forward run: Nx = 512, Np = 192, l0 = 3, m0 = 0,
+    T0 = 1 keV with a seeded 8-mode Fourier perturbation (<= 5%): its 200 steps vs
     the oracle at full size.
 C3  2D magnetised hot spot at full size: 256 x 256, Np = 128 to pmax = 0.25,
     l0 = m0 = 4 (complex harmonics), T = 0.5 + 1.5 exp(-r^2/(2 (20 lambda_ei)^2))
     keV, n = 1, uniform Bz with omega_c/nu_ei = 0.5, collisions on: number and
-    energy conservation over 2 steps and the FP32-split error per field
-    (the oracle would need hours at this size).
+    energy over the configuration's 500 steps and the FP32-split error per field
+    after them; one FP64 step at full size against the oracle (500 oracle steps
+    at this size would take hours).
 
 lambda_ei and nu_ei follow the NRL formulary (nu_ei = 2.91e-6 Z n lnL T^-1.5
 s^-1, T in eV, n in cm^-3, lnL = 10) at the reference density 1e21 cm^-3;
@@ -21,7 +22,7 @@ lengths are in c/omega_p, times in 1/omega_p.
 import numpy as np
 import pytest
 
-from oracle.oracle import Config, Oracle, num_harmonics
+from oracle.oracle import Config, Oracle, hind, num_harmonics
 
 pytestmark = pytest.mark.gpu
 
@@ -87,10 +88,10 @@ def test_c1_heat_flow_full_size_vs_oracle():
     assert e["f00"] < 1e-6 and e["f10"] < 1e-4 and e["Ex"] < 1e-4
 
 
-def test_c5_shape_vs_oracle():
+def test_c5_forward_200_steps_vs_oracle():
     import oshun_amd
-    print("case C5 shape, Nx=512 Np=192 l0=3, 5 steps", flush=True)
-    nx, np_, l0, pmax, h, steps = 512, 192, 3, 0.25, 0.1, 5
+    print("case C5 forward, Nx=512 Np=192 l0=3, 200 steps", flush=True)
+    nx, np_, l0, pmax, h, steps = 512, 192, 3, 0.25, 0.1, 200
     L = 100.0 * lambda_ei(1.0)
     dx = L / nx
     x = np.arange(nx) * dx
@@ -117,9 +118,9 @@ def test_c5_shape_vs_oracle():
 
 def test_c3_hot_spot_full_size():
     import oshun_amd
-    print("case C3 hot spot, 256x256 Np=128 l0=m0=4, 2 steps", flush=True)
+    print("case C3 hot spot, 256x256 Np=128 l0=m0=4, 500 steps", flush=True)
     nx = ny = 256
-    np_, l0, m0, pmax, h, steps = 128, 4, 4, 0.25, 0.5, 2
+    np_, l0, m0, pmax, h, steps = 128, 4, 4, 0.25, 0.5, 500
     lam = lambda_ei(0.5)
     dx = dy = 20.0 * lam / 16.0
     x = (np.arange(nx) - nx / 2) * dx
@@ -147,9 +148,52 @@ def test_c3_hot_spot_full_size():
     E0 = (m_0[0] * (m_0[3] + m_0[4] + m_0[5])).sum()
     E1 = (m_1[0] * (m_1[3] + m_1[4] + m_1[5])).sum()
     print(f"C3 dN/N {(N1 - N0) / N0:.2e} dE/E {(E1 - E0) / E0:.2e}")
-    assert abs(N1 - N0) < 1e-9 * N0
-    assert abs(E1 - E0) < 1e-6 * E0
+    # The scheme itself is not exactly conservative here (pmax = 4 p_t at the spot's centre, the
+    # pmax closure of the E-field p derivative and CLEAN_LOW_PC): the oracle on a 16 x 16 copy of
+    # this spot drifts by dN/N = 1.5e-7 and dE/E = 5.0e-6 over 500 steps; the GPU at full size by
+    # 1.6e-7 and 5.3e-6.
+    assert abs(N1 - N0) < 1e-6 * N0
+    assert abs(E1 - E0) < 5e-5 * E0
     (f32, fld32), _, _ = res[oshun_amd.FP32_SPLIT]
-    e = {"f00": l2rel(f32[0], fg[0]), "f11": l2rel(f32[2], fg[2]), "Bz": l2rel(fld32[5].real, fldg[5].real)}
-    print("C3 FP32-split relative L2 error vs FP64 after 2 steps:", {k: f"{v:.2e}" for k, v in e.items()})
-    assert e["f00"] < 1e-6 and e["Bz"] < 1e-6
+    e = {"f00": l2rel(f32[0], fg[0]), "f10": l2rel(f32[1], fg[1]), "f11": l2rel(f32[2], fg[2]),
+         "Ex": l2rel(fld32[0].real, fldg[0].real), "Ey": l2rel(fld32[1].real, fldg[1].real),
+         "Bz": l2rel(fld32[5].real, fldg[5].real)}
+    print(f"C3 FP32-split relative L2 error vs FP64 after {steps} steps:", {k: f"{v:.2e}" for k, v in e.items()},
+          flush=True)
+    assert e["f00"] < 1e-5 and e["Ex"] < 1e-4 and e["Ey"] < 1e-4 and e["Bz"] < 1e-5
+
+
+def test_c3_full_size_one_step_vs_oracle():
+    """One FP64 step of the full C3 shape (l0 = m0 = 4, 256 x 256, Np = 128, Bz, collisions) against the
+    oracle at <= 1e-12 (the oracle takes about half a minute for it)."""
+    import oshun_amd
+    print("case C3 hot spot vs oracle, 256x256 Np=128 l0=m0=4, 1 step", flush=True)
+    nx = ny = 256
+    np_, l0, m0, pmax, h = 128, 4, 4, 0.25, 0.5
+    lam = lambda_ei(0.5)
+    dx = dy = 20.0 * lam / 16.0
+    x = (np.arange(nx) - nx / 2) * dx
+    y = (np.arange(ny) - ny / 2) * dy
+    T = 0.5 + 1.5 * np.exp(-(x[None, :] ** 2 + y[:, None] ** 2) / (2 * (20.0 * lam) ** 2))
+    f = np.zeros((num_harmonics(l0, m0), ny, nx, np_), np.complex128)
+    f[0] = maxwellian(np_, pmax, T)
+    rng = np.random.default_rng(33)
+    for l in range(1, l0 + 1):  # every harmonic nonzero (m = 0 ones real), so that every coupling is exercised
+        for m in range(min(l, m0) + 1):
+            r = rng.uniform(-1, 1, f[0].shape) + (1j * rng.uniform(-1, 1, f[0].shape) if m else 0.0)
+            f[hind(l, m, m0)] = 1e-3 * r * f[0].real
+    fld = np.zeros((6, ny, nx), np.complex128)
+    fld[5] = 0.5 * nu_ei_over_wp(0.5)
+    fld[0] = 1e-4 * np.sin(2 * np.pi * np.arange(nx) / nx)[None, :]
+    kw = dict(pmax=pmax, dx=dx, dy=dy, density_np=DENSITY, small_dt=h)
+    o = Oracle(Config(l0=l0, m0=m0, np=np_, nx=nx, ny=ny, nb=4, **kw))
+    o.set_state(f, fld)
+    o.step(h, 1)
+    fo, fldo = o.get_state()
+    ctx = oshun_amd.Context(l0, m0, np_, nx, ny, nb_algorithms=4, **kw)
+    ctx.upload(f, fld)
+    ctx.step(h, 1)
+    fg, fldg = ctx.download()
+    print(f"C3 vs oracle: f {rel(fg, fo):.1e} fields {rel(fldg.real, fldo.real):.1e}", flush=True)
+    assert rel(fg, fo) < 1e-12
+    assert rel(fldg.real, fldo.real) < 1e-12
