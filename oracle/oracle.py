@@ -21,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboshun_oracle.so")
 REF_BIN = os.path.join(HERE, "_ref", "oshun_ref")
+SHIM_BIN = os.path.join(HERE, "_ref", "oshun_ref_shim")
 
 # planes of Oracle.moments(), in the order of the library's oshun_moments
 MOMENT_NAMES = ("n", "Vx", "Vy", "VxVx", "VyVy", "VzVz", "Qx", "VxVy", "VxVz", "VyVz", "Vsq", "T_eV", "P_Mbar",
@@ -269,11 +270,13 @@ def have_ref():
 
 
 def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
-            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0):
+            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, binary=None):
     """Run the reference driver; returns dict(dx, dy, h, t, p, f, fld).
-    op 5 (implicit-E step) takes es = if_implicitES."""
-    if not have_ref():
-        raise RuntimeError("reference driver not built (make -C oracle ref)")
+    op 5 (implicit-E step) takes es = if_implicitES.  binary: another build of
+    the driver (SHIM_BIN: the same driver with the GPU shim linked in)."""
+    binary = binary or REF_BIN
+    if not os.path.exists(binary):
+        raise RuntimeError(f"driver not built: {binary} (make -C oracle ref / shim)")
     nh = num_harmonics(l0, m0)
     f = np.ascontiguousarray(f, np.complex128).reshape(nh, ny, nx, np_)
     fld = np.ascontiguousarray(fld, np.complex128).reshape(6, ny, nx)
@@ -286,7 +289,9 @@ def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0
                      np.float64).tofile(fh)
             f.tofile(fh)
             fld.tofile(fh)
-        subprocess.run([REF_BIN, fin, fout], check=True, capture_output=True)
+        r = subprocess.run([binary, fin, fout], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{os.path.basename(binary)} failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")
         raw = np.fromfile(fout, np.float64)
     hdr = raw[:4]
     p = raw[4:4 + np_]

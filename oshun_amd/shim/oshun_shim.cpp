@@ -1,0 +1,232 @@
+// oshun_shim.cpp -- OSHUN-1's RK4 driver, its implicit-E RK2 driver and the
+// collision-step bodies, running on an MI355X through the C ABI of oshun_hip.h.
+//
+// Drop-in at link time: the class declarations stay the reference's own
+// (decl-RK.h:48-79 and :169-203, decl-fokkerplanck.h:140-163 and :289-311,
+// decl-implicitE.h:146-171); this file defines the member functions that do
+// the work of one step of main_loop (main.cpp:74-107):
+//   Runge_Kutta_4::Runge_Kutta_4, ::advance_p1             def-RK.cpp:49-62, 110-145
+//   Runge_Kutta_2_Imp::Runge_Kutta_2_Imp, ::advance_p1/p2  def-RK.cpp:400-413, 470-524
+//   Explicit_EC_FP::loop                                   def-FokkerPlanck.cpp:590-625
+//   Euler_Backward::advance_1, ::advance_lm                def-FokkerPlanck.cpp:1270-1304
+//   Electric_Field_Methods::Implicit_E_Field::advance      def-ImplicitE.cpp:472-606
+// Link this object ahead of def-RK.o, def-FokkerPlanck.o and def-ImplicitE.o,
+// either after deleting those bodies from the reference files or with
+// -Wl,--allow-multiple-definition (then the first definition, this one, is
+// the one every caller and every vtable binds to; the reference files stay
+// untouched).  oracle/Makefile's `shim` target does the latter.
+//
+// One device state per process, as main.cpp builds one Stat Y: the first
+// integrator call creates the device context from Inputdata::IN() and
+// uploads the host Stat (after main_loop has read a restart, main.cpp:49-64);
+// the host Stat is refreshed by a download at the end of every output
+// interval -- every numh() steps of the driver, main.cpp:77-131 -- so that
+// CX.Output and Write_Restart read current data.  The guard refresh
+// CX.Neighbor_Communications then only rewrites host guard cells.  Laser and
+// inverse-bremsstrahlung sources (def-PLSource.cpp), which change the host
+// Stat between the integrators' calls, are not supported.  Errors print and
+// exit(1), as the reference does.
+//
+// The implicit-E decks the library runs are the 1D variant (if_implicit1D,
+// m = 0 harmonics, Ey = Ez = B = 0): the device context then holds the m = 0
+// harmonics only, and the whole step (RK2 halves, e-e, field solve, implicit
+// collisions) is oshun_step_implicit_1d inside Runge_Kutta_2_Imp::advance_p1;
+// the later calls of the step only move their clocks.
+
+#include <cstdlib>
+#include <iostream>
+#include <vector>
+#include <valarray>
+#include <complex>
+#include <limits>
+#include <math.h>
+#include <stdio.h>
+#include <float.h>
+
+#include "matrices.h"
+#include "decl-nmethods.h"
+#include "decl-input.h"
+#include "decl-state.h"
+#include "decl-vlasovmax.h"
+#include "decl-actions.h"
+#include "decl-RK.h"
+#include "decl-fokkerplanck.h"
+#include "decl-implicitE.h"
+
+#include "oshun_hip.h"
+
+namespace {
+
+struct Session {
+  oshun_ctx* ctx = nullptr;
+  Stat* Y = nullptr;     // the host Stat the device state mirrors
+  bool implicit = false; // Runge_Kutta_2_Imp drives the step
+  bool m0_only = false;  // the device holds the m = 0 harmonics only
+  bool rk_driven = false;
+  size_t numh = 1;       // steps per output interval of the driver
+  size_t steps = 0;      // steps since the last download
+};
+
+Session& session() {
+  static Session s;
+  return s;
+}
+
+[[noreturn]] void die(const char* where) {
+  Session& s = session();
+  std::cout << "oshun shim: " << where;
+  if (s.ctx) std::cout << ": " << oshun_last_error(s.ctx);
+  std::cout << "\n";
+  std::exit(1);
+}
+
+void check(int rc, const char* where) {
+  if (rc != OSHUN_OK) die(where);
+}
+
+// host pointers of the harmonics the device holds (ind(l,m) order) and of the six fields
+void pointers(Stat& Y, std::vector<double*>& sh, std::vector<double*>& fld) {
+  Session& s = session();
+  const size_t l0 = Inputdata::IN().inp().l0;
+  sh.clear();
+  fld.clear();
+  if (s.m0_only) {
+    for (size_t l = 0; l <= l0; ++l) sh.push_back(reinterpret_cast<double*>(&Y.SH(l, 0)(0, 0, 0)));
+  } else {
+    for (size_t i = 0; i < Y.DF().dim(); ++i) sh.push_back(reinterpret_cast<double*>(&Y.SH(i)(0, 0, 0)));
+  }
+  for (size_t k = 0; k < 6; ++k) fld.push_back(reinterpret_cast<double*>(&Y.FLD(k)(0, 0)));
+}
+
+// the device context of this process, created and loaded from Y on first use
+void attach(Stat& Y, double t) {
+  Session& s = session();
+  if (s.ctx) {
+    if (s.Y != &Y) die("one state per process: the integrators were built on different Stat objects");
+    return;
+  }
+  using Inputdata::IN;
+  const int nbc = IN().list().RKLevel;
+  s.m0_only = s.implicit && IN().list().if_implicit1D;
+  oshun_grid g;
+  g.l0 = static_cast<int>(IN().inp().l0);
+  g.m0 = s.m0_only ? 0 : static_cast<int>(IN().inp().m0);
+  g.nump = static_cast<int>(IN().inp().pr.dim());
+  g.numx = static_cast<int>(IN().inp().x.dim()) - 2 * nbc;
+  g.numy = static_cast<int>(IN().inp().y.dim()) - 2 * nbc;
+  g.pmin = IN().inp().pr(0);
+  g.pmax = IN().inp().pr(IN().inp().pr.dim() - 1);
+  g.dx = IN().inp().x.dx();
+  g.dy = IN().inp().y.dx();
+  g.precision = OSHUN_FP64;
+  oshun_phys p;
+  p.density_np = IN().list().density_np;
+  p.zeta = IN().list().Zeta;
+  p.small_dt = IN().list().small_dt;
+  p.nb_algorithms = IN().list().NB_algorithms;
+  p.collisions = OSHUN_COLL_EE | OSHUN_COLL_FLM;
+  p.ref_quirks = 1;
+  check(oshun_create(&g, &p, &s.ctx), "oshun_create");
+  s.Y = &Y;
+  std::vector<double*> sh, fld;
+  pointers(Y, sh, fld);
+  check(oshun_upload_harmonics(s.ctx, sh.data(), fld.data(), nbc), "oshun_upload_harmonics");
+  check(oshun_set_time(s.ctx, t), "oshun_set_time");
+}
+
+void download() {
+  Session& s = session();
+  std::vector<double*> sh, fld;
+  pointers(*s.Y, sh, fld);
+  check(oshun_download_harmonics(s.ctx, sh.data(), fld.data(), Inputdata::IN().list().RKLevel),
+        "oshun_download_harmonics");
+}
+
+// end of a step (after advance_lm): refresh the host Stat when an output interval ends
+void step_done() {
+  Session& s = session();
+  if (++s.steps >= s.numh) {
+    download();
+    s.steps = 0;
+  }
+}
+
+}  // namespace
+
+// ---- Runge_Kutta_4 (def-RK.cpp:49-145) --------------------------------------
+Runge_Kutta_4::Runge_Kutta_4(Stat& Yin, int tout_start)
+    : Y(Yin), Y0(Yin), Y1(Yin), Yh(Yin), Act(Yh),
+      t(static_cast<double>(tout_start) * Inputdata::IN().cont().dt_out),
+      Tout(t + Inputdata::IN().cont().dt_out) {
+  // the step count and size of the reference (def-RK.cpp:60-61)
+  num_h = size_t(static_cast<int>(Inputdata::IN().cont().dt_out / Inputdata::IN().inp().CLF)) + 1;
+  h = Inputdata::IN().cont().dt_out / static_cast<double>(num_h);
+  Session& s = session();
+  s.rk_driven = true;
+  s.numh = num_h;
+}
+
+Runge_Kutta_4& Runge_Kutta_4::advance_p1() {
+  attach(Y, t);
+  check(oshun_rk4_step(session().ctx, h), "Runge_Kutta_4::advance_p1");
+  t += h;  // the device clock advances by the same h
+  return *this;
+}
+
+// ---- Runge_Kutta_2_Imp (def-RK.cpp:400-524), 1D implicit-E variant ------------
+Runge_Kutta_2_Imp::Runge_Kutta_2_Imp(Stat& Yin, int tout_start)
+    : Y(Yin), Y0(Yin), Yh(Yin), Act(Yh),
+      t(static_cast<double>(tout_start) * Inputdata::IN().cont().dt_out),
+      Tout(t + Inputdata::IN().cont().dt_out) {
+  num_h = size_t(static_cast<int>(Inputdata::IN().cont().dt_out / Inputdata::IN().inp().CLF)) + 1;  // :411-412
+  h = Inputdata::IN().cont().dt_out / static_cast<double>(num_h);
+  Session& s = session();
+  s.implicit = true;
+  s.rk_driven = true;
+  s.numh = num_h;
+}
+
+Runge_Kutta_2_Imp& Runge_Kutta_2_Imp::advance_p1() {
+  if (!Inputdata::IN().list().if_implicit1D)
+    die("Runge_Kutta_2_Imp: the library runs the 1D implicit-E variant (if_implicit1D) only");
+  attach(Y, t);
+  check(oshun_step_implicit_1d(session().ctx, h, 1, Inputdata::IN().list().if_implicitES ? 1 : 0),
+        "Runge_Kutta_2_Imp::advance_p1");
+  t += h;
+  return *this;
+}
+
+Runge_Kutta_2_Imp& Runge_Kutta_2_Imp::advance_p2() { return *this; }  // done in advance_p1
+
+void Electric_Field_Methods::Implicit_E_Field::advance(Explicit_Method*, Euler_Backward*) {
+  // the field solve is part of oshun_step_implicit_1d (advance_p1)
+}
+
+// ---- collisions (def-FokkerPlanck.cpp:590-625, 1270-1304) ---------------------
+void Explicit_EC_FP::loop(const double& tnew) {
+  Session& s = session();
+  if (!s.implicit) {
+    attach(Y, t);
+    check(oshun_collide_ee(s.ctx, tnew), "Explicit_EC_FP::loop");  // numh from tnew - t as at :597-599
+    if (!s.rk_driven) download();  // used without a driver: the caller reads the Stat next
+  }
+  t = tnew;
+}
+
+void Euler_Backward::advance_1(const double& tnew) {
+  Session& s = session();
+  if (!s.implicit) {
+    attach(Y, t);
+    check(oshun_collide_flm(s.ctx, tnew, 1), "Euler_Backward::advance_1");
+  }
+}
+
+void Euler_Backward::advance_lm(const double& tnew) {
+  Session& s = session();
+  if (!s.implicit) {
+    attach(Y, t);
+    check(oshun_collide_flm(s.ctx, tnew, 2), "Euler_Backward::advance_lm");
+  }
+  t = tnew;
+  step_done();
+}
