@@ -67,16 +67,27 @@ def make_ctx(precision, dx):
                              zeta=1.0, small_dt=H, nb_algorithms=NB)
 
 
+def conservation(m0, m1):
+    """relative change of the total electron number and kinetic energy between two moment sets"""
+    n0, n1 = m0[0].sum(), m1[0].sum()
+    e0 = (m0[0] * (m0[3] + m0[4] + m0[5])).sum()
+    e1 = (m1[0] * (m1[3] + m1[4] + m1[5])).sum()
+    return {"dN_over_N": float((n1 - n0) / n0), "dE_over_E": float((e1 - e0) / e0)}
+
+
 def run_path(precision, steps, warmup, f, fld, dx):
     ctx = make_ctx(precision, dx)
     ctx.upload(f, fld)
     ctx.step(H, warmup)
     ctx.synchronize()
+    m_start = ctx.moments()
     t0 = time.perf_counter()
     ctx.step(H, steps)
     ctx.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / steps
+    ctx.cons = conservation(m_start, ctx.moments())
+    ctx.final = ctx.download()  # after warmup + steps, for the FP32-split error against FP64
     # per-kernel durations: HIP events on the library's stream over `steps` more steps
     ctx.profile(True)
     ctx.step(H, steps)
@@ -88,8 +99,42 @@ def run_path(precision, steps, warmup, f, fld, dx):
     return ctx, ms, prof
 
 
+def host_cores():
+    """CPU threads this process may use (the GPU host's share: at most 16)"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _ref_wall(procs, nsteps, inp, td):
+    """wall time of `procs` concurrent reference drivers running `nsteps` steps of the same slab"""
+    import subprocess
+    from oracle import oracle as orc
+    with open(inp, "rb") as fh:
+        raw = bytearray(fh.read())
+    path = os.path.join(td, f"in_{nsteps}.bin")
+    hdr = np.frombuffer(raw[:40], np.int32).copy()
+    hdr[6] = nsteps
+    raw[:40] = hdr.tobytes()
+    with open(path, "wb") as fh:
+        fh.write(raw)
+    t0 = time.perf_counter()
+    ps = [subprocess.Popen([orc.REF_BIN, path, os.path.join(td, f"out_{nsteps}_{k}.bin")],
+                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL) for k in range(procs)]
+    rcs = [p.wait() for p in ps]
+    if any(rcs):
+        raise RuntimeError(f"reference driver failed: {rcs}")
+    return time.perf_counter() - t0
+
+
 def cpu_baseline(budget_s=20.0):
-    """The reference on a bounded sample of the workload (same l0, Np, h, collisions; Nx = 64)."""
+    """The reference on a bounded sample of the workload (same l0, Np, h, collisions) on the host's cores:
+    P concurrent reference drivers (oracle/_ref, the unmodified reference classes), one periodic x slab of
+    Nx = 64 each, as the reference's MPI ranks would hold slabs (its only parallelism); the halo exchange
+    between ranks is not included."""
+    import tempfile
     from oracle import oracle as orc
     nx = 64
     f, fld, dx = target_state(nx=nx)
@@ -100,24 +145,22 @@ def cpu_baseline(budget_s=20.0):
         f2 = np.zeros((nh2, 1, nx, NP), np.complex128)
         for l in range(L0 + 1):
             f2[orc.hind(l, 0, 2)] = f[l]
-        nsteps = 1
-        t0 = time.perf_counter()
-        orc.run_ref(L0, 2, NP, nx, 1, f2, fld, op=1, nsteps=nsteps, nb=NB, pmax=PMAX, xmin=xmin, xmax=xmax,
-                    h=H, small_dt=H, density_np=DENSITY)
-        t1 = time.perf_counter()
-        setup = t1 - t0
-        nsteps = max(1, min(6, int(budget_s / max(setup, 1e-3))))
-        t0 = time.perf_counter()
-        orc.run_ref(L0, 2, NP, nx, 1, f2, fld, op=1, nsteps=nsteps + 1, nb=NB, pmax=PMAX, xmin=xmin,
-                    xmax=xmax, h=H, small_dt=H, density_np=DENSITY)
-        t1 = time.perf_counter()
-        per_step = max((t1 - t0) - setup, 1e-9) / nsteps
-        cells = (L0 + 1) * NP * nx
-        return {"value": cells / per_step, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-                "sample": f"OSHUN-1 classes (oracle/_ref), l0=8 as m0=2 y-uniform stand-in (24 harmonics "
-                          f"stored, 9 useful), Np=256, Nx=64 (+4 guards each side, 9 y rows incl. guards), "
-                          f"{nsteps} RK4+FP steps minus a 1-step run; value counts the 9 useful harmonics",
-                "s_per_step": per_step}
+        P = host_cores()
+        with tempfile.TemporaryDirectory() as td:
+            inp = os.path.join(td, "in.bin")
+            orc.write_ref_input(inp, L0, 2, NP, nx, 1, f2, fld, op=1, nsteps=1, nb=NB, pmax=PMAX, xmin=xmin,
+                                xmax=xmax, h=H, small_dt=H, density_np=DENSITY)
+            setup = _ref_wall(P, 1, inp, td)
+            nsteps = max(1, min(6, int(budget_s / max(setup, 1e-3))))
+            wall = _ref_wall(P, nsteps + 1, inp, td)
+        per_step = max(wall - setup, 1e-9) / nsteps  # wall time of one step of all P slabs together
+        cells = (L0 + 1) * NP * nx * P
+        return {"value": cells / per_step, "unit": "cell-updates/s", "cores": P, "kind": "reference",
+                "sample": f"OSHUN-1 classes (oracle/_ref) as {P} concurrent processes, one per core, each a "
+                          f"periodic slab of Nx=64 (+4 guards each side): l0=8 as m0=2 y-uniform stand-in (24 "
+                          f"harmonics stored, 9 useful), Np=256, {nsteps} RK4+FP steps minus a 1-step run; value "
+                          f"counts the 9 useful harmonics of all {P} slabs",
+                "s_per_step": per_step, "slab_cells": (L0 + 1) * NP * nx}
     # fall back to the oracle's restatement
     o = orc.Oracle(orc.Config(l0=L0, m0=M0, np=NP, nx=nx, pmax=PMAX, dx=dx, density_np=DENSITY, small_dt=H,
                               nb=NB))
@@ -484,6 +527,12 @@ def main():
             r64["traffic"] = sum(v["read_bytes"] + v["write_bytes"] for v in pairs) / len(pairs)
             r64["traffic_source"] = os.path.relpath(tpath, ROOT)
     r32 = roof(prof32, S32, args.steps)
+    (f64, fl64), (f32, fl32) = ctx64.final, ctx32.final
+
+    def l2(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    split_err = {"err_f00": l2(f32[0], f64[0]), "err_f10": l2(f32[1], f64[1]), "err_Ex": l2(fl32[0], fl64[0]),
+                 "after_steps": args.warmup + args.steps, "norm": "relative L2 against the FP64 path"}
     step_frac64 = 16.0 * S64 / (ms64 * 1e-3) / HBM_PEAK
     step_frac32 = 16.0 * S32 / (ms32 * 1e-3) / HBM_PEAK
     cpu = None if args.no_cpu else cpu_baseline()
@@ -507,8 +556,10 @@ def main():
                    "cells": cells, "state_bytes_fp64": S64, "state_bytes_fp32": S32},
         "roofline": r64,
         "step_roofline_frac": step_frac64,
+        "conservation": ctx64.cons,
         "fp32_split": {"value": cells / (ms32 * 1e-3), "ms_per_step": ms32, "dtype": "f32 f / f64 fields+collisions",
-                       "speedup_vs_fp64": ms64 / ms32, "roofline": r32, "step_roofline_frac": step_frac32},
+                       "speedup_vs_fp64": ms64 / ms32, "roofline": r32, "step_roofline_frac": step_frac32,
+                       "conservation": ctx32.cons, **split_err},
         "kernels_fp64": breakdown(prof64, args.steps),
         "kernels_fp32_split": breakdown(prof32, args.steps),
         "cpu_baseline": cpu,
