@@ -127,6 +127,11 @@ def _load():
         lib.oo_tridiag.argtypes = [I, dp, dp, dp, dp, dp]
         lib.oo_axes.argtypes = [P, dp, dp, dp]
         lib.oo_moments.argtypes = [P, dp]
+        lib.oo_set_sources.argtypes = [P, dp, dp, dp, dp, I]
+        lib.oo_laser_source.argtypes = [P, D]
+        lib.oo_laser_source.restype = I
+        lib.oo_inverse_bremsstrahlung.argtypes = [P, D]
+        lib.oo_laser_efield.argtypes = [P, D]
         lib.oo_tlm_1d.argtypes = [P, dp, dp, dp, dp, D, I, dp, dp, dp, dp]
         lib.oo_tlm_1d.restype = I
         _lib = lib
@@ -217,6 +222,28 @@ class Oracle:
         self._lib.oo_flm_coeffs(self._h, _dptr(f00), float(dt), *[_dptr(o) for o in out])
         return out  # sub, diag, sup, scatt
 
+    def set_sources(self, src, x, y, xglob):
+        """Source parameters (SOURCE_KEYS) and the positions of the interior cells (x, y) and of the
+        global x axis (def-PLSource.cpp constructors); the source clocks start at the current time."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        xg = np.ascontiguousarray(xglob, np.float64)
+        sv = source_vector(src)
+        self._lib.oo_set_sources(self._h, _dptr(sv), _dptr(x), _dptr(y), _dptr(xg), int(xg.size))
+
+    def laser_source(self, tnew):
+        """PLaserSource::Laser_Source (def-PLSource.cpp:138-252); False when a heating rate alpha
+        leaves [0, 0.99] (the reference exits there)."""
+        return self._lib.oo_laser_source(self._h, float(tnew)) == 0
+
+    def inverse_bremsstrahlung(self, tnew):
+        """InverseBremsstrahlung::loop (def-PLSource.cpp:780-861)."""
+        self._lib.oo_inverse_bremsstrahlung(self._h, float(tnew))
+
+    def laser_efield(self, tnew):
+        """ExternalEfield::Laser_Efield (def-PLSource.cpp:368-418)."""
+        self._lib.oo_laser_efield(self._h, float(tnew))
+
     def moments(self):
         """The 18 moments of MOMENT_NAMES per (y, x) (def-Parallel.cpp:1677-2253, FP64)."""
         out = np.zeros((len(MOMENT_NAMES), self.cfg.ny, self.cfg.nx))
@@ -269,21 +296,47 @@ def have_ref():
     return os.path.exists(REF_BIN)
 
 
+# the deck's source parameters, in the order of the driver's source block and of oo_set_sources
+SOURCE_KEYS = ("linear_Ivst", "polynomial_Ivst", "inverse_bremsstrahlung", "polarization_direction", "I_0",
+               "lambda_0", "ab_eta", "ab_depth", "cntr_x0", "cntr_r0", "spot_w0", "rise_time", "flat_time",
+               "fall_time", "smaller_dt")
+
+
+def source_vector(src):
+    return np.array([float(src.get(k, 0.0)) for k in SOURCE_KEYS], np.float64)
+
+
+def deck_axis(n, vmin, vmax, guards=4):
+    """Interior cell positions of the reference's x (or y) axis and its global axis, computed as
+    Inputdata::decl_input does (def-Input.cpp:313-325, Axis of matrices.h:860-866): the global axis
+    i*d + min, then the local one from its first cell minus the guards, cumulatively by glob.dx()."""
+    d = (vmax - vmin) / float(n - 1)
+    glob = np.arange(n) * d + vmin
+    gdx = glob[1] - glob[0]
+    loc = np.empty(n + 2 * guards)
+    loc[0] = glob[0] - guards * gdx
+    for i in range(1, loc.size):
+        loc[i] = loc[i - 1] + gdx
+    return loc[guards:guards + n].copy(), glob
+
+
 def write_ref_input(path, l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
-                    ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0):
+                    ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, src=None):
     """The driver's input file (layout in ref_harness.cpp's header comment)."""
     nh = num_harmonics(l0, m0)
     f = np.ascontiguousarray(f, np.complex128).reshape(nh, ny, nx, np_)
     fld = np.ascontiguousarray(fld, np.complex128).reshape(6, ny, nx)
     with open(path, "wb") as fh:
-        np.array([l0, m0, np_, nx, ny, op, nsteps, nb, es, 0], np.int32).tofile(fh)
+        np.array([l0, m0, np_, nx, ny, op, nsteps, nb, es, 1 if src is not None else 0], np.int32).tofile(fh)
         np.array([pmax, xmin, xmax, ymin, ymax, h, small_dt, density_np, zeta], np.float64).tofile(fh)
+        if src is not None:
+            source_vector(src).tofile(fh)
         f.tofile(fh)
         fld.tofile(fh)
 
 
 def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
-            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, binary=None):
+            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, binary=None, src=None):
     """Run the reference driver; returns dict(dx, dy, h, t, p, f, fld).
     op 5 (implicit-E step) takes es = if_implicitES.  binary: another build of
     the driver (SHIM_BIN: the same driver with the GPU shim linked in)."""
@@ -295,7 +348,7 @@ def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
         write_ref_input(fin, l0, m0, np_, nx, ny, f, fld, op, nsteps, nb, pmax, xmin, xmax, ymin, ymax, h,
-                        small_dt, density_np, zeta, es)
+                        small_dt, density_np, zeta, es, src)
         r = subprocess.run([binary, fin, fout], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"{os.path.basename(binary)} failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")

@@ -706,6 +706,297 @@ struct Oracle {
     t_ee = tnew;
   }
 
+  // --------------------------------------------------------- sources ----
+  // def-PLSource.cpp: PLaserSource::Laser_Source (:138-252), the
+  // InverseBremsstrahlung operator (:476-777, :780-861) and
+  // ExternalEfield::Laser_Efield (:368-418).  main.cpp:90-92 calls
+  // Laser_Source and InverseBremsstrahlung::loop after the e-e collisions;
+  // Laser_Efield is commented out there (:91) and is a separate call here.
+  // Profiles and rates are built as the constructors build them
+  // (:73-133, :308-353, :861-946) from the cell positions the caller passes
+  // (the reference's x and y axes at the interior cells and its global x axis).
+  struct Src {
+    int linear = 0, poly = 0, ib = 0, pol = 0;
+    double I_0 = 0, lambda_0 = 1, ab_eta = 0, ab_depth = 1, cntr_x0 = 0, cntr_r0 = 0, spot_w0 = 1;
+    double rise = 0, flat = 0, fall = 0, smaller_dt = 1;
+    std::vector<double> ILx, ILy, Ty, ILxe, ILye, xpos;  // PLS (normalised x), IB/EFLD profiles, x of cells
+    std::vector<double> gam;                             // gamma(p)
+    double nmin = 1e-5, IL = 0, IL_sqrt = 0, IL_min = 0, T0 = 0, acoeff = 0;
+    double t_pls = 0, t_ib = 0, t_efld = 0;
+    double vos = 0, w0overwp = 0;
+    // IB_Operator tables (:476-545)
+    std::vector<double> vr, U4, U4m1, U2, U2m1, Inv_Uav6, Qn, Pn, U2p, U2pm1;
+    double Inv_Uav6_nm1 = 0, Pnm1 = 0, vw_coeff_cube = 0, Qn_coeff = 0;
+    bool alpha_error = false;
+  } src;
+
+  void set_sources(const double* sv, const double* x, const double* y, const double* xglob, int nxg, double t0) {
+    Src& S = src;
+    S.linear = static_cast<int>(sv[0]); S.poly = static_cast<int>(sv[1]); S.ib = static_cast<int>(sv[2]);
+    S.pol = static_cast<int>(sv[3]);
+    S.I_0 = sv[4]; S.lambda_0 = sv[5]; S.ab_eta = sv[6]; S.ab_depth = sv[7]; S.cntr_x0 = sv[8]; S.cntr_r0 = sv[9];
+    S.spot_w0 = sv[10]; S.rise = sv[11]; S.flat = sv[12]; S.fall = sv[13]; S.smaller_dt = sv[14];
+    const double xgdx = xglob[1] - xglob[0];
+    // PLaserSource (:73-133)
+    S.IL = S.I_0 * S.ab_eta;
+    S.IL_sqrt = std::sqrt(S.IL);
+    S.IL_min = 0.00001 * S.IL;
+    S.T0 = 1000.0 * std::sqrt(1.0e-14 * S.lambda_0 * S.lambda_0);
+    S.acoeff = 2.08050325 * 1.0e+12 / (xgdx * c.density_np * S.lambda_0);
+    double sumI = 0.0;
+    for (int i = 0; i < nxg; ++i) sumI += std::exp((-1.0) * std::pow((xglob[i] - S.cntr_x0) / S.ab_depth, 2));
+    S.ILx.resize(nx); S.ILxe.resize(nx); S.xpos.assign(x, x + nx);
+    for (int i = 0; i < nx; ++i) {
+      S.ILx[i] = std::exp((-1.0) * std::pow((x[i] - S.cntr_x0) / S.ab_depth, 2));
+      S.ILxe[i] = S.ILx[i];  // IB and the external field keep the profile unnormalised (:336-338, :891-893)
+      S.ILx[i] *= 1.0 / sumI;
+    }
+    S.ILy.resize(ny); S.ILye.resize(ny); S.Ty.resize(ny);
+    for (int j = 0; j < ny; ++j) {
+      const double tmp2 = (y[j] - S.cntr_r0) / S.spot_w0;
+      S.ILy[j] = std::exp(-2.0 * tmp2 * tmp2);
+      S.ILye[j] = S.ILy[j];
+      S.Ty[j] = S.T0 * std::sqrt(S.ILy[j] * S.IL);
+    }
+    S.gam.resize(np);
+    for (int i = 0; i < np; ++i) S.gam[i] = std::sqrt(1.0 + pd[i] * pd[i]);
+    // ExternalEfield / InverseBremsstrahlung constants (:308-318, :861-875)
+    const double omega_0 = 3.0e+10 * 2.0 * M_PI / (1.0e-4 * S.lambda_0);
+    const double omega_p = 5.64 * 1.0e+4 * std::sqrt(c.density_np);
+    S.w0overwp = omega_0 / omega_p;
+    S.vos = S.lambda_0 * std::sqrt(7.3e-19 * S.I_0);
+    if (std::abs(S.pol) > 3) S.pol = 0;
+    if (std::abs(S.ib) > 1) S.ib = 0;
+    // IB_Operator (:476-545)
+    const double re = 2.8179402894e-13;
+    const double kp = std::sqrt(4.0 * M_PI * (c.density_np) * re);
+    S.vw_coeff_cube = omega_p / omega_0 * kp * re;
+    S.Qn_coeff = kp * re / 6.0;
+    const int N = np;
+    S.vr.assign(N, 0.0); S.U4.assign(N, 0.0); S.U4m1.assign(N, 0.0); S.U2.assign(N, 0.0); S.U2m1.assign(N, 0.0);
+    S.Inv_Uav6.assign(N, 0.0); S.Qn.assign(N, 0.0); S.Pn.assign(N, 0.0); S.U2p.assign(N, 0.0);
+    S.U2pm1.assign(N, 0.0);
+    for (int i = 0; i < N; ++i) { S.vr[i] = pd[i]; S.vr[i] = S.vr[i] / (std::sqrt(1.0 + S.vr[i] * S.vr[i])); }
+    const std::vector<double>& v = S.vr;
+    for (int i = 1; i < N; ++i) {
+      S.U4[i] = 0.5 * std::pow(v[i], 4) * (v[i] - v[i - 1]);
+      S.U4m1[i] = 0.5 * std::pow(v[i - 1], 4) * (v[i] - v[i - 1]);
+      S.U2[i] = 0.5 * v[i] * v[i] * (v[i] - v[i - 1]);
+      S.U2m1[i] = 0.5 * v[i - 1] * v[i - 1] * (v[i] - v[i - 1]);
+    }
+    S.Inv_Uav6_nm1 = std::pow(2.0 / v[0], 6);
+    for (int i = 0; i < N - 1; ++i) S.Inv_Uav6[i] = std::pow(2.0 / (v[i + 1] + v[i]), 6);  // [N-1] is never used
+    S.Qn[0] = 1.0 / ((v[0] * v[0] * v[1]) / 2.0);
+    for (int i = 1; i < N - 1; ++i) S.Qn[i] = 1.0 / (v[i] * v[i] * (v[i + 1] - v[i - 1]) / 2.0);
+    S.Pnm1 = 2.0 / (v[0] * v[0]);
+    for (int i = 0; i < N - 1; ++i) S.Pn[i] = 1.0 / ((v[i + 1] - v[i]) / 2.0 * (v[i + 1] + v[i]));
+    // density weights of the IB renormalisation, on p (:902-905)
+    for (int i = 1; i < N; ++i) {
+      S.U2p[i] = 0.5 * pd[i] * pd[i] * (pd[i] - pd[i - 1]);
+      S.U2pm1[i] = 0.5 * pd[i - 1] * pd[i - 1] * (pd[i] - pd[i - 1]);
+    }
+    S.t_pls = S.t_ib = S.t_efld = t0;
+  }
+
+  // the intensity envelope times (t2 - t1) of PLaserSource / ExternalEfield (:259-289, :428-458)
+  double t_profile(double t2, double t1) const {
+    const Src& S = src;
+    double t;
+    if (t2 < S.rise) {
+      t = t2 / S.rise;
+      if (S.poly) return (10 * std::pow(t, 3) - 15 * std::pow(t, 4) + 6 * std::pow(t, 5)) * (t2 - t1);
+      if (S.linear) return t * (t2 - t1);
+      return 0;
+    }
+    if (t2 < S.rise + S.flat) return t2 - t1;
+    if (t2 < S.rise + S.flat + S.fall) {
+      t = (S.rise + S.flat + S.fall - t2) / S.fall;
+      if (S.poly) return (10 * std::pow(t, 3) - 15 * std::pow(t, 4) + 6 * std::pow(t, 5)) * (t2 - t1);
+      if (S.linear) return t * (t2 - t1);
+      return 0;
+    }
+    return -1.0;
+  }
+
+  // Make_Gaussian_f (:1057-1124): unit-density Maxwellian of temperature Thot [eV] in f; Thot
+  // becomes the temperature of the discretised profile (-1 when it does not resolve p1)
+  void make_gaussian(std::vector<double>& f, double& Thot) const {
+    const std::vector<double>& p = pd;
+    const std::vector<double>& g = src.gam;
+    const int N = np;
+    const double PTH = 4.19 / 3000.0 * std::sqrt(Thot);
+    if (!(PTH > p[1])) { Thot = -1.0; return; }
+    f = p;
+    for (int i = 0; i < N; ++i) f[i] *= p[i];
+    for (int i = 0; i < N; ++i) f[i] *= -0.5 / (PTH * PTH);
+    for (int i = 0; i < N; ++i) f[i] = std::exp(f[i]);
+    for (int i = 0; i < N; ++i) f[i] *= 1.0 / std::pow(std::sqrt(2 * M_PI) * PTH, 3);
+    double n0 = f[0] * p[0] * p[0];
+    n0 += f[N - 1] * p[N - 1] * p[N - 1];
+    n0 *= 0.5;
+    for (int i = 1; i < N - 1; ++i) n0 += f[i] * p[i] * p[i];
+    const double dp = p[1] - p[0];
+    const double p0p1_sq = p[0] * p[0] / (p[1] * p[1]), inv_mp0p1_sq = 1.0 / (1.0 - p0p1_sq);
+    double f00 = f[1] / 5.0 * p0p1_sq;
+    f00 += (f[0] - f[1] * p0p1_sq) * inv_mp0p1_sq * (1.0 / 3.0 - 1.0 / 5.0 * p0p1_sq);
+    f00 *= p[0] * (p[0] / dp) * p[0];
+    n0 += f00;
+    n0 *= 4.0 * M_PI * dp;
+    for (int i = 0; i < N; ++i) f[i] *= 1.0 / n0;
+    double vtsq = f[0] * std::pow(p[0], 4) / std::pow(g[0], 2);
+    vtsq += f[N - 1] * std::pow(p[N - 1], 4) / std::pow(g[N - 1], 2);
+    vtsq *= 0.5;
+    for (int i = 1; i < N - 1; ++i) vtsq += f[i] * std::pow(p[i], 4) / std::pow(g[i], 2);
+    f00 = f[1] / 7.0 * p0p1_sq;
+    f00 += (f[0] - f[1] * p0p1_sq) * inv_mp0p1_sq * (1.0 / 5.0 - 1.0 / 7.0 * p0p1_sq);
+    f00 *= std::pow(p[0], 4) * (p[0] / dp);
+    vtsq += f00;
+    vtsq *= 4.0 * M_PI / 3.0 * dp;
+    Thot = vtsq * (3000.0 / 4.19) * (3000.0 / 4.19);
+  }
+
+  // PLaserSource::Laser_Source (:138-252) on the interior cells (the
+  // reference also heats guard cells, which its exchange then overwrites)
+  void laser_source(double tnew) {
+    Src& S = src;
+    const double I0_t = t_profile(tnew, S.t_pls);
+    const double dt = tnew - S.t_pls;
+    const double ILt = I0_t / dt;
+    S.t_pls = tnew;
+    if (I0_t < 0) return;
+    const std::vector<double>& p = pd;
+    const std::vector<double>& g = S.gam;
+    const int N = np;
+    const double dp = p[1] - p[0];
+    const double p0p1_sq = p[0] * p[0] / (p[1] * p[1]), inv_mp0p1_sq = 1.0 / (1.0 - p0p1_sq);
+    std::vector<double> fp4(N);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        cd* col = Y.h(0) + at(0, ix, iy);
+        double n0 = col[0].real() * p[0] * p[0];
+        n0 += col[N - 1].real() * p[N - 1] * p[N - 1];
+        n0 *= 0.5;
+        for (int i = 1; i < N - 1; ++i) n0 += p[i] * p[i] * col[i].real();
+        double f00 = col[1].real() / 5.0 * p0p1_sq;
+        f00 += (col[0].real() - col[1].real() * p0p1_sq) * inv_mp0p1_sq * (1.0 / 3.0 - 1.0 / 5.0 * p0p1_sq);
+        f00 *= p[0] * (p[0] / dp) * p[0];
+        n0 += f00;
+        n0 *= 4.0 * M_PI * dp;
+        double vsq = col[0].real() * std::pow(p[0], 4) / std::pow(g[0], 2);
+        vsq += col[N - 1].real() * std::pow(p[N - 1], 4) / std::pow(g[N - 1], 2);
+        vsq *= 0.5;
+        for (int i = 1; i < N - 1; ++i) vsq += std::pow(p[i], 4) / std::pow(g[i], 2) * col[i].real();
+        f00 = col[1].real() / 7.0 * p0p1_sq;
+        f00 += (col[0].real() - col[1].real() * p0p1_sq) * inv_mp0p1_sq * (1.0 / 5.0 - 1.0 / 7.0 * p0p1_sq);
+        f00 *= std::pow(p[0], 4) * (p[0] / dp);
+        vsq += f00;
+        vsq *= 4.0 * M_PI / 3.0 * dp;
+        vsq /= n0;
+        vsq *= (3000.0 / 4.19) * (3000.0 / 4.19);  // temperature in eV
+        double T_hot = S.Ty[iy] + vsq;
+        make_gaussian(fp4, T_hot);
+        if ((n0 > S.nmin) && (S.ILy[iy] * S.ILx[ix] * ILt * S.IL > S.IL_min) && (T_hot > vsq)) {
+          double alpha = S.acoeff * S.IL_sqrt * dt / n0;
+          alpha *= S.ILx[ix] * std::sqrt(S.ILy[iy] * ILt);
+          if ((alpha < 0) || (alpha > 0.99)) { S.alpha_error = true; continue; }  // the reference exits (:230-233)
+          for (int i = 0; i < N; ++i) {
+            col[i] *= (1.0 - alpha);
+            col[i] += alpha * fp4[i] * n0;
+          }
+        }
+      }
+  }
+
+  // ZLOGei_v2 (:42-67)
+  double zlogei_v2(double ne, double Te) const {
+    if (ne > 0.0000001) {
+      Te /= (3.0 * ne);
+      Te *= 511000;
+      ne *= c.density_np;
+      Te = std::log(Te);
+      ne = std::log(ne);
+      const double lnei = 24.0 - 0.5 * ne + Te;
+      if (lnei > 2.0) return lnei * c.zeta;
+    }
+    return 2.0 * c.zeta;
+  }
+
+  // IB_Operator::Getslope (:550-608); fh[N-1] is never written (stays 0)
+  void ib_slope(const std::vector<double>& fin, double vos, std::vector<double>& fh) const {
+    const Src& S = src;
+    const int N = np;
+    double p0overp1_sq = pd[0] / pd[1];
+    p0overp1_sq *= p0overp1_sq;
+    const double f00 = (fin[0] - fin[1] * p0overp1_sq) / (1.0 - p0overp1_sq);
+    double I4 = 0;
+    for (int n_ = 1; n_ < N; ++n_) I4 += S.U4[n_] * fin[n_] + S.U4m1[n_] * fin[n_ - 1];
+    double I2 = 0;
+    for (int n_ = 1; n_ < N; ++n_) I2 += S.U2[n_] * fin[n_] + S.U2m1[n_] * fin[n_ - 1];
+    const double ZLn_ei = zlogei_v2(4.0 * M_PI * I2, 4.0 * M_PI * I4);
+    const double vw_cube = S.vw_coeff_cube * ZLn_ei * 4.0 * M_PI * I2;
+    std::vector<double> gn(N);
+    for (int i = 0; i < N; ++i) {
+      gn[i] = S.Inv_Uav6[i] * vw_cube * vw_cube;
+      gn[i] = 1.0 / (1.0 + gn[i]);
+    }
+    const double qn_c = S.Qn_coeff * vos * vos * ZLn_ei * 4.0 * M_PI * I2;
+    fh[0] = qn_c * S.Qn[0] *
+            (S.Pn[0] * gn[0] * (fin[1] - fin[0]) -
+             S.Pnm1 / (1.0 + S.Inv_Uav6_nm1 * vw_cube * vw_cube) * (fin[0] - f00));
+    for (int i = 1; i < N - 1; ++i)
+      fh[i] = qn_c * S.Qn[i] *
+              (S.Pn[i] * gn[i] * (fin[i + 1] - fin[i]) - S.Pn[i - 1] * gn[i - 1] * (fin[i] - fin[i - 1]));
+  }
+
+  // InverseBremsstrahlung::loop (:780-861): RungeKutta4_ib sub-steps (:713-757) per cell, then the
+  // density restored
+  void inverse_bremsstrahlung(double tnew) {
+    Src& S = src;
+    const int N = np;
+    const int numh = static_cast<int>(static_cast<size_t>(static_cast<int>((tnew - S.t_ib) / S.smaller_dt)) + 1);
+    const double h = (tnew - S.t_ib) / static_cast<double>(static_cast<size_t>(numh));
+    std::vector<double> fc(N), f0(N), f1(N), fh(N, 0.0);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const double vos = S.vos * std::sqrt(S.ILxe[ix] * S.ILye[iy]);
+        cd* col = Y.h(0) + at(0, ix, iy);
+        for (int i = 0; i < N; ++i) fc[i] = col[i].real();
+        double I2_before = 0.0;
+        for (int n_ = 1; n_ < N; ++n_) I2_before += S.U2p[n_] * fc[n_] + S.U2pm1[n_] * fc[n_ - 1];
+        for (int s = 0; s < numh; ++s) {
+          f0 = fc; f1 = fc;
+          ib_slope(f1, vos, fh);
+          for (int i = 0; i < N; ++i) { fh[i] *= 0.5 * h; f1[i] += fh[i]; fh[i] *= 1.0 / 3.0; fc[i] += fh[i]; }
+          ib_slope(f1, vos, fh);
+          f1 = f0;
+          for (int i = 0; i < N; ++i) { fh[i] *= 0.5 * h; f1[i] += fh[i]; fh[i] *= 2.0 / 3.0; fc[i] += fh[i]; }
+          ib_slope(f1, vos, fh);
+          for (int i = 0; i < N; ++i) { fh[i] *= h; f0[i] += fh[i]; fh[i] *= 1.0 / 3.0; fc[i] += fh[i]; }
+          ib_slope(f0, vos, fh);
+          for (int i = 0; i < N; ++i) { fh[i] *= h / 6.0; fc[i] += fh[i]; }
+        }
+        double I2_after = 0.0;
+        for (int n_ = 1; n_ < N; ++n_) I2_after += S.U2p[n_] * fc[n_] + S.U2pm1[n_] * fc[n_ - 1];
+        for (int i = 0; i < N; ++i) fc[i] *= I2_before / I2_after;
+        for (int i = 0; i < N; ++i) col[i] = fc[i];
+      }
+    S.t_ib = tnew;
+  }
+
+  // ExternalEfield::Laser_Efield (:368-418): the chosen field component set on every cell
+  void laser_efield(double tnew) {
+    Src& S = src;
+    const double dt = tnew - S.t_efld;
+    const double env = t_profile(tnew, S.t_efld) / dt;
+    if (S.pol >= 1 && S.pol <= 3) {
+      cd* F = Y.fld.data() + (size_t)(S.pol - 1) * nxy;
+      for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+          F[(size_t)j * nx + i] = S.vos * S.ILxe[i] * S.ILye[j] * env * std::sin(S.w0overwp * (tnew - S.xpos[i]));
+    }
+    S.t_efld = tnew;
+  }
+
   // Implicit_FLM_step::reset_coeff, def-FokkerPlanck.cpp:808-939 (tridiagonal part)
   struct Coef {
     std::vector<double> sub, diag, sup, scatt;  // Alpha_Tri bands (already * -lnee kpre Dt), scattering term
@@ -947,6 +1238,8 @@ struct Oracle {
     rk4(h);
     double t = t_rk;
     if (c.coll_ee) collide_ee(t); else t_ee = t;
+    if (src.linear || src.poly) laser_source(t);  // main.cpp:90-92
+    if (src.ib == 1) inverse_bremsstrahlung(t);
     if (c.coll_flm) {
       if (c.l0 >= 1) collide_flm(t - t_eb, 1);   // Euler_Backward::advance_1
       collide_flm(t - t_eb, 2);                  // Euler_Backward::advance_lm
@@ -1448,6 +1741,20 @@ void oo_step(void* o, double h, int nsteps) {
   for (int s = 0; s < nsteps; ++s) q->step(h);
 }
 void oo_moments(void* o, double* out) { static_cast<Oracle*>(o)->moments(out); }
+// sources: sv = {linear, poly, ib, pol, I_0, lambda_0, ab_eta, ab_depth, cntr_x0, cntr_r0, spot_w0,
+// rise, flat, fall, smaller_dt}; x/y: positions of the interior cells; xglob: the global x axis
+void oo_set_sources(void* o, const double* sv, const double* x, const double* y, const double* xglob, int nxg) {
+  Oracle* q = static_cast<Oracle*>(o);
+  q->set_sources(sv, x, y, xglob, nxg, q->t_rk);
+}
+int oo_laser_source(void* o, double tnew) {
+  Oracle* q = static_cast<Oracle*>(o);
+  q->src.alpha_error = false;
+  q->laser_source(tnew);
+  return q->src.alpha_error ? -1 : 0;
+}
+void oo_inverse_bremsstrahlung(void* o, double tnew) { static_cast<Oracle*>(o)->inverse_bremsstrahlung(tnew); }
+void oo_laser_efield(void* o, double tnew) { static_cast<Oracle*>(o)->laser_efield(tnew); }
 // nsteps implicit-E steps of the 1D variant (l0 >= 3); es = if_implicitES
 int oo_step_implicit_1d(void* o, double h, int nsteps, int es) {
   Oracle* q = static_cast<Oracle*>(o);

@@ -1,9 +1,10 @@
 // Test driver for the UNMODIFIED reference classes of OSHUN-1.
 //
 // TEST INFRASTRUCTURE ONLY (oracle side).  It is compiled by oracle/Makefile
-// together with eight reference translation units taken as they lie under
+// together with nine reference translation units taken as they lie under
 // the reference tree (def-Input, def-Nmethods, def-State, def-VlasovMax,
-// def-Actions, def-RK, def-FokkerPlanck, def-ImplicitE); none of them uses MPI.  Nothing of
+// def-Actions, def-RK, def-FokkerPlanck, def-ImplicitE, def-PLSource); none of
+// them uses MPI.  Nothing of
 // the reference is copied into this repository; the binary lands in
 // oracle/_ref/, which git ignores.
 //
@@ -18,8 +19,11 @@
 // and after each step, which is what the single-node periodic exchange does.
 //
 // Usage:  oshun_ref <in.bin> <out.bin>
-//   in.bin : int32[10] {l0,m0,np,nx,ny,op,nsteps,nb,es,pad}  (es: if_implicitES, op 5)
+//   in.bin : int32[10] {l0,m0,np,nx,ny,op,nsteps,nb,es,src}  (es: if_implicitES, op 5)
 //            double[9] {pmax,xmin,xmax,ymin,ymax,h,small_dt,density_np,zeta}
+//            if src: double[15] {linear_Ivst, polynomial_Ivst, inverse_bremsstrahlung,
+//                   polarization_direction, I_0, lambda_0, ab_eta, ab_depth, cntr_x0,
+//                   cntr_r0, spot_w0, rise_time, flat_time, fall_time, smaller_dt}
 //            complex<double>[nh][ny][nx][np]  (interior f, p fastest)
 //            complex<double>[6][ny][nx]       (interior Ex..Bz)
 //   op     : 0 = Actions::Adv slope of the input state
@@ -30,6 +34,11 @@
 //            5 = nsteps x {RK2 advance_p1, e-e, implicit E, RK2 advance_p2,
 //                f1, flm, halo} with implicit_E and if_implicit1D on
 //                (main.cpp:79-107 with Runge_Kutta_2_Imp and Implicit_E_Field)
+//            op 1 also runs PLaserSource::Laser_Source and InverseBremsstrahlung::loop
+//            after the e-e collisions when the deck turns them on (main.cpp:90-92)
+//            6 = nsteps x {PLaserSource::Laser_Source}   (tnew advanced by h each time)
+//            7 = nsteps x {InverseBremsstrahlung::loop}  (same)
+//            8 = nsteps x {ExternalEfield::Laser_Efield} (same)
 //   out.bin: double[4] {dx_used, dy_used, h_used, t_final},
 //            double[np] p axis, then f and fields as in in.bin.
 
@@ -50,6 +59,7 @@
 #include "decl-RK.h"
 #include "decl-fokkerplanck.h"
 #include "decl-implicitE.h"
+#include "decl-plsource.h"
 
 typedef std::complex<double> cd;
 
@@ -110,6 +120,8 @@ int main(int argc, char** argv) {
   if (std::fread(dv, sizeof(double), 9, fi) != 9) die("short header");
   const int l0 = iv[0], m0 = iv[1], np = iv[2], nx = iv[3], ny = iv[4], op = iv[5], nsteps = iv[6], nb = iv[7];
   const int es = iv[8];
+  double sv[15] = {0};
+  if (iv[9] && std::fread(sv, sizeof(double), 15, fi) != 15) die("short source block");
   const int nh = ((m0 + 1) * (2 * l0 - m0 + 2)) / 2;
   const int nbc = 4;
 
@@ -128,6 +140,11 @@ int main(int argc, char** argv) {
   L.if_tridiagonal = true; L.implicit_E = (op == 5); L.if_implicit1D = (op == 5); L.if_implicitES = (op == 5 && es);
   L.density_np = dv[7]; L.Zeta = dv[8];
   L.bndX = 0; L.bndY = 0;
+  L.linear_Ivst = sv[0] != 0; L.polynomial_Ivst = sv[1] != 0;
+  L.inverse_bremsstrahlung = static_cast<int>(sv[2]); L.polarization_direction = static_cast<int>(sv[3]);
+  L.I_0 = sv[4]; L.lambda_0 = sv[5]; L.ab_eta = sv[6]; L.ab_depth = sv[7]; L.cntr_x0 = sv[8];
+  L.cntr_r0 = sv[9]; L.spot_w0 = sv[10]; L.rise_time = sv[11]; L.flat_time = sv[12]; L.fall_time = sv[13];
+  L.smaller_dt = sv[14];
   Inputdata::IN().inp() = Inputdata::decl_input(L);
   Inputdata::IN().cont() = Inputdata::decl_control(L);
   if (ny == 1) {
@@ -165,10 +182,14 @@ int main(int argc, char** argv) {
     Runge_Kutta_4 rk(Y, 0);
     Explicit_EC_FP ee(Y, 0);
     Euler_Backward eb(Y, 0);
+    PLaserSource PLS(Y, 0);
+    InverseBremsstrahlung IBs(Y, 0);
     for (int s = 0; s < nsteps; ++s) {
       rk.advance_p1();
       if (op == 1) {
         ee.loop(rk.time());
+        if (PLS.PLSOURCE()) PLS.Laser_Source(rk.time());
+        if (IBs.IBSOURCE() == 1) IBs.loop(rk.time());
         eb.advance_1(rk.time());
         eb.advance_lm(rk.time());
       }
@@ -215,6 +236,20 @@ int main(int argc, char** argv) {
       fill_guards_periodic(Y, nbc);
     }
     t_final = rk.time();
+    Out = Y;
+  } else if (op >= 6 && op <= 8) {
+    PLaserSource PLS(Y, 0);
+    InverseBremsstrahlung IBs(Y, 0);
+    ExternalEfield EFLD(Y, 0);
+    double t = 0.0;
+    for (int s = 0; s < nsteps; ++s) {
+      t += dv[5];
+      if (op == 6) PLS.Laser_Source(t);
+      else if (op == 7) IBs.loop(t);
+      else EFLD.Laser_Efield(t);
+      fill_guards_periodic(Y, nbc);
+    }
+    t_final = t;
     Out = Y;
   } else {
     die("bad op");

@@ -25,6 +25,13 @@ Cases (the reference needs l0 >= 3 and m0 >= 2, def-Input.cpp:73):
                  m0=2 y-uniform stand-in, if_implicitES on
   imp_1d_m0_l8   the same at l0=8, Np=64, if_implicitES off (Faraday in
                  Adv1; the transverse fields are zero)
+  pls_only       3 PLaserSource::Laser_Source calls (polynomial rise), 2D
+  ib_only        3 InverseBremsstrahlung::loop calls, 2D
+  efield_only    3 ExternalEfield::Laser_Efield calls (Ex polarisation), 2D
+  step_src       2 full steps with the laser source and inverse
+                 bremsstrahlung on (main.cpp:79-107 with :90-92), 2D
+
+`python tests/golden/make_golden.py name ...` writes only the named fixtures.
 """
 import os
 import sys
@@ -35,6 +42,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from oracle.oracle import hind, num_harmonics, run_ref  # noqa: E402
+
+# the source decks: 351 nm light, polynomial ramp over 1/omega_p, flat top to 5
+SRC = dict(linear_Ivst=0, polynomial_Ivst=1, inverse_bremsstrahlung=0, polarization_direction=0, I_0=3.0e14,
+           lambda_0=0.351, ab_eta=0.5, ab_depth=8.0, cntr_x0=20.0, cntr_r0=10.0, spot_w0=15.0, rise_time=1.0,
+           flat_time=4.0, fall_time=1.0, smaller_dt=0.2)
 
 
 def p_axis(np_, pmax):
@@ -100,7 +112,27 @@ def save(name, cfg, fin, fldin, res, keep_m0_only=False, m0_phys=None):
     print(f"{name}: {os.path.getsize(path)} bytes")
 
 
-def main():
+def source_cases(only):
+    base = dict(l0=3, m0=2, np=48, nx=6, ny=3, pmax=0.3, xmin=0.0, xmax=50.0, ymin=0.0, ymax=30.0,
+                nsteps=3, nb=4, h=0.5, small_dt=0.2, density_np=1e21, zeta=1.0)
+    cases = (("pls_only", 6, dict(SRC)), ("ib_only", 7, dict(SRC, polynomial_Ivst=0, inverse_bremsstrahlung=1)),
+             ("efield_only", 8, dict(SRC, polynomial_Ivst=0, polarization_direction=1)),
+             ("step_src", 1, dict(SRC, inverse_bremsstrahlung=1)))
+    for name, op, src in cases:
+        if only and name not in only:
+            continue
+        c = dict(base, op=op, nsteps=2 if op == 1 else 3)
+        f, fld = maxwellian_state(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], c["pmax"], 40 + op)
+        r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], f, fld, op=op, nsteps=c["nsteps"],
+                    pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"], h=c["h"],
+                    small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"], src=src)
+        save(name, dict(c, **{"src_" + k: v for k, v in src.items()}), f, fld, r)
+
+
+def main(only=()):
+    if only:
+        source_cases(only)
+        return
     # 1) RHS, 2D complex
     c = dict(l0=4, m0=3, np=16, nx=6, ny=4, pmax=0.3, xmin=-2.0, xmax=3.0, ymin=-1.0, ymax=2.0,
              op=0, nsteps=0, nb=4, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0)
@@ -182,6 +214,8 @@ def main():
         assert not np.any(r["fld"][1:])
         save(name, c, f, fld, r, keep_m0_only=True, m0_phys=0)
 
+    source_cases(())
+
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))
