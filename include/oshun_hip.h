@@ -118,6 +118,40 @@ int oshun_step(oshun_ctx* ctx, double h, int nsteps);
  * otherwise OSHUN_ERR_ARG. */
 int oshun_step_implicit_1d(oshun_ctx* ctx, double h, int nsteps, int if_implicitES);
 
+/* ---- Sources (def-PLSource.cpp) -------------------------------------------
+ * The deck's laser parameters (Inputdata input_list, decl-input.h:104-109). */
+typedef struct oshun_sources {
+  int linear_Ivst, polynomial_Ivst; /* PLaserSource on when either is set (decl-plsource.h:47)         */
+  int inverse_bremsstrahlung;       /* InverseBremsstrahlung on when 1 (def-PLSource.cpp:772)          */
+  int polarization_direction;       /* ExternalEfield: 1 Ex, 2 Ey, 3 Ez (def-PLSource.cpp:356-364)     */
+  double I_0, lambda_0;             /* intensity [W/cm^2], wavelength [um]                              */
+  double ab_eta, ab_depth;          /* absorbed fraction, absorption depth                              */
+  double cntr_x0, cntr_r0, spot_w0; /* profile centre in x, in y, and spot size                         */
+  double rise_time, flat_time, fall_time; /* time envelope                                             */
+  double smaller_dt;                /* inverse-bremsstrahlung sub-step: numh = int(Dt/smaller_dt)+1     */
+} oshun_sources;
+/* Sets the sources of the context (the constructors of PLaserSource,
+ * InverseBremsstrahlung and ExternalEfield, def-PLSource.cpp:73-133, :308-353,
+ * :861-946).  x[nx], y[ny]: positions of the interior cells on the
+ * reference's axes (Inputdata::decl_input, def-Input.cpp:313-325); xglob:
+ * its global x axis (nxglob points, the laser profile's normalisation).  The
+ * source clocks start at the context's time.  oshun_step then runs
+ * PLaserSource::Laser_Source and InverseBremsstrahlung::loop after the e-e
+ * collisions of every step, where main_loop calls them (main.cpp:90-92), as
+ * does oshun_step_implicit_1d. */
+int oshun_set_sources(oshun_ctx* ctx, const oshun_sources* src, const double* x, const double* y,
+                      const double* xglob, int nxglob);
+/* PLaserSource::Laser_Source(tnew) (def-PLSource.cpp:138-252).  A heating
+ * rate outside [0, 0.99], where the reference stops the run, returns
+ * OSHUN_ERR_STATE (those cells are left unheated); oshun_step checks it too. */
+int oshun_laser_source(oshun_ctx* ctx, double tnew);
+/* InverseBremsstrahlung::loop(tnew) (def-PLSource.cpp:780-861). */
+int oshun_inverse_bremsstrahlung(oshun_ctx* ctx, double tnew);
+/* ExternalEfield::Laser_Efield(tnew) (def-PLSource.cpp:368-418): sets the
+ * polarisation's field component on every cell; main_loop has this call
+ * commented out (main.cpp:91), so no step runs it. */
+int oshun_laser_efield(oshun_ctx* ctx, double tnew);
+
 /* Nonrelativistic moments of the current state in double
  * (Parallel_Output::nonrelativistic_moments, def-Parallel.cpp:1677-2253;
  * the reference computes them in float).  out = double[OSHUN_NMOMENTS][ny][nx]

@@ -72,6 +72,16 @@ class Phys(ctypes.Structure):
                 ("nb_algorithms", ctypes.c_int), ("collisions", ctypes.c_int), ("ref_quirks", ctypes.c_int)]
 
 
+class Sources(ctypes.Structure):
+    """oshun_sources: the deck's laser parameters (def-PLSource.cpp)."""
+    _fields_ = [("linear_Ivst", ctypes.c_int), ("polynomial_Ivst", ctypes.c_int),
+                ("inverse_bremsstrahlung", ctypes.c_int), ("polarization_direction", ctypes.c_int),
+                ("I_0", ctypes.c_double), ("lambda_0", ctypes.c_double), ("ab_eta", ctypes.c_double),
+                ("ab_depth", ctypes.c_double), ("cntr_x0", ctypes.c_double), ("cntr_r0", ctypes.c_double),
+                ("spot_w0", ctypes.c_double), ("rise_time", ctypes.c_double), ("flat_time", ctypes.c_double),
+                ("fall_time", ctypes.c_double), ("smaller_dt", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -122,6 +132,10 @@ def _load():
     lib.oshun_stage_kernel.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.oshun_pair_rows.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.oshun_moments.argtypes = [P, dp]
+    lib.oshun_set_sources.argtypes = [P, ctypes.POINTER(Sources), dp, dp, dp, I]
+    lib.oshun_laser_source.argtypes = [P, D]
+    lib.oshun_inverse_bremsstrahlung.argtypes = [P, D]
+    lib.oshun_laser_efield.argtypes = [P, D]
     lib.oshun_moments_noalloc.argtypes = [P, P]
     lib.oshun_batch_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Phys), dp, I, ctypes.POINTER(P)]
     lib.oshun_batch_destroy.argtypes = [P]
@@ -259,6 +273,28 @@ class Context:
 
     def step(self, h, nsteps=1):
         self._chk(self._lib.oshun_step(self._h, float(h), int(nsteps)))
+
+    def set_sources(self, src, x, y, xglob):
+        """Laser parameters (a dict of oshun_sources' fields) and the positions of the interior cells
+        (x, y) and of the global x axis; oshun_step then runs the enabled sources every step."""
+        s = Sources(**{name: (int(src.get(name, 0)) if typ is ctypes.c_int else float(src.get(name, 0.0)))
+                       for name, typ in Sources._fields_})
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        xg = np.ascontiguousarray(xglob, np.float64)
+        self._chk(self._lib.oshun_set_sources(self._h, ctypes.byref(s), _dptr(x), _dptr(y), _dptr(xg), int(xg.size)))
+
+    def laser_source(self, tnew):
+        """PLaserSource::Laser_Source (def-PLSource.cpp:138-252)."""
+        self._chk(self._lib.oshun_laser_source(self._h, float(tnew)))
+
+    def inverse_bremsstrahlung(self, tnew):
+        """InverseBremsstrahlung::loop (def-PLSource.cpp:780-861)."""
+        self._chk(self._lib.oshun_inverse_bremsstrahlung(self._h, float(tnew)))
+
+    def laser_efield(self, tnew):
+        """ExternalEfield::Laser_Efield (def-PLSource.cpp:368-418)."""
+        self._chk(self._lib.oshun_laser_efield(self._h, float(tnew)))
 
     def step_implicit_1d(self, h, nsteps=1, es=0):
         """main.cpp:79-107 with implicit_E and if_implicit1D (Runge_Kutta_2_Imp +

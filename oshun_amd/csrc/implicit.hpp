@@ -198,6 +198,7 @@ int implicit_step_1d(oshun_ctx* c, double h, int es) {
     if ((rc = do_collide(c, 1, 0, 0, numh, hee, 0.0))) return rc;
   }
   c->t_ee = t;
+  if (sources_in_step(c) && (rc = step_sources(c, t))) return rc;  // main.cpp:90-92
   // Implicit_E_Field::advance, 1D variant.  The reference runs its two probes
   // on the state and restores f00, f1m, f2m afterwards (def-ImplicitE.cpp:
   // 491-520); here they run on a copy in B2 (S and B2 swap around them), so S

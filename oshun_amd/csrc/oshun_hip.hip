@@ -92,6 +92,16 @@ struct oshun_ctx {
   int rows_segw = 16;       // lanes per column of collide_rows (16 or 32)
   int rows_npl2 = 0, rows_segw2 = 16;
   oshun::CollTables ct;
+  // sources of def-PLSource.cpp (oshun_set_sources): switches, clocks, constants
+  bool src_on = false;
+  oshun_sources src = {};
+  int src_pls = 0, src_ib = 0, src_pol = 0;
+  double src_t_pls = 0.0, src_t_ib = 0.0, src_t_efld = 0.0;
+  double src_IL = 0.0, src_IL_sqrt = 0.0, src_IL_min = 0.0, src_acoeff = 0.0, src_vos = 0.0, src_w0overwp = 0.0;
+  double src_vw_coeff_cube = 0.0, src_Qn_coeff = 0.0, src_Inv_Uav6_nm1 = 0.0, src_Pnm1 = 0.0, src_p0 = 0.0, src_p1 = 0.0;
+  double* d_src = nullptr;   // per-p and per-cell source tables
+  double* d_ibw = nullptr;   // inverse bremsstrahlung work planes (first use)
+  int* d_srcflag = nullptr;  // a heating rate out of range
   hipStream_t stream = nullptr;
   int device = 0;  // the HIP device current at oshun_create
   double t_rk = 0.0, t_ee = 0.0, t_eb = 0.0;
@@ -585,6 +595,11 @@ int do_collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double 
                  : collide<double>(c, do_ee, do_f1, do_flm, numh, h_ee, dt_flm);
 }
 
+// sources of one step after the e-e collisions (main.cpp:90-92; sources.hpp)
+int step_sources(oshun_ctx* c, double t);
+int check_source_flag(oshun_ctx* c);
+bool sources_in_step(const oshun_ctx* c) { return c->src_on && (c->src_pls || c->src_ib == 1); }
+
 // numh and sub-step of Explicit_EC_FP::loop, def-FokkerPlanck.cpp:597-599
 void ee_substeps(const oshun_ctx* c, double tnew, int* numh, double* h) {
   size_t n = static_cast<size_t>(static_cast<int>((tnew - c->t_ee) / c->ph.small_dt)) + 1;
@@ -602,8 +617,10 @@ int full_step(oshun_ctx* c, double h, bool next1) {
   const int do_flm = (coll & OSHUN_COLL_FLM) ? 1 : 0;
   // (the launch keeps up to oshun::kFuseTiles current partials per column in LDS)
   const int npt = c->fp32 ? pair_tiles<float>(c) : pair_tiles<double>(c);
+  // with sources the e-e and implicit collisions are separate launches around them
+  const bool srcs = sources_in_step(c);
   const bool fuse = c->use_pair && c->ny == 1 && do_ee && do_flm && c->rows_npl && c->rows_segw == c->rows_segw2 &&
-                    c->rows_npl == c->rows_npl2 && npt <= oshun::kFuseTiles;
+                    c->rows_npl == c->rows_npl2 && npt <= oshun::kFuseTiles && !srcs;
   // without collisions the step's last field stage and the next step's first
   // run as one field launch after the RK4 (fuse_fields = 3)
   const bool f41 = !fuse && next1 && c->use_pair && c->ny == 1 && !do_ee && !do_flm;
@@ -622,7 +639,13 @@ int full_step(oshun_ctx* c, double h, bool next1) {
   int numh = 0;
   double hee = 0.0;
   if (do_ee) ee_substeps(c, t, &numh, &hee);
-  if (do_ee || do_flm) rc = do_collide(c, do_ee, do_flm, do_flm, numh, hee, t - c->t_eb);
+  if (srcs) {
+    if (do_ee) rc = do_collide(c, 1, 0, 0, numh, hee, 0.0);
+    if (!rc) rc = step_sources(c, t);
+    if (!rc && do_flm) rc = do_collide(c, 0, 1, 1, 0, 0.0, t - c->t_eb);
+  } else if (do_ee || do_flm) {
+    rc = do_collide(c, do_ee, do_flm, do_flm, numh, hee, t - c->t_eb);
+  }
   const int ff = c->fuse_fields;
   c->fuse_fields = 0;
   if (rc) return rc;
@@ -748,7 +771,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_ibw, c->d_srcflag};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
@@ -1156,6 +1179,7 @@ int oshun_download_state(oshun_ctx* c, double* f, double* fld, int nbc) {
 int oshun_set_time(oshun_ctx* c, double t) {
   if (!sane(c)) return OSHUN_ERR_ARG;
   c->t_rk = c->t_ee = c->t_eb = t;
+  c->src_t_pls = c->src_t_ib = c->src_t_efld = t;
   return OSHUN_OK;
 }
 
@@ -1207,7 +1231,8 @@ int oshun_step(oshun_ctx* c, double h, int nsteps) {
     int rc = full_step(c, h, s + 1 < nsteps);
     if (rc) { c->next1_ready = false; return rc; }
   }
-  return OSHUN_OK;
+  // a laser-source heating rate out of range stops the reference (def-PLSource.cpp:230-233)
+  return (c->src_on && c->src_pls) ? check_source_flag(c) : OSHUN_OK;
 }
 
 int oshun_synchronize(oshun_ctx* c) {
@@ -1361,5 +1386,7 @@ int oshun_diag_copy(oshun_ctx* c, long long nbytes) {
 #include "batch.hpp"
 // adjoint of the step (oshun_adjoint_*)
 #include "adjoint.hpp"
-
+// laser heating, inverse bremsstrahlung, external field (oshun_set_sources, ...)
+#include "sources.hpp"
+// implicit-E step, 1D variant (oshun_step_implicit_1d)
 #include "implicit.hpp"

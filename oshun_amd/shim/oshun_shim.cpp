@@ -10,7 +10,9 @@
 //   Explicit_EC_FP::loop                                   def-FokkerPlanck.cpp:590-625
 //   Euler_Backward::advance_1, ::advance_lm                def-FokkerPlanck.cpp:1270-1304
 //   Electric_Field_Methods::Implicit_E_Field::advance      def-ImplicitE.cpp:472-606
-// Link this object ahead of def-RK.o, def-FokkerPlanck.o and def-ImplicitE.o,
+//   PLaserSource::Laser_Source                             def-PLSource.cpp:138-252
+//   InverseBremsstrahlung::loop                            def-PLSource.cpp:780-861
+// Link this object ahead of def-RK.o, def-FokkerPlanck.o, def-ImplicitE.o and def-PLSource.o,
 // either after deleting those bodies from the reference files or with
 // -Wl,--allow-multiple-definition (then the first definition, this one, is
 // the one every caller and every vtable binds to; the reference files stay
@@ -22,10 +24,10 @@
 // the host Stat is refreshed by a download at the end of every output
 // interval -- every numh() steps of the driver, main.cpp:77-131 -- so that
 // CX.Output and Write_Restart read current data.  The guard refresh
-// CX.Neighbor_Communications then only rewrites host guard cells.  Laser and
-// inverse-bremsstrahlung sources (def-PLSource.cpp), which change the host
-// Stat between the integrators' calls, are not supported.  Errors print and
-// exit(1), as the reference does.
+// CX.Neighbor_Communications then only rewrites host guard cells.  The laser
+// and inverse-bremsstrahlung sources (def-PLSource.cpp) run on the device too,
+// with the deck's parameters and axes.  Errors print and exit(1), as the
+// reference does.
 //
 // The implicit-E decks the library runs are the 1D variant (if_implicit1D,
 // m = 0 harmonics, Ey = Ez = B = 0): the device context then holds the m = 0
@@ -52,6 +54,7 @@
 #include "decl-RK.h"
 #include "decl-fokkerplanck.h"
 #include "decl-implicitE.h"
+#include "decl-plsource.h"
 
 #include "oshun_hip.h"
 
@@ -132,6 +135,23 @@ void attach(Stat& Y, double t) {
   pointers(Y, sh, fld);
   check(oshun_upload_harmonics(s.ctx, sh.data(), fld.data(), nbc), "oshun_upload_harmonics");
   check(oshun_set_time(s.ctx, t), "oshun_set_time");
+  // the deck's laser sources, with the cell positions of its own axes (def-Input.cpp:313-325)
+  const Inputdata::input_list& L = IN().list();
+  if (L.linear_Ivst || L.polynomial_Ivst || L.inverse_bremsstrahlung == 1) {
+    oshun_sources src;
+    src.linear_Ivst = L.linear_Ivst; src.polynomial_Ivst = L.polynomial_Ivst;
+    src.inverse_bremsstrahlung = L.inverse_bremsstrahlung; src.polarization_direction = L.polarization_direction;
+    src.I_0 = L.I_0; src.lambda_0 = L.lambda_0; src.ab_eta = L.ab_eta; src.ab_depth = L.ab_depth;
+    src.cntr_x0 = L.cntr_x0; src.cntr_r0 = L.cntr_r0; src.spot_w0 = L.spot_w0;
+    src.rise_time = L.rise_time; src.flat_time = L.flat_time; src.fall_time = L.fall_time;
+    src.smaller_dt = L.smaller_dt;
+    std::vector<double> xs(g.numx), ys(g.numy), xg(IN().inp().xglob.dim());
+    for (int i = 0; i < g.numx; ++i) xs[i] = IN().inp().x(i + nbc);
+    for (int j = 0; j < g.numy; ++j) ys[j] = IN().inp().y(j + nbc);
+    for (size_t i = 0; i < xg.size(); ++i) xg[i] = IN().inp().xglob(i);
+    check(oshun_set_sources(s.ctx, &src, xs.data(), ys.data(), xg.data(), static_cast<int>(xg.size())),
+          "oshun_set_sources");
+  }
 }
 
 void download() {
@@ -209,6 +229,27 @@ void Explicit_EC_FP::loop(const double& tnew) {
     attach(Y, t);
     check(oshun_collide_ee(s.ctx, tnew), "Explicit_EC_FP::loop");  // numh from tnew - t as at :597-599
     if (!s.rk_driven) download();  // used without a driver: the caller reads the Stat next
+  }
+  t = tnew;
+}
+
+// ---- sources (def-PLSource.cpp:138-252, 780-861); inside the implicit step's device call there
+void PLaserSource::Laser_Source(const double& tnew) {
+  Session& s = session();
+  if (!s.implicit) {
+    // the class holds f00 only, not the Stat: the device state comes from the step's integrators
+    if (!s.ctx) die("PLaserSource::Laser_Source before the first integrator call");
+    check(oshun_laser_source(s.ctx, tnew), "PLaserSource::Laser_Source");
+  }
+  t1 = tnew;
+}
+
+void InverseBremsstrahlung::loop(const double& tnew) {
+  Session& s = session();
+  if (!s.implicit) {
+    attach(Y, t);
+    check(oshun_inverse_bremsstrahlung(s.ctx, tnew), "InverseBremsstrahlung::loop");
+    if (!s.rk_driven) download();
   }
   t = tnew;
 }

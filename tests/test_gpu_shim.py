@@ -5,7 +5,8 @@ with the shim (oshun_amd/shim/oshun_shim.cpp) linked ahead of the reference
 translation units, so that every step runs on the GPU through the C ABI
 (oracle/_ref/oshun_ref_shim, built by `make -C oracle shim`).
 
-* RK4 decks (fixtures step_2d_c3, step_1d_m0) and the collision-only drivers
+* RK4 decks (fixtures step_2d_c3, step_1d_m0, and step_src with the laser
+  source and inverse bremsstrahlung on) and the collision-only drivers
   (ee_only, flm_only): the shim's output against the compiled reference's
   fixture at <= 1e-12 normwise in f and the fields.
 * implicit-E 1D decks (imp_1d_m0, imp_1d_m0_l8): bitwise equal to the same
@@ -51,7 +52,8 @@ def _run_shim(g):
                   nsteps=int(c["nsteps"]), nb=int(c["nb"]), pmax=float(c["pmax"]), xmin=float(c["xmin"]),
                   xmax=float(c["xmax"]), ymin=float(c["ymin"]), ymax=float(c["ymax"]), h=float(c["h"]),
                   small_dt=float(c["small_dt"]), density_np=float(c["density_np"]), zeta=float(c["zeta"]),
-                  es=int(c.get("es", 0)), binary=SHIM_BIN)
+                  es=int(c.get("es", 0)), binary=SHIM_BIN,
+                  src={k[4:]: float(v) for k, v in c.items() if k.startswith("src_")} or None)
     fo, flo = out["f"], out["fld"]
     if m0 != int(c["m0"]):
         assert np.abs(np.delete(fo, [hind(l, 0, m0) for l in range(l0 + 1)], axis=0)).max() == 0.0
@@ -59,7 +61,7 @@ def _run_shim(g):
     return fo, flo[:g["fld_out"].shape[0]]
 
 
-@pytest.mark.parametrize("name", ["step_2d_c3", "step_1d_m0", "ee_only", "flm_only"])
+@pytest.mark.parametrize("name", ["step_2d_c3", "step_1d_m0", "ee_only", "flm_only", "step_src"])
 def test_reference_classes_through_the_shim_match_the_reference(golden, name):
     g = golden(name)
     print(f"case shim {name}", flush=True)

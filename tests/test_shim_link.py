@@ -22,9 +22,11 @@ REPLACED = ["Runge_Kutta_4::Runge_Kutta_4(Stat&, int)", "Runge_Kutta_4::advance_
             "Runge_Kutta_2_Imp::Runge_Kutta_2_Imp(Stat&, int)", "Runge_Kutta_2_Imp::advance_p1()",
             "Runge_Kutta_2_Imp::advance_p2()", "Explicit_EC_FP::loop(double const&)",
             "Euler_Backward::advance_1(double const&)", "Euler_Backward::advance_lm(double const&)",
-            "Electric_Field_Methods::Implicit_E_Field::advance(Explicit_Method*, Euler_Backward*)"]
+            "Electric_Field_Methods::Implicit_E_Field::advance(Explicit_Method*, Euler_Backward*)",
+            "PLaserSource::Laser_Source(double const&)", "InverseBremsstrahlung::loop(double const&)"]
 ABI = ["oshun_create", "oshun_upload_harmonics", "oshun_download_harmonics", "oshun_set_time", "oshun_rk4_step",
-       "oshun_collide_ee", "oshun_collide_flm", "oshun_step_implicit_1d"]
+       "oshun_collide_ee", "oshun_collide_flm", "oshun_step_implicit_1d", "oshun_set_sources", "oshun_laser_source",
+       "oshun_inverse_bremsstrahlung"]
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree absent (the link needs its sources)")
@@ -52,6 +54,8 @@ def test_shim_links_with_the_reference_translation_units():
                      ("Runge_Kutta_2_Imp::advance_p1()", "oshun_step_implicit_1d"),
                      ("Explicit_EC_FP::loop(double const&)", "oshun_collide_ee"),
                      ("Euler_Backward::advance_1(double const&)", "oshun_collide_flm"),
-                     ("Euler_Backward::advance_lm(double const&)", "oshun_collide_flm")):
+                     ("Euler_Backward::advance_lm(double const&)", "oshun_collide_flm"),
+                     ("PLaserSource::Laser_Source(double const&)", "oshun_laser_source"),
+                     ("InverseBremsstrahlung::loop(double const&)", "oshun_inverse_bremsstrahlung")):
         body = dis.split(f"<{fn}>:\n", 1)[1].split("\n\n", 1)[0]
         assert f"<{call}@plt>" in body, fn
