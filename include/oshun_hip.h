@@ -117,6 +117,30 @@ int oshun_step(oshun_ctx* ctx, double h, int nsteps);
  * (def-Actions.cpp:118-132).  1D runs with m0 = 0, 3 <= l0 <= 12, FP64;
  * otherwise OSHUN_ERR_ARG. */
 int oshun_step_implicit_1d(oshun_ctx* ctx, double h, int nsteps, int if_implicitES);
+/* nsteps x the same loop without if_implicit1D, the variant of the production
+ * decks (def-Input.cpp:87-95 of each POP deck): Runge_Kutta_2_Imp::advance_p1 (Adv1:
+ * advection, and B rotation + Faraday unless if_implicitES), e-e, sources,
+ * Implicit_E_Field::advance (def-ImplicitE.cpp:472-606: J0 and the Ex, Ey, Ez
+ * probes through Implicit_Ex / Implicit_Eyz, def-VlasovMax.cpp:157-291, each
+ * followed by Euler_Backward::advance_1; JN from Ampere or, if_implicitES, the
+ * current set below; the 3x3 conductivity solve by Cramer's rule,
+ * def-Nmethods.cpp:215-251, retried up to 4 times on zero determinants),
+ * advance_p2 (Exyz + CLEAN_LOW_PC), advance_1/advance_lm.  l0 >= 3, m0 >= 2,
+ * Np <= 512, FP64; otherwise OSHUN_ERR_ARG.  More than 8 cells with a zero
+ * determinant after the last attempt, where the reference stops the run
+ * (:597-600), return OSHUN_ERR_STATE after the step's field solve. */
+int oshun_step_implicit_2d(oshun_ctx* ctx, double h, int nsteps, int if_implicitES);
+/* JN of the electrostatic 2D variant.  With if_implicitES the reference skips
+ * Ampere and JN keeps what Current_xyz's constructor gave it: a copy of the Ex
+ * field at the solver's construction, in all three components
+ * (def-ImplicitE.cpp:56-58, 446).  jn: 3 complex fields (x, y, z) laid out
+ * like three of oshun_upload_state's fields, guards of width nbc included;
+ * the real parts are used.  Zero until set. */
+int oshun_set_implicit_es_current(oshun_ctx* ctx, const double* jn, int nbc);
+/* Cells whose conductivity determinant was still zero after the last
+ * oshun_step_implicit_2d step's final attempt (their field is set to 0, as
+ * the reference does; it prints a warning). */
+int oshun_implicit_zero_dets(const oshun_ctx* ctx);
 
 /* ---- Sources (def-PLSource.cpp) -------------------------------------------
  * The deck's laser parameters (Inputdata input_list, decl-input.h:104-109). */

@@ -121,6 +121,9 @@ def _load():
         lib.oo_step.argtypes = [P, D, I]
         lib.oo_step_implicit_1d.argtypes = [P, D, I, I]
         lib.oo_step_implicit_1d.restype = I
+        lib.oo_step_implicit_2d.argtypes = [P, D, I, I]
+        lib.oo_step_implicit_2d.restype = I
+        lib.oo_set_es_current.argtypes = [P, dp]
         lib.oo_ee_getslope.argtypes = [P, dp, dp]
         lib.oo_flm_solve.argtypes = [P, dp, D, I, dp]
         lib.oo_flm_coeffs.argtypes = [P, dp, D, dp, dp, dp, dp]
@@ -202,6 +205,22 @@ class Oracle:
         """main.cpp:79-107 with implicit_E and if_implicit1D (RK2, implicit Ex); l0 >= 3."""
         if self._lib.oo_step_implicit_1d(self._h, float(h), int(nsteps), int(es)) != 0:
             raise ValueError("the 1D implicit-E step needs l0 >= 3")
+
+    def step_implicit_2d(self, h, nsteps=1, es=0):
+        """main.cpp:79-107 with implicit_E and not if_implicit1D (RK2, 3x3 implicit field
+        solve); l0 >= 3, m0 >= 2.  Returns the largest number of zero-determinant cells
+        left in a step; raises where the reference exits (more than 8)."""
+        z = self._lib.oo_step_implicit_2d(self._h, float(h), int(nsteps), int(es))
+        if z < 0:
+            raise ValueError("the 2D implicit-E step needs l0 >= 3 and m0 >= 2")
+        if z > 8:
+            raise FloatingPointError(f"implicit field: zero determinant in {z} cells")
+        return z
+
+    def set_es_current(self, jn):
+        """JN of the electrostatic 2D implicit field: complex [3][ny][nx]."""
+        jn = np.ascontiguousarray(jn, np.complex128).reshape(3, self.cfg.ny, self.cfg.nx)
+        self._lib.oo_set_es_current(self._h, _dptr(jn.view(np.float64)))
 
     def ee_getslope(self, f00):
         f00 = np.ascontiguousarray(f00, np.float64)
@@ -338,7 +357,7 @@ def write_ref_input(path, l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=
 def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
             ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, binary=None, src=None):
     """Run the reference driver; returns dict(dx, dy, h, t, p, f, fld).
-    op 5 (implicit-E step) takes es = if_implicitES.  binary: another build of
+    op 5 and 9 (implicit-E step, 1D and 2D variant) take es = if_implicitES.  binary: another build of
     the driver (SHIM_BIN: the same driver with the GPU shim linked in)."""
     binary = binary or REF_BIN
     if not os.path.exists(binary):

@@ -343,8 +343,23 @@ struct Oracle {
     else if (m >= 2) out.push_back({l - 1, m - 1, 2, eC4[l2i(l, m)], false});
   }
 
-  void exyz(const State& Yin, State& Yh) const {
+  // probe = 0: ElectricField::Exyz.  probe = 1..3: the implicit field's
+  // probe couplings, i.e. the terms of Exyz with sources l <= 3 and targets
+  // l <= 2 --
+  //   1  Implicit_Ex (def-VlasovMax.cpp:157-196): Ex only, sources m <= 1;
+  //   2  Implicit_Eyz with Em = Ep = Ey (ImpEy, def-Actions.cpp:187-201);
+  //   3  Implicit_Eyz with Em = -i Ez, Ep = i Ez (ImpEz, :207-223).
+  // Implicit_Eyz (def-VlasovMax.cpp:231-291) is exactly the Em/Ep terms of
+  // Exyz inside those bounds, the (3,3) -> (2,2) term only when m0 > 2.
+  // Multipliers are running products as in the reference (:359-484): each of
+  // the three field matrices Ex, Em, Ep is scaled by the first coefficient it
+  // meets and then by the ratio of each coefficient to the one before it on
+  // the same matrix (the "C4(l0,1) = B2[l0]" of :418 is that previous
+  // coefficient), so the multipliers round as the reference's do.
+  void exyz(const State& Yin, State& Yh, int probe = 0) const {
     std::vector<cd> G(np), Hh(np);
+    const int lsrc = probe ? std::min(3, c.l0) : c.l0;
+    const int ltgt = probe ? 2 : c.l0;
     for (int iy = 0; iy < ny; ++iy)
       for (int ix = 0; ix < nx; ++ix) {
         size_t ixy = (size_t)iy * nx + ix;
@@ -353,31 +368,40 @@ struct Oracle {
         cd ii(0.0, 1.0);
         cd Em = Ez; Em *= (-1.0) * ii; Em += Ey;   // def-VlasovMax.cpp:352
         cd Ep = Ez; Ep *= ii; Ep += Ey;            // :353
-        cd fl[3] = {Ex, Em, Ep};
+        if (probe == 2) { Em = Ey; Ep = Ey; }
+        if (probe == 3) { Em = Ez; Em *= (-1.0) * ii; Ep = Ez; Ep *= ii; }
+        cd cur[3] = {Ex, Em, Ep}, prev[3];
+        bool started[3] = {false, false, false};
+        auto keep = [&](const ETerm& t, int m) {
+          if (!probe) return true;
+          if (t.tl > ltgt) return false;
+          if (probe == 1) return t.field == 0 && m <= 1;
+          return t.field != 0;
+        };
+        auto apply = [&](const ETerm& t, const std::vector<cd>& V) {
+          const int s = t.field;
+          if (!started[s]) cur[s] *= t.coef;
+          else cur[s] *= t.coef / prev[s];
+          started[s] = true;
+          prev[s] = t.coef;
+          cd* dst = Yh.h(H(t.tl, t.tm)) + at(0, ix, iy);
+          for (int ip = 0; ip < np; ++ip) {
+            cd v = V[ip] * cur[s];
+            dst[ip] += t.re ? cd(v.real(), 0.0) : v;
+          }
+        };
         for (int m = 0; m <= c.m0; ++m)
-          for (int l = m; l <= c.l0; ++l) {
+          for (int l = m; l <= lsrc; ++l) {
             const cd* fcol = Yin.h(H(l, m)) + at(0, ix, iy);
             std::vector<ETerm> tg, th;
             if (l == 0) g00_col(fcol, G.data());
             else make_gh_col(l, fcol, G.data(), Hh.data());
             if (l >= 1) e_targets_H(l, m, th);
             e_targets_G(l, m, tg);
-            for (const ETerm& t : th) {
-              cd mult = fl[t.field] * t.coef;
-              cd* dst = Yh.h(H(t.tl, t.tm)) + at(0, ix, iy);
-              for (int ip = 0; ip < np; ++ip) {
-                cd v = Hh[ip] * mult;
-                dst[ip] += t.re ? cd(v.real(), 0.0) : v;
-              }
-            }
-            for (const ETerm& t : tg) {
-              cd mult = fl[t.field] * t.coef;
-              cd* dst = Yh.h(H(t.tl, t.tm)) + at(0, ix, iy);
-              for (int ip = 0; ip < np; ++ip) {
-                cd v = G[ip] * mult;
-                dst[ip] += t.re ? cd(v.real(), 0.0) : v;
-              }
-            }
+            for (const ETerm& t : th)
+              if (keep(t, m)) apply(t, Hh);
+            for (const ETerm& t : tg)
+              if (keep(t, m)) apply(t, G);
           }
       }
   }
@@ -1379,6 +1403,8 @@ struct Oracle {
     t_rk += h;
     double t = t_rk;
     if (c.coll_ee) collide_ee(t); else t_ee = t;
+    if (src.linear || src.poly) laser_source(t);  // main.cpp:90-92
+    if (src.ib == 1) inverse_bremsstrahlung(t);
     implicit_e_1d(h);
     rk2([&](const State& a, State& b) { adv2_1d(a, b); }, h);
     if (c.coll_flm) {
@@ -1386,6 +1412,154 @@ struct Oracle {
       collide_flm(t - t_eb, 2);
     }
     t_eb = t;
+  }
+
+  // ------------------------------------------- implicit E, 2D variant ----
+  // Current_xyz::calculate_J (def-ImplicitE.cpp:125-180): x from f10, y and z
+  // from f11 (Jy = Re, Jz = Re(i J) of the -8pi/3 sum), p0 and pmax at
+  // weight 1 on interior cells as in the x component above.
+  void current_imp(const State& Yin, std::vector<cd>& jx, std::vector<cd>& jy, std::vector<cd>& jz) const {
+    current_imp_x(Yin, jx);
+    jy.assign(nxy, cd(0.0, 0.0));
+    jz.assign(nxy, cd(0.0, 0.0));
+    const cd c01(0.0, 1.0);
+    for (int iy = 0; iy < ny; ++iy)
+      for (int ix = 0; ix < nx; ++ix) {
+        const cd* f = Yin.h(H(1, 1)) + at(0, ix, iy);
+        cd j = f[0] * p3og[0];
+        j += f[np - 1] * p3og[np - 1];
+        for (int ip = 1; ip < np - 1; ++ip) j += f[ip] * p3og[ip];
+        j *= Delta_p;
+        j += small * (f[1] * p3og[1]);
+        j *= (-8.0) * M_PI / 3.0;
+        const size_t k = (size_t)iy * nx + ix;
+        jy[k] = cd(j.real(), 0.0);
+        j *= c01;
+        jz[k] = cd(j.real(), 0.0);
+      }
+  }
+  // JN of Implicit_E_Field::Ampere (def-ImplicitE.cpp:652-678): the same
+  // differences as MaxwellEq::Ampere, assigned instead of added
+  void ampere_jn(std::vector<cd>* jn) const {
+    State z;
+    z.init(0, 0, nxy);
+    for (int k = 3; k < 6; ++k) std::copy(Y.F(k), Y.F(k) + nxy, z.F(k));
+    ampere(z, z);
+    for (int k = 0; k < 3; ++k) jn[k].assign(z.F(k), z.F(k) + nxy);
+  }
+  // JN when if_implicitES: Ampere is skipped and JN keeps what Current_xyz's
+  // constructor gave it, a copy of the Ex field at the solver's construction
+  // in all three components (def-ImplicitE.cpp:56-58, 446); the caller sets it.
+  std::vector<cd> es_jn[3];
+
+  // Implicit_E_Field::advance, 2D variant (def-ImplicitE.cpp:472-606):
+  // J0 = current after Euler_Backward::advance_1 with the fields untouched;
+  // J_Ex, J_Ey, J_Ez = current after an RK2 of the probe couplings (exyz with
+  // probe 1, 2, 3) with Ex = DE.Ex, Ey = DE.Ey, Ez = DE.Ez in turn and
+  // advance_1; f00, f1m, f2m restored after each.  Then per cell the 3x3
+  // conductivity sgm(i,j) = (J_Ej - J0)_i / DE_j and E = sgm^-1 (JN - J0) by
+  // Cramer's rule (Det33, Detx33, Dety33, Detz33: def-Nmethods.cpp:215-251).
+  // Cells whose |Re det| <= 6 DBL_MIN get E = 0, DE.Ex *= 2, DE.Ey *= 1.3,
+  // and the whole probe sequence runs again, at most 4 attempts.  Returns the
+  // number of such cells left after the last attempt (the reference warns,
+  // and exits when there are more than 8).
+  int implicit_e_2d(double h, bool es) {
+    const double Eps = 16.0 * std::numeric_limits<double>::epsilon();
+    const double LargeEps = std::sqrt(Eps);
+    std::vector<cd> DE[3];
+    for (int k = 0; k < 3; ++k) DE[k].assign(nxy, cd(0.0, 0.0));
+    for (int k = 0; k < nxy; ++k) {  // FindDE, :628-650
+      cd ex = Y.F(0)[k], ey = Y.F(1)[k], ez = Y.F(2)[k];
+      ex *= ex; ey *= ey; ez *= ez;
+      ex += ey; ex += ez;
+      ex = std::sqrt(ex);
+      ex *= LargeEps;
+      ex += Eps;
+      DE[0][k] = DE[1][k] = DE[2][k] = ex;
+    }
+    std::vector<int> saved;
+    for (int l = 0; l <= 2; ++l)
+      for (int m = 0; m <= std::min(l, c.m0); ++m) saved.push_back(H(l, m));
+    std::vector<std::vector<cd>> keep;
+    for (int i : saved) keep.emplace_back(Y.h(i), Y.h(i) + n);
+    auto restore = [&](int lmax) {
+      for (size_t k = 0; k < saved.size(); ++k) {
+        bool in = false;
+        for (int l = 0; l <= lmax; ++l)
+          for (int m = 0; m <= std::min(l, c.m0); ++m) in = in || (saved[k] == H(l, m));
+        if (in) std::copy(keep[k].begin(), keep[k].end(), Y.h(saved[k]));
+      }
+    };
+    const double Dt = t_rk - t_eb;
+    std::vector<cd> J0[3], JE[3][3], JN[3], EN[3];
+    for (int k = 0; k < 3; ++k) EN[k].assign(nxy, cd(0.0, 0.0));
+    int zeros = 1, attempt = 0;
+    while (zeros > 0 && attempt < 4) {
+      zeros = 0;
+      ++attempt;
+      if (c.coll_flm) collide_flm(Dt, 1);
+      current_imp(Y, J0[0], J0[1], J0[2]);
+      restore(1);
+      for (int d = 0; d < 3; ++d) {
+        std::copy(DE[d].begin(), DE[d].end(), Y.F(d));
+        rk2([&](const State& a, State& b) { zero_state(b); exyz(a, b, d + 1); }, h);
+        if (c.coll_flm) collide_flm(Dt, 1);
+        current_imp(Y, JE[d][0], JE[d][1], JE[d][2]);
+        restore(2);
+      }
+      if (!es) ampere_jn(JN);
+      else for (int k = 0; k < 3; ++k) JN[k] = es_jn[k].empty() ? std::vector<cd>(nxy, cd(0.0, 0.0)) : es_jn[k];
+      for (int k = 0; k < nxy; ++k) {
+        cd A[3][3], D[3];
+        for (int i = 0; i < 3; ++i) {
+          for (int j = 0; j < 3; ++j) A[i][j] = (JE[j][i][k] - J0[i][k]) / DE[j][k];
+          D[i] = JN[i][k] - J0[i][k];
+        }
+        const cd det = A[0][0] * (A[1][1] * A[2][2] - A[2][1] * A[1][2]) -
+                       A[1][0] * (A[0][1] * A[2][2] - A[2][1] * A[0][2]) +
+                       A[2][0] * (A[0][1] * A[1][2] - A[1][1] * A[0][2]);
+        if (std::abs(det.real()) > 6.0 * std::numeric_limits<double>::min()) {
+          EN[0][k] = (D[0] * (A[1][1] * A[2][2] - A[2][1] * A[1][2]) -
+                      D[1] * (A[0][1] * A[2][2] - A[2][1] * A[0][2]) +
+                      D[2] * (A[0][1] * A[1][2] - A[1][1] * A[0][2])) / det;
+          EN[1][k] = (A[0][0] * (D[1] * A[2][2] - D[2] * A[1][2]) -
+                      A[1][0] * (D[0] * A[2][2] - D[2] * A[0][2]) +
+                      A[2][0] * (D[0] * A[1][2] - D[1] * A[0][2])) / det;
+          EN[2][k] = (A[0][0] * (A[1][1] * D[2] - A[2][1] * D[1]) -
+                      A[1][0] * (A[0][1] * D[2] - A[2][1] * D[0]) +
+                      A[2][0] * (A[0][1] * D[1] - A[1][1] * D[0])) / det;
+        } else {
+          ++zeros;
+          DE[0][k] *= 2;
+          DE[1][k] *= 1.3;
+          EN[0][k] = EN[1][k] = EN[2][k] = 0.0;
+        }
+      }
+    }
+    for (int k = 0; k < 3; ++k) std::copy(EN[k].begin(), EN[k].end(), Y.F(k));
+    return zeros;
+  }
+  // Implicit_Actions::Adv2 without if_implicit1D (def-Actions.cpp:138-160)
+  void adv2_2d(const State& Yin, State& Yh) const {
+    zero_state(Yh);
+    exyz(Yin, Yh);
+    clean_low_pc(Yh);
+  }
+  int step_implicit_2d(double h, bool es) {
+    rk2([&](const State& a, State& b) { adv1(a, b, es); }, h);
+    t_rk += h;
+    double t = t_rk;
+    if (c.coll_ee) collide_ee(t); else t_ee = t;
+    if (src.linear || src.poly) laser_source(t);  // main.cpp:90-92
+    if (src.ib == 1) inverse_bremsstrahlung(t);
+    const int zeros = implicit_e_2d(h, es);
+    rk2([&](const State& a, State& b) { adv2_2d(a, b); }, h);
+    if (c.coll_flm) {
+      collide_flm(t - t_eb, 1);
+      collide_flm(t - t_eb, 2);
+    }
+    t_eb = t;
+    return zeros;
   }
 };
 
@@ -1761,6 +1935,22 @@ int oo_step_implicit_1d(void* o, double h, int nsteps, int es) {
   if (q->c.l0 < 3) return -1;
   for (int s = 0; s < nsteps; ++s) q->step_implicit_1d(h, es != 0);
   return 0;
+}
+// nsteps implicit-E steps of the 2D variant (l0 >= 3, m0 >= 2); es =
+// if_implicitES.  Returns the largest count of zero-determinant cells left in
+// a step, or -1 for a configuration the variant does not run.
+int oo_step_implicit_2d(void* o, double h, int nsteps, int es) {
+  Oracle* q = static_cast<Oracle*>(o);
+  if (q->c.l0 < 3 || q->c.m0 < 2) return -1;
+  int worst = 0;
+  for (int s = 0; s < nsteps; ++s) worst = std::max(worst, q->step_implicit_2d(h, es != 0));
+  return worst;
+}
+// JN of the electrostatic 2D variant: jn = complex [3][ny][nx] (x, y, z)
+void oo_set_es_current(void* o, const double* jn) {
+  Oracle* q = static_cast<Oracle*>(o);
+  const cd* j = reinterpret_cast<const cd*>(jn);
+  for (int k = 0; k < 3; ++k) q->es_jn[k].assign(j + (size_t)k * q->nxy, j + (size_t)(k + 1) * q->nxy);
 }
 // Tangent-linear model of nsteps steps of a 1D m0 = 0 run from t = 0.
 // f, df: real [l][x][p]; fld, dfld: real [6][x]; outputs the final state and

@@ -34,7 +34,9 @@
 //            5 = nsteps x {RK2 advance_p1, e-e, implicit E, RK2 advance_p2,
 //                f1, flm, halo} with implicit_E and if_implicit1D on
 //                (main.cpp:79-107 with Runge_Kutta_2_Imp and Implicit_E_Field)
-//            op 1 also runs PLaserSource::Laser_Source and InverseBremsstrahlung::loop
+//            9 = the same with if_implicit1D off (the 2D variant: Implicit_Ex,
+//                Implicit_Eyz probes and the 3x3 conductivity solve)
+//            ops 1, 5 and 9 also run PLaserSource::Laser_Source and InverseBremsstrahlung::loop
 //            after the e-e collisions when the deck turns them on (main.cpp:90-92)
 //            6 = nsteps x {PLaserSource::Laser_Source}   (tnew advanced by h each time)
 //            7 = nsteps x {InverseBremsstrahlung::loop}  (same)
@@ -137,7 +139,8 @@ int main(int argc, char** argv) {
   L.n_outsteps = 1; L.t_stop = dv[5];  // dt_out = h
   L.small_dt = dv[6];
   L.NB_algorithms = nb;
-  L.if_tridiagonal = true; L.implicit_E = (op == 5); L.if_implicit1D = (op == 5); L.if_implicitES = (op == 5 && es);
+  L.if_tridiagonal = true; L.implicit_E = (op == 5 || op == 9); L.if_implicit1D = (op == 5);
+  L.if_implicitES = ((op == 5 || op == 9) && es);
   L.density_np = dv[7]; L.Zeta = dv[8];
   L.bndX = 0; L.bndY = 0;
   L.linear_Ivst = sv[0] != 0; L.polynomial_Ivst = sv[1] != 0;
@@ -213,11 +216,13 @@ int main(int argc, char** argv) {
     }
     t_final = t;
     Out = Y;
-  } else if (op == 5) {
+  } else if (op == 5 || op == 9) {
     Runge_Kutta_2_Imp rk(Y, 0);
     Explicit_EC_FP ee(Y, 0);
     Euler_Backward eb(Y, 0);
     Electric_Field_Methods::Implicit_E_Field eim(Y, 0);
+    PLaserSource PLS(Y, 0);
+    InverseBremsstrahlung IBs(Y, 0);
     for (int s = 0; s < nsteps; ++s) {
       rk.advance_p1();
       // Neighbor_ImplicitE_Communications moves the B guards only
@@ -229,6 +234,8 @@ int main(int argc, char** argv) {
         for (int k = 3; k < 6; ++k) Y.FLD(k) = G.FLD(k);
       }
       ee.loop(rk.time());
+      if (PLS.PLSOURCE()) PLS.Laser_Source(rk.time());
+      if (IBs.IBSOURCE() == 1) IBs.loop(rk.time());
       eim.advance(&rk, &eb);
       rk.advance_p2();
       eb.advance_1(rk.time());

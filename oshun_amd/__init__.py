@@ -121,6 +121,9 @@ def _load():
     lib.oshun_collide_flm.argtypes = [P, D, I]
     lib.oshun_step.argtypes = [P, D, I]
     lib.oshun_step_implicit_1d.argtypes = [P, D, I, I]
+    lib.oshun_step_implicit_2d.argtypes = [P, D, I, I]
+    lib.oshun_set_implicit_es_current.argtypes = [P, dp, I]
+    lib.oshun_implicit_zero_dets.argtypes = [P]
     lib.oshun_synchronize.argtypes = [P]
     lib.oshun_get_stream.argtypes = [P, ctypes.POINTER(P)]
     lib.oshun_device_state.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
@@ -300,6 +303,19 @@ class Context:
         """main.cpp:79-107 with implicit_E and if_implicit1D (Runge_Kutta_2_Imp +
         Implicit_E_Field); 1D, m0 = 0, l0 >= 3, FP64."""
         self._chk(self._lib.oshun_step_implicit_1d(self._h, float(h), int(nsteps), int(es)))
+
+    def step_implicit_2d(self, h, nsteps=1, es=0):
+        """main.cpp:79-107 with implicit_E, not if_implicit1D (Runge_Kutta_2_Imp +
+        Implicit_E_Field with the 3x3 conductivity solve); l0 >= 3, m0 >= 2, FP64.
+        Returns the cells left with a zero determinant in the last step."""
+        self._chk(self._lib.oshun_step_implicit_2d(self._h, float(h), int(nsteps), int(es)))
+        return self._lib.oshun_implicit_zero_dets(self._h)
+
+    def set_implicit_es_current(self, jn, nbc=0):
+        """JN of the electrostatic 2D implicit field: complex [3][ny+2nbc][nx+2nbc]."""
+        shp = self.fld_shape(nbc)
+        jn = np.ascontiguousarray(jn, np.complex128).reshape((3,) + tuple(shp[1:]))
+        self._chk(self._lib.oshun_set_implicit_es_current(self._h, _dptr(jn.view(np.float64)), int(nbc)))
 
     def synchronize(self):
         self._chk(self._lib.oshun_synchronize(self._h))

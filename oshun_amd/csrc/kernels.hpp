@@ -49,7 +49,7 @@ struct StageArgs {
   T c05h, hh, h6, third, twothird;
   int np, nx, ny, nc;
   long long plane;
-  int mode;  // 0: slope, 1..4: RK4 stage
+  int mode;  // 0: slope, 1..4: RK4 stage, 5..6: RK2 stage (vlasov_stage only)
   int c10, c11r, c11i;
 };
 
@@ -147,6 +147,16 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage(StageArgs<T> a) {
           const T kh = acc * a.hh;
           a.out[o] = a.y0[o] + kh;
           a.acc[o] = a.acc[o] + kh * a.third;
+        } break;
+        case 5: {  // Runge_Kutta_2_Imp stage 1: Y0 = Y + h k, Y += (h k) / 2   (def-RK.cpp:478-482)
+          const T kh = acc * a.hh;
+          const T y0 = a.y0[o];
+          a.out[o] = y0 + kh;
+          a.acc[o] = y0 + kh * T(0.5);
+        } break;
+        case 6: {  // stage 2: Y += (h / 2) k   (:486-488)
+          const T kh = acc * a.c05h;
+          a.out[o] = a.acc[o] + kh;
         } break;
         default: {
           const T kh = acc * a.h6;

@@ -36,6 +36,13 @@ struct ProfRec {
   hipEvent_t e0, e1;
 };
 
+// a coupling program on the device (terms, per-plane starts, CLEAN_LOW_PC rows)
+struct DevProg {
+  Term* terms = nullptr;
+  int* tstart = nullptr;
+  int* clean = nullptr;
+};
+
 struct oshun_ctx {
   oshun_grid g;
   oshun_phys ph;
@@ -102,6 +109,15 @@ struct oshun_ctx {
   double* d_src = nullptr;   // per-p and per-cell source tables
   double* d_ibw = nullptr;   // inverse bremsstrahlung work planes (first use)
   int* d_srcflag = nullptr;  // a heating rate out of range
+  // implicit-E step, 2D variant (implicit2d.hpp), set up by its first call:
+  // the action programs (Adv1 with B, Adv1 electrostatic, Adv2, probes) and
+  // d_imp = probe fields [3][6][nxy] | currents J0, J_Ex, J_Ey, J_Ez [4][3][nxy]
+  // | electrostatic JN [3][nxy]; d_imp_zero counts zero determinants
+  DevProg iprog[4];
+  double* d_imp = nullptr;
+  int* d_imp_zero = nullptr;
+  int* h_imp_zero = nullptr;  // pinned
+  int imp_zero_last = 0;
   hipStream_t stream = nullptr;
   int device = 0;  // the HIP device current at oshun_create
   double t_rk = 0.0, t_ee = 0.0, t_eb = 0.0;
@@ -239,10 +255,12 @@ int launch_stage_1d(oshun_ctx* c, int mode, const void* in, const void* y0, void
   return OSHUN_OK;
 }
 
+// prog: a coupling program other than the full one (term-list interpreter,
+// the first nc planes only)
 template <typename T>
 int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* out, void* acc, const double* fin,
-                 double h) {
-  if (c->use1d) return launch_stage_1d<T>(c, mode, in, y0, out, acc, fin, h);
+                 double h, const DevProg* prog = nullptr, int nc = 0) {
+  if (c->use1d && !prog) return launch_stage_1d<T>(c, mode, in, y0, out, acc, fin, h);
   oshun::StageArgs<T> a;
   a.in = static_cast<const T*>(in);
   a.y0 = static_cast<const T*>(y0);
@@ -274,11 +292,17 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
   a.c11r = c->T.m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, c->T.m0)] : -1;
   a.c11i = c->T.m0 >= 1 ? c->T.comp_im[oshun::hind(1, 1, c->T.m0)] : -1;
   a.coef = c->d_coef;
+  if (prog) {
+    a.terms = prog->terms;
+    a.tstart = prog->tstart;
+    a.clean = prog->clean;
+    a.nc = nc;
+  }
   dim3 block(oshun::kTileX, oshun::kTileP, 1);
   dim3 grid((c->nx + oshun::kTileX - 1) / oshun::kTileX, c->npb, c->ny);
   ProfRec r;
   prof_begin(c, 0, &r);
-  if (c->spec >= 0 && !c->force_general) {
+  if (c->spec >= 0 && !c->force_general && !prog) {
     switch (c->spec) {
 #define OSHUN_SPEC_LAUNCH(ID, L, M, TD)                                                        \
       case ID:                                                                                 \
@@ -771,9 +795,13 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_ibw, c->d_srcflag};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_ibw, c->d_srcflag, c->d_imp, c->d_imp_zero};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  for (DevProg& q : c->iprog)
+    for (void* p : {(void*)q.terms, (void*)q.tstart, (void*)q.clean})
+      if (p) hipFree(p);
+  if (c->h_imp_zero) hipHostFree(c->h_imp_zero);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
   c->prec.clear();
   if (c->stream) hipStreamDestroy(c->stream);
@@ -1390,3 +1418,4 @@ int oshun_diag_copy(oshun_ctx* c, long long nbytes) {
 #include "sources.hpp"
 // implicit-E step, 1D variant (oshun_step_implicit_1d)
 #include "implicit.hpp"
+#include "implicit2d.hpp"

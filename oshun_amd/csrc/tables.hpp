@@ -397,4 +397,42 @@ inline void build_tables(HostTables& T, int l0, int m0, int np, int ny, double p
   T.g_den = 1.0 - (v[0] * v[0]) / (v[1] * v[1]);
 }
 
+// Coupling programs of the implicit-E step's actions, taken from the full
+// program above (same per-target term order, same coefficients):
+//   kProgAdv1    Implicit_Actions::Adv1 (def-Actions.cpp:118-132): x- and
+//                y-advection, and the B-field rotation unless es; no
+//                CLEAN_LOW_PC
+//   kProgAdv2    Adv2 without if_implicit1D (:138-160): the E-field
+//                couplings of Exyz, with CLEAN_LOW_PC
+//   kProgProbe   Implicit_Ex and Implicit_Eyz (def-VlasovMax.cpp:157-196,
+//                231-291): the E-field couplings with sources l <= 3 and
+//                targets l <= 2, Ex only from sources m <= 1.  Run with one
+//                field component nonzero (Ex, Ey or Ez), it is ImpEx, ImpEy
+//                (Em = Ep = Ey) or ImpEz (Em = -i Ez, Ep = i Ez).
+enum { kProgAdv1 = 0, kProgAdv2 = 1, kProgProbe = 2 };
+struct Program {
+  std::vector<Term> terms;
+  std::vector<int> tstart, clean;
+};
+inline Program derive_program(const HostTables& T, int kind, bool es) {
+  Program P;
+  P.tstart.assign(T.nc + 1, 0);
+  P.clean.assign(T.nc, 0);
+  for (int tc = 0; tc < T.nc; ++tc) {
+    P.tstart[tc] = static_cast<int>(P.terms.size());
+    for (int k = T.tstart[tc]; k < T.tstart[tc + 1]; ++k) {
+      const Term& t = T.terms[k];
+      const bool efield = t.op == OP_G || t.op == OP_H || t.op == OP_G00;
+      bool keep;
+      if (kind == kProgAdv1) keep = t.op == OP_DX || t.op == OP_DY || (t.op == OP_ID && !es);
+      else if (kind == kProgAdv2) keep = efield;
+      else keep = efield && T.comp_l[t.src] <= 3 && T.comp_l[tc] <= 2 && (t.mul != MUL_EX || T.comp_m[t.src] <= 1);
+      if (keep) P.terms.push_back(t);
+    }
+    if (kind == kProgAdv2) P.clean[tc] = T.clean[tc];
+  }
+  P.tstart[T.nc] = static_cast<int>(P.terms.size());
+  return P;
+}
+
 }  // namespace oshun

@@ -29,11 +29,14 @@
 // with the deck's parameters and axes.  Errors print and exit(1), as the
 // reference does.
 //
-// The implicit-E decks the library runs are the 1D variant (if_implicit1D,
-// m = 0 harmonics, Ey = Ez = B = 0): the device context then holds the m = 0
-// harmonics only, and the whole step (RK2 halves, e-e, field solve, implicit
-// collisions) is oshun_step_implicit_1d inside Runge_Kutta_2_Imp::advance_p1;
-// the later calls of the step only move their clocks.
+// Implicit-E decks: the whole step (RK2 halves, e-e, sources, field solve,
+// implicit collisions) is one device call inside Runge_Kutta_2_Imp::
+// advance_p1 -- oshun_step_implicit_1d for the 1D variant (if_implicit1D: the
+// device context then holds the m = 0 harmonics only), oshun_step_implicit_2d
+// otherwise; the later calls of the step only move their clocks.  With
+// if_implicitES the 2D field solve's JN is the Ex the field solver was built
+// with (def-ImplicitE.cpp:56-58, 446): the RK2 constructor, which main.cpp
+// runs just before the solver's, keeps a copy of it for the device.
 
 #include <cstdlib>
 #include <iostream>
@@ -68,6 +71,7 @@ struct Session {
   bool rk_driven = false;
   size_t numh = 1;       // steps per output interval of the driver
   size_t steps = 0;      // steps since the last download
+  std::vector<double> es_ex;  // Ex at the implicit field solver's construction (2D, if_implicitES)
 };
 
 Session& session() {
@@ -135,6 +139,11 @@ void attach(Stat& Y, double t) {
   pointers(Y, sh, fld);
   check(oshun_upload_harmonics(s.ctx, sh.data(), fld.data(), nbc), "oshun_upload_harmonics");
   check(oshun_set_time(s.ctx, t), "oshun_set_time");
+  if (!s.es_ex.empty()) {
+    std::vector<double> jn;
+    for (int k = 0; k < 3; ++k) jn.insert(jn.end(), s.es_ex.begin(), s.es_ex.end());
+    check(oshun_set_implicit_es_current(s.ctx, jn.data(), nbc), "oshun_set_implicit_es_current");
+  }
   // the deck's laser sources, with the cell positions of its own axes (def-Input.cpp:313-325)
   const Inputdata::input_list& L = IN().list();
   if (L.linear_Ivst || L.polynomial_Ivst || L.inverse_bremsstrahlung == 1) {
@@ -193,7 +202,7 @@ Runge_Kutta_4& Runge_Kutta_4::advance_p1() {
   return *this;
 }
 
-// ---- Runge_Kutta_2_Imp (def-RK.cpp:400-524), 1D implicit-E variant ------------
+// ---- Runge_Kutta_2_Imp (def-RK.cpp:400-524) --------------------------------------
 Runge_Kutta_2_Imp::Runge_Kutta_2_Imp(Stat& Yin, int tout_start)
     : Y(Yin), Y0(Yin), Yh(Yin), Act(Yh),
       t(static_cast<double>(tout_start) * Inputdata::IN().cont().dt_out),
@@ -204,14 +213,25 @@ Runge_Kutta_2_Imp::Runge_Kutta_2_Imp(Stat& Yin, int tout_start)
   s.implicit = true;
   s.rk_driven = true;
   s.numh = num_h;
+  const Inputdata::input_list& L = Inputdata::IN().list();
+  if (!L.if_implicit1D && L.if_implicitES) {
+    const Field2D& ex = Yin.EMF().Ex();
+    const double* p = reinterpret_cast<const double*>(&Yin.FLD(0)(0, 0));
+    s.es_ex.assign(p, p + 2 * ex.numx() * ex.numy());
+  }
 }
 
 Runge_Kutta_2_Imp& Runge_Kutta_2_Imp::advance_p1() {
-  if (!Inputdata::IN().list().if_implicit1D)
-    die("Runge_Kutta_2_Imp: the library runs the 1D implicit-E variant (if_implicit1D) only");
+  const Inputdata::input_list& L = Inputdata::IN().list();
   attach(Y, t);
-  check(oshun_step_implicit_1d(session().ctx, h, 1, Inputdata::IN().list().if_implicitES ? 1 : 0),
-        "Runge_Kutta_2_Imp::advance_p1");
+  Session& s = session();
+  if (L.if_implicit1D) {
+    check(oshun_step_implicit_1d(s.ctx, h, 1, L.if_implicitES ? 1 : 0), "Runge_Kutta_2_Imp::advance_p1");
+  } else {
+    check(oshun_step_implicit_2d(s.ctx, h, 1, L.if_implicitES ? 1 : 0), "Runge_Kutta_2_Imp::advance_p1");
+    const int zeros = oshun_implicit_zero_dets(s.ctx);
+    if (zeros > 0) std::cout << "WARNING, Det = 0 in " << zeros << "locations \n";  // def-ImplicitE.cpp:597-598
+  }
   t += h;
   return *this;
 }
@@ -219,7 +239,7 @@ Runge_Kutta_2_Imp& Runge_Kutta_2_Imp::advance_p1() {
 Runge_Kutta_2_Imp& Runge_Kutta_2_Imp::advance_p2() { return *this; }  // done in advance_p1
 
 void Electric_Field_Methods::Implicit_E_Field::advance(Explicit_Method*, Euler_Backward*) {
-  // the field solve is part of oshun_step_implicit_1d (advance_p1)
+  // the field solve is part of oshun_step_implicit_1d/_2d (advance_p1)
 }
 
 // ---- collisions (def-FokkerPlanck.cpp:590-625, 1270-1304) ---------------------

@@ -30,6 +30,13 @@ Cases (the reference needs l0 >= 3 and m0 >= 2, def-Input.cpp:73):
   efield_only    3 ExternalEfield::Laser_Efield calls (Ex polarisation), 2D
   step_src       2 full steps with the laser source and inverse
                  bremsstrahlung on (main.cpp:79-107 with :90-92), 2D
+  imp_2d_m2      2 implicit-E steps without if_implicit1D (the 3x3 field
+                 solve, Ampere JN), l0=3 m0=2, 2D, E and B nonzero
+  imp_2d_m3_es   2 such steps, l0=4 m0=3, if_implicitES (JN = the input Ex)
+  imp_2d_l7_quirk 1 step, l0=7 m0=3 (the y-advection slip inside Adv1)
+  imp_2d_dilute  1 step with two cells diluted by 1e-150 (zero determinant:
+                 4 attempts, E = 0 there)
+  imp_2d_src     2 steps with the laser source and inverse bremsstrahlung
 
 `python tests/golden/make_golden.py name ...` writes only the named fixtures.
 """
@@ -129,9 +136,63 @@ def source_cases(only):
         save(name, dict(c, **{"src_" + k: v for k, v in src.items()}), f, fld, r)
 
 
+def implicit_2d_state(c, seed):
+    """Perturbed Maxwellian with all three E components and B nonzero."""
+    f, fld = maxwellian_state(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], c["pmax"], seed, bz=2e-3)
+    x = np.arange(c["nx"]) / c["nx"]
+    y = np.arange(c["ny"]) / c["ny"]
+    fld[3] = 1e-3 * np.cos(2 * np.pi * y)[:, None] * np.ones((1, c["nx"]))
+    fld[4] = -1.5e-3 * np.sin(2 * np.pi * x)[None, :] * np.ones((c["ny"], 1))
+    return f, fld
+
+
+def implicit_2d_cases(only):
+    """Implicit-E steps without if_implicit1D (harness op 9): the 3x3 field solve."""
+    base = dict(pmax=0.25, xmin=0.0, xmax=60.0, ymin=0.0, ymax=40.0, op=9, nb=4, h=0.5, small_dt=0.2,
+                density_np=1e21, zeta=1.0)
+    cases = (
+        ("imp_2d_m2", dict(l0=3, m0=2, np=24, nx=6, ny=4, nsteps=2, es=0), 51, None, None),
+        # electrostatic: JN is the Ex the solver was built with (the input's)
+        ("imp_2d_m3_es", dict(l0=4, m0=3, np=32, nx=5, ny=4, nsteps=2, es=1), 52, None, None),
+        # l0 = 7, m0 = 3: the y-advection coefficient slip inside Adv1
+        ("imp_2d_l7_quirk", dict(l0=7, m0=3, np=20, nx=4, ny=5, nsteps=1, es=0), 53, None, None),
+        # a 5 x 6 block of cells diluted by 1e-150: the two cells two cells
+        # inside it stay dilute through Adv1's two advection stages, their
+        # determinant underflows to zero, the probes are retried 4 times and
+        # those cells get E = 0.  They lie 4 or more cells from the edges, so
+        # that no guard cell holds a copy of them (the reference counts zero
+        # determinants on guard cells too and stops the run above 8)
+        ("imp_2d_dilute", dict(l0=3, m0=2, np=16, nx=11, ny=9, nsteps=1, es=0), 54,
+         ((slice(2, 7), slice(3, 9)), ((4, 5), (4, 6))), None),
+        # sources after the e-e collisions (polynomial laser heating, inverse bremsstrahlung)
+        ("imp_2d_src", dict(l0=3, m0=2, np=48, nx=6, ny=3, nsteps=2, es=0, pmax=0.3, xmax=50.0, ymax=30.0), 55,
+         None, dict(SRC, inverse_bremsstrahlung=1)),
+    )
+    for name, cc, seed, dilute, src in cases:
+        if only and name not in only:
+            continue
+        c = dict(base, **cc)
+        f, fld = implicit_2d_state(c, seed)
+        if dilute:
+            f[:, dilute[0][0], dilute[0][1], :] *= 1e-150
+        r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], f, fld, op=9, nsteps=c["nsteps"],
+                    pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"], h=c["h"],
+                    small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"], es=c["es"], src=src)
+        assert np.all(np.isfinite(r["f"])) and np.all(np.isfinite(r["fld"])), name
+        extra = {"src_" + k: v for k, v in src.items()} if src else {}
+        if dilute:
+            ys, xs = dilute[0]
+            extra["dilute_block"] = np.array([ys.start, ys.stop, xs.start, xs.stop])
+            extra["zero_cells"] = np.array(dilute[1])
+            zero = np.all(r["fld"][:3] == 0, axis=0)
+            assert sorted(zip(*np.nonzero(zero))) == sorted(dilute[1]), (name, np.nonzero(zero))
+        save(name, dict(c, **extra), f, fld, r)
+
+
 def main(only=()):
     if only:
         source_cases(only)
+        implicit_2d_cases(only)
         return
     # 1) RHS, 2D complex
     c = dict(l0=4, m0=3, np=16, nx=6, ny=4, pmax=0.3, xmin=-2.0, xmax=3.0, ymin=-1.0, ymax=2.0,
@@ -215,6 +276,7 @@ def main(only=()):
         save(name, c, f, fld, r, keep_m0_only=True, m0_phys=0)
 
     source_cases(())
+    implicit_2d_cases(())
 
 
 if __name__ == "__main__":

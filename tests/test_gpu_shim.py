@@ -9,9 +9,11 @@ translation units, so that every step runs on the GPU through the C ABI
   source and inverse bremsstrahlung on) and the collision-only drivers
   (ee_only, flm_only): the shim's output against the compiled reference's
   fixture at <= 1e-12 normwise in f and the fields.
-* implicit-E 1D decks (imp_1d_m0, imp_1d_m0_l8): bitwise equal to the same
-  step called directly through the C ABI (oshun_step_implicit_1d); that path
-  is held to the fixtures in test_gpu_implicit.py.
+* implicit-E decks, 1D variant (imp_1d_m0, imp_1d_m0_l8) and 2D variant
+  (imp_2d_m2; imp_2d_m3_es, whose JN is the Ex the shim's RK2 constructor
+  copied): bitwise equal to the same steps called directly through the C ABI
+  (oshun_step_implicit_1d/_2d); those paths are held to the fixtures in
+  test_gpu_implicit.py and test_gpu_implicit2d.py.
 Fixtures of m0 = 0 physics ran the reference as an m0 = 2 y-uniform stand-in;
 the shim gets the same stand-in deck and input.
 """
@@ -84,3 +86,19 @@ def test_implicit_e_through_the_shim_equals_the_abi(golden, name):
     fa, flda = ctx.download()
     assert np.array_equal(f, fa)
     assert np.array_equal(fld, flda[:fld.shape[0]])
+
+
+@pytest.mark.parametrize("name", ["imp_2d_m2", "imp_2d_m3_es"])
+def test_implicit_e_2d_through_the_shim_equals_the_abi(golden, name):
+    g = golden(name)
+    c = g["cfg"]
+    print(f"case shim {name}", flush=True)
+    f, fld = _run_shim(g)
+    ctx = ctx_from_golden(g)
+    ctx.upload(g["f_in"], g["fld_in"])
+    if int(c["es"]):
+        ctx.set_implicit_es_current(np.stack([g["fld_in"][0]] * 3))
+    ctx.step_implicit_2d(float(c["h"]), int(c["nsteps"]), int(c["es"]))
+    fa, flda = ctx.download()
+    assert np.array_equal(f, fa)
+    assert np.array_equal(fld, flda)
