@@ -449,6 +449,101 @@ def run_implicit(steps, warmup, no_cpu=False):
             "cpu_baseline": cpu}
 
 
+# the three production decks (def-Input.cpp:60-98 of POP-marocchino-secIII.B, POP-tzoufras-secIII.A,
+# POP-tzoufras-secIV.B): implicit_E without if_implicit1D, RKLevel 2; the x, y extents are the decks'
+# global grids, run here on one GPU (the decks spread them over 32, 64 and 128 MPI ranks)
+POP_DECKS = (
+    ("marocchino-secIII.B", dict(l0=48, m0=2, np=288, nx=640, ny=2, pmax=0.4, xmax=8574.383315, ymax=5000.0, es=1)),
+    ("tzoufras-secIII.A", dict(l0=31, m0=2, np=432, nx=192, ny=2, pmax=0.6, xmax=39880.82246, ymax=1000.0, es=1)),
+    ("tzoufras-secIV.B", dict(l0=16, m0=3, np=300, nx=96, ny=48, pmax=0.6, xmax=17835.246, ymax=44588.115, es=0)),
+)
+
+
+def pop_state(d, nx=None, seed=7):
+    """Maxwellian f00 (T = 1 keV, 5% density and 10% temperature ripples in x), 1e-3 random f_lm, small E and B"""
+    from oracle.oracle import hind, num_harmonics
+    l0, m0, np_, ny = d["l0"], d["m0"], d["np"], d["ny"]
+    nx = nx or d["nx"]
+    rng = np.random.default_rng(seed)
+    pmin = d["pmax"] / (2.0 * np_)
+    p = np.arange(np_) * ((d["pmax"] - pmin) / (np_ - 1)) + pmin
+    x = np.arange(nx) / nx
+    y = np.arange(ny) / ny
+    T = (1.0 + 0.1 * np.cos(2 * np.pi * x))[None, :] * np.ones((ny, 1))
+    pt = np.sqrt(T / 511.0)
+    n = (1.0 + 0.05 * np.sin(2 * np.pi * x))[None, :] * np.ones((ny, 1))
+    f = np.zeros((num_harmonics(l0, m0), ny, nx, np_), np.complex128)
+    f[0] = (n / (np.sqrt(2 * np.pi) * 2 * np.pi * pt ** 3))[..., None] * np.exp(-p ** 2 / (2 * pt[..., None] ** 2))
+    for l in range(1, l0 + 1):
+        for m in range(min(l, m0) + 1):
+            r = rng.uniform(-1, 1, (ny, nx, np_)) + (1j * rng.uniform(-1, 1, (ny, nx, np_)) if m else 0.0)
+            f[hind(l, m, m0)] = 1e-3 * r * f[0].real
+    fld = np.zeros((6, ny, nx), np.complex128)
+    fld[0] = 1e-4 * np.sin(2 * np.pi * x)[None, :]
+    fld[1] = 2e-5 * np.cos(2 * np.pi * x)[None, :]
+    fld[2] = -1e-5 * np.sin(2 * np.pi * y)[:, None]
+    fld[5] = 2e-3
+    return f, fld
+
+
+def run_pop(steps, warmup, no_cpu=False, h=1.0):
+    """The implicit-E step without if_implicit1D (oshun_step_implicit_2d) on the three production decks'
+    grids, FP64; the reference's op-9 driver (oracle/_ref) timed on a 4-cell x slab of each."""
+    import oshun_amd
+    from oracle import oracle as orc
+    out = []
+    for name, d in POP_DECKS:
+        dx = d["xmax"] / (d["nx"] - 1)
+        dy = d["ymax"] / (d["ny"] - 1)
+        f, fld = pop_state(d)
+        ctx = oshun_amd.Context(l0=d["l0"], m0=d["m0"], nump=d["np"], numx=d["nx"], numy=d["ny"], pmax=d["pmax"],
+                                dx=dx, dy=dy, density_np=DENSITY, small_dt=0.5 * h, nb_algorithms=NB)
+        ctx.upload(f, fld)
+        ctx.step_implicit_2d(h, warmup, d["es"])
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.step_implicit_2d(h, steps, d["es"])
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ctx.profile(True)
+        ctx.step_implicit_2d(h, steps, d["es"])
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        fo, fldo = ctx.download()
+        if not (np.all(np.isfinite(fo)) and np.all(np.isfinite(fldo))):
+            raise RuntimeError(f"{name}: non-finite state after the timed steps")
+        nh = orc.num_harmonics(d["l0"], d["m0"])
+        cells = nh * d["np"] * d["nx"] * d["ny"]
+        rec = {"deck": name, "l0": d["l0"], "m0": d["m0"], "np": d["np"], "nx": d["nx"], "ny": d["ny"],
+               "es": d["es"], "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "cells": cells,
+               "state_bytes": ctx.state_bytes(),
+               "kernels": {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps}
+                           for k, v in prof.items() if v[1]}}
+        del ctx
+        if not no_cpu and orc.have_ref():
+            nxs = 4
+            fs, flds = pop_state(d, nx=nxs)
+            kw = dict(nb=NB, pmax=d["pmax"], xmin=0.0, xmax=dx * (nxs - 1), ymin=0.0, ymax=d["ymax"], h=h,
+                      small_dt=0.5 * h, density_np=DENSITY, es=d["es"])
+            t0 = time.perf_counter()
+            orc.run_ref(d["l0"], d["m0"], d["np"], nxs, d["ny"], fs, flds, op=9, nsteps=1, **kw)
+            t1 = time.perf_counter()
+            orc.run_ref(d["l0"], d["m0"], d["np"], nxs, d["ny"], fs, flds, op=9, nsteps=2, **kw)
+            t2 = time.perf_counter()
+            per_step = max((t2 - t1) - (t1 - t0), 1e-9)
+            sc = nh * d["np"] * nxs * d["ny"]
+            rec["cpu_baseline"] = {"value": sc / per_step, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                                   "sample": f"OSHUN-1 classes (oracle/_ref, op 9), the deck's l0, m0, Np and Ny on "
+                                             f"Nx = {nxs} (+ guards), a 2-step minus a 1-step run",
+                                   "s_per_step": per_step}
+        out.append(rec)
+        print(json.dumps(rec), file=sys.stderr, flush=True)
+    return {"metric": "phase-space cell-updates/s per implicit-E RK2+FP step (2D variant, FP64), production decks",
+            "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "dtype": "f64",
+            "data": "synthetic (Maxwellian with ripples and 1e-3 random harmonics on each deck's grid)",
+            "decks": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -466,6 +561,8 @@ def main():
     ap.add_argument("--config", default=None, help="with --configs: only the configuration of this name (C3)")
     ap.add_argument("--implicit", action="store_true",
                     help="the implicit-E step of the 1D variant instead of the RK4 step (FP64)")
+    ap.add_argument("--pop", action="store_true",
+                    help="the implicit-E step without if_implicit1D on the three production decks' grids (FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -478,6 +575,9 @@ def main():
         return
     if args.implicit:
         print(json.dumps(run_implicit(args.steps, args.warmup, args.no_cpu)))
+        return
+    if args.pop:
+        print(json.dumps(run_pop(args.steps, args.warmup, args.no_cpu)))
         return
     if args.batch:
         print(json.dumps(run_ensemble(args.steps, args.warmup)))
