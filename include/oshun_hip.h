@@ -56,6 +56,13 @@ typedef struct oshun_grid {
                         /* (matrices.h:860-866; decks use pmin = pmax/(2 nump))  */
   double dx, dy;        /* cell sizes (the reference's x.dx(), y.dx())            */
   int precision;        /* OSHUN_FP64 or OSHUN_FP32_SPLIT                         */
+  int bndx, bndy;       /* boundaries: 0 periodic, 1 mirror (bndX, bndY of the
+                           deck, def-Parallel.cpp:1121-1307)                     */
+  int nbc;              /* with a mirror boundary: the guard cells per side
+                           (RKLevel); the device then keeps the whole arrays,
+                           guards included, and refreshes the guards at the end
+                           of each oshun_step as Neighbor_Communications does;
+                           numx, numy >= nbc.  Ignored when both are periodic. */
 } oshun_grid;
 
 typedef struct oshun_phys {
@@ -85,7 +92,9 @@ int oshun_num_harmonics(const oshun_ctx* ctx);
 
 /* Stat <-> device.  One host pointer per harmonic and per field (the shape
  * of a reference Stat, decl-state.h:302-334); guards are skipped on upload
- * and refilled with periodic images on download. */
+ * and refilled with periodic images on download.  A mirror-boundary context
+ * takes nbc = its guard count (the whole arrays, guards copied as they are)
+ * or nbc = 0 (the interior; upload then refreshes the guards). */
 int oshun_upload_harmonics(oshun_ctx* ctx, const double* const* sh, const double* const* fld, int nbc);
 int oshun_download_harmonics(oshun_ctx* ctx, double* const* sh, double* const* fld, int nbc);
 /* Same with all harmonics (resp. all fields) contiguous in one buffer. */
@@ -107,8 +116,14 @@ int oshun_collide_ee(oshun_ctx* ctx, double tnew);
 /* Euler_Backward::advance_1 (which = 1: l = 1) and advance_lm (which = 2:
  * l >= 2, then t_flm = tnew). */
 int oshun_collide_flm(oshun_ctx* ctx, double tnew, int which);
-/* nsteps x main.cpp:79-107 (RK4, e-e, f1, flm; guards are implicit). */
+/* nsteps x main.cpp:79-107 (RK4, e-e, f1, flm, guard refresh; with periodic
+ * boundaries the guards are implicit). */
 int oshun_step(oshun_ctx* ctx, double h, int nsteps);
+/* Parallel_Environment::Neighbor_Communications of one node
+ * (def-Parallel.cpp:3040-3127 with sameNode_periodic_X/Y and
+ * sameNode_mirror_X/Y, :1121-1307) on a mirror-boundary context; oshun_step
+ * runs it at the end of every step.  OSHUN_ERR_ARG on periodic contexts. */
+int oshun_fill_guards(oshun_ctx* ctx);
 /* nsteps x main.cpp:79-107 with implicit_E and if_implicit1D (the RKLevel = 2
  * loop): Runge_Kutta_2_Imp::advance_p1 (def-RK.cpp:470-498), e-e,
  * Implicit_E_Field::advance (def-ImplicitE.cpp:472-606, 1D variant),

@@ -340,13 +340,15 @@ def deck_axis(n, vmin, vmax, guards=4):
 
 
 def write_ref_input(path, l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
-                    ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, src=None):
+                    ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, src=None,
+                    bnd=(0, 0)):
     """The driver's input file (layout in ref_harness.cpp's header comment)."""
     nh = num_harmonics(l0, m0)
     f = np.ascontiguousarray(f, np.complex128).reshape(nh, ny, nx, np_)
     fld = np.ascontiguousarray(fld, np.complex128).reshape(6, ny, nx)
     with open(path, "wb") as fh:
-        np.array([l0, m0, np_, nx, ny, op, nsteps, nb, es, 1 if src is not None else 0], np.int32).tofile(fh)
+        np.array([l0, m0, np_, nx, ny, op, nsteps, nb, es, 1 if src is not None else 0, int(bnd[0]), int(bnd[1])],
+                 np.int32).tofile(fh)
         np.array([pmax, xmin, xmax, ymin, ymax, h, small_dt, density_np, zeta], np.float64).tofile(fh)
         if src is not None:
             source_vector(src).tofile(fh)
@@ -355,10 +357,13 @@ def write_ref_input(path, l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=
 
 
 def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
-            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, binary=None, src=None):
+            ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, binary=None, src=None,
+            bnd=(0, 0), nbc=4):
     """Run the reference driver; returns dict(dx, dy, h, t, p, f, fld).
     op 5 and 9 (implicit-E step, 1D and 2D variant) take es = if_implicitES.  binary: another build of
-    the driver (SHIM_BIN: the same driver with the GPU shim linked in)."""
+    the driver (SHIM_BIN: the same driver with the GPU shim linked in).  bnd = (bndX, bndY): with a
+    mirror boundary the result also holds the whole arrays with their nbc guards, of the initial
+    state after its guard fill (f_ext0, fld_ext0) and of the final state (f_ext, fld_ext)."""
     binary = binary or REF_BIN
     if not os.path.exists(binary):
         raise RuntimeError(f"driver not built: {binary} (make -C oracle ref / shim)")
@@ -367,7 +372,7 @@ def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
         write_ref_input(fin, l0, m0, np_, nx, ny, f, fld, op, nsteps, nb, pmax, xmin, xmax, ymin, ymax, h,
-                        small_dt, density_np, zeta, es, src)
+                        small_dt, density_np, zeta, es, src, bnd)
         r = subprocess.run([binary, fin, fout], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"{os.path.basename(binary)} failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")
@@ -376,6 +381,15 @@ def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0
     p = raw[4:4 + np_]
     rest = raw[4 + np_:].view(np.complex128)
     nf = nh * ny * nx * np_
-    return dict(dx=hdr[0], dy=hdr[1], h=hdr[2], t=hdr[3], p=p,
-                f=rest[:nf].reshape(nh, ny, nx, np_).copy(),
-                fld=rest[nf:].reshape(6, ny, nx).copy())
+    out = dict(dx=hdr[0], dy=hdr[1], h=hdr[2], t=hdr[3], p=p,
+               f=rest[:nf].reshape(nh, ny, nx, np_).copy(),
+               fld=rest[nf:nf + 6 * ny * nx].reshape(6, ny, nx).copy())
+    if bnd[0] or bnd[1]:
+        nxg, nyg = nx + 2 * nbc, ny + 2 * nbc
+        ne, nfe = nh * nyg * nxg * np_, 6 * nyg * nxg
+        ext = rest[nf + 6 * ny * nx:]
+        for k, tag in enumerate(("0", "")):
+            blk = ext[k * (ne + nfe):(k + 1) * (ne + nfe)]
+            out["f_ext" + tag] = blk[:ne].reshape(nh, nyg, nxg, np_).copy()
+            out["fld_ext" + tag] = blk[ne:].reshape(6, nyg, nxg).copy()
+    return out

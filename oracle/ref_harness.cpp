@@ -19,7 +19,8 @@
 // and after each step, which is what the single-node periodic exchange does.
 //
 // Usage:  oshun_ref <in.bin> <out.bin>
-//   in.bin : int32[10] {l0,m0,np,nx,ny,op,nsteps,nb,es,src}  (es: if_implicitES, op 5)
+//   in.bin : int32[12] {l0,m0,np,nx,ny,op,nsteps,nb,es,src,bndX,bndY}  (es: if_implicitES, op 5/9;
+//            bndX/bndY: 0 periodic, 1 mirror guards, def-Parallel.cpp:1121-1307)
 //            double[9] {pmax,xmin,xmax,ymin,ymax,h,small_dt,density_np,zeta}
 //            if src: double[15] {linear_Ivst, polynomial_Ivst, inverse_bremsstrahlung,
 //                   polarization_direction, I_0, lambda_0, ab_eta, ab_depth, cntr_x0,
@@ -42,7 +43,10 @@
 //            7 = nsteps x {InverseBremsstrahlung::loop}  (same)
 //            8 = nsteps x {ExternalEfield::Laser_Efield} (same)
 //   out.bin: double[4] {dx_used, dy_used, h_used, t_final},
-//            double[np] p axis, then f and fields as in in.bin.
+//            double[np] p axis, then f and fields as in in.bin; with a mirror
+//            boundary also the whole arrays, guards included, of the initial
+//            state (after its first guard fill) and of the final state:
+//            complex<double>[nh][ny+2g][nx+2g][np], complex<double>[6][ny+2g][nx+2g].
 
 #include <cmath>
 #include <complex>
@@ -112,16 +116,126 @@ static void fill_guards_periodic(Stat& Y, int nbc) {
   }
 }
 
+// Guard fill of a single node with a mirror boundary on at least one side:
+// Node_Communications::sameNode_bound_X then sameNode_bound_Y (the order of
+// Parallel_Environment::Neighbor_Communications, def-Parallel.cpp:3040-3127).
+// Restated from sameNode_periodic_X/Y and sameNode_mirror_X/Y
+// (def-Parallel.cpp:1137-1307), which live with the MPI code and are not
+// built here.  As in the reference, the x mirror refreshes harmonics l < l0
+// only and the y mirror 1 <= l < l0 only; the others keep their guard values.
+// x mirror: guard at distance d from the face = interior cell at distance d,
+// times (-1)^(l+m); Ey, Ez, Bx change sign.  y mirror: the same copy, then
+// v -> (-1)^m conj(v) (as v -= 2 Re v; v *= -(-1)^m); Ex, Ez, By change sign.
+static void fill_guards_bnd(Stat& Y, int nbc, int bndX, int bndY) {
+  const int l0 = Y.DF().l0(), m0 = Y.DF().m0();
+  const int npx = Y.SH(0).nump(), nxg = Y.SH(0).numx(), nyg = Y.SH(0).numy();
+  // x direction, whole columns (all rows)
+  auto xcopy_h = [&](SHarmonic& s, int dst, int src, double sg) {
+    for (int iy = 0; iy < nyg; ++iy)
+      for (int ip = 0; ip < npx; ++ip) s(ip, dst, iy) = s(ip, src, iy) * sg;
+  };
+  auto xcopy_f = [&](Field2D& f, int dst, int src, double sg) {
+    for (int iy = 0; iy < nyg; ++iy) f(dst, iy) = f(src, iy) * sg;
+  };
+  for (int l = 0; l <= l0; ++l)
+    for (int m = 0; m <= std::min(l, m0); ++m) {
+      SHarmonic& s = Y.SH(l, m);
+      if (bndX == 0) {
+        for (int g = 0; g < nbc; ++g) {
+          xcopy_h(s, g, nxg - 2 * nbc + g, 1.0);  // right bound -> left guard
+          xcopy_h(s, nxg - nbc + g, nbc + g, 1.0);  // left bound -> right guard
+        }
+      } else if (l < l0) {
+        const double sg = ((l + m) % 2) ? -1.0 : 1.0;
+        for (int c = 0; c < nbc; ++c) xcopy_h(s, nxg - c - 1, nxg - 2 * nbc + c, sg);
+        for (int c = 0; c < nbc; ++c) xcopy_h(s, c, 2 * nbc - c - 1, sg);
+      }
+    }
+  for (int k = 0; k < 6; ++k) {
+    Field2D& f = Y.FLD(k);
+    const double sg = (bndX != 0 && (k == 1 || k == 2 || k == 3)) ? -1.0 : 1.0;
+    for (int g = 0; g < nbc; ++g) {
+      if (bndX == 0) {
+        xcopy_f(f, g, nxg - 2 * nbc + g, 1.0);
+        xcopy_f(f, nxg - nbc + g, nbc + g, 1.0);
+      } else {
+        xcopy_f(f, nxg - g - 1, nxg - 2 * nbc + g, sg);
+        xcopy_f(f, g, 2 * nbc - g - 1, sg);
+      }
+    }
+  }
+  // y direction, whole rows (all columns)
+  for (int l = 0; l <= l0; ++l)
+    for (int m = 0; m <= std::min(l, m0); ++m) {
+      SHarmonic& s = Y.SH(l, m);
+      auto ycopy = [&](int dst, int src, bool mir, double sg) {
+        for (int ix = 0; ix < nxg; ++ix)
+          for (int ip = 0; ip < npx; ++ip) {
+            cd v = s(ip, ix, src);
+            if (mir) {
+              const cd w = v;
+              v -= 2.0 * w.real();
+              v *= sg;
+            }
+            s(ip, ix, dst) = v;
+          }
+      };
+      if (bndY == 0) {
+        for (int g = 0; g < nbc; ++g) {
+          ycopy(g, nyg - 2 * nbc + g, false, 1.0);
+          ycopy(nyg - nbc + g, nbc + g, false, 1.0);
+        }
+      } else if (l >= 1 && l < l0) {
+        const double sg = -((m % 2) ? -1.0 : 1.0);
+        for (int c = 0; c < nbc; ++c) ycopy(nyg - c - 1, nyg - 2 * nbc + c, true, sg);
+        for (int c = 0; c < nbc; ++c) ycopy(c, 2 * nbc - c - 1, true, sg);
+      }
+    }
+  for (int k = 0; k < 6; ++k) {
+    Field2D& f = Y.FLD(k);
+    const double sg = (bndY != 0 && (k == 0 || k == 2 || k == 4)) ? -1.0 : 1.0;
+    auto ycopy = [&](int dst, int src) {
+      for (int ix = 0; ix < nxg; ++ix) f(ix, dst) = f(ix, src) * sg;
+    };
+    for (int g = 0; g < nbc; ++g) {
+      if (bndY == 0) {
+        for (int ix = 0; ix < nxg; ++ix) {
+          f(ix, g) = f(ix, nyg - 2 * nbc + g);
+          f(ix, nyg - nbc + g) = f(ix, nbc + g);
+        }
+      } else {
+        ycopy(nyg - g - 1, nyg - 2 * nbc + g);
+        ycopy(g, 2 * nbc - g - 1);
+      }
+    }
+  }
+}
+
+static void write_ext(FILE* fo, Stat& Y) {
+  const int nh = Y.DF().dim();
+  for (int h = 0; h < nh; ++h) {
+    const SHarmonic& s = Y.SH(h);
+    const size_t n = (size_t)s.nump() * s.numx() * s.numy();
+    std::fwrite(&Y.SH(h)(0, 0, 0), sizeof(cd), n, fo);
+  }
+  for (int k = 0; k < 6; ++k) {
+    const size_t n = (size_t)Y.FLD(k).numx() * Y.FLD(k).numy();
+    std::fwrite(&Y.FLD(k)(0, 0), sizeof(cd), n, fo);
+  }
+}
+
 int main(int argc, char** argv) {
   if (argc != 3) die("usage: oshun_ref in.bin out.bin");
   FILE* fi = std::fopen(argv[1], "rb");
   if (!fi) die("cannot open input");
-  int iv[10];
+  int iv[12];
   double dv[9];
-  if (std::fread(iv, sizeof(int), 10, fi) != 10) die("short header");
+  if (std::fread(iv, sizeof(int), 12, fi) != 12) die("short header");
   if (std::fread(dv, sizeof(double), 9, fi) != 9) die("short header");
   const int l0 = iv[0], m0 = iv[1], np = iv[2], nx = iv[3], ny = iv[4], op = iv[5], nsteps = iv[6], nb = iv[7];
   const int es = iv[8];
+  const int bndX = iv[10], bndY = iv[11];
+  const bool mirror = bndX != 0 || bndY != 0;
   double sv[15] = {0};
   if (iv[9] && std::fread(sv, sizeof(double), 15, fi) != 15) die("short source block");
   const int nh = ((m0 + 1) * (2 * l0 - m0 + 2)) / 2;
@@ -142,7 +256,7 @@ int main(int argc, char** argv) {
   L.if_tridiagonal = true; L.implicit_E = (op == 5 || op == 9); L.if_implicit1D = (op == 5);
   L.if_implicitES = ((op == 5 || op == 9) && es);
   L.density_np = dv[7]; L.Zeta = dv[8];
-  L.bndX = 0; L.bndY = 0;
+  L.bndX = bndX; L.bndY = bndY;
   L.linear_Ivst = sv[0] != 0; L.polynomial_Ivst = sv[1] != 0;
   L.inverse_bremsstrahlung = static_cast<int>(sv[2]); L.polarization_direction = static_cast<int>(sv[3]);
   L.I_0 = sv[4]; L.lambda_0 = sv[5]; L.ab_eta = sv[6]; L.ab_depth = sv[7]; L.cntr_x0 = sv[8];
@@ -172,7 +286,12 @@ int main(int argc, char** argv) {
   for (int k = 0; k < 6; ++k)
     for (int iy = 0; iy < ny; ++iy)
       for (int ix = 0; ix < nx; ++ix) Y.FLD(k)(ix + nbc, iy + nbc) = fb[((size_t)k * ny + iy) * nx + ix];
-  fill_guards_periodic(Y, nbc);
+  auto fill = [&](Stat& S) {
+    if (mirror) fill_guards_bnd(S, nbc, bndX, bndY);
+    else fill_guards_periodic(S, nbc);
+  };
+  fill(Y);
+  Stat Y0(Y);  // the initial state with its guards
 
   double t_final = 0.0, h_used = dv[5];
   Stat Out(Y);
@@ -196,7 +315,7 @@ int main(int argc, char** argv) {
         eb.advance_1(rk.time());
         eb.advance_lm(rk.time());
       }
-      fill_guards_periodic(Y, nbc);
+      fill(Y);
     }
     t_final = rk.time();
     Out = Y;
@@ -212,7 +331,7 @@ int main(int argc, char** argv) {
         eb.advance_1(t);
         eb.advance_lm(t);
       }
-      fill_guards_periodic(Y, nbc);
+      fill(Y);
     }
     t_final = t;
     Out = Y;
@@ -230,7 +349,7 @@ int main(int argc, char** argv) {
       // full exchange below (the operators in between are x-local)
       {
         Stat G(Y);
-        fill_guards_periodic(G, nbc);
+        fill(G);
         for (int k = 3; k < 6; ++k) Y.FLD(k) = G.FLD(k);
       }
       ee.loop(rk.time());
@@ -240,7 +359,7 @@ int main(int argc, char** argv) {
       rk.advance_p2();
       eb.advance_1(rk.time());
       eb.advance_lm(rk.time());
-      fill_guards_periodic(Y, nbc);
+      fill(Y);
     }
     t_final = rk.time();
     Out = Y;
@@ -254,7 +373,7 @@ int main(int argc, char** argv) {
       if (op == 6) PLS.Laser_Source(t);
       else if (op == 7) IBs.loop(t);
       else EFLD.Laser_Efield(t);
-      fill_guards_periodic(Y, nbc);
+      fill(Y);
     }
     t_final = t;
     Out = Y;
@@ -278,6 +397,10 @@ int main(int argc, char** argv) {
       for (int ix = 0; ix < nx; ++ix) fb[((size_t)k * ny + iy) * nx + ix] = Out.FLD(k)(ix + nbc, iy + nbc);
   std::fwrite(buf.data(), sizeof(cd), buf.size(), fo);
   std::fwrite(fb.data(), sizeof(cd), fb.size(), fo);
+  if (mirror) {
+    write_ext(fo, Y0);
+    write_ext(fo, Out);
+  }
   std::fclose(fo);
   return 0;
 }

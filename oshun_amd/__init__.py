@@ -64,7 +64,8 @@ def deck_pmin(pmax, nump):
 class Grid(ctypes.Structure):
     _fields_ = [("l0", ctypes.c_int), ("m0", ctypes.c_int), ("nump", ctypes.c_int), ("numx", ctypes.c_int),
                 ("numy", ctypes.c_int), ("pmin", ctypes.c_double), ("pmax", ctypes.c_double),
-                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("precision", ctypes.c_int)]
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("precision", ctypes.c_int),
+                ("bndx", ctypes.c_int), ("bndy", ctypes.c_int), ("nbc", ctypes.c_int)]
 
 
 class Phys(ctypes.Structure):
@@ -124,6 +125,7 @@ def _load():
     lib.oshun_step_implicit_2d.argtypes = [P, D, I, I]
     lib.oshun_set_implicit_es_current.argtypes = [P, dp, I]
     lib.oshun_implicit_zero_dets.argtypes = [P]
+    lib.oshun_fill_guards.argtypes = [P]
     lib.oshun_synchronize.argtypes = [P]
     lib.oshun_get_stream.argtypes = [P, ctypes.POINTER(P)]
     lib.oshun_device_state.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
@@ -168,16 +170,18 @@ class Context:
 
     def __init__(self, l0, m0, nump, numx, numy=1, pmax=0.3, pmin=None, dx=1.0, dy=1.0, precision=FP64,
                  density_np=1e21, zeta=1.0, small_dt=1.0, nb_algorithms=4, collisions=COLL_EE | COLL_FLM,
-                 ref_quirks=1):
+                 ref_quirks=1, bndx=0, bndy=0, nbc=0):
+        """bndx, bndy: 0 periodic, 1 mirror; with a mirror the device keeps nbc guard cells per side
+        (the deck's RKLevel) and upload/download take nbc = that number (whole arrays) or 0."""
         lib = _load()
         if pmin is None:
             pmin = deck_pmin(pmax, nump)
-        self.grid = Grid(l0, m0, nump, numx, numy, pmin, pmax, dx, dy, precision)
+        self.grid = Grid(l0, m0, nump, numx, numy, pmin, pmax, dx, dy, precision, bndx, bndy, nbc)
         self.phys = Phys(density_np, zeta, small_dt, nb_algorithms, collisions, ref_quirks)
         h = ctypes.c_void_p()
         rc = lib.oshun_create(ctypes.byref(self.grid), ctypes.byref(self.phys), ctypes.byref(h))
         if rc != 0:
-            raise OshunError(f"oshun_create failed ({rc}) for {l0=} {m0=} {nump=} {numx=} {numy=}")
+            raise OshunError(f"oshun_create failed ({rc}) for {l0=} {m0=} {nump=} {numx=} {numy=} {bndx=} {bndy=} {nbc=}")
         self._h = h
         self._lib = lib
         self._owned = True
@@ -316,6 +320,10 @@ class Context:
         shp = self.fld_shape(nbc)
         jn = np.ascontiguousarray(jn, np.complex128).reshape((3,) + tuple(shp[1:]))
         self._chk(self._lib.oshun_set_implicit_es_current(self._h, _dptr(jn.view(np.float64)), int(nbc)))
+
+    def fill_guards(self):
+        """Parallel_Environment::Neighbor_Communications of one node (mirror-boundary contexts)."""
+        self._chk(self._lib.oshun_fill_guards(self._h))
 
     def synchronize(self):
         self._chk(self._lib.oshun_synchronize(self._h))

@@ -256,6 +256,7 @@ extern "C" {
 
 int oshun_step_implicit_1d(oshun_ctx* c, double h, int nsteps, int if_implicitES) {
   if (!sane(c) || nsteps < 0 || !(h > 0.0)) return OSHUN_ERR_ARG;
+  if (c->ext) return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_1d: periodic boundaries only");
   if (!c->use1d || c->fp32 || c->T.l0 < 3 || c->T.l0 > 12)
     return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_1d: needs a 1D run with m0 = 0, 3 <= l0 <= 12, FP64");
   if (!c->use_pair || !c->rows_npl)

@@ -374,6 +374,7 @@ extern "C" {
 
 int oshun_step_implicit_2d(oshun_ctx* c, double h, int nsteps, int if_implicitES) {
   if (!sane(c) || nsteps < 0 || !(h > 0.0)) return OSHUN_ERR_ARG;
+  if (c->ext) return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_2d: periodic boundaries only");
   if (c->fp32 || c->T.l0 < 3 || c->T.m0 < 2)
     return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_2d: needs l0 >= 3, m0 >= 2, FP64");
   if (c->np > oshun::kImpRows)

@@ -51,17 +51,46 @@ struct StageArgs {
   long long plane;
   int mode;  // 0: slope, 1..4: RK4 stage, 5..6: RK2 stage (vlasov_stage only)
   int c10, c11r, c11i;
+  // 1: the planes hold the reference's whole arrays, guard cells included,
+  // and x/y differences follow its flattened-array rule (ext_offsets)
+  int ext = 0;
 };
+
+// x and y neighbours of a point of the whole (guard-including) arrays as
+// Matrix3D::Dd2 / Dd3 (matrices.h:811-829) and Matrix2D::Dd1 / Dd2 (:511-533)
+// take them: minus central differences over the flattened array, so that
+//   x = 0 of row y > 0 pairs with x = nx-1 of row y-1, x = nx-1 of row y < ny-1
+//   with x = 0 of row y+1; the very first point takes f(x=0) - f(x=2) and the
+//   very last keeps f itself (no subtraction);
+//   y = 0 takes f(y=0) - f(y=2), y = ny-1 keeps f.
+// Offsets are in units of the x stride 1 and the row stride ys; *last set
+// means "no subtrahend" (the minus-neighbour offset is then 0: f itself).
+__host__ __device__ __forceinline__ void ext_offsets(int x, int y, int nx, int ny, long long ys, long long* xm,
+                                                     long long* xp, long long* ym, long long* yp, bool* xlast,
+                                                     bool* ylast) {
+  *xlast = false;
+  *ylast = false;
+  if (x > 0) *xm = -1;
+  else if (y > 0) *xm = -ys + (nx - 1);
+  else *xm = 0;
+  if (x == 0 && y == 0) *xp = 2;
+  else if (x < nx - 1) *xp = 1;
+  else if (y < ny - 1) *xp = ys - (nx - 1);
+  else { *xm = 0; *xp = 0; *xlast = true; }
+  if (y == 0) { *ym = 0; *yp = 2 * ys; }
+  else if (y < ny - 1) { *ym = -ys; *yp = ys; }
+  else { *ym = 0; *yp = 0; *ylast = true; }
+}
 
 template <typename T>
 __device__ __forceinline__ T op_value(const Term& tm, const T* __restrict__ s, long long idx, int p, int np, int nx,
                                       long long xm_off, long long xp_off, long long ym_off, long long yp_off,
-                                      const StageArgs<T>& a) {
+                                      bool xlast, bool ylast, const StageArgs<T>& a) {
   switch (tm.op) {
     case OP_DX:
-      return s[idx + xm_off] - s[idx + xp_off];
+      return s[idx + xm_off] - (xlast ? T(0) : s[idx + xp_off]);
     case OP_DY:
-      return s[idx + ym_off] - s[idx + yp_off];
+      return s[idx + ym_off] - (ylast ? T(0) : s[idx + yp_off]);
     case OP_ID:
       return s[idx];
     default: {
@@ -96,11 +125,13 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage(StageArgs<T> a) {
   double j10 = 0.0, j11r = 0.0, j11i = 0.0;
   if (x < nx && p < np) {
     const long long idx = ((long long)y * np + p) * nx + x;
-    const long long xm_off = (x == 0) ? (nx - 1) : -1;
-    const long long xp_off = (x == nx - 1) ? -(long long)(nx - 1) : 1;
     const long long ystride = (long long)np * nx;
-    const long long ym_off = (y == 0) ? (long long)(ny - 1) * ystride : -ystride;
-    const long long yp_off = (y == ny - 1) ? -(long long)(ny - 1) * ystride : ystride;
+    long long xm_off = (x == 0) ? (nx - 1) : -1;
+    long long xp_off = (x == nx - 1) ? -(long long)(nx - 1) : 1;
+    long long ym_off = (y == 0) ? (long long)(ny - 1) * ystride : -ystride;
+    long long yp_off = (y == ny - 1) ? -(long long)(ny - 1) * ystride : ystride;
+    bool xlast = false, ylast = false;
+    if (a.ext) ext_offsets(x, y, nx, ny, ystride, &xm_off, &xp_off, &ym_off, &yp_off, &xlast, &ylast);
     const int ixy = y * nx + x;
     double F[6];
 #pragma unroll
@@ -112,7 +143,7 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage(StageArgs<T> a) {
       for (int t = a.tstart[tc]; t < t1; ++t) {
         const Term tm = a.terms[t];
         const T* s = a.in + (long long)tm.src * a.plane;
-        T val = op_value<T>(tm, s, idx, p, np, nx, xm_off, xp_off, ym_off, yp_off, a);
+        T val = op_value<T>(tm, s, idx, p, np, nx, xm_off, xp_off, ym_off, yp_off, xlast, ylast, a);
         if (tm.op == OP_G00 && a.bgG) val += a.bgG[p];  // G00 is linear: G00(bg + df) = G00(bg) + G00(df)
         T mult;
         switch (tm.mul) {  // wave-uniform; keeps F[] in registers
@@ -669,6 +700,7 @@ struct FieldArgs {
   long long plane;
   double p3og1, small, delta_p, midx, midy;
   double c05h, hh, h6, third, twothird;
+  int ext = 0;  // whole arrays with guards: flattened-array differences (ext_offsets)
 };
 
 __device__ __forceinline__ void rk_combine(int mode, double k, double y0, double* out, double* acc, double c05h,
@@ -803,8 +835,15 @@ __device__ __forceinline__ void field_body(const FieldArgs<T>& a, const int bloc
   }
   if (i >= nxy || row != 0) return;
   const int x = i % nx, y = i / nx;
-  const int xm = y * nx + (x == 0 ? nx - 1 : x - 1), xp = y * nx + (x == nx - 1 ? 0 : x + 1);
-  const int ym = (y == 0 ? ny - 1 : y - 1) * nx + x, yp = (y == ny - 1 ? 0 : y + 1) * nx + x;
+  int xm = y * nx + (x == 0 ? nx - 1 : x - 1), xp = y * nx + (x == nx - 1 ? 0 : x + 1);
+  int ym = (y == 0 ? ny - 1 : y - 1) * nx + x, yp = (y == ny - 1 ? 0 : y + 1) * nx + x;
+  bool xlast = false, ylast = false;
+  if (a.ext) {
+    long long oxm, oxp, oym, oyp;
+    ext_offsets(x, y, nx, ny, nx, &oxm, &oxp, &oym, &oyp, &xlast, &ylast);
+    xm = i + static_cast<int>(oxm); xp = i + static_cast<int>(oxp);
+    ym = i + static_cast<int>(oym); yp = i + static_cast<int>(oyp);
+  }
   const double* Ex = a.fin;
   const double* Ey = a.fin + nxy;
   const double* Ez = a.fin + 2 * nxy;
@@ -835,6 +874,7 @@ __device__ __forceinline__ void field_body(const FieldArgs<T>& a, const int bloc
   maxwell_terms(
       [&](int c, int at) {
         const double* F = (c == 0) ? Ex : (c == 1) ? Ey : (c == 2) ? Ez : (c == 3) ? Bx : (c == 4) ? By : Bz;
+        if ((at == 1 && xlast) || (at == 3 && ylast)) return 0.0;
         return F[at == 0 ? xm : at == 1 ? xp : at == 2 ? ym : yp];
       },
       a.midx, a.midy, k);
@@ -1024,6 +1064,9 @@ struct CollArgs {
   CollTables t;
   const int* comp_l;   // l of each plane
   int nc, np, nx, ny, nb, l0;
+  // the columns collided: x in [cx0, cx0 + cnx), y in [cy0, cy0 + cny) (the
+  // interior when the planes keep guard cells); nx is the row stride
+  int cx0 = 0, cy0 = 0, cnx = 0, cny = 0;
   long long plane;
   int do_ee, do_f1, do_flm, numh;
   double h_ee, dt_flm;
@@ -1423,11 +1466,12 @@ constexpr int kCollCols = 8;  // x columns (one wavefront each) per workgroup
 template <typename T, int NPL>
 __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
   const int nx = a.nx, np = a.np;
-  const int ntx = (nx + kCollCols - 1) / kCollCols;
+  const int ntx = (a.cnx + kCollCols - 1) / kCollCols;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int y = tile / ntx;
-  const int x = (tile % ntx) * kCollCols + (threadIdx.x >> 6);
-  if (x >= nx) return;  // whole wavefront leaves together
+  const int y = tile / ntx + a.cy0;
+  const int xw = (tile % ntx) * kCollCols + (threadIdx.x >> 6);
+  if (xw >= a.cnx) return;  // whole wavefront leaves together
+  const int x = xw + a.cx0;
   const int lane = lane_id();
   const long long base = (long long)y * np * nx + x;
   EETab<NPL> tb;
@@ -1961,9 +2005,9 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   for (int k = 0; k < kTbPer; ++k) tbv[k] = a.rt[threadIdx.x + 256 * k];
   const double bgv = (a.bg && static_cast<int>(threadIdx.x) < a.np) ? a.bg[threadIdx.x] : 0.0;
   const int nx = a.nx, np = a.np;
-  const int ntx = (nx + kCols - 1) / kCols;
-  const int y = tile / ntx;
-  const int x0 = (tile % ntx) * kCols;
+  const int ntx = (a.cnx + kCols - 1) / kCols;
+  const int y = tile / ntx + a.cy0;
+  const int x0 = (tile % ntx) * kCols + a.cx0;
   // fused field stages: their inputs are fetched now, while the collisions run
   // (stage-3 fields at x0-2 .. x0+kCols+1, accumulator at x0-1 .. x0+kCols,
   // f10(p1) and the current partials of the stage-4 input at the columns)
@@ -1999,7 +2043,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   const int lrow = (threadIdx.x / SEGW) * kS + p0 + seg;  // LDS index of lane-layout row i: lrow + i
   // row-order layout: thread t moves column cc = t % kCols, rows r0 + k*SEGW (k < NPL)
   const int cc = threadIdx.x % kCols, r0 = threadIdx.x / kCols;
-  const bool ccv = x0 + cc < nx;
+  const bool ccv = x0 + cc < a.cx0 + a.cnx;
   const int crow = cc * kS + r0 + r0 / NPL;  // LDS index of row-order element k: crow + k*(SEGW + SEGW/NPL)
   constexpr int kStepL = SEGW + SEGW / NPL;
   // byte offsets within a plane; rows past np and idle columns read 0 / drop their stores

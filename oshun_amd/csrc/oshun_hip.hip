@@ -49,6 +49,10 @@ struct oshun_ctx {
   HostTables T;
   int nc = 0, np = 0, nx = 0, ny = 0, nxy = 0, npb = 0;
   long long plane = 0;
+  // mirror boundaries (ext): nx, ny, plane above count the whole arrays,
+  // gn guard cells on each side; nxi, nyi the interior (= nx, ny otherwise)
+  bool ext = false;
+  int gn = 0, nxi = 0, nyi = 0;
   bool fp32 = false;
   size_t esz = 8;
   void *S = nullptr, *A = nullptr, *B1 = nullptr, *B2 = nullptr;
@@ -67,6 +71,8 @@ struct oshun_ctx {
   int* d_tstart = nullptr;
   int* d_clean = nullptr;
   int* d_comp_l = nullptr;
+  int* d_comp_m = nullptr;     // m of each plane (guard refresh)
+  int* d_comp_isim = nullptr;  // 1 on imaginary planes (guard refresh)
   void* d_vt = nullptr;  // Vlasov tables in storage precision: invpax | vtab | cg | hp0
   double* d_p3og = nullptr;
   double* d_e1d = nullptr;  // 1D m0 = 0 coefficient rows: eA1 | eA2 | xA1 | xA2
@@ -292,6 +298,7 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
   a.c11r = c->T.m0 >= 1 ? c->T.comp_re[oshun::hind(1, 1, c->T.m0)] : -1;
   a.c11i = c->T.m0 >= 1 ? c->T.comp_im[oshun::hind(1, 1, c->T.m0)] : -1;
   a.coef = c->d_coef;
+  a.ext = c->ext ? 1 : 0;
   if (prog) {
     a.terms = prog->terms;
     a.tstart = prog->tstart;
@@ -341,6 +348,7 @@ oshun::FieldArgs<T> field_args(oshun_ctx* c, int mode, const double* fin, const 
   a.delta_p = c->T.delta_p;
   a.midx = c->T.midx; a.midy = c->T.midy;
   a.c05h = 0.5 * h; a.hh = h; a.h6 = h / 6.0; a.third = 1.0 / 3.0; a.twothird = 2.0 / 3.0;
+  a.ext = c->ext ? 1 : 0;
   return a;
 }
 
@@ -519,8 +527,8 @@ template <typename T, int NPL>
 int launch_collide_npl(oshun_ctx* c, oshun::CollArgs<T>& a) {
   ProfRec r;
   prof_begin(c, 2, &r);
-  const int ntx = (c->nx + oshun::kCollCols - 1) / oshun::kCollCols;
-  hipLaunchKernelGGL((oshun::collide<T, NPL>), dim3(ntx * c->ny), dim3(64 * oshun::kCollCols), 0, c->stream, a);
+  const int ntx = (c->nxi + oshun::kCollCols - 1) / oshun::kCollCols;
+  hipLaunchKernelGGL((oshun::collide<T, NPL>), dim3(ntx * c->nyi), dim3(64 * oshun::kCollCols), 0, c->stream, a);
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
@@ -529,8 +537,8 @@ int launch_collide_npl(oshun_ctx* c, oshun::CollArgs<T>& a) {
 template <typename T, int NPL, int SEGW, int MODE>
 int launch_collide_rows(oshun_ctx* c, oshun::CollArgs<T>& a) {
   const int cols = 256 / SEGW;
-  const int ntx = (c->nx + cols - 1) / cols;
-  hipLaunchKernelGGL((oshun::collide_rows<T, NPL, SEGW, MODE>), dim3(ntx * c->ny), dim3(256), 0, c->stream, a);
+  const int ntx = (c->nxi + cols - 1) / cols;
+  hipLaunchKernelGGL((oshun::collide_rows<T, NPL, SEGW, MODE>), dim3(ntx * c->nyi), dim3(256), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
@@ -554,7 +562,8 @@ oshun::CollArgs<T> coll_args(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int
   a.r01 = c->T.v0sq / c->T.v1sq; a.inv_gden = 1.0 / c->T.g_den; a.c2v0 = 2.0 / c->T.v0sq;
   a.fuse = 0;
   a.fz = {};
-  return a;
+  a.cx0 = c->gn; a.cy0 = c->gn; a.cnx = c->nxi; a.cny = c->nyi;  // interior columns (Euler_Backward
+  return a;                                                       // and Explicit_EC_FP skip guards)
 }
 
 // the field stages a fused collision launch takes over (see oshun_ctx::fuse_fields)
@@ -622,6 +631,7 @@ int do_collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double 
 // sources of one step after the e-e collisions (main.cpp:90-92; sources.hpp)
 int step_sources(oshun_ctx* c, double t);
 int check_source_flag(oshun_ctx* c);
+int fill_guards(oshun_ctx* c);  // mirror-boundary contexts: Neighbor_Communications (guards.hpp)
 bool sources_in_step(const oshun_ctx* c) { return c->src_on && (c->src_pls || c->src_ib == 1); }
 
 // numh and sub-step of Explicit_EC_FP::loop, def-FokkerPlanck.cpp:597-599
@@ -679,7 +689,7 @@ int full_step(oshun_ctx* c, double h, bool next1) {
   }
   c->t_ee = t;
   c->t_eb = t;
-  return OSHUN_OK;
+  return c->ext ? fill_guards(c) : OSHUN_OK;  // CX.Neighbor_Communications(Y), main.cpp:104
 }
 int full_step(oshun_ctx* c, double h) { return full_step(c, h, false); }
 
@@ -795,7 +805,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_ibw, c->d_srcflag, c->d_imp, c->d_imp_zero};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_ibw, c->d_srcflag, c->d_imp, c->d_imp_zero, c->d_comp_m, c->d_comp_isim};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (DevProg& q : c->iprog)
@@ -819,7 +829,8 @@ bool sane(const oshun_ctx* c) { return c && c->S; }
 static void plan_kernels(oshun_ctx* c, std::vector<double>* spec_coef) {
   const oshun_grid& g = c->g;
   const unsigned long long state_bytes = (unsigned long long)c->nc * c->plane * c->esz;
-  c->use1d = (g.m0 == 0 && g.numy == 1 && g.l0 <= kMaxL0Special);
+  // whole arrays with guards (mirror boundaries) run on the term-list interpreter
+  c->use1d = (g.m0 == 0 && g.numy == 1 && g.l0 <= kMaxL0Special) && !c->ext;
   c->use_pair = c->use1d && state_bytes < (1ull << 31);
   c->pair_r = 3;
   if (c->use_pair) {
@@ -831,7 +842,7 @@ static void plan_kernels(oshun_ctx* c, std::vector<double>* spec_coef) {
   }
   c->npt = c->use_pair ? pair_tiles_r(c->np, c->pair_r) : 0;
   c->spec = -1;
-  if (!c->use1d && (unsigned long long)c->plane * sizeof(double) < (1ull << 32)) {  // 32-bit plane offsets
+  if (!c->use1d && !c->ext && (unsigned long long)c->plane * sizeof(double) < (1ull << 32)) {  // 32-bit offsets
     std::vector<double> coef;
 #define OSHUN_SPEC_FIND(ID, L, M, TD) \
     if (c->spec < 0 && oshun::spec_coefficients<L, M, TD>(c->T, coef)) c->spec = ID;
@@ -937,6 +948,10 @@ static int launch_moments(oshun_ctx* c, double* part, int nchunk, double* out) {
   return OSHUN_OK;
 }
 
+namespace {
+int fill_guards(oshun_ctx* c);  // guards.hpp
+}  // namespace
+
 extern "C" {
 
 int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out) {
@@ -950,15 +965,22 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
   if (phys->nb_algorithms < 0 || phys->nb_algorithms > g.nump || !(phys->small_dt > 0.0) ||
       !(phys->density_np > 0.0))
     return OSHUN_ERR_ARG;
+  const bool ext = g.bndx != 0 || g.bndy != 0;
+  if (g.bndx < 0 || g.bndx > 1 || g.bndy < 0 || g.bndy > 1) return OSHUN_ERR_ARG;
+  if (ext && (g.nbc < 1 || g.numx < g.nbc || g.numy < g.nbc || g.precision != OSHUN_FP64)) return OSHUN_ERR_ARG;
   oshun_ctx* c = new oshun_ctx();
   c->g = g;
   c->ph = *phys;
   c->fp32 = g.precision == OSHUN_FP32_SPLIT;
   c->esz = c->fp32 ? 4 : 8;
-  oshun::build_tables(c->T, g.l0, g.m0, g.nump, g.numy, g.pmin, g.pmax, g.dx, g.dy, phys->density_np,
+  c->ext = ext;
+  c->gn = ext ? g.nbc : 0;
+  c->nxi = g.numx; c->nyi = g.numy;
+  const int nxa = g.numx + 2 * c->gn, nya = g.numy + 2 * c->gn;  // the arrays the device holds
+  oshun::build_tables(c->T, g.l0, g.m0, g.nump, nya, g.pmin, g.pmax, g.dx, g.dy, phys->density_np,
                       phys->nb_algorithms, phys->ref_quirks);
-  c->nc = c->T.nc; c->np = g.nump; c->nx = g.numx; c->ny = g.numy; c->nxy = g.numx * g.numy;
-  c->plane = (long long)g.numx * g.numy * g.nump;
+  c->nc = c->T.nc; c->np = g.nump; c->nx = nxa; c->ny = nya; c->nxy = nxa * nya;
+  c->plane = (long long)nxa * nya * g.nump;
   std::vector<double> spec_coef;
   plan_kernels(c, &spec_coef);
   const int prows = c->use1d ? oshun::kRows1D : oshun::kTileP;
@@ -998,6 +1020,7 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
   }
   if (upload_vec(c, &c->d_terms, c->T.terms) || upload_vec(c, &c->d_tstart, c->T.tstart) ||
       upload_vec(c, &c->d_clean, c->T.clean) || upload_vec(c, &c->d_comp_l, c->T.comp_l) ||
+      upload_vec(c, &c->d_comp_m, c->T.comp_m) || upload_vec(c, &c->d_comp_isim, c->T.comp_isim) ||
       upload_vec(c, &c->d_p3og, c->T.p3og))
     return bail(OSHUN_ERR_HIP);
   if (c->spec >= 0 && upload_vec(c, &c->d_coef, spec_coef)) return bail(OSHUN_ERR_HIP);
@@ -1121,10 +1144,14 @@ int oshun_kernel_plan(const oshun_grid* grid, const oshun_phys* phys, int* stage
   c.ph = *phys;
   c.fp32 = g.precision == OSHUN_FP32_SPLIT;
   c.esz = c.fp32 ? 4 : 8;
-  oshun::build_tables(c.T, g.l0, g.m0, g.nump, g.numy, g.pmin, g.pmax, g.dx, g.dy, phys->density_np,
+  c.ext = g.bndx != 0 || g.bndy != 0;
+  c.gn = c.ext ? g.nbc : 0;
+  c.nxi = g.numx; c.nyi = g.numy;
+  const int nxa = g.numx + 2 * c.gn, nya = g.numy + 2 * c.gn;
+  oshun::build_tables(c.T, g.l0, g.m0, g.nump, nya, g.pmin, g.pmax, g.dx, g.dy, phys->density_np,
                       phys->nb_algorithms, phys->ref_quirks);
-  c.nc = c.T.nc; c.np = g.nump; c.nx = g.numx; c.ny = g.numy; c.nxy = g.numx * g.numy;
-  c.plane = (long long)g.numx * g.numy * g.nump;
+  c.nc = c.T.nc; c.np = g.nump; c.nx = nxa; c.ny = nya; c.nxy = nxa * nya;
+  c.plane = (long long)nxa * nya * g.nump;
   plan_kernels(&c, nullptr);
   *stage = c.use_pair ? 0 : (c.use1d ? 1 : (c.spec >= 0 ? 2 : 3));
   *collide = c.rows_npl ? 0 : 1;
@@ -1144,8 +1171,43 @@ const char* oshun_last_error(const oshun_ctx* ctx) { return ctx ? ctx->err.c_str
 
 int oshun_num_harmonics(const oshun_ctx* ctx) { return ctx ? ctx->T.nh : OSHUN_ERR_ARG; }
 
+static int upload_raw(oshun_ctx* c, const double* const* sh, const double* const* fld, int nbc);
+
+// Whole arrays (nbc guards) of a mirror-boundary context, built on the host
+// from interior-only ones (nbc = 0): guards zero until the device fills them.
+static void ext_whole(const oshun_ctx* c, const double* const* in, int n, int per_cell,
+                      std::vector<std::vector<double>>& buf, std::vector<const double*>& ptr) {
+  buf.assign(n, std::vector<double>((size_t)2 * per_cell * c->nx * c->ny, 0.0));
+  ptr.clear();
+  for (int k = 0; k < n; ++k) {
+    for (int y = 0; y < c->nyi; ++y)
+      for (int x = 0; x < c->nxi; ++x)
+        std::copy(in[k] + 2 * (size_t)per_cell * ((size_t)y * c->nxi + x),
+                  in[k] + 2 * (size_t)per_cell * ((size_t)y * c->nxi + x + 1),
+                  buf[k].begin() + 2 * (size_t)per_cell * ((size_t)(y + c->gn) * c->nx + x + c->gn));
+    ptr.push_back(buf[k].data());
+  }
+}
+
 int oshun_upload_harmonics(oshun_ctx* c, const double* const* sh, const double* const* fld, int nbc) {
   if (!sane(c) || nbc < 0) return OSHUN_ERR_ARG;
+  if (c->ext) {  // the device holds whole arrays: nbc = its guards (copied as they are), or 0 (guards filled)
+    if (nbc == c->gn) return upload_raw(c, sh, fld, 0);
+    if (nbc != 0) return fail(c, OSHUN_ERR_ARG, "upload: nbc must be the context's guard count or 0");
+    std::vector<std::vector<double>> bh, bf;
+    std::vector<const double*> ph, pf;
+    if (sh) ext_whole(c, sh, c->T.nh, c->np, bh, ph);
+    if (fld) ext_whole(c, fld, 6, 1, bf, pf);
+    int rc = upload_raw(c, sh ? ph.data() : nullptr, fld ? pf.data() : nullptr, 0);
+    return rc ? rc : fill_guards(c);
+  }
+  return upload_raw(c, sh, fld, nbc);
+}
+
+// the device arrays from host arrays with nbc guard cells (skipped) -- or,
+// in a mirror-boundary context, from the whole arrays (nbc = 0 on the
+// device's own whole dimensions)
+static int upload_raw(oshun_ctx* c, const double* const* sh, const double* const* fld, int nbc) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (sh) {
     for (int h = 0; h < c->T.nh; ++h)
@@ -1175,15 +1237,43 @@ int oshun_upload_harmonics(oshun_ctx* c, const double* const* sh, const double* 
   return OSHUN_OK;
 }
 
-int oshun_download_harmonics(oshun_ctx* c, double* const* sh, double* const* fld, int nbc) {
-  if (!sane(c) || nbc < 0) return OSHUN_ERR_ARG;
-  return download_raw(c, c->S, c->Sf, sh, fld, nbc, true);
+// device arrays to host arrays with nbc guards; a mirror-boundary context
+// gives its whole arrays (nbc = its guard count) or the interior (nbc = 0)
+static int download_any(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* const* sh,
+                        double* const* fld, int nbc, bool state) {
+  if (!c->ext) return download_raw(c, fsrc, fldsrc, sh, fld, nbc, state);
+  if (nbc == c->gn) return download_raw(c, fsrc, fldsrc, sh, fld, 0, state);
+  if (nbc != 0) return fail(c, OSHUN_ERR_ARG, "download: nbc must be the context's guard count or 0");
+  std::vector<std::vector<double>> bh(sh ? c->T.nh : 0, std::vector<double>(2 * (size_t)c->np * c->nxy)),
+      bf(fld ? 6 : 0, std::vector<double>(2 * (size_t)c->nxy));
+  std::vector<double*> ph, pf;
+  for (auto& b : bh) ph.push_back(b.data());
+  for (auto& b : bf) pf.push_back(b.data());
+  int rc = download_raw(c, fsrc, fldsrc, sh ? ph.data() : nullptr, fld ? pf.data() : nullptr, 0, state);
+  if (rc) return rc;
+  auto cut = [&](std::vector<std::vector<double>>& b, double* const* out, int per_cell) {
+    for (size_t k = 0; k < b.size(); ++k)
+      for (int y = 0; y < c->nyi; ++y)
+        for (int x = 0; x < c->nxi; ++x)
+          std::copy(b[k].begin() + 2 * (size_t)per_cell * ((size_t)(y + c->gn) * c->nx + x + c->gn),
+                    b[k].begin() + 2 * (size_t)per_cell * ((size_t)(y + c->gn) * c->nx + x + c->gn + 1),
+                    out[k] + 2 * (size_t)per_cell * ((size_t)y * c->nxi + x));
+  };
+  cut(bh, sh, c->np);
+  cut(bf, fld, 1);
+  return OSHUN_OK;
 }
 
+int oshun_download_harmonics(oshun_ctx* c, double* const* sh, double* const* fld, int nbc) {
+  if (!sane(c) || nbc < 0) return OSHUN_ERR_ARG;
+  return download_any(c, c->S, c->Sf, sh, fld, nbc, true);
+}
+
+// host arrays (interior plus nbc guards per side) of a state laid out harmonic after harmonic
 static void split_ptrs(const oshun_ctx* c, const double* f, const double* fld, int nbc,
                        std::vector<const double*>& hp, std::vector<const double*>& fp) {
-  const size_t hsz = 2 * (size_t)c->np * (c->nx + 2 * nbc) * (c->ny + 2 * nbc);
-  const size_t fsz = 2 * (size_t)(c->nx + 2 * nbc) * (c->ny + 2 * nbc);
+  const size_t hsz = 2 * (size_t)c->np * (c->nxi + 2 * nbc) * (c->nyi + 2 * nbc);
+  const size_t fsz = 2 * (size_t)(c->nxi + 2 * nbc) * (c->nyi + 2 * nbc);
   hp.clear(); fp.clear();
   if (f) for (int h = 0; h < c->T.nh; ++h) hp.push_back(f + h * hsz);
   if (fld) for (int k = 0; k < 6; ++k) fp.push_back(fld + k * fsz);
@@ -1224,7 +1314,7 @@ int oshun_rhs(oshun_ctx* c, double* f_slope, double* fld_slope) {
   if (rc) return rc;
   std::vector<const double*> hp, fp;
   split_ptrs(c, f_slope, fld_slope, 0, hp, fp);
-  return download_raw(c, c->B1, c->B1f, f_slope ? const_cast<double* const*>(hp.data()) : nullptr,
+  return download_any(c, c->B1, c->B1f, f_slope ? const_cast<double* const*>(hp.data()) : nullptr,
                       fld_slope ? const_cast<double* const*>(fp.data()) : nullptr, 0, false);
 }
 
@@ -1340,6 +1430,7 @@ int oshun_profile_read(oshun_ctx* c, double* ms_total, long long* launches, int 
 
 int oshun_moments(oshun_ctx* c, double* out) {
   if (!sane(c) || !out) return OSHUN_ERR_ARG;
+  if (c->ext) return fail(c, OSHUN_ERR_ARG, "moments: not on a mirror-boundary context (whole arrays)");
   const int nchunk = moment_chunks(c);
   if (!c->d_mom) {  // output and partial-sum buffers on the first call
     if (hipMalloc((void**)&c->d_mom, (size_t)oshun::kMomOut * c->nxy * sizeof(double)) != hipSuccess ||
@@ -1357,6 +1448,7 @@ int oshun_moments(oshun_ctx* c, double* out) {
 
 int oshun_moments_noalloc(oshun_ctx* c, double* out_dev) {
   if (!sane(c) || !out_dev) return OSHUN_ERR_ARG;
+  if (c->ext) return fail(c, OSHUN_ERR_ARG, "moments: not on a mirror-boundary context (whole arrays)");
   const size_t per_chunk = (size_t)oshun::kMomSums * c->nxy;  // doubles of one chunk's partial sums
   if (per_chunk > kNoAllocScratchDoubles)
     return fail(c, OSHUN_ERR_ARG, "oshun_moments_noalloc: nx*ny = " + std::to_string(c->nxy) + " exceeds " +
@@ -1419,3 +1511,4 @@ int oshun_diag_copy(oshun_ctx* c, long long nbytes) {
 // implicit-E step, 1D variant (oshun_step_implicit_1d)
 #include "implicit.hpp"
 #include "implicit2d.hpp"
+#include "guards.hpp"

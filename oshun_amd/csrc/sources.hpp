@@ -360,6 +360,7 @@ extern "C" {
 int oshun_set_sources(oshun_ctx* c, const oshun_sources* s, const double* x, const double* y, const double* xglob,
                       int nxglob) {
   if (!sane(c) || !s || !x || !y || !xglob || nxglob < 2) return OSHUN_ERR_ARG;
+  if (c->ext) return fail(c, OSHUN_ERR_ARG, "oshun_set_sources: periodic boundaries only");
   if (!(s->lambda_0 > 0.0) || !(s->ab_depth != 0.0) || !(s->spot_w0 != 0.0) ||
       (s->inverse_bremsstrahlung == 1 && !(s->smaller_dt > 0.0)))
     return fail(c, OSHUN_ERR_ARG, "oshun_set_sources: lambda_0 > 0, ab_depth and spot_w0 nonzero, smaller_dt > 0");

@@ -12,6 +12,9 @@
 //   Electric_Field_Methods::Implicit_E_Field::advance      def-ImplicitE.cpp:472-606
 //   PLaserSource::Laser_Source                             def-PLSource.cpp:138-252
 //   InverseBremsstrahlung::loop                            def-PLSource.cpp:780-861
+// With a mirror boundary (bndX or bndY = 1) the device keeps the whole arrays,
+// guards included, and refreshes the guards at the end of each step
+// (advance_lm) as Neighbor_Communications does (def-Parallel.cpp:1121-1307).
 // Link this object ahead of def-RK.o, def-FokkerPlanck.o, def-ImplicitE.o and def-PLSource.o,
 // either after deleting those bodies from the reference files or with
 // -Wl,--allow-multiple-definition (then the first definition, this one, is
@@ -126,6 +129,9 @@ void attach(Stat& Y, double t) {
   g.dx = IN().inp().x.dx();
   g.dy = IN().inp().y.dx();
   g.precision = OSHUN_FP64;
+  g.bndx = IN().list().bndX;  // mirror boundaries: the device keeps the whole arrays
+  g.bndy = IN().list().bndY;
+  g.nbc = nbc;
   oshun_phys p;
   p.density_np = IN().list().density_np;
   p.zeta = IN().list().Zeta;
@@ -287,6 +293,9 @@ void Euler_Backward::advance_lm(const double& tnew) {
   if (!s.implicit) {
     attach(Y, t);
     check(oshun_collide_flm(s.ctx, tnew, 2), "Euler_Backward::advance_lm");
+    // the step's guard refresh (CX.Neighbor_Communications, main.cpp:104) where the device keeps guards
+    const Inputdata::input_list& L = Inputdata::IN().list();
+    if (L.bndX || L.bndY) check(oshun_fill_guards(s.ctx), "Neighbor_Communications");
   }
   t = tnew;
   step_done();

@@ -37,6 +37,10 @@ Cases (the reference needs l0 >= 3 and m0 >= 2, def-Input.cpp:73):
   imp_2d_dilute  1 step with two cells diluted by 1e-150 (zero determinant:
                  4 attempts, E = 0 there)
   imp_2d_src     2 steps with the laser source and inverse bremsstrahlung
+  mirror_x_2d    3 full steps with an x mirror boundary (whole arrays with
+                 guards kept: f_ext0, f_ext), l0=4 m0=2
+  mirror_y_2d    3 full steps with a y mirror boundary, l0=3 m0=3
+  mirror_xy_rk4  4 RK4 steps with both mirrors, l0=3 m0=2
 
 `python tests/golden/make_golden.py name ...` writes only the named fixtures.
 """
@@ -189,10 +193,42 @@ def implicit_2d_cases(only):
         save(name, dict(c, **extra), f, fld, r)
 
 
+def mirror_cases(only):
+    """Full steps with mirror boundaries (the harness's restatement of sameNode_mirror_X/Y), whole
+    arrays with their 4 guards stored before the first step and after the last."""
+    base = dict(pmax=0.25, xmin=0.0, xmax=60.0, ymin=0.0, ymax=40.0, nb=4, h=0.4, small_dt=0.15,
+                density_np=1e22, zeta=1.0, nbc=4)
+    cases = (
+        # x mirror: the l = l0 harmonics are never refreshed (def-Parallel.cpp:1168)
+        ("mirror_x_2d", dict(l0=4, m0=2, np=16, nx=6, ny=5, op=1, nsteps=3, bndX=1, bndY=0), 61),
+        # y mirror: l = 0 and l = l0 are never refreshed (:1258); conjugate images
+        ("mirror_y_2d", dict(l0=3, m0=3, np=12, nx=5, ny=6, op=1, nsteps=3, bndX=0, bndY=1), 62),
+        # both, RK4 only
+        ("mirror_xy_rk4", dict(l0=3, m0=2, np=12, nx=5, ny=5, op=2, nsteps=4, bndX=1, bndY=1), 63),
+    )
+    for name, cc, seed in cases:
+        if only and name not in only:
+            continue
+        c = dict(base, **cc)
+        f, fld = implicit_2d_state(c, seed)
+        r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], f, fld, op=c["op"], nsteps=c["nsteps"],
+                    pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"], h=c["h"],
+                    small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"], zeta=c["zeta"],
+                    bnd=(c["bndX"], c["bndY"]), nbc=c["nbc"])
+        for k in ("f", "fld", "f_ext0", "fld_ext0", "f_ext", "fld_ext"):
+            assert np.all(np.isfinite(r[k])), (name, k)
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, f_in=f, fld_in=fld, f_out=r["f"], fld_out=r["fld"], p=r["p"], dx=r["dx"],
+                            dy=r["dy"], h=r["h"], t=r["t"], f_ext0=r["f_ext0"], fld_ext0=r["fld_ext0"],
+                            f_ext=r["f_ext"], fld_ext=r["fld_ext"], **{"cfg_" + k: v for k, v in c.items()})
+        print(f"{name}: {os.path.getsize(path)} bytes")
+
+
 def main(only=()):
     if only:
         source_cases(only)
         implicit_2d_cases(only)
+        mirror_cases(only)
         return
     # 1) RHS, 2D complex
     c = dict(l0=4, m0=3, np=16, nx=6, ny=4, pmax=0.3, xmin=-2.0, xmax=3.0, ymin=-1.0, ymax=2.0,
@@ -277,6 +313,7 @@ def main(only=()):
 
     source_cases(())
     implicit_2d_cases(())
+    mirror_cases(())
 
 
 if __name__ == "__main__":

@@ -9,6 +9,9 @@ translation units, so that every step runs on the GPU through the C ABI
   source and inverse bremsstrahlung on) and the collision-only drivers
   (ee_only, flm_only): the shim's output against the compiled reference's
   fixture at <= 1e-12 normwise in f and the fields.
+* mirror-boundary decks (mirror_x_2d, mirror_y_2d): the whole arrays, guards
+  included, against the fixture at <= 1e-12 (the device keeps the guards and
+  refreshes them in the shim's Euler_Backward::advance_lm).
 * implicit-E decks, 1D variant (imp_1d_m0, imp_1d_m0_l8) and 2D variant
   (imp_2d_m2; imp_2d_m3_es, whose JN is the Ex the shim's RK2 constructor
   copied): bitwise equal to the same steps called directly through the C ABI
@@ -55,7 +58,10 @@ def _run_shim(g):
                   xmax=float(c["xmax"]), ymin=float(c["ymin"]), ymax=float(c["ymax"]), h=float(c["h"]),
                   small_dt=float(c["small_dt"]), density_np=float(c["density_np"]), zeta=float(c["zeta"]),
                   es=int(c.get("es", 0)), binary=SHIM_BIN,
-                  src={k[4:]: float(v) for k, v in c.items() if k.startswith("src_")} or None)
+                  src={k[4:]: float(v) for k, v in c.items() if k.startswith("src_")} or None,
+                  bnd=(int(c.get("bndX", 0)), int(c.get("bndY", 0))), nbc=int(c.get("nbc", 4)))
+    if "bndX" in c:
+        return out
     fo, flo = out["f"], out["fld"]
     if m0 != int(c["m0"]):
         assert np.abs(np.delete(fo, [hind(l, 0, m0) for l in range(l0 + 1)], axis=0)).max() == 0.0
@@ -102,3 +108,13 @@ def test_implicit_e_2d_through_the_shim_equals_the_abi(golden, name):
     fa, flda = ctx.download()
     assert np.array_equal(f, fa)
     assert np.array_equal(fld, flda)
+
+
+@pytest.mark.parametrize("name", ["mirror_x_2d", "mirror_y_2d"])
+def test_mirror_deck_through_the_shim(golden, name):
+    g = golden(name)
+    print(f"case shim {name}", flush=True)
+    out = _run_shim(g)
+    assert rel(out["f_ext"], g["f_ext"]) < TOL
+    assert rel(out["fld_ext"].real, g["fld_ext"].real) < TOL
+    assert rel(out["f"], g["f_out"]) < TOL
