@@ -115,9 +115,9 @@ def _ref_wall(procs, nsteps, inp, td):
     with open(inp, "rb") as fh:
         raw = bytearray(fh.read())
     path = os.path.join(td, f"in_{nsteps}.bin")
-    hdr = np.frombuffer(raw[:48], np.int32).copy()
+    hdr = np.frombuffer(raw[:52], np.int32).copy()
     hdr[6] = nsteps
-    raw[:48] = hdr.tobytes()
+    raw[:52] = hdr.tobytes()
     with open(path, "wb") as fh:
         fh.write(raw)
     t0 = time.perf_counter()

@@ -150,7 +150,8 @@ int oshun_step_implicit_2d(oshun_ctx* ctx, double h, int nsteps, int if_implicit
  * field at the solver's construction, in all three components
  * (def-ImplicitE.cpp:56-58, 446).  jn: 3 complex fields (x, y, z) laid out
  * like three of oshun_upload_state's fields, guards of width nbc included;
- * the real parts are used.  Zero until set. */
+ * the real parts are used (a mirror-boundary context takes its whole arrays:
+ * nbc = its guard count).  Zero until set. */
 int oshun_set_implicit_es_current(oshun_ctx* ctx, const double* jn, int nbc);
 /* Cells whose conductivity determinant was still zero after the last
  * oshun_step_implicit_2d step's final attempt (their field is set to 0, as

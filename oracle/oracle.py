@@ -341,14 +341,14 @@ def deck_axis(n, vmin, vmax, guards=4):
 
 def write_ref_input(path, l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,
                     ymin=-1.0, ymax=1.0, h=0.01, small_dt=1.0, density_np=1e21, zeta=1.0, es=0, src=None,
-                    bnd=(0, 0)):
+                    bnd=(0, 0), nbc=4):
     """The driver's input file (layout in ref_harness.cpp's header comment)."""
     nh = num_harmonics(l0, m0)
     f = np.ascontiguousarray(f, np.complex128).reshape(nh, ny, nx, np_)
     fld = np.ascontiguousarray(fld, np.complex128).reshape(6, ny, nx)
     with open(path, "wb") as fh:
-        np.array([l0, m0, np_, nx, ny, op, nsteps, nb, es, 1 if src is not None else 0, int(bnd[0]), int(bnd[1])],
-                 np.int32).tofile(fh)
+        np.array([l0, m0, np_, nx, ny, op, nsteps, nb, es, 1 if src is not None else 0, int(bnd[0]), int(bnd[1]),
+                  int(nbc)], np.int32).tofile(fh)
         np.array([pmax, xmin, xmax, ymin, ymax, h, small_dt, density_np, zeta], np.float64).tofile(fh)
         if src is not None:
             source_vector(src).tofile(fh)
@@ -372,7 +372,7 @@ def run_ref(l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0
         fin = os.path.join(td, "in.bin")
         fout = os.path.join(td, "out.bin")
         write_ref_input(fin, l0, m0, np_, nx, ny, f, fld, op, nsteps, nb, pmax, xmin, xmax, ymin, ymax, h,
-                        small_dt, density_np, zeta, es, src, bnd)
+                        small_dt, density_np, zeta, es, src, bnd, nbc)
         r = subprocess.run([binary, fin, fout], capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"{os.path.basename(binary)} failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")

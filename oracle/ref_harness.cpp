@@ -19,8 +19,9 @@
 // and after each step, which is what the single-node periodic exchange does.
 //
 // Usage:  oshun_ref <in.bin> <out.bin>
-//   in.bin : int32[12] {l0,m0,np,nx,ny,op,nsteps,nb,es,src,bndX,bndY}  (es: if_implicitES, op 5/9;
-//            bndX/bndY: 0 periodic, 1 mirror guards, def-Parallel.cpp:1121-1307)
+//   in.bin : int32[13] {l0,m0,np,nx,ny,op,nsteps,nb,es,src,bndX,bndY,nbc}  (es: if_implicitES, op 5/9;
+//            bndX/bndY: 0 periodic, 1 mirror guards, def-Parallel.cpp:1121-1307;
+//            nbc: guard cells per side = RKLevel, 0 for the default 4)
 //            double[9] {pmax,xmin,xmax,ymin,ymax,h,small_dt,density_np,zeta}
 //            if src: double[15] {linear_Ivst, polynomial_Ivst, inverse_bremsstrahlung,
 //                   polarization_direction, I_0, lambda_0, ab_eta, ab_depth, cntr_x0,
@@ -228,9 +229,9 @@ int main(int argc, char** argv) {
   if (argc != 3) die("usage: oshun_ref in.bin out.bin");
   FILE* fi = std::fopen(argv[1], "rb");
   if (!fi) die("cannot open input");
-  int iv[12];
+  int iv[13];
   double dv[9];
-  if (std::fread(iv, sizeof(int), 12, fi) != 12) die("short header");
+  if (std::fread(iv, sizeof(int), 13, fi) != 13) die("short header");
   if (std::fread(dv, sizeof(double), 9, fi) != 9) die("short header");
   const int l0 = iv[0], m0 = iv[1], np = iv[2], nx = iv[3], ny = iv[4], op = iv[5], nsteps = iv[6], nb = iv[7];
   const int es = iv[8];
@@ -239,7 +240,8 @@ int main(int argc, char** argv) {
   double sv[15] = {0};
   if (iv[9] && std::fread(sv, sizeof(double), 15, fi) != 15) die("short source block");
   const int nh = ((m0 + 1) * (2 * l0 - m0 + 2)) / 2;
-  const int nbc = 4;
+  const int nbc = iv[12] > 0 ? iv[12] : 4;
+  if ((op == 1 || op == 2) && nbc < 4) die("RK4 steps need 4 guard cells");
 
   Inputdata::input_list& L = Inputdata::IN().list();
   L.NnodesX = 1; L.NnodesY = 1;

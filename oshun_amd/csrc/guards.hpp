@@ -5,7 +5,8 @@
 // Parallel_Environment::Neighbor_Communications of one node
 // (def-Parallel.cpp:3040-3127): the x direction, then the y direction, each
 // periodic (sameNode_periodic_X/Y, :1137-1156, :1229-1248) or mirror
-// (sameNode_mirror_X/Y, :1160-1200, :1251-1307).  As there:
+// (sameNode_mirror_X/Y, :1160-1200, :1251-1307); and the implicit-E step's
+// magnetic-field refresh after its first RK2 half (b_only).  As there:
 //   * periodic: the last nbc interior columns (rows) go to the left guards,
 //     the first nbc to the right ones, every harmonic and field;
 //   * x mirror: guard at distance d from the face = interior cell at distance
@@ -28,6 +29,7 @@ struct GuardArgs {
   const int* comp_m;
   const int* comp_isim;  // 1 on the imaginary plane of a harmonic
   int nc, np, nx, ny, g, l0, mirror;
+  int b_only;  // Neighbor_ImplicitE_Communications: the magnetic field only
   long long plane;
 };
 
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(256) void guards_x(GuardArgs a) {
     else { const int c = j - g; dst = nx - c - 1; src = nx - 2 * g + c; }  // right guard nx-c-1 <- nx-2g+c
   }
   const long long row = ((long long)y * np + p) * nx;
-  for (int c = 0; c < a.nc; ++c) {
+  for (int c = 0; c < (a.b_only ? 0 : a.nc); ++c) {
     const int l = a.comp_l[c];
     if (a.mirror && l >= a.l0) continue;
     const double sg = (a.mirror && ((l + a.comp_m[c]) & 1)) ? -1.0 : 1.0;
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(256) void guards_x(GuardArgs a) {
   }
   if (p == 0) {
     const long long frow = (long long)y * nx;
-    for (int k = 0; k < 6; ++k) {
+    for (int k = a.b_only ? 3 : 0; k < 6; ++k) {
       const double sg = (a.mirror && (k == 1 || k == 2 || k == 3)) ? -1.0 : 1.0;  // Ey, Ez, Bx
       double* F = a.fld + (long long)k * nx * ny + frow;
       F[dst] = F[src] * sg;
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(256) void guards_y(GuardArgs a) {
     else { const int c = j - g; dst = ny - c - 1; src = ny - 2 * g + c; }
   }
   const long long od = ((long long)dst * np + p) * nx + x, os = ((long long)src * np + p) * nx + x;
-  for (int c = 0; c < a.nc; ++c) {
+  for (int c = 0; c < (a.b_only ? 0 : a.nc); ++c) {
     const int l = a.comp_l[c];
     double* pl = a.f + (long long)c * a.plane;
     if (!a.mirror) {
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256) void guards_y(GuardArgs a) {
     }
   }
   if (p == 0) {
-    for (int k = 0; k < 6; ++k) {
+    for (int k = a.b_only ? 3 : 0; k < 6; ++k) {
       const double sg = (a.mirror && (k == 0 || k == 2 || k == 4)) ? -1.0 : 1.0;  // Ex, Ez, By
       double* F = a.fld + (long long)k * nx * ny;
       F[(long long)dst * nx + x] = F[(long long)src * nx + x] * sg;
@@ -113,9 +115,14 @@ __global__ __launch_bounds__(256) void guards_y(GuardArgs a) {
 
 namespace {
 
-int fill_guards(oshun_ctx* c) {
+// b_only: Parallel_Environment::Neighbor_ImplicitE_Communications
+// (def-Parallel.cpp:2945-3033 with Node_ImplicitE_Communications::
+// sameNode_periodic/mirror_X/Y, :497-680): Bx, By, Bz only, with Bx negated
+// across an x mirror and By across a y mirror
+int fill_guards(oshun_ctx* c, bool b_only) {
   if (!c->ext) return OSHUN_OK;
   oshun::GuardArgs a;
+  a.b_only = b_only ? 1 : 0;
   a.f = static_cast<double*>(c->S);
   a.fld = c->Sf;
   a.comp_l = c->d_comp_l;
@@ -140,7 +147,7 @@ extern "C" {
 int oshun_fill_guards(oshun_ctx* c) {
   if (!sane(c)) return OSHUN_ERR_ARG;
   if (!c->ext) return fail(c, OSHUN_ERR_ARG, "oshun_fill_guards: the context keeps no guard cells");
-  return fill_guards(c);
+  return fill_guards(c, false);
 }
 
 }  // extern "C"

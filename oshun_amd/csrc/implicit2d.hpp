@@ -31,6 +31,12 @@
 // changes.  Each current is Current_xyz::calculate_J (def-ImplicitE.cpp:
 // 125-180), whose trapezoid end weight lands on a guard row, so interior
 // cells sum p0 and pmax with weight 1.
+//
+// With mirror boundaries (whole arrays with guards) every kernel here runs on
+// all cells, guards included, as the reference's loops do (the field solve
+// over szx x szy, def-ImplicitE.cpp:536-577, counting zero determinants on
+// guard cells too); the collisions stay on the interior; the magnetic-field
+// guards are refreshed after the first RK2 half and all guards at the end.
 
 namespace oshun {
 
@@ -39,6 +45,7 @@ struct ImpFarArgs {
   double* y0;   // fields of the RK2's Y0 [6][nxy]
   int nx, ny;
   double midx, midy, h, h2;
+  int ext;  // whole arrays with guards: flattened-array differences (ext_offsets)
 };
 
 // MaxwellEq::Faraday (def-VlasovMax.cpp:979-1005) for both RK2 stages of
@@ -50,17 +57,24 @@ __global__ __launch_bounds__(256) void imp_faraday_2d(ImpFarArgs a, int mode) {
   const long long k = blockIdx.x * 256LL + threadIdx.x;
   if (k >= nxy) return;
   const int x = static_cast<int>(k % nx), y = static_cast<int>(k / nx);
-  const long long xm = (x == 0) ? k + nx - 1 : k - 1, xp = (x == nx - 1) ? k - (nx - 1) : k + 1;
-  const long long ym = (y == 0) ? k + (long long)(ny - 1) * nx : k - nx;
-  const long long yp = (y == ny - 1) ? k - (long long)(ny - 1) * nx : k + nx;
+  long long xm = (x == 0) ? k + nx - 1 : k - 1, xp = (x == nx - 1) ? k - (nx - 1) : k + 1;
+  long long ym = (y == 0) ? k + (long long)(ny - 1) * nx : k - nx;
+  long long yp = (y == ny - 1) ? k - (long long)(ny - 1) * nx : k + nx;
+  bool xlast = false, ylast = false;
+  if (a.ext) {
+    long long oxm, oxp, oym, oyp;
+    ext_offsets(x, y, nx, ny, nx, &oxm, &oxp, &oym, &oyp, &xlast, &ylast);
+    xm = k + oxm; xp = k + oxp; ym = k + oym; yp = k + oyp;
+  }
   const double* Ex = a.fld;
   const double* Ey = a.fld + nxy;
   const double* Ez = a.fld + 2 * nxy;
+  auto X = [&](const double* F, long long i, bool zero) { return zero ? 0.0 : F[i]; };
   double k3 = 0.0, k4 = 0.0, k5 = 0.0;
-  k3 += Ez[ym] * (-a.midy) - Ez[yp] * (-a.midy);
-  k4 += Ez[xm] * a.midx - Ez[xp] * a.midx;
-  k5 += Ex[ym] * a.midy - Ex[yp] * a.midy;
-  k5 += Ey[xm] * (-a.midx) - Ey[xp] * (-a.midx);
+  k3 += Ez[ym] * (-a.midy) - X(Ez, yp, ylast) * (-a.midy);
+  k4 += Ez[xm] * a.midx - X(Ez, xp, xlast) * a.midx;
+  k5 += Ex[ym] * a.midy - X(Ex, yp, ylast) * a.midy;
+  k5 += Ey[xm] * (-a.midx) - X(Ey, xp, xlast) * (-a.midx);
   const double kc[3] = {k3, k4, k5};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -101,6 +115,7 @@ struct ImpCurArgs {
   long long plane;
   int c10, c11r, c11i, np, nx, ny;
   double delta_p, small;
+  int ext;  // whole arrays: row 0 is the guard row the end-weight subscript lands on
 };
 
 // Current_xyz::calculate_J (def-ImplicitE.cpp:125-180): Jx from f10 with
@@ -129,7 +144,15 @@ __global__ __launch_bounds__(256) void imp_current_2d(ImpCurArgs a) {
       chunk[ip][tx] = ok ? f[(long long)ip * nx] * a.p3og[ip] : 0.0;
     __syncthreads();
     if (ok && ty == 0) {
-      for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip][tx];
+      if (a.ext && y == 0) {
+        // the guard row that jayx(ix) *= 0.5 addresses (an (ix, 0) subscript):
+        // halved once before its p sum and once more for every later row
+        j *= 0.5;
+        for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip][tx];
+        for (int r = 1; r < a.ny; ++r) j *= 0.5;
+      } else {
+        for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip][tx];
+      }
       j *= a.delta_p;
       j += a.small * (f[nx] * a.p3og[1]);
     }
@@ -153,6 +176,7 @@ struct ImpSolveArgs {
   int* zeros;
   int nx, ny;
   double midx, midy;
+  int ext;  // whole arrays with guards: flattened-array differences (ext_offsets)
 };
 
 // the 3x3 conductivity solve of Implicit_E_Field::advance (def-ImplicitE.cpp:
@@ -171,16 +195,23 @@ __global__ __launch_bounds__(256) void imp_solve_2d(ImpSolveArgs a) {
     for (int i = 0; i < 3; ++i) jn[i] = a.jn[i * nxy + k];
   } else {
     const int x = static_cast<int>(k % nx), y = static_cast<int>(k / nx);
-    const long long xm = (x == 0) ? k + nx - 1 : k - 1, xp = (x == nx - 1) ? k - (nx - 1) : k + 1;
-    const long long ym = (y == 0) ? k + (long long)(ny - 1) * nx : k - nx;
-    const long long yp = (y == ny - 1) ? k - (long long)(ny - 1) * nx : k + nx;
+    long long xm = (x == 0) ? k + nx - 1 : k - 1, xp = (x == nx - 1) ? k - (nx - 1) : k + 1;
+    long long ym = (y == 0) ? k + (long long)(ny - 1) * nx : k - nx;
+    long long yp = (y == ny - 1) ? k - (long long)(ny - 1) * nx : k + nx;
+    bool xlast = false, ylast = false;
+    if (a.ext) {
+      long long oxm, oxp, oym, oyp;
+      ext_offsets(x, y, nx, ny, nx, &oxm, &oxp, &oym, &oyp, &xlast, &ylast);
+      xm = k + oxm; xp = k + oxp; ym = k + oym; yp = k + oyp;
+    }
     const double* Bx = a.fld + 3 * nxy;
     const double* By = a.fld + 4 * nxy;
     const double* Bz = a.fld + 5 * nxy;
-    jn[0] = Bz[ym] * a.midy - Bz[yp] * a.midy;
-    jn[1] = Bz[xm] * (-a.midx) - Bz[xp] * (-a.midx);
-    jn[2] = Bx[ym] * (-a.midy) - Bx[yp] * (-a.midy);
-    jn[2] += By[xm] * a.midx - By[xp] * a.midx;
+    auto X = [&](const double* F, long long i, bool zero) { return zero ? 0.0 : F[i]; };
+    jn[0] = Bz[ym] * a.midy - X(Bz, yp, ylast) * a.midy;
+    jn[1] = Bz[xm] * (-a.midx) - X(Bz, xp, xlast) * (-a.midx);
+    jn[2] = Bx[ym] * (-a.midy) - X(Bx, yp, ylast) * (-a.midy);
+    jn[2] += By[xm] * a.midx - X(By, xp, xlast) * a.midx;
   }
   double A[3][3], D[3];
 #pragma unroll
@@ -254,6 +285,7 @@ oshun::ImpFarArgs imp_far_args(oshun_ctx* c, double h) {
   a.nx = c->nx; a.ny = c->ny;
   a.midx = c->T.midx; a.midy = c->T.midy;
   a.h = h; a.h2 = 0.5 * h;
+  a.ext = c->ext ? 1 : 0;
   return a;
 }
 
@@ -269,6 +301,7 @@ int imp_current(oshun_ctx* c, const void* buf, int slot) {
   a.c11i = c->T.comp_im[oshun::hind(1, 1, c->T.m0)];
   a.np = c->np; a.nx = c->nx; a.ny = c->ny;
   a.delta_p = c->T.delta_p; a.small = c->T.small;
+  a.ext = c->ext ? 1 : 0;
   ProfRec r;
   prof_begin(c, 1, &r);
   hipLaunchKernelGGL(oshun::imp_current_2d, dim3((c->nx + oshun::kImpCols - 1) / oshun::kImpCols, c->ny), dim3(256),
@@ -317,6 +350,8 @@ int implicit_step_2d(oshun_ctx* c, double h, int es) {
   std::swap(c->S, c->A);
   c->t_rk += h;
   const double t = c->t_rk;
+  // CX.Neighbor_ImplicitE_Communications(Y) (main.cpp:84-86): the B guards
+  if (c->ext && (rc = fill_guards(c, true))) return rc;
   const int coll = c->ph.collisions;
   const int do_ee = (coll & OSHUN_COLL_EE) ? 1 : 0;
   const int do_flm = (coll & OSHUN_COLL_FLM) ? 1 : 0;
@@ -339,6 +374,7 @@ int implicit_step_2d(oshun_ctx* c, double h, int es) {
   oshun::ImpSolveArgs sa;
   sa.fld = c->Sf; sa.pf = imp_pf(c); sa.j = imp_j(c, 0); sa.jn = es ? imp_jn(c) : nullptr;
   sa.zeros = c->d_imp_zero; sa.nx = c->nx; sa.ny = c->ny; sa.midx = c->T.midx; sa.midy = c->T.midy;
+  sa.ext = c->ext ? 1 : 0;
   int zeros = 0;
   for (int attempt = 0; attempt < 4; ++attempt) {
     for (int d = 0; d < (attempt ? 2 : 3); ++d)
@@ -365,7 +401,7 @@ int implicit_step_2d(oshun_ctx* c, double h, int es) {
   std::swap(c->S, c->A);
   if (do_flm && (rc = do_collide(c, 0, 1, 1, 0, 0.0, dt_flm))) return rc;
   c->t_eb = t;
-  return OSHUN_OK;
+  return c->ext ? fill_guards(c, false) : OSHUN_OK;  // CX.Neighbor_Communications(Y), main.cpp:104
 }
 
 }  // namespace
@@ -374,7 +410,6 @@ extern "C" {
 
 int oshun_step_implicit_2d(oshun_ctx* c, double h, int nsteps, int if_implicitES) {
   if (!sane(c) || nsteps < 0 || !(h > 0.0)) return OSHUN_ERR_ARG;
-  if (c->ext) return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_2d: periodic boundaries only");
   if (c->fp32 || c->T.l0 < 3 || c->T.m0 < 2)
     return fail(c, OSHUN_ERR_ARG, "oshun_step_implicit_2d: needs l0 >= 3, m0 >= 2, FP64");
   if (c->np > oshun::kImpRows)
@@ -391,12 +426,16 @@ int oshun_set_implicit_es_current(oshun_ctx* c, const double* jn, int nbc) {
   if (!sane(c) || !jn || nbc < 0) return OSHUN_ERR_ARG;
   int rc;
   if ((rc = imp2d_setup(c))) return rc;
-  const int nxg = c->nx + 2 * nbc, nyg = c->ny + 2 * nbc;
+  // a mirror-boundary context holds JN on its whole arrays: nbc must be its guard count
+  if (c->ext && nbc != c->gn)
+    return fail(c, OSHUN_ERR_ARG, "oshun_set_implicit_es_current: nbc must be the context's guard count");
+  const int e = c->ext ? 0 : nbc;  // guards of the caller's arrays beyond the device's
+  const int nxg = c->nx + 2 * e, nyg = c->ny + 2 * e;
   std::vector<double> h((size_t)3 * c->nxy);
   for (int k = 0; k < 3; ++k)
     for (int y = 0; y < c->ny; ++y)
       for (int x = 0; x < c->nx; ++x)
-        h[((size_t)k * c->ny + y) * c->nx + x] = jn[2 * (((size_t)k * nyg + y + nbc) * nxg + x + nbc)];
+        h[((size_t)k * c->ny + y) * c->nx + x] = jn[2 * (((size_t)k * nyg + y + e) * nxg + x + e)];
   HIPCHK(c, hipMemcpyAsync(imp_jn(c), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return OSHUN_OK;

@@ -631,7 +631,7 @@ int do_collide(oshun_ctx* c, int do_ee, int do_f1, int do_flm, int numh, double 
 // sources of one step after the e-e collisions (main.cpp:90-92; sources.hpp)
 int step_sources(oshun_ctx* c, double t);
 int check_source_flag(oshun_ctx* c);
-int fill_guards(oshun_ctx* c);  // mirror-boundary contexts: Neighbor_Communications (guards.hpp)
+int fill_guards(oshun_ctx* c, bool b_only = false);  // mirror-boundary contexts (guards.hpp)
 bool sources_in_step(const oshun_ctx* c) { return c->src_on && (c->src_pls || c->src_ib == 1); }
 
 // numh and sub-step of Explicit_EC_FP::loop, def-FokkerPlanck.cpp:597-599
@@ -689,7 +689,7 @@ int full_step(oshun_ctx* c, double h, bool next1) {
   }
   c->t_ee = t;
   c->t_eb = t;
-  return c->ext ? fill_guards(c) : OSHUN_OK;  // CX.Neighbor_Communications(Y), main.cpp:104
+  return c->ext ? fill_guards(c, false) : OSHUN_OK;  // CX.Neighbor_Communications(Y), main.cpp:104
 }
 int full_step(oshun_ctx* c, double h) { return full_step(c, h, false); }
 
@@ -948,9 +948,6 @@ static int launch_moments(oshun_ctx* c, double* part, int nchunk, double* out) {
   return OSHUN_OK;
 }
 
-namespace {
-int fill_guards(oshun_ctx* c);  // guards.hpp
-}  // namespace
 
 extern "C" {
 
@@ -1199,7 +1196,7 @@ int oshun_upload_harmonics(oshun_ctx* c, const double* const* sh, const double* 
     if (sh) ext_whole(c, sh, c->T.nh, c->np, bh, ph);
     if (fld) ext_whole(c, fld, 6, 1, bf, pf);
     int rc = upload_raw(c, sh ? ph.data() : nullptr, fld ? pf.data() : nullptr, 0);
-    return rc ? rc : fill_guards(c);
+    return rc ? rc : fill_guards(c, false);
   }
   return upload_raw(c, sh, fld, nbc);
 }

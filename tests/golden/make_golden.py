@@ -41,6 +41,13 @@ Cases (the reference needs l0 >= 3 and m0 >= 2, def-Input.cpp:73):
                  guards kept: f_ext0, f_ext), l0=4 m0=2
   mirror_y_2d    3 full steps with a y mirror boundary, l0=3 m0=3
   mirror_xy_rk4  4 RK4 steps with both mirrors, l0=3 m0=2
+  mirror_imp_x   2 implicit-E steps (2D variant, RKLevel-2 guards), x mirror
+  mirror_imp_x_es 2 such steps, l0=3 m0=3, if_implicitES (JN = the initial
+                 Ex, guards included); both with the reference's own one-ulp
+                 noise floor of the field solve (floor_f, floor_fld).  (A y
+                 mirror leaves f00's guards stale, def-Parallel.cpp:1258, and
+                 the field solve next to it is then ill-conditioned: ~1e-3
+                 one-ulp floor; not a useful fixture.)
 
 `python tests/golden/make_golden.py name ...` writes only the named fixtures.
 """
@@ -205,23 +212,45 @@ def mirror_cases(only):
         ("mirror_y_2d", dict(l0=3, m0=3, np=12, nx=5, ny=6, op=1, nsteps=3, bndX=0, bndY=1), 62),
         # both, RK4 only
         ("mirror_xy_rk4", dict(l0=3, m0=2, np=12, nx=5, ny=5, op=2, nsteps=4, bndX=1, bndY=1), 63),
+        # implicit-E steps (op 9) with RKLevel = 2 guards, as the production deck with bndX = 1
+        ("mirror_imp_x", dict(l0=4, m0=2, np=16, nx=6, ny=5, op=9, nsteps=2, es=0, bndX=1, bndY=0, nbc=2), 64),
+        ("mirror_imp_x_es", dict(l0=3, m0=3, np=12, nx=5, ny=6, op=9, nsteps=2, es=1, bndX=1, bndY=0, nbc=2), 65),
     )
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
     for name, cc, seed in cases:
         if only and name not in only:
             continue
         c = dict(base, **cc)
         f, fld = implicit_2d_state(c, seed)
-        r = run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], f, fld, op=c["op"], nsteps=c["nsteps"],
-                    pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"], h=c["h"],
-                    small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"], zeta=c["zeta"],
-                    bnd=(c["bndX"], c["bndY"]), nbc=c["nbc"])
+
+        def ref(fin):
+            return run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], fin, fld, op=c["op"], nsteps=c["nsteps"],
+                           pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"],
+                           h=c["h"], small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"],
+                           zeta=c["zeta"], es=c.get("es", 0), bnd=(c["bndX"], c["bndY"]), nbc=c["nbc"])
+        r = ref(f)
         for k in ("f", "fld", "f_ext0", "fld_ext0", "f_ext", "fld_ext"):
             assert np.all(np.isfinite(r[k])), (name, k)
+        extra = {}
+        if c["op"] == 9:
+            # the implicit field solve's noise floor, measured on the reference itself: the largest
+            # change of its whole arrays when the input f is perturbed by one ulp (3 draws)
+            rng = np.random.default_rng(7)
+            wf, we = 0.0, np.zeros(6)
+            for _ in range(3):
+                r1 = ref(f * (1.0 + 2.220446049250313e-16 * rng.choice([-1.0, 1.0], f.shape)))
+                wf = max(wf, rel(r1["f_ext"], r["f_ext"]))
+                we = np.maximum(we, [rel(r1["fld_ext"][k], r["fld_ext"][k]) if np.any(r["fld_ext"][k]) else 0.0
+                                     for k in range(6)])
+            extra = dict(floor_f=wf, floor_fld=we)
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, f_in=f, fld_in=fld, f_out=r["f"], fld_out=r["fld"], p=r["p"], dx=r["dx"],
                             dy=r["dy"], h=r["h"], t=r["t"], f_ext0=r["f_ext0"], fld_ext0=r["fld_ext0"],
-                            f_ext=r["f_ext"], fld_ext=r["fld_ext"], **{"cfg_" + k: v for k, v in c.items()})
-        print(f"{name}: {os.path.getsize(path)} bytes")
+                            f_ext=r["f_ext"], fld_ext=r["fld_ext"], **extra, **{"cfg_" + k: v for k, v in c.items()})
+        print(f"{name}: {os.path.getsize(path)} bytes", extra)
 
 
 def main(only=()):

@@ -6,7 +6,8 @@ def-Parallel.cpp:1160-1307, with its loop bounds: l0 across x, l = 0 and l0
 across y are not refreshed and evolve in their guards).
 
 Against the fixtures mirror_x_2d (x mirror, 3 full steps), mirror_y_2d (y
-mirror, 3 full steps) and mirror_xy_rk4 (both, 4 RK4 steps), written by the
+mirror, 3 full steps), mirror_xy_rk4 (both, 4 RK4 steps) and the implicit-E
+steps mirror_imp_x / mirror_imp_x_es (x mirror, RKLevel-2 guards), written by the
 compiled reference's classes with the driver's restatement of the fill: the
 whole arrays after the steps, guards included, at <= 1e-12 normwise, and the
 interior download.  Also: an interior-only upload fills the guards exactly as
@@ -20,12 +21,14 @@ from test_gpu_parity import TOL, rel
 pytestmark = pytest.mark.gpu
 
 CASES = ["mirror_x_2d", "mirror_y_2d", "mirror_xy_rk4"]
+IMP_CASES = ["mirror_imp_x", "mirror_imp_x_es"]
+FLOOR_MULT = 5.0
 
 
 def _ctx(g):
     import oshun_amd
     c = g["cfg"]
-    coll = oshun_amd.COLL_EE | oshun_amd.COLL_FLM if int(c["op"]) == 1 else 0
+    coll = oshun_amd.COLL_EE | oshun_amd.COLL_FLM if int(c["op"]) in (1, 9) else 0  # op 2: RK4 alone
     return oshun_amd.Context(l0=int(c["l0"]), m0=int(c["m0"]), nump=int(c["np"]), numx=int(c["nx"]),
                              numy=int(c["ny"]), pmax=float(c["pmax"]), dx=float(g["dx"]), dy=float(g["dy"]),
                              density_np=float(c["density_np"]), zeta=float(c["zeta"]),
@@ -63,13 +66,39 @@ def test_interior_upload_fills_the_guards(golden, name):
     assert np.array_equal(fld, g["fld_ext0"])
 
 
+@pytest.mark.parametrize("name", IMP_CASES)
+def test_mirror_implicit_steps_vs_reference(golden, name):
+    """implicit-E steps without if_implicit1D with an x mirror and RKLevel-2 guards
+    (the production deck's boundary): whole arrays within FLOOR_MULT times the
+    reference's own one-ulp noise floor of its field solve (stored in the fixture)"""
+    g = golden(name)
+    c = g["cfg"]
+    nb = int(c["nbc"])
+    print(f"case mirror implicit {name}", flush=True)
+    ctx = _ctx(g)
+    ctx.upload(g["f_ext0"], g["fld_ext0"], nbc=nb)
+    if int(c["es"]):
+        ctx.set_implicit_es_current(np.stack([g["fld_ext0"][0]] * 3), nbc=nb)
+    z = ctx.step_implicit_2d(float(c["h"]), int(c["nsteps"]), int(c["es"]))
+    f, fld = ctx.download(nbc=nb)
+    ef = rel(f, g["f_ext"])
+    ee = [rel(fld[k].real, g["fld_ext"][k].real) if np.any(g["fld_ext"][k]) else float(np.abs(fld[k]).max())
+          for k in range(6)]
+    wf, we = float(g["floor_f"]), g["floor_fld"]
+    print(f"  f {ef:.2e} (floor {wf:.1e}), fields {['%.1e' % v for v in ee]} (floors {['%.1e' % v for v in we]})",
+          flush=True)
+    assert z == 0
+    assert wf < 1e-10 and np.all(we < 1e-4)
+    assert ef <= max(TOL, FLOOR_MULT * wf)
+    for k in range(6):
+        assert ee[k] <= max(TOL, FLOOR_MULT * we[k]), k
+
+
 def test_mirror_context_rejects_what_it_does_not_run():
     import oshun_amd
     ctx = oshun_amd.Context(l0=3, m0=2, nump=12, numx=5, numy=5, pmax=0.25, bndx=1, nbc=4)
     with pytest.raises(oshun_amd.OshunError):
         ctx.moments()
-    with pytest.raises(oshun_amd.OshunError):
-        ctx.step_implicit_2d(0.1, 1, 0)
     with pytest.raises(oshun_amd.OshunError):
         ctx.upload(np.zeros(ctx.f_shape(2), np.complex128), np.zeros(ctx.fld_shape(2), np.complex128), nbc=2)
     with pytest.raises(oshun_amd.OshunError):

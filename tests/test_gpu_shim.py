@@ -118,3 +118,26 @@ def test_mirror_deck_through_the_shim(golden, name):
     assert rel(out["f_ext"], g["f_ext"]) < TOL
     assert rel(out["fld_ext"].real, g["fld_ext"].real) < TOL
     assert rel(out["f"], g["f_out"]) < TOL
+
+
+@pytest.mark.parametrize("name", ["mirror_imp_x", "mirror_imp_x_es"])
+def test_mirror_implicit_deck_through_the_shim_equals_the_abi(golden, name):
+    """the production deck's integrator and boundary (implicit E, 2D variant, x mirror,
+    RKLevel-2 guards) through the reference's classes: bitwise the ABI's steps"""
+    import oshun_amd
+    g = golden(name)
+    c = g["cfg"]
+    nb = int(c["nbc"])
+    print(f"case shim {name}", flush=True)
+    out = _run_shim(g)
+    ctx = oshun_amd.Context(l0=int(c["l0"]), m0=int(c["m0"]), nump=int(c["np"]), numx=int(c["nx"]),
+                            numy=int(c["ny"]), pmax=float(c["pmax"]), dx=float(g["dx"]), dy=float(g["dy"]),
+                            density_np=float(c["density_np"]), zeta=float(c["zeta"]), small_dt=float(c["small_dt"]),
+                            nb_algorithms=int(c["nb"]), bndx=int(c["bndX"]), bndy=int(c["bndY"]), nbc=nb)
+    ctx.upload(g["f_ext0"], g["fld_ext0"], nbc=nb)
+    if int(c["es"]):
+        ctx.set_implicit_es_current(np.stack([g["fld_ext0"][0]] * 3), nbc=nb)
+    ctx.step_implicit_2d(float(c["h"]), int(c["nsteps"]), int(c["es"]))
+    f, fld = ctx.download(nbc=nb)
+    assert np.array_equal(out["f_ext"], f)
+    assert np.array_equal(out["fld_ext"], fld)
