@@ -173,7 +173,11 @@ typedef struct oshun_sources {
 /* Sets the sources of the context (the constructors of PLaserSource,
  * InverseBremsstrahlung and ExternalEfield, def-PLSource.cpp:73-133, :308-353,
  * :861-946).  x[nx], y[ny]: positions of the interior cells on the
- * reference's axes (Inputdata::decl_input, def-Input.cpp:313-325); xglob:
+ * reference's axes (Inputdata::decl_input, def-Input.cpp:313-325) -- of all
+ * cells, guards included, on a mirror-boundary context, where the laser
+ * source and the external field act on every cell and inverse bremsstrahlung
+ * on the interior, as in the reference (def-PLSource.cpp:152-230, 395-410,
+ * 838-839); xglob:
  * its global x axis (nxglob points, the laser profile's normalisation).  The
  * source clocks start at the context's time.  oshun_step then runs
  * PLaserSource::Laser_Source and InverseBremsstrahlung::loop after the e-e

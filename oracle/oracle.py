@@ -325,10 +325,11 @@ def source_vector(src):
     return np.array([float(src.get(k, 0.0)) for k in SOURCE_KEYS], np.float64)
 
 
-def deck_axis(n, vmin, vmax, guards=4):
+def deck_axis(n, vmin, vmax, guards=4, whole=False):
     """Interior cell positions of the reference's x (or y) axis and its global axis, computed as
     Inputdata::decl_input does (def-Input.cpp:313-325, Axis of matrices.h:860-866): the global axis
-    i*d + min, then the local one from its first cell minus the guards, cumulatively by glob.dx()."""
+    i*d + min, then the local one from its first cell minus the guards, cumulatively by glob.dx().
+    whole: the positions of all n + 2 guards cells instead of the interior's."""
     d = (vmax - vmin) / float(n - 1)
     glob = np.arange(n) * d + vmin
     gdx = glob[1] - glob[0]
@@ -336,7 +337,7 @@ def deck_axis(n, vmin, vmax, guards=4):
     loc[0] = glob[0] - guards * gdx
     for i in range(1, loc.size):
         loc[i] = loc[i - 1] + gdx
-    return loc[guards:guards + n].copy(), glob
+    return (loc.copy() if whole else loc[guards:guards + n].copy()), glob
 
 
 def write_ref_input(path, l0, m0, np_, nx, ny, f, fld, op, nsteps=1, nb=4, pmax=0.3, xmin=-1.0, xmax=1.0,

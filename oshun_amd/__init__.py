@@ -289,6 +289,9 @@ class Context:
         x = np.ascontiguousarray(x, np.float64)
         y = np.ascontiguousarray(y, np.float64)
         xg = np.ascontiguousarray(xglob, np.float64)
+        gd = self.grid.nbc if (self.grid.bndx or self.grid.bndy) else 0  # mirror: every cell, guards included
+        if x.size != self.nx + 2 * gd or y.size != self.ny + 2 * gd:
+            raise ValueError(f"set_sources: {self.nx + 2 * gd} x and {self.ny + 2 * gd} y positions expected")
         self._chk(self._lib.oshun_set_sources(self._h, ctypes.byref(s), _dptr(x), _dptr(y), _dptr(xg), int(xg.size)))
 
     def laser_source(self, tnew):

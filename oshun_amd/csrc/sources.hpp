@@ -45,6 +45,7 @@ struct SrcArgs {
   double* fld;             // EFLD: the field plane written
   int* flag;               // set when a heating rate alpha leaves [0, 0.99]
   int np, nx, ny, numh;
+  int cx0, cy0, cnx, cny;  // InverseBremsstrahlung's cells: the interior (def-PLSource.cpp:838-839)
   double dp, p0p1_sq, inv_mp0p1_sq;
   // PLaserSource
   double ILt, dt, IL, IL_sqrt, IL_min, acoeff, nmin;
@@ -191,9 +192,10 @@ __device__ void ib_slope(const SrcArgs<T>& a, const double* in, double* fh, int 
 template <typename T>
 __global__ __launch_bounds__(256) void ib_kernel(SrcArgs<T> a) {
   const int nxy = a.nx * a.ny;
-  const int cell = blockIdx.x * 256 + threadIdx.x;
-  if (cell >= nxy) return;
-  const int ix = cell % a.nx, iy = cell / a.nx;
+  const int w = blockIdx.x * 256 + threadIdx.x;
+  if (w >= a.cnx * a.cny) return;
+  const int ix = a.cx0 + w % a.cnx, iy = a.cy0 + w / a.cnx;
+  const int cell = iy * a.nx + ix;
   const int N = a.np;
   T* col = a.f00 + (long long)iy * N * a.nx + ix;
   const double vos = a.vos * sqrt(a.ILxe[ix] * a.ILye[iy]);
@@ -285,6 +287,7 @@ oshun::SrcArgs<T> src_args(oshun_ctx* c) {
   a.xpos = b + 2 * nx + 3 * ny;
   a.flag = c->d_srcflag;
   a.np = c->np; a.nx = c->nx; a.ny = c->ny;
+  a.cx0 = c->gn; a.cy0 = c->gn; a.cnx = c->nxi; a.cny = c->nyi;
   const double p0 = c->src_p0, p1 = c->src_p1;
   a.dp = p1 - p0;
   a.p0p1_sq = p0 * p0 / (p1 * p1);
@@ -326,7 +329,7 @@ int launch_ib(oshun_ctx* c, double tnew) {
   a.numh = numh;
   a.h = (tnew - c->src_t_ib) / static_cast<double>(static_cast<size_t>(numh));
   c->src_t_ib = tnew;
-  hipLaunchKernelGGL(oshun::ib_kernel<T>, dim3((c->nxy + 255) / 256), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(oshun::ib_kernel<T>, dim3((c->nxi * c->nyi + 255) / 256), dim3(256), 0, c->stream, a);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
@@ -360,7 +363,6 @@ extern "C" {
 int oshun_set_sources(oshun_ctx* c, const oshun_sources* s, const double* x, const double* y, const double* xglob,
                       int nxglob) {
   if (!sane(c) || !s || !x || !y || !xglob || nxglob < 2) return OSHUN_ERR_ARG;
-  if (c->ext) return fail(c, OSHUN_ERR_ARG, "oshun_set_sources: periodic boundaries only");
   if (!(s->lambda_0 > 0.0) || !(s->ab_depth != 0.0) || !(s->spot_w0 != 0.0) ||
       (s->inverse_bremsstrahlung == 1 && !(s->smaller_dt > 0.0)))
     return fail(c, OSHUN_ERR_ARG, "oshun_set_sources: lambda_0 > 0, ab_depth and spot_w0 nonzero, smaller_dt > 0");

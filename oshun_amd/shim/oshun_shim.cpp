@@ -160,9 +160,11 @@ void attach(Stat& Y, double t) {
     src.cntr_x0 = L.cntr_x0; src.cntr_r0 = L.cntr_r0; src.spot_w0 = L.spot_w0;
     src.rise_time = L.rise_time; src.flat_time = L.flat_time; src.fall_time = L.fall_time;
     src.smaller_dt = L.smaller_dt;
-    std::vector<double> xs(g.numx), ys(g.numy), xg(IN().inp().xglob.dim());
-    for (int i = 0; i < g.numx; ++i) xs[i] = IN().inp().x(i + nbc);
-    for (int j = 0; j < g.numy; ++j) ys[j] = IN().inp().y(j + nbc);
+    // positions of the cells the device holds: the interior, or every cell with a mirror boundary
+    const int gx = (g.bndx || g.bndy) ? nbc : 0;
+    std::vector<double> xs(g.numx + 2 * gx), ys(g.numy + 2 * gx), xg(IN().inp().xglob.dim());
+    for (size_t i = 0; i < xs.size(); ++i) xs[i] = IN().inp().x(i + nbc - gx);
+    for (size_t j = 0; j < ys.size(); ++j) ys[j] = IN().inp().y(j + nbc - gx);
     for (size_t i = 0; i < xg.size(); ++i) xg[i] = IN().inp().xglob(i);
     check(oshun_set_sources(s.ctx, &src, xs.data(), ys.data(), xg.data(), static_cast<int>(xg.size())),
           "oshun_set_sources");

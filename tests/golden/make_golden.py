@@ -48,6 +48,8 @@ Cases (the reference needs l0 >= 3 and m0 >= 2, def-Input.cpp:73):
                  mirror leaves f00's guards stale, def-Parallel.cpp:1258, and
                  the field solve next to it is then ill-conditioned: ~1e-3
                  one-ulp floor; not a useful fixture.)
+  mirror_imp_x_src 2 such steps, x mirror, with linear laser heating and
+                 inverse bremsstrahlung (the production deck's whole step)
 
 `python tests/golden/make_golden.py name ...` writes only the named fixtures.
 """
@@ -215,6 +217,11 @@ def mirror_cases(only):
         # implicit-E steps (op 9) with RKLevel = 2 guards, as the production deck with bndX = 1
         ("mirror_imp_x", dict(l0=4, m0=2, np=16, nx=6, ny=5, op=9, nsteps=2, es=0, bndX=1, bndY=0, nbc=2), 64),
         ("mirror_imp_x_es", dict(l0=3, m0=3, np=12, nx=5, ny=6, op=9, nsteps=2, es=1, bndX=1, bndY=0, nbc=2), 65),
+        # the production deck's whole step: implicit E, x mirror, RKLevel 2, linear laser heating
+        # (and inverse bremsstrahlung)
+        ("mirror_imp_x_src", dict(l0=3, m0=2, np=48, nx=6, ny=4, op=9, nsteps=2, es=0, bndX=1, bndY=0, nbc=2,
+                                  pmax=0.3, xmax=50.0, ymax=30.0,
+                                  src=dict(SRC, linear_Ivst=1, polynomial_Ivst=0, inverse_bremsstrahlung=1)), 66),
     )
 
     def rel(a, b):
@@ -224,13 +231,14 @@ def mirror_cases(only):
         if only and name not in only:
             continue
         c = dict(base, **cc)
+        src = c.pop("src", None)
         f, fld = implicit_2d_state(c, seed)
 
         def ref(fin):
             return run_ref(c["l0"], c["m0"], c["np"], c["nx"], c["ny"], fin, fld, op=c["op"], nsteps=c["nsteps"],
                            pmax=c["pmax"], xmin=c["xmin"], xmax=c["xmax"], ymin=c["ymin"], ymax=c["ymax"],
                            h=c["h"], small_dt=c["small_dt"], density_np=c["density_np"], nb=c["nb"],
-                           zeta=c["zeta"], es=c.get("es", 0), bnd=(c["bndX"], c["bndY"]), nbc=c["nbc"])
+                           zeta=c["zeta"], es=c.get("es", 0), bnd=(c["bndX"], c["bndY"]), nbc=c["nbc"], src=src)
         r = ref(f)
         for k in ("f", "fld", "f_ext0", "fld_ext0", "f_ext", "fld_ext"):
             assert np.all(np.isfinite(r[k])), (name, k)
@@ -249,7 +257,8 @@ def mirror_cases(only):
         path = os.path.join(HERE, name + ".npz")
         np.savez_compressed(path, f_in=f, fld_in=fld, f_out=r["f"], fld_out=r["fld"], p=r["p"], dx=r["dx"],
                             dy=r["dy"], h=r["h"], t=r["t"], f_ext0=r["f_ext0"], fld_ext0=r["fld_ext0"],
-                            f_ext=r["f_ext"], fld_ext=r["fld_ext"], **extra, **{"cfg_" + k: v for k, v in c.items()})
+                            f_ext=r["f_ext"], fld_ext=r["fld_ext"], **extra, **{"cfg_" + k: v for k, v in c.items()},
+                            **{"cfg_src_" + k: v for k, v in (src or {}).items()})
         print(f"{name}: {os.path.getsize(path)} bytes", extra)
 
 

@@ -7,7 +7,8 @@ across y are not refreshed and evolve in their guards).
 
 Against the fixtures mirror_x_2d (x mirror, 3 full steps), mirror_y_2d (y
 mirror, 3 full steps), mirror_xy_rk4 (both, 4 RK4 steps) and the implicit-E
-steps mirror_imp_x / mirror_imp_x_es (x mirror, RKLevel-2 guards), written by the
+steps mirror_imp_x / mirror_imp_x_es / mirror_imp_x_src (x mirror, RKLevel-2
+guards; the last with laser heating and inverse bremsstrahlung), written by the
 compiled reference's classes with the driver's restatement of the fill: the
 whole arrays after the steps, guards included, at <= 1e-12 normwise, and the
 interior download.  Also: an interior-only upload fills the guards exactly as
@@ -21,7 +22,7 @@ from test_gpu_parity import TOL, rel
 pytestmark = pytest.mark.gpu
 
 CASES = ["mirror_x_2d", "mirror_y_2d", "mirror_xy_rk4"]
-IMP_CASES = ["mirror_imp_x", "mirror_imp_x_es"]
+IMP_CASES = ["mirror_imp_x", "mirror_imp_x_es", "mirror_imp_x_src"]
 FLOOR_MULT = 5.0
 
 
@@ -79,6 +80,11 @@ def test_mirror_implicit_steps_vs_reference(golden, name):
     ctx.upload(g["f_ext0"], g["fld_ext0"], nbc=nb)
     if int(c["es"]):
         ctx.set_implicit_es_current(np.stack([g["fld_ext0"][0]] * 3), nbc=nb)
+    if "src_I_0" in c:  # the laser on every cell of the whole arrays, inverse bremsstrahlung on the interior
+        from oracle.oracle import SOURCE_KEYS, deck_axis
+        x, xg = deck_axis(int(c["nx"]), float(c["xmin"]), float(c["xmax"]), guards=nb, whole=True)
+        y, _ = deck_axis(int(c["ny"]), float(c["ymin"]), float(c["ymax"]), guards=nb, whole=True)
+        ctx.set_sources({k: float(c["src_" + k]) for k in SOURCE_KEYS}, x, y, xg)
     z = ctx.step_implicit_2d(float(c["h"]), int(c["nsteps"]), int(c["es"]))
     f, fld = ctx.download(nbc=nb)
     ef = rel(f, g["f_ext"])
