@@ -449,14 +449,36 @@ def run_implicit(steps, warmup, no_cpu=False):
             "cpu_baseline": cpu}
 
 
-# the three production decks (def-Input.cpp:60-98 of POP-marocchino-secIII.B, POP-tzoufras-secIII.A,
-# POP-tzoufras-secIV.B): implicit_E without if_implicit1D, RKLevel 2; the x, y extents are the decks'
-# global grids, run here on one GPU (the decks spread them over 32, 64 and 128 MPI ranks)
+# the three production decks (def-Input.cpp:60-200 of POP-marocchino-secIII.B, POP-tzoufras-secIII.A,
+# POP-tzoufras-secIV.B): implicit_E without if_implicit1D, RKLevel 2, their own grids, boundaries,
+# collision and laser parameters and time step (h = dt_out / num_h with CLF = clf_dp * dp, def-RK.cpp:
+# 407-413); the decks spread these grids over 32, 64 and 128 MPI ranks, here they run on one GPU
+_LASER = dict(lambda_0=0.351, I_0=1.0e16, ab_eta=1.0, cntr_r0=0.0, smaller_dt=0.005)
 POP_DECKS = (
-    ("marocchino-secIII.B", dict(l0=48, m0=2, np=288, nx=640, ny=2, pmax=0.4, xmax=8574.383315, ymax=5000.0, es=1)),
-    ("tzoufras-secIII.A", dict(l0=31, m0=2, np=432, nx=192, ny=2, pmax=0.6, xmax=39880.82246, ymax=1000.0, es=1)),
-    ("tzoufras-secIV.B", dict(l0=16, m0=3, np=300, nx=96, ny=48, pmax=0.6, xmax=17835.246, ymax=44588.115, es=0)),
+    ("marocchino-secIII.B", dict(l0=48, m0=2, np=288, nx=640, ny=2, pmax=0.4, xmin=-8574.383315, xmax=8574.383315,
+                                 ymin=-5000.0, ymax=5000.0, es=1, bndx=0, density_np=1.5e21, zeta=1.0,
+                                 small_dt=0.16, nb=4, t_stop=72801.67589, n_out=300, src=None)),
+    ("tzoufras-secIII.A", dict(l0=31, m0=2, np=432, nx=192, ny=2, pmax=0.6, xmin=-39880.82246, xmax=39880.82246,
+                               ymin=-1000.0, ymax=1000.0, es=1, bndx=0, density_np=4.5e20, zeta=2.0, small_dt=0.15,
+                               nb=6, t_stop=358927.4021, n_out=300,
+                               src=dict(_LASER, inverse_bremsstrahlung=1, spot_w0=99999725.41, cntr_x0=0.0,
+                                        ab_depth=5640.0, rise_time=1e-9, flat_time=358927.4021,
+                                        fall_time=178352.46))),
+    ("tzoufras-secIV.B", dict(l0=16, m0=3, np=300, nx=96, ny=48, pmax=0.6, xmin=-23780.328, xmax=17835.246,
+                              ymin=-44588.115, ymax=44588.115, es=0, bndx=1, density_np=1.0e23, zeta=4.0,
+                              small_dt=0.09, nb=6, t_stop=2140299.52, n_out=960,
+                              src=dict(_LASER, linear_Ivst=1, spot_w0=14862.705, cntr_x0=-10701.0, ab_depth=7134.0,
+                                       rise_time=178352.46, flat_time=1783524.6, fall_time=178352.46))),
 )
+POP_NBC = 2  # RKLevel
+
+
+def pop_h(d):
+    """the deck's step: num_h = int(dt_out / CLF) + 1, h = dt_out / num_h, CLF = clf_dp * dp (clf_dp = 30)"""
+    pmin = d["pmax"] / (2.0 * d["np"])
+    clf = 30.0 * (d["pmax"] - pmin) / (d["np"] - 1)
+    dt_out = d["t_stop"] / d["n_out"]
+    return dt_out / (int(dt_out / clf) + 1)
 
 
 def pop_state(d, nx=None, seed=7):
@@ -486,19 +508,27 @@ def pop_state(d, nx=None, seed=7):
     return f, fld
 
 
-def run_pop(steps, warmup, no_cpu=False, h=1.0):
+def run_pop(steps, warmup, no_cpu=False):
     """The implicit-E step without if_implicit1D (oshun_step_implicit_2d) on the three production decks'
-    grids, FP64; the reference's op-9 driver (oracle/_ref) timed on a 4-cell x slab of each."""
+    grids, boundaries (tzoufras-secIV.B: x mirror with RKLevel-2 guards), collision and laser settings and
+    time step, FP64; the reference's op-9 driver (oracle/_ref) timed on a 4-cell x slab of each."""
     import oshun_amd
     from oracle import oracle as orc
     out = []
     for name, d in POP_DECKS:
-        dx = d["xmax"] / (d["nx"] - 1)
-        dy = d["ymax"] / (d["ny"] - 1)
+        h = pop_h(d)
+        dx = (d["xmax"] - d["xmin"]) / (d["nx"] - 1)
+        dy = (d["ymax"] - d["ymin"]) / (d["ny"] - 1)
         f, fld = pop_state(d)
+        nbc = POP_NBC if d["bndx"] else 0
         ctx = oshun_amd.Context(l0=d["l0"], m0=d["m0"], nump=d["np"], numx=d["nx"], numy=d["ny"], pmax=d["pmax"],
-                                dx=dx, dy=dy, density_np=DENSITY, small_dt=0.5 * h, nb_algorithms=NB)
+                                dx=dx, dy=dy, density_np=d["density_np"], zeta=d["zeta"], small_dt=d["small_dt"],
+                                nb_algorithms=d["nb"], bndx=d["bndx"], nbc=nbc)
         ctx.upload(f, fld)
+        if d["src"]:
+            x, xg = orc.deck_axis(d["nx"], d["xmin"], d["xmax"], guards=POP_NBC, whole=bool(d["bndx"]))
+            y, _ = orc.deck_axis(d["ny"], d["ymin"], d["ymax"], guards=POP_NBC, whole=bool(d["bndx"]))
+            ctx.set_sources(d["src"], x, y, xg)
         ctx.step_implicit_2d(h, warmup, d["es"])
         ctx.synchronize()
         t0 = time.perf_counter()
@@ -515,27 +545,30 @@ def run_pop(steps, warmup, no_cpu=False, h=1.0):
         nh = orc.num_harmonics(d["l0"], d["m0"])
         cells = nh * d["np"] * d["nx"] * d["ny"]
         rec = {"deck": name, "l0": d["l0"], "m0": d["m0"], "np": d["np"], "nx": d["nx"], "ny": d["ny"],
-               "es": d["es"], "ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "cells": cells,
-               "state_bytes": ctx.state_bytes(),
+               "es": d["es"], "bndx": d["bndx"], "sources": bool(d["src"]), "h": h, "ms_per_step": ms,
+               "cell_updates_per_s": cells / (ms * 1e-3), "cells": cells, "state_bytes": ctx.state_bytes(),
                "kernels": {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps}
                            for k, v in prof.items() if v[1]}}
         del ctx
         if not no_cpu and orc.have_ref():
             nxs = 4
             fs, flds = pop_state(d, nx=nxs)
-            kw = dict(nb=NB, pmax=d["pmax"], xmin=0.0, xmax=dx * (nxs - 1), ymin=0.0, ymax=d["ymax"], h=h,
-                      small_dt=0.5 * h, density_np=DENSITY, es=d["es"])
+            kw = dict(nb=d["nb"], pmax=d["pmax"], xmin=0.0, xmax=dx * (nxs - 1), ymin=d["ymin"], ymax=d["ymax"],
+                      h=h, small_dt=d["small_dt"], density_np=d["density_np"], zeta=d["zeta"], es=d["es"],
+                      src=d["src"], bnd=(d["bndx"], 0), nbc=POP_NBC)
             t0 = time.perf_counter()
             orc.run_ref(d["l0"], d["m0"], d["np"], nxs, d["ny"], fs, flds, op=9, nsteps=1, **kw)
             t1 = time.perf_counter()
-            orc.run_ref(d["l0"], d["m0"], d["np"], nxs, d["ny"], fs, flds, op=9, nsteps=2, **kw)
+            orc.run_ref(d["l0"], d["m0"], d["np"], nxs, d["ny"], fs, flds, op=9, nsteps=3, **kw)
             t2 = time.perf_counter()
-            per_step = max((t2 - t1) - (t1 - t0), 1e-9)
+            per_step = max((t2 - t1) - (t1 - t0), 1e-9) / 2
             sc = nh * d["np"] * nxs * d["ny"]
             rec["cpu_baseline"] = {"value": sc / per_step, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
-                                   "sample": f"OSHUN-1 classes (oracle/_ref, op 9), the deck's l0, m0, Np and Ny on "
-                                             f"Nx = {nxs} (+ guards), a 2-step minus a 1-step run",
+                                   "sample": f"OSHUN-1 classes (oracle/_ref, op 9), the deck's l0, m0, Np, Ny, "
+                                             f"boundary and sources on Nx = {nxs} (+ guards), a 3-step minus a "
+                                             f"1-step run",
                                    "s_per_step": per_step}
+            rec["gpu_over_cpu_core"] = rec["cell_updates_per_s"] / rec["cpu_baseline"]["value"]
         out.append(rec)
         print(json.dumps(rec), file=sys.stderr, flush=True)
     return {"metric": "phase-space cell-updates/s per implicit-E RK2+FP step (2D variant, FP64), production decks",
