@@ -508,7 +508,7 @@ def pop_state(d, nx=None, seed=7):
     return f, fld
 
 
-def run_pop(steps, warmup, no_cpu=False):
+def run_pop(steps, warmup, no_cpu=False, deck=None):
     """The implicit-E step without if_implicit1D (oshun_step_implicit_2d) on the three production decks'
     grids, boundaries (tzoufras-secIV.B: x mirror with RKLevel-2 guards), collision and laser settings and
     time step, FP64; the reference's op-9 driver (oracle/_ref) timed on a 4-cell x slab of each."""
@@ -516,6 +516,8 @@ def run_pop(steps, warmup, no_cpu=False):
     from oracle import oracle as orc
     out = []
     for name, d in POP_DECKS:
+        if deck and name != deck:
+            continue
         h = pop_h(d)
         dx = (d["xmax"] - d["xmin"]) / (d["nx"] - 1)
         dy = (d["ymax"] - d["ymin"]) / (d["ny"] - 1)
@@ -596,6 +598,8 @@ def main():
                     help="the implicit-E step of the 1D variant instead of the RK4 step (FP64)")
     ap.add_argument("--pop", action="store_true",
                     help="the implicit-E step without if_implicit1D on the three production decks' grids (FP64)")
+    ap.add_argument("--deck", default=None, choices=[n for n, _ in POP_DECKS],
+                    help="with --pop: only the deck of this name")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -610,7 +614,7 @@ def main():
         print(json.dumps(run_implicit(args.steps, args.warmup, args.no_cpu)))
         return
     if args.pop:
-        print(json.dumps(run_pop(args.steps, args.warmup, args.no_cpu)))
+        print(json.dumps(run_pop(args.steps, args.warmup, args.no_cpu, args.deck)))
         return
     if args.batch:
         print(json.dumps(run_ensemble(args.steps, args.warmup)))

@@ -81,7 +81,8 @@ int oshun_destroy(oshun_ctx* ctx);
 /* The kernels a context of this grid would run, decided on the host alone
  * (nothing is allocated; no GPU is needed): *stage = 0 the 1D stage-pair
  * kernels, 1 the 1D single-stage kernel, 2 the compile-time specialised
- * general kernel, 3 the term-list interpreter; *collide = 0 the lane-layout
+ * general kernel, 3 the term-list interpreter, 4 the l-streaming general
+ * kernel (m0 = 1..4, any l0); *collide = 0 the lane-layout
  * collision kernel, 1 one wavefront per column.  The stage pairs address the
  * whole state with 32-bit byte offsets and the lane-layout kernel one plane,
  * so states of 2 GiB or more (planes of 2 GiB) take the 64-bit kernels.  No
@@ -248,8 +249,9 @@ int oshun_profile_read(oshun_ctx* ctx, double* ms_total, long long* launches, in
 int oshun_diag_copy(oshun_ctx* ctx, long long nbytes);
 /* Diagnostics: which kernel runs the Vlasov stage of this context
  * (*which = 0 the 1D m0 = 0 kernels, 1 the compile-time specialised general
- * kernel, 2 the term-list interpreter).  set = 1 forces the interpreter,
- * 0 restores the default, -1 only queries.  Both general kernels evaluate
+ * kernel, 2 the term-list interpreter, 3 the l-streaming general kernel).
+ * set = 1 forces the interpreter (also for the implicit step's programs),
+ * 0 restores the default, -1 only queries.  The general kernels evaluate
  * Actions::Adv (def-Actions.cpp:73-94) with the same per-target sums. */
 int oshun_stage_kernel(oshun_ctx* ctx, int set, int* which);
 /* Tile height (p rows) of the 1D stage-pair kernels: 10 by default, 6 where

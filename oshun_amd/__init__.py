@@ -390,11 +390,12 @@ class Context:
 
     def stage_kernel(self, set=-1):
         """Which kernel runs the Vlasov stage: "1d" (m0 = 0, one y row), "spec" (compile-time
-        specialised general kernel) or "general" (term-list interpreter).  set=1 forces the
-        interpreter, set=0 restores the default, set=-1 only queries."""
+        specialised general kernel), "general" (term-list interpreter) or "lm" (l-streaming
+        general kernel, m0 = 1..4).  set=1 forces the interpreter, set=0 restores the default,
+        set=-1 only queries."""
         w = ctypes.c_int(-1)
         self._chk(self._lib.oshun_stage_kernel(self._h, int(set), ctypes.byref(w)))
-        return ("1d", "spec", "general")[w.value]
+        return ("1d", "spec", "general", "lm")[w.value]
 
     def pair_rows(self, set=0):
         """Tile height (p rows) of the 1D stage-pair kernels; set=6 or 10 chooses it, 0 only queries."""

@@ -9,6 +9,7 @@
 #pragma once
 
 #include "kernels.hpp"
+#include "stage_lm.hpp"
 #include "stage_spec.hpp"
 
 // X(T, L0): 1D stage kernel, l0 = 1..12
@@ -62,6 +63,17 @@
   OSHUN_ROWS_LAYOUTS(OSHUN_I_ROWSB, EXT, T, 3)
 
 // general system: the term-list interpreter and the specialised shapes
+// the l-streaming kernel, m0 = 1..4: every program (kind) in FP64, the full
+// program in FP32; with and without y advection
+#define OSHUN_I_LM(EXT, T, M0, KIND, TD) \
+  EXT template __global__ void oshun::vlasov_stage_lm<T, M0, KIND, TD>(oshun::StageArgs<T>);
+#define OSHUN_LM_TD(EXT, T, M0, KIND) OSHUN_I_LM(EXT, T, M0, KIND, false) OSHUN_I_LM(EXT, T, M0, KIND, true)
+#define OSHUN_LM_KINDS(EXT, T, M0) OSHUN_LM_TD(EXT, T, M0, 0) OSHUN_LM_TD(EXT, T, M0, 1) OSHUN_LM_TD(EXT, T, M0, 2) \
+  OSHUN_LM_TD(EXT, T, M0, 3)
+#define OSHUN_FAMILY_LM_F64(EXT) OSHUN_LM_KINDS(EXT, double, 1) OSHUN_LM_KINDS(EXT, double, 2) \
+  OSHUN_LM_KINDS(EXT, double, 3) OSHUN_LM_KINDS(EXT, double, 4)
+#define OSHUN_FAMILY_LM_F32(EXT) OSHUN_LM_TD(EXT, float, 1, 0) OSHUN_LM_TD(EXT, float, 2, 0) \
+  OSHUN_LM_TD(EXT, float, 3, 0) OSHUN_LM_TD(EXT, float, 4, 0)
 #define OSHUN_SPEC_F64(ID, L, M, TD) OSHUN_I_SPEC(OSHUN_SPEC_EXT, double, ID, L, M, TD)
 #define OSHUN_SPEC_F32(ID, L, M, TD) OSHUN_I_SPEC(OSHUN_SPEC_EXT, float, ID, L, M, TD)
 
@@ -70,4 +82,5 @@
   OSHUN_FAMILY_PAIRB(extern, double) OSHUN_FAMILY_PAIRB(extern, float) OSHUN_FAMILY_PAIRB_F32_R4(extern) \
   OSHUN_FAMILY_COLL(extern, double) OSHUN_FAMILY_COLL(extern, float)                          \
   OSHUN_FAMILY_COLLB(extern, double) OSHUN_FAMILY_COLLB(extern, float)                        \
-  OSHUN_I_GEN(extern, double) OSHUN_I_GEN(extern, float)
+  OSHUN_I_GEN(extern, double) OSHUN_I_GEN(extern, float)                                      \
+  OSHUN_FAMILY_LM_F64(extern) OSHUN_FAMILY_LM_F32(extern)

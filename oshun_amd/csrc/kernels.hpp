@@ -44,10 +44,13 @@ struct StageArgs {
   const T* hp0;
   const double* p3og;
   const T* bgG;  // G00 of the f00 background (plane 0 stores f00 minus it), or null
-  const double* coef;  // term coefficients in visit order (vlasov_stage_spec only)
+  // term coefficients in visit order (vlasov_stage_spec); add_complex c and d
+  // per target (vlasov_stage_lm, which reads tstart as their per-plane start)
+  const double* coef;
   T p0p1_sq, inv_mp0p1_sq, g_r;
   T c05h, hh, h6, third, twothird;
   int np, nx, ny, nc;
+  int l0;  // vlasov_stage_lm only
   long long plane;
   int mode;  // 0: slope, 1..4: RK4 stage, 5..6: RK2 stage (vlasov_stage only)
   int c10, c11r, c11i;

@@ -16,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "instances.hpp"
@@ -41,6 +42,9 @@ struct DevProg {
   Term* terms = nullptr;
   int* tstart = nullptr;
   int* clean = nullptr;
+  int lm = -1;             // LmKind of the l-streaming kernel that runs it, -1: the interpreter
+  int* gstart = nullptr;   // that kernel's coefficients (LmPlan)
+  double* coef = nullptr;
 };
 
 struct oshun_ctx {
@@ -84,6 +88,7 @@ struct oshun_ctx {
   // kernel for this (l0, m0, 2D), -1 for the term-list interpreter
   int spec = -1;
   bool force_general = false;
+  DevProg full;  // lm and coef of the full program (the interpreter reads d_terms, d_tstart, d_clean)
   double* d_coef = nullptr;  // term coefficients in the spec kernel's visit order
   double* d_coll = nullptr;
   double* d_rt = nullptr;   // collide_rows tables of the e-e launch (interleaved for rows_npl/rows_segw)
@@ -152,6 +157,46 @@ int upload_vec(oshun_ctx* c, V** dst, const std::vector<V>& src) {
   size_t n = std::max<size_t>(src.size(), 1) * sizeof(V);
   HIPCHK(c, hipMalloc((void**)dst, n));
   if (!src.empty()) HIPCHK(c, hipMemcpy(*dst, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice));
+  return OSHUN_OK;
+}
+
+// The l-streaming kernel's plan of a coupling program (terms, tstart over all
+// planes): q->lm = the program kind it runs and q->gstart, q->coef its
+// coefficients, or q->lm = -1 (the interpreter runs the program; FP32
+// storage has the full program's kernel only)
+int lm_prepare(oshun_ctx* c, const std::vector<Term>& terms, const std::vector<int>& tstart, DevProg* q) {
+  q->lm = -1;
+  if (c->use1d) return OSHUN_OK;
+  oshun::LmPlan P;
+  oshun::lm_build(c->T, terms, tstart, &P);
+  if (P.kind < 0 || (c->fp32 && P.kind != oshun::kLmFull)) return OSHUN_OK;
+  if (int rc = upload_vec(c, &q->gstart, P.gstart)) return rc;
+  if (int rc = upload_vec(c, &q->coef, P.coef)) return rc;
+  q->lm = P.kind;
+  return OSHUN_OK;
+}
+
+template <typename T, int M0, int KIND>
+void launch_lm_td(oshun_ctx* c, const oshun::StageArgs<T>& a, dim3 grid, dim3 block) {
+  if (c->T.two_d)
+    hipLaunchKernelGGL((oshun::vlasov_stage_lm<T, M0, KIND, true>), grid, block, 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((oshun::vlasov_stage_lm<T, M0, KIND, false>), grid, block, 0, c->stream, a);
+}
+template <typename T, int M0>
+int launch_lm_kind(oshun_ctx* c, const oshun::StageArgs<T>& a, int kind, dim3 grid, dim3 block) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (kind != oshun::kLmFull) return OSHUN_ERR_ARG;
+    launch_lm_td<T, M0, oshun::kLmFull>(c, a, grid, block);
+  } else {
+    switch (kind) {
+      case oshun::kLmFull: launch_lm_td<T, M0, oshun::kLmFull>(c, a, grid, block); break;
+      case oshun::kLmAdv1: launch_lm_td<T, M0, oshun::kLmAdv1>(c, a, grid, block); break;
+      case oshun::kLmAdv1Es: launch_lm_td<T, M0, oshun::kLmAdv1Es>(c, a, grid, block); break;
+      case oshun::kLmAdv2: launch_lm_td<T, M0, oshun::kLmAdv2>(c, a, grid, block); break;
+      default: return OSHUN_ERR_ARG;
+    }
+  }
   return OSHUN_OK;
 }
 
@@ -299,6 +344,7 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
   a.c11i = c->T.m0 >= 1 ? c->T.comp_im[oshun::hind(1, 1, c->T.m0)] : -1;
   a.coef = c->d_coef;
   a.ext = c->ext ? 1 : 0;
+  a.l0 = c->T.l0;
   if (prog) {
     a.terms = prog->terms;
     a.tstart = prog->tstart;
@@ -307,9 +353,25 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
   }
   dim3 block(oshun::kTileX, oshun::kTileP, 1);
   dim3 grid((c->nx + oshun::kTileX - 1) / oshun::kTileX, c->npb, c->ny);
+  // the l-streaming kernel: programs over all planes that it matches, the
+  // full program where no specialised kernel is built for the shape
+  const DevProg* lmp = prog ? (nc == c->nc ? prog : nullptr) : (c->spec < 0 ? &c->full : nullptr);
+  const int lm = (lmp && !c->force_general) ? lmp->lm : -1;
   ProfRec r;
   prof_begin(c, 0, &r);
-  if (c->spec >= 0 && !c->force_general && !prog) {
+  if (lm >= 0) {
+    a.coef = lmp->coef;
+    a.tstart = lmp->gstart;
+    int rc = OSHUN_ERR_ARG;
+    switch (c->T.m0) {
+      case 1: rc = launch_lm_kind<T, 1>(c, a, lm, grid, block); break;
+      case 2: rc = launch_lm_kind<T, 2>(c, a, lm, grid, block); break;
+      case 3: rc = launch_lm_kind<T, 3>(c, a, lm, grid, block); break;
+      case 4: rc = launch_lm_kind<T, 4>(c, a, lm, grid, block); break;
+      default: break;
+    }
+    if (rc) return fail(c, OSHUN_ERR_ARG, "no l-streaming stage kernel for this m0 and program");
+  } else if (c->spec >= 0 && !c->force_general && !prog) {
     switch (c->spec) {
 #define OSHUN_SPEC_LAUNCH(ID, L, M, TD)                                                        \
       case ID:                                                                                 \
@@ -809,8 +871,10 @@ void free_all(oshun_ctx* c) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (DevProg& q : c->iprog)
-    for (void* p : {(void*)q.terms, (void*)q.tstart, (void*)q.clean})
+    for (void* p : {(void*)q.terms, (void*)q.tstart, (void*)q.clean, (void*)q.gstart, (void*)q.coef})
       if (p) hipFree(p);
+  if (c->full.gstart) hipFree(c->full.gstart);
+  if (c->full.coef) hipFree(c->full.coef);
   if (c->h_imp_zero) hipHostFree(c->h_imp_zero);
   for (auto& r : c->prec) { hipEventDestroy(r.e0); hipEventDestroy(r.e1); }
   c->prec.clear();
@@ -1021,6 +1085,7 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
       upload_vec(c, &c->d_p3og, c->T.p3og))
     return bail(OSHUN_ERR_HIP);
   if (c->spec >= 0 && upload_vec(c, &c->d_coef, spec_coef)) return bail(OSHUN_ERR_HIP);
+  if (lm_prepare(c, c->T.terms, c->T.tstart, &c->full)) return bail(OSHUN_ERR_HIP);
   {
     std::vector<double> e;
     e.insert(e.end(), c->T.e1d_A1.begin(), c->T.e1d_A1.end());
@@ -1150,7 +1215,10 @@ int oshun_kernel_plan(const oshun_grid* grid, const oshun_phys* phys, int* stage
   c.nc = c.T.nc; c.np = g.nump; c.nx = nxa; c.ny = nya; c.nxy = nxa * nya;
   c.plane = (long long)nxa * nya * g.nump;
   plan_kernels(&c, nullptr);
-  *stage = c.use_pair ? 0 : (c.use1d ? 1 : (c.spec >= 0 ? 2 : 3));
+  oshun::LmPlan lp;
+  if (!c.use1d && c.spec < 0) oshun::lm_build(c.T, c.T.terms, c.T.tstart, &lp);
+  const bool lm = lp.kind == oshun::kLmFull;
+  *stage = c.use_pair ? 0 : (c.use1d ? 1 : (c.spec >= 0 ? 2 : (lm ? 4 : 3)));
   *collide = c.rows_npl ? 0 : 1;
   return OSHUN_OK;
 }
@@ -1479,7 +1547,7 @@ int oshun_stage_kernel(oshun_ctx* c, int set, int* which) {
   if (!sane(c)) return OSHUN_ERR_ARG;
   if (set == 0 || set == 1) c->force_general = (set == 1);
   else if (set != -1) return fail(c, OSHUN_ERR_ARG, "set must be -1, 0 or 1");
-  if (which) *which = c->use1d ? 0 : (c->spec >= 0 && !c->force_general ? 1 : 2);
+  if (which) *which = c->use1d ? 0 : (c->force_general ? 2 : (c->spec >= 0 ? 1 : (c->full.lm >= 0 ? 3 : 2)));
   return OSHUN_OK;
 }
 

@@ -74,4 +74,6 @@ def test_kernel_plan_keeps_32bit_offsets_in_range():
     nx_big = 0x7ffffff0 // (256 * 8) + 1
     assert _plan(8, 0, 256, nx_big, 1) == (1, 1)
     assert _plan(4, 4, 128, 256, 256) == (2, 0)               # C3: specialised general kernel
-    assert _plan(5, 4, 64, 16, 16)[0] == 3                    # no specialisation: interpreter
+    assert _plan(5, 4, 64, 16, 16)[0] == 4                    # no specialisation: l-streaming kernel
+    assert _plan(48, 2, 288, 640, 2)[0] == 4                  # a production deck's shape
+    assert _plan(6, 5, 64, 16, 16)[0] == 3                    # m0 > 4: interpreter

@@ -62,11 +62,11 @@ def test_spec_steps_equal_interpreter(l0, m0, np_, nx, ny, prec):
     assert np.array_equal(out[0][1], out[1][1])
 
 
-def test_shapes_without_spec_use_the_interpreter():
+def test_shapes_without_spec_use_the_lm_kernel():
     import oshun_amd
     print("case stage spec dispatch", flush=True)
-    for shape, want in (((5, 1, 16, 8, 1), "general"), ((3, 1, 16, 8, 3), "general"), ((4, 4, 16, 8, 2), "spec"),
-                        ((4, 0, 16, 8, 1), "1d")):
+    for shape, want in (((5, 1, 16, 8, 1), "lm"), ((3, 1, 16, 8, 3), "lm"), ((4, 4, 16, 8, 2), "spec"),
+                        ((4, 0, 16, 8, 1), "1d"), ((6, 5, 16, 8, 2), "general")):
         ctx = _ctx(*shape, oshun_amd.FP64)
         assert ctx.stage_kernel() == want, shape
         ctx.close()
