@@ -118,7 +118,6 @@ struct oshun_ctx {
   double src_IL = 0.0, src_IL_sqrt = 0.0, src_IL_min = 0.0, src_acoeff = 0.0, src_vos = 0.0, src_w0overwp = 0.0;
   double src_vw_coeff_cube = 0.0, src_Qn_coeff = 0.0, src_Inv_Uav6_nm1 = 0.0, src_Pnm1 = 0.0, src_p0 = 0.0, src_p1 = 0.0;
   double* d_src = nullptr;   // per-p and per-cell source tables
-  double* d_ibw = nullptr;   // inverse bremsstrahlung work planes (first use)
   int* d_srcflag = nullptr;  // a heating rate out of range
   // implicit-E step, 2D variant (implicit2d.hpp), set up by its first call:
   // the action programs (Adv1 with B, Adv1 electrostatic, Adv2, probes) and
@@ -867,7 +866,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_ibw, c->d_srcflag, c->d_imp, c->d_imp_zero, c->d_comp_m, c->d_comp_isim};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_srcflag, c->d_imp, c->d_imp_zero, c->d_comp_m, c->d_comp_isim};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (DevProg& q : c->iprog)

@@ -11,13 +11,13 @@
 //                density restored
 //   efld_kernel  ExternalEfield::Laser_Efield (:368-418)
 //
-// One thread per (x,y) cell with the p loops in the reference's order, x
-// fastest across threads so that every p row is one coalesced access: these
-// operators are per-cell sequences of p sums and p-local updates, applied
-// once per step.  Per-p and per-cell tables are formed on the host with the
-// reference's expressions (oshun_set_sources).  The inverse bremsstrahlung
-// sub-steps keep their four work columns (the integrated f00, the two RK4
-// stage inputs and the slope) in double planes [p][y][x].
+// These operators are per-cell sequences of p sums and p-local updates,
+// applied once per step.  One wavefront per (x,y) cell: the cell's column
+// (and, for the inverse bremsstrahlung sub-steps, the integrated f00, the two
+// RK4 stage inputs and the slope) in LDS, each p sum in one lane in the
+// reference's order, the p-local work across the lanes.  Per-p and per-cell
+// tables are formed on the host with the reference's expressions
+// (oshun_set_sources).
 
 namespace oshun {
 
@@ -41,7 +41,6 @@ struct SrcArgs {
   const double* bg;        // FP32 split: f00 background per p (null in FP64)
   const double* tab;       // ST_NROWS rows of np
   const double *ILx, *ILy, *Ty, *ILxe, *ILye, *xpos;
-  double *fc, *f0, *f1, *fh;  // IB work planes [p][nxy]
   double* fld;             // EFLD: the field plane written
   int* flag;               // set when a heating rate alpha leaves [0, 0.99]
   int np, nx, ny, numh;
@@ -60,69 +59,90 @@ __device__ __forceinline__ double src_row(const SrcArgs<T>& a, int k, int p) {
   return a.tab[k * a.np + p];
 }
 
+// Serial p sums of the operators below run in one lane, in the reference's
+// order (so every sum is the reference's to the last bit); the p-local work
+// runs across the wavefront's lanes.  LDS holds the cell's column.
+constexpr int kSrcLanes = 64;
+
 template <typename T>
-__global__ __launch_bounds__(256) void pls_kernel(SrcArgs<T> a) {
-  const int nxy = a.nx * a.ny;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= nxy) return;
+__global__ __launch_bounds__(kSrcLanes) void pls_kernel(SrcArgs<T> a) {
+  extern __shared__ double pls_sh[];  // the column F(p), then the unit Maxwellian gauss(p)
+  __shared__ double red[4];
+  const int N = a.np, lane = threadIdx.x;
+  const int i = blockIdx.x;
   const int ix = i % a.nx, iy = i / a.nx;
-  const int N = a.np;
+  double* Fc = pls_sh;
+  double* gs = pls_sh + N;
   T* col = a.f00 + (long long)iy * N * a.nx + ix;
-  auto F = [&](int p) -> double {
-    return static_cast<double>(col[(long long)p * a.nx]) + (a.bg ? a.bg[p] : 0.0);
-  };
+  for (int p = lane; p < N; p += kSrcLanes) Fc[p] = static_cast<double>(col[(long long)p * a.nx]) + (a.bg ? a.bg[p] : 0.0);
+  __syncthreads();
   const double* pr = a.tab + ST_P * N;
-  // density (:149-170)
-  double n0 = F(0) * pr[0] * pr[0];
-  n0 += F(N - 1) * pr[N - 1] * pr[N - 1];
-  n0 *= 0.5;
-  for (int p = 1; p < N - 1; ++p) n0 += pr[p] * pr[p] * F(p);
-  double f00 = F(1) / 5.0 * a.p0p1_sq;
-  f00 += (F(0) - F(1) * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 3.0 - 1.0 / 5.0 * a.p0p1_sq);
-  f00 *= pr[0] * (pr[0] / a.dp) * pr[0];
-  n0 += f00;
-  n0 *= 4.0 * M_PI * a.dp;
-  // temperature in eV (:176-206)
-  double vsq = F(0) * src_row(a, ST_P4, 0) / src_row(a, ST_G2, 0);
-  vsq += F(N - 1) * src_row(a, ST_P4, N - 1) / src_row(a, ST_G2, N - 1);
-  vsq *= 0.5;
-  for (int p = 1; p < N - 1; ++p) vsq += src_row(a, ST_W4, p) * F(p);
-  f00 = F(1) / 7.0 * a.p0p1_sq;
-  f00 += (F(0) - F(1) * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 5.0 - 1.0 / 7.0 * a.p0p1_sq);
-  f00 *= src_row(a, ST_P4, 0) * (pr[0] / a.dp);
-  vsq += f00;
-  vsq *= 4.0 * M_PI / 3.0 * a.dp;
+  const double* P4 = a.tab + ST_P4 * N;
+  const double* G2 = a.tab + ST_G2 * N;
+  const double* W4 = a.tab + ST_W4 * N;
+  if (lane == 0) {  // density (:149-170)
+    double n0 = Fc[0] * pr[0] * pr[0];
+    n0 += Fc[N - 1] * pr[N - 1] * pr[N - 1];
+    n0 *= 0.5;
+    for (int p = 1; p < N - 1; ++p) n0 += pr[p] * pr[p] * Fc[p];
+    double f00 = Fc[1] / 5.0 * a.p0p1_sq;
+    f00 += (Fc[0] - Fc[1] * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 3.0 - 1.0 / 5.0 * a.p0p1_sq);
+    f00 *= pr[0] * (pr[0] / a.dp) * pr[0];
+    n0 += f00;
+    n0 *= 4.0 * M_PI * a.dp;
+    red[0] = n0;
+  } else if (lane == 1) {  // temperature in eV (:176-206), before the division by n0
+    double vsq = Fc[0] * P4[0] / G2[0];
+    vsq += Fc[N - 1] * P4[N - 1] / G2[N - 1];
+    vsq *= 0.5;
+    for (int p = 1; p < N - 1; ++p) vsq += W4[p] * Fc[p];
+    double f00 = Fc[1] / 7.0 * a.p0p1_sq;
+    f00 += (Fc[0] - Fc[1] * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 5.0 - 1.0 / 7.0 * a.p0p1_sq);
+    f00 *= P4[0] * (pr[0] / a.dp);
+    vsq += f00;
+    vsq *= 4.0 * M_PI / 3.0 * a.dp;
+    red[1] = vsq;
+  }
+  __syncthreads();
+  const double n0 = red[0];
+  double vsq = red[1];
   vsq /= n0;
   vsq *= (3000.0 / 4.19) * (3000.0 / 4.19);
   // Make_Gaussian_f (:1057-1124) at T_hot = Ty + T
   double T_hot = a.Ty[iy] + vsq;
   const double PTH = 4.19 / 3000.0 * sqrt(T_hot);
-  double gnorm = 0.0;  // 1 / (its density before normalisation)
   const double e_c = -0.5 / (PTH * PTH);
   const double g_c = 1.0 / pow(sqrt(2 * M_PI) * PTH, 3);
-  auto gauss = [&](int p) { return exp((pr[p] * pr[p]) * e_c) * g_c; };
-  if (PTH > pr[1]) {
-    double g0 = gauss(0) * pr[0] * pr[0];
-    g0 += gauss(N - 1) * pr[N - 1] * pr[N - 1];
-    g0 *= 0.5;
-    for (int p = 1; p < N - 1; ++p) g0 += gauss(p) * pr[p] * pr[p];
-    double gf = gauss(1) / 5.0 * a.p0p1_sq;
-    gf += (gauss(0) - gauss(1) * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 3.0 - 1.0 / 5.0 * a.p0p1_sq);
-    gf *= pr[0] * (pr[0] / a.dp) * pr[0];
-    g0 += gf;
-    g0 *= 4.0 * M_PI * a.dp;
-    gnorm = 1.0 / g0;
-    auto fg = [&](int p) { return gauss(p) * gnorm; };
-    double vt = fg(0) * src_row(a, ST_P4, 0) / src_row(a, ST_G2, 0);
-    vt += fg(N - 1) * src_row(a, ST_P4, N - 1) / src_row(a, ST_G2, N - 1);
-    vt *= 0.5;
-    for (int p = 1; p < N - 1; ++p) vt += fg(p) * src_row(a, ST_P4, p) / src_row(a, ST_G2, p);
-    gf = fg(1) / 7.0 * a.p0p1_sq;
-    gf += (fg(0) - fg(1) * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 5.0 - 1.0 / 7.0 * a.p0p1_sq);
-    gf *= src_row(a, ST_P4, 0) * (pr[0] / a.dp);
-    vt += gf;
-    vt *= 4.0 * M_PI / 3.0 * a.dp;
-    T_hot = vt * (3000.0 / 4.19) * (3000.0 / 4.19);
+  const bool hot = PTH > pr[1];
+  if (hot) {
+    for (int p = lane; p < N; p += kSrcLanes) gs[p] = exp((pr[p] * pr[p]) * e_c) * g_c;
+    __syncthreads();
+    if (lane == 0) {
+      double g0 = gs[0] * pr[0] * pr[0];
+      g0 += gs[N - 1] * pr[N - 1] * pr[N - 1];
+      g0 *= 0.5;
+      for (int p = 1; p < N - 1; ++p) g0 += gs[p] * pr[p] * pr[p];
+      double gf = gs[1] / 5.0 * a.p0p1_sq;
+      gf += (gs[0] - gs[1] * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 3.0 - 1.0 / 5.0 * a.p0p1_sq);
+      gf *= pr[0] * (pr[0] / a.dp) * pr[0];
+      g0 += gf;
+      g0 *= 4.0 * M_PI * a.dp;
+      const double gnorm = 1.0 / g0;  // 1 / (its density before normalisation)
+      auto fg = [&](int p) { return gs[p] * gnorm; };
+      double vt = fg(0) * P4[0] / G2[0];
+      vt += fg(N - 1) * P4[N - 1] / G2[N - 1];
+      vt *= 0.5;
+      for (int p = 1; p < N - 1; ++p) vt += fg(p) * P4[p] / G2[p];
+      gf = fg(1) / 7.0 * a.p0p1_sq;
+      gf += (fg(0) - fg(1) * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 5.0 - 1.0 / 7.0 * a.p0p1_sq);
+      gf *= P4[0] * (pr[0] / a.dp);
+      vt += gf;
+      vt *= 4.0 * M_PI / 3.0 * a.dp;
+      red[2] = gnorm;
+      red[3] = vt * (3000.0 / 4.19) * (3000.0 / 4.19);
+    }
+    __syncthreads();
+    T_hot = red[3];
   } else {
     T_hot = -1.0;
   }
@@ -130,13 +150,14 @@ __global__ __launch_bounds__(256) void pls_kernel(SrcArgs<T> a) {
     double alpha = a.acoeff * a.IL_sqrt * a.dt / n0;
     alpha *= a.ILx[ix] * sqrt(a.ILy[iy] * a.ILt);
     if ((alpha < 0) || (alpha > 0.99)) {  // the reference stops the run here (:230-233)
-      atomicOr(a.flag, 1);
+      if (lane == 0) atomicOr(a.flag, 1);
       return;
     }
-    for (int p = 0; p < N; ++p) {
-      double v = F(p);
+    const double gnorm = red[2];
+    for (int p = lane; p < N; p += kSrcLanes) {
+      double v = Fc[p];
       v *= (1.0 - alpha);
-      v += alpha * (gauss(p) * gnorm) * n0;
+      v += alpha * (gs[p] * gnorm) * n0;
       col[(long long)p * a.nx] = static_cast<T>(v - (a.bg ? a.bg[p] : 0.0));
     }
   }
@@ -156,16 +177,26 @@ __device__ __forceinline__ double zlogei_v2(double ne, double Te, double density
   return 2.0 * zeta;
 }
 
-// IB_Operator::Getslope (:550-608) of the column in plane `in` into plane `fh` (rows 0..np-2)
+// sum over n = 1..N-1 of A[n] in[n] + B[n] in[n-1], in order
+__device__ __forceinline__ double ib_psum(const double* A, const double* B, const double* in, int N) {
+  double s = 0.0;
+  for (int n = 1; n < N; ++n) s += A[n] * in[n] + B[n] * in[n - 1];
+  return s;
+}
+
+// IB_Operator::Getslope (:550-608) of the column `in` into fh (rows 0..np-2),
+// both in LDS; lanes 0 and 1 form the sums I4 and I2
 template <typename T>
-__device__ void ib_slope(const SrcArgs<T>& a, const double* in, double* fh, int cell, int nxy, double vos) {
-  const int N = a.np;
-  auto fin = [&](int p) { return in[(long long)p * nxy + cell]; };
-  const double f00 = (fin(0) - fin(1) * a.p0overp1_sq) / (1.0 - a.p0overp1_sq);
-  double I4 = 0;
-  for (int n = 1; n < N; ++n) I4 += src_row(a, ST_U4, n) * fin(n) + src_row(a, ST_U4M1, n) * fin(n - 1);
-  double I2 = 0;
-  for (int n = 1; n < N; ++n) I2 += src_row(a, ST_U2, n) * fin(n) + src_row(a, ST_U2M1, n) * fin(n - 1);
+__device__ void ib_slope(const SrcArgs<T>& a, const double* in, double* fh, double* red, double vos) {
+  const int N = a.np, lane = threadIdx.x;
+  if (lane < 2) {
+    const double* A = a.tab + (lane == 0 ? ST_U4 : ST_U2) * N;
+    const double* B = a.tab + (lane == 0 ? ST_U4M1 : ST_U2M1) * N;
+    red[lane] = ib_psum(A, B, in, N);
+  }
+  __syncthreads();
+  const double I4 = red[0], I2 = red[1];
+  const double f00 = (in[0] - in[1] * a.p0overp1_sq) / (1.0 - a.p0overp1_sq);
   const double ZLn_ei = zlogei_v2(4.0 * M_PI * I2, 4.0 * M_PI * I4, a.density_np, a.zeta);
   const double vw_cube = a.vw_coeff_cube * ZLn_ei * 4.0 * M_PI * I2;
   auto gn = [&](int i) {
@@ -173,72 +204,84 @@ __device__ void ib_slope(const SrcArgs<T>& a, const double* in, double* fh, int 
     return 1.0 / (1.0 + g);
   };
   const double qn_c = a.Qn_coeff * vos * vos * ZLn_ei * 4.0 * M_PI * I2;
-  double gprev = gn(0);
-  double fprev = fin(0), fcur = fin(1);
-  fh[cell] = qn_c * src_row(a, ST_QN, 0) *
-             (src_row(a, ST_PN, 0) * gprev * (fcur - fprev) -
-              a.Pnm1 / (1.0 + a.Inv_Uav6_nm1 * vw_cube * vw_cube) * (fprev - f00));
-  for (int i = 1; i < N - 1; ++i) {
-    const double fnext = fin(i + 1), gi = gn(i);
-    fh[(long long)i * nxy + cell] = qn_c * src_row(a, ST_QN, i) *
-                                    (src_row(a, ST_PN, i) * gi * (fnext - fcur) -
-                                     src_row(a, ST_PN, i - 1) * gprev * (fcur - fprev));
-    gprev = gi;
-    fprev = fcur;
-    fcur = fnext;
+  for (int i = lane; i < N - 1; i += kSrcLanes) {
+    if (i == 0) {
+      fh[0] = qn_c * src_row(a, ST_QN, 0) *
+              (src_row(a, ST_PN, 0) * gn(0) * (in[1] - in[0]) -
+               a.Pnm1 / (1.0 + a.Inv_Uav6_nm1 * vw_cube * vw_cube) * (in[0] - f00));
+    } else {
+      fh[i] = qn_c * src_row(a, ST_QN, i) *
+              (src_row(a, ST_PN, i) * gn(i) * (in[i + 1] - in[i]) -
+               src_row(a, ST_PN, i - 1) * gn(i - 1) * (in[i] - in[i - 1]));
+    }
   }
+  __syncthreads();
 }
 
+// InverseBremsstrahlung::loop (:780-861) of one interior cell per workgroup
 template <typename T>
-__global__ __launch_bounds__(256) void ib_kernel(SrcArgs<T> a) {
-  const int nxy = a.nx * a.ny;
-  const int w = blockIdx.x * 256 + threadIdx.x;
-  if (w >= a.cnx * a.cny) return;
+__global__ __launch_bounds__(kSrcLanes) void ib_kernel(SrcArgs<T> a) {
+  extern __shared__ double ib_sh[];  // the integrated column fc, the stage inputs f0, f1 and the slope fh
+  __shared__ double red[2];
+  const int N = a.np, lane = threadIdx.x;
+  const int w = blockIdx.x;
   const int ix = a.cx0 + w % a.cnx, iy = a.cy0 + w / a.cnx;
-  const int cell = iy * a.nx + ix;
-  const int N = a.np;
+  double* fc = ib_sh;
+  double* f0 = ib_sh + N;
+  double* f1 = ib_sh + 2 * N;
+  double* fh = ib_sh + 3 * N;
   T* col = a.f00 + (long long)iy * N * a.nx + ix;
   const double vos = a.vos * sqrt(a.ILxe[ix] * a.ILye[iy]);
-  auto W = [&](double* w, int p) -> double& { return w[(long long)p * nxy + cell]; };
-  for (int p = 0; p < N; ++p) W(a.fc, p) = static_cast<double>(col[(long long)p * a.nx]) + (a.bg ? a.bg[p] : 0.0);
-  W(a.fh, N - 1) = 0.0;  // never written by Getslope
-  double I2_before = 0.0;
-  for (int n = 1; n < N; ++n)
-    I2_before += src_row(a, ST_U2P, n) * W(a.fc, n) + src_row(a, ST_U2PM1, n) * W(a.fc, n - 1);
+  for (int p = lane; p < N; p += kSrcLanes) {
+    fc[p] = static_cast<double>(col[(long long)p * a.nx]) + (a.bg ? a.bg[p] : 0.0);
+    fh[p] = 0.0;  // row np-1 is never written by Getslope
+  }
+  __syncthreads();
+  if (lane == 0) red[0] = ib_psum(a.tab + ST_U2P * N, a.tab + ST_U2PM1 * N, fc, N);
+  __syncthreads();
+  const double I2_before = red[0];
   const double h = a.h;
   for (int s = 0; s < a.numh; ++s) {  // RungeKutta4_ib::advance (:713-757)
-    for (int p = 0; p < N; ++p) { W(a.f0, p) = W(a.fc, p); W(a.f1, p) = W(a.fc, p); }
-    ib_slope(a, a.f1, a.fh, cell, nxy, vos);
-    for (int p = 0; p < N; ++p) {
-      double& fh = W(a.fh, p);
-      fh *= (0.5 * h); W(a.f1, p) += fh;
-      fh *= (1.0 / 3.0); W(a.fc, p) += fh;
+    for (int p = lane; p < N; p += kSrcLanes) { f0[p] = fc[p]; f1[p] = fc[p]; }
+    __syncthreads();
+    ib_slope(a, f1, fh, red, vos);
+    for (int p = lane; p < N; p += kSrcLanes) {
+      double q = fh[p];
+      q *= (0.5 * h); f1[p] += q;
+      q *= (1.0 / 3.0); fc[p] += q;
+      fh[p] = q;
     }
-    ib_slope(a, a.f1, a.fh, cell, nxy, vos);
-    for (int p = 0; p < N; ++p) {
-      double& fh = W(a.fh, p);
-      W(a.f1, p) = W(a.f0, p);
-      fh *= (0.5 * h); W(a.f1, p) += fh;
-      fh *= (2.0 / 3.0); W(a.fc, p) += fh;
+    __syncthreads();
+    ib_slope(a, f1, fh, red, vos);
+    for (int p = lane; p < N; p += kSrcLanes) {
+      double q = fh[p];
+      f1[p] = f0[p];
+      q *= (0.5 * h); f1[p] += q;
+      q *= (2.0 / 3.0); fc[p] += q;
+      fh[p] = q;
     }
-    ib_slope(a, a.f1, a.fh, cell, nxy, vos);
-    for (int p = 0; p < N; ++p) {
-      double& fh = W(a.fh, p);
-      fh *= h; W(a.f0, p) += fh;
-      fh *= (1.0 / 3.0); W(a.fc, p) += fh;
+    __syncthreads();
+    ib_slope(a, f1, fh, red, vos);
+    for (int p = lane; p < N; p += kSrcLanes) {
+      double q = fh[p];
+      q *= h; f0[p] += q;
+      q *= (1.0 / 3.0); fc[p] += q;
+      fh[p] = q;
     }
-    ib_slope(a, a.f0, a.fh, cell, nxy, vos);
-    for (int p = 0; p < N; ++p) {
-      double& fh = W(a.fh, p);
-      fh *= (h / 6.0); W(a.fc, p) += fh;
+    __syncthreads();
+    ib_slope(a, f0, fh, red, vos);
+    for (int p = lane; p < N; p += kSrcLanes) {
+      double q = fh[p];
+      q *= (h / 6.0); fc[p] += q;
+      fh[p] = q;
     }
+    __syncthreads();
   }
-  double I2_after = 0.0;
-  for (int n = 1; n < N; ++n)
-    I2_after += src_row(a, ST_U2P, n) * W(a.fc, n) + src_row(a, ST_U2PM1, n) * W(a.fc, n - 1);
-  const double r = I2_before / I2_after;
-  for (int p = 0; p < N; ++p) {
-    double v = W(a.fc, p);
+  if (lane == 0) red[0] = ib_psum(a.tab + ST_U2P * N, a.tab + ST_U2PM1 * N, fc, N);
+  __syncthreads();
+  const double r = I2_before / red[0];
+  for (int p = lane; p < N; p += kSrcLanes) {
+    double v = fc[p];
     v *= r;
     col[(long long)p * a.nx] = static_cast<T>(v - (a.bg ? a.bg[p] : 0.0));
   }
@@ -313,23 +356,21 @@ int launch_pls(oshun_ctx* c, double tnew) {
   oshun::SrcArgs<T> a = src_args<T>(c);
   a.ILt = ILt;
   a.dt = dt;
-  hipLaunchKernelGGL(oshun::pls_kernel<T>, dim3((c->nxy + 255) / 256), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(oshun::pls_kernel<T>, dim3(c->nxy), dim3(oshun::kSrcLanes), 2 * c->np * sizeof(double), c->stream,
+                     a);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
 
 template <typename T>
 int launch_ib(oshun_ctx* c, double tnew) {
-  const size_t work = (size_t)c->np * c->nxy;
-  if (!c->d_ibw && hipMalloc((void**)&c->d_ibw, 4 * work * sizeof(double)) != hipSuccess)
-    return fail(c, OSHUN_ERR_ALLOC, "inverse bremsstrahlung: work planes");
   oshun::SrcArgs<T> a = src_args<T>(c);
-  a.fc = c->d_ibw; a.f0 = c->d_ibw + work; a.f1 = c->d_ibw + 2 * work; a.fh = c->d_ibw + 3 * work;
   const int numh = static_cast<int>(static_cast<size_t>(static_cast<int>((tnew - c->src_t_ib) / c->src.smaller_dt)) + 1);
   a.numh = numh;
   a.h = (tnew - c->src_t_ib) / static_cast<double>(static_cast<size_t>(numh));
   c->src_t_ib = tnew;
-  hipLaunchKernelGGL(oshun::ib_kernel<T>, dim3((c->nxi * c->nyi + 255) / 256), dim3(256), 0, c->stream, a);
+  hipLaunchKernelGGL(oshun::ib_kernel<T>, dim3(c->nxi * c->nyi), dim3(oshun::kSrcLanes), 4 * c->np * sizeof(double),
+                     c->stream, a);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
