@@ -267,7 +267,7 @@ int imp2d_setup(oshun_ctx* c) {
     if ((rc = upload_vec(c, &c->iprog[q].terms, P.terms))) return rc;
     if ((rc = upload_vec(c, &c->iprog[q].tstart, P.tstart))) return rc;
     if ((rc = upload_vec(c, &c->iprog[q].clean, P.clean))) return rc;
-    if ((rc = lm_prepare(c, P.terms, P.tstart, &c->iprog[q]))) return rc;
+    if ((rc = lm_prepare(c, P.terms, P.tstart, P.clean, &c->iprog[q]))) return rc;
   }
   const size_t n = (size_t)(18 + 12 + 3) * c->nxy;
   HIPCHK(c, hipMalloc((void**)&c->d_imp, n * sizeof(double)));

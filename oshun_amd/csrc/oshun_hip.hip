@@ -43,8 +43,9 @@ struct DevProg {
   int* tstart = nullptr;
   int* clean = nullptr;
   int lm = -1;             // LmKind of the l-streaming kernel that runs it, -1: the interpreter
-  int* gstart = nullptr;   // that kernel's coefficients (LmPlan)
+  int* gstart = nullptr;   // that kernel's coefficients and interior layout (LmPlan)
   double* coef = nullptr;
+  int lm_istart = 0, lm_istride = 0, lm_ioff[5] = {0, 0, 0, 0, 0};
 };
 
 struct oshun_ctx {
@@ -163,14 +164,18 @@ int upload_vec(oshun_ctx* c, V** dst, const std::vector<V>& src) {
 // planes): q->lm = the program kind it runs and q->gstart, q->coef its
 // coefficients, or q->lm = -1 (the interpreter runs the program; FP32
 // storage has the full program's kernel only)
-int lm_prepare(oshun_ctx* c, const std::vector<Term>& terms, const std::vector<int>& tstart, DevProg* q) {
+int lm_prepare(oshun_ctx* c, const std::vector<Term>& terms, const std::vector<int>& tstart,
+               const std::vector<int>& clean, DevProg* q) {
   q->lm = -1;
   if (c->use1d) return OSHUN_OK;
   oshun::LmPlan P;
-  oshun::lm_build(c->T, terms, tstart, &P);
+  oshun::lm_build(c->T, terms, tstart, clean, &P);
   if (P.kind < 0 || (c->fp32 && P.kind != oshun::kLmFull)) return OSHUN_OK;
   if (int rc = upload_vec(c, &q->gstart, P.gstart)) return rc;
   if (int rc = upload_vec(c, &q->coef, P.coef)) return rc;
+  q->lm_istart = P.istart;
+  q->lm_istride = P.istride;
+  for (int m = 0; m <= oshun::kLmMaxM0; ++m) q->lm_ioff[m] = P.ioff[m];
   q->lm = P.kind;
   return OSHUN_OK;
 }
@@ -355,12 +360,21 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
   // the l-streaming kernel: programs over all planes that it matches, the
   // full program where no specialised kernel is built for the shape
   const DevProg* lmp = prog ? (nc == c->nc ? prog : nullptr) : (c->spec < 0 ? &c->full : nullptr);
-  const int lm = (lmp && !c->force_general) ? lmp->lm : -1;
+  int lm = (lmp && !c->force_general) ? lmp->lm : -1;
+  // the kernels run the full program in modes 0..4 (1 with y0 the stage
+  // input) and the implicit ones in modes 0, 6 and 5 (y0 the stage input);
+  // lane byte offsets within a plane are 32-bit
+  if (lm == oshun::kLmFull && !(mode == 0 || mode == 2 || mode == 3 || mode == 4 || (mode == 1 && y0 == in))) lm = -1;
+  if (lm > 0 && !(mode == 0 || mode == 6 || (mode == 5 && y0 == in))) lm = -1;
+  if ((unsigned long long)c->plane * sizeof(T) >= (1ull << 32)) lm = -1;
   ProfRec r;
   prof_begin(c, 0, &r);
   if (lm >= 0) {
     a.coef = lmp->coef;
     a.tstart = lmp->gstart;
+    a.lm_istart = lmp->lm_istart;
+    a.lm_istride = lmp->lm_istride;
+    for (int m = 0; m <= oshun::kLmMaxM0; ++m) a.lm_ioff[m] = lmp->lm_ioff[m];
     int rc = OSHUN_ERR_ARG;
     switch (c->T.m0) {
       case 1: rc = launch_lm_kind<T, 1>(c, a, lm, grid, block); break;
@@ -1084,7 +1098,7 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
       upload_vec(c, &c->d_p3og, c->T.p3og))
     return bail(OSHUN_ERR_HIP);
   if (c->spec >= 0 && upload_vec(c, &c->d_coef, spec_coef)) return bail(OSHUN_ERR_HIP);
-  if (lm_prepare(c, c->T.terms, c->T.tstart, &c->full)) return bail(OSHUN_ERR_HIP);
+  if (lm_prepare(c, c->T.terms, c->T.tstart, c->T.clean, &c->full)) return bail(OSHUN_ERR_HIP);
   {
     std::vector<double> e;
     e.insert(e.end(), c->T.e1d_A1.begin(), c->T.e1d_A1.end());
@@ -1215,7 +1229,7 @@ int oshun_kernel_plan(const oshun_grid* grid, const oshun_phys* phys, int* stage
   c.plane = (long long)nxa * nya * g.nump;
   plan_kernels(&c, nullptr);
   oshun::LmPlan lp;
-  if (!c.use1d && c.spec < 0) oshun::lm_build(c.T, c.T.terms, c.T.tstart, &lp);
+  if (!c.use1d && c.spec < 0) oshun::lm_build(c.T, c.T.terms, c.T.tstart, c.T.clean, &lp);
   const bool lm = lp.kind == oshun::kLmFull;
   *stage = c.use_pair ? 0 : (c.use1d ? 1 : (c.spec >= 0 ? 2 : (lm ? 4 : 3)));
   *collide = c.rows_npl ? 0 : 1;

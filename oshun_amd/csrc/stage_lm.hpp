@@ -7,10 +7,10 @@
 // (def-VlasovMax.cpp:346-644, 757-951), so the thread walks l upward and
 // keeps three layers of source values in registers: for layer l-1 the G of
 // MakeGH / G00 and the x, y differences, for layer l the value itself (the B
-// rotation), for layer l+1 the H and the differences.  Each layer's loads are
-// issued one l ahead of its use, after the RK operands of the layer being
-// written (loads complete in order, so the update's wait leaves the
-// look-ahead in flight).
+// rotation), for layer l+1 the H and the differences.  At each l the stencil
+// of layer l+1 is loaded and formed together with the RK operands of layer l;
+// the other resident workgroups cover that wait (no look-ahead buffer, which
+// would cost the registers that keep several waves per SIMD resident).
 //
 // The targets' sums are formed in the coupling program's order (tables.hpp
 // build_tables: E field by source, m outer and l inner; then x advection, y
@@ -45,6 +45,12 @@ __host__ __device__ constexpr int lm_mask(int kind) {
   return kind == kLmFull ? (LM_E | LM_X | LM_Y | LM_B)
                          : (kind == kLmAdv1 ? (LM_X | LM_Y | LM_B) : (kind == kLmAdv1Es ? (LM_X | LM_Y) : LM_E));
 }
+// CLEAN_LOW_PC rows of a plane of harmonic l (def-Actions.cpp:240-253) in the
+// programs that have them (Actions::Adv and Adv2; tables.hpp build_tables)
+__host__ __device__ constexpr int lm_clean_rows(int kind, int l) {
+  return (kind == kLmFull || kind == kLmAdv2) ? (l >= 1 ? (l - 1 < 80 ? l - 1 : 80) : 0) : 0;
+}
+
 constexpr int kLmMaxM0 = 4;
 constexpr int kLmNone = -1;  // no field term in Mr (Mi)
 
@@ -177,6 +183,9 @@ struct LmPlan {
   int kind = -1;  // LmKind, or -1: the interpreter runs the program
   std::vector<int> gstart;
   std::vector<double> coef;
+  // interior layers l = m0+1 .. l0-1: the coefficients of target (l, m)
+  // start at istart + (l - m0 - 1) istride + ioff[m]
+  int istart = 0, istride = 0, ioff[kLmMaxM0 + 1] = {0, 0, 0, 0, 0};
 };
 
 // True (and *P filled) when the program (terms, tstart over all planes)
@@ -184,8 +193,11 @@ struct LmPlan {
 // calls for KIND, in the same order (source plane, operator, multiplier, and
 // source l), with each call's c and d consistent across its terms.
 template <int M0, int KIND, bool TD>
-bool lm_build_kind(const HostTables& T, const std::vector<Term>& terms, const std::vector<int>& tstart, LmPlan* P) {
-  if (T.m0 != M0 || T.two_d != TD || (int)tstart.size() != T.nc + 1) return false;
+bool lm_build_kind(const HostTables& T, const std::vector<Term>& terms, const std::vector<int>& tstart,
+                   const std::vector<int>& clean, LmPlan* P) {
+  if (T.m0 != M0 || T.two_d != TD || (int)tstart.size() != T.nc + 1 || (int)clean.size() != T.nc) return false;
+  for (int c = 0; c < T.nc; ++c)
+    if (clean[c] != lm_clean_rows(KIND, T.comp_l[c])) return false;
   P->gstart.assign(T.nc + 1, 0);
   P->coef.clear();
   for (int l = 0; l <= T.l0; ++l)
@@ -222,28 +234,44 @@ bool lm_build_kind(const HostTables& T, const std::vector<Term>& terms, const st
       }
     }
   P->gstart[T.nc] = (int)P->coef.size();
+  // the interior layout (the same calls at every interior l)
+  const int l1 = M0 + 1;
+  if (l1 <= T.l0 - 1) {
+    P->istart = P->gstart[lm_layer_start(l1, M0)];
+    P->istride = l1 + 1 <= T.l0 - 1 ? P->gstart[lm_layer_start(l1 + 1, M0)] - P->istart : 0;
+    for (int m = 0; m <= M0; ++m) P->ioff[m] = P->gstart[lm_layer_start(l1, M0) + lm_slot(m, 0)] - P->istart;
+    for (int l = l1; l <= T.l0 - 1; ++l)
+      for (int m = 0; m <= M0; ++m)
+        if (P->gstart[lm_layer_start(l, M0) + lm_slot(m, 0)] != P->istart + (l - l1) * P->istride + P->ioff[m])
+          return false;
+  }
   P->kind = KIND;
   return true;
 }
 
 template <int M0, bool TD>
-bool lm_build_m(const HostTables& T, const std::vector<Term>& terms, const std::vector<int>& tstart, LmPlan* P) {
-  return lm_build_kind<M0, kLmFull, TD>(T, terms, tstart, P) || lm_build_kind<M0, kLmAdv1, TD>(T, terms, tstart, P) ||
-         lm_build_kind<M0, kLmAdv1Es, TD>(T, terms, tstart, P) || lm_build_kind<M0, kLmAdv2, TD>(T, terms, tstart, P);
+bool lm_build_m(const HostTables& T, const std::vector<Term>& terms, const std::vector<int>& tstart,
+                const std::vector<int>& clean, LmPlan* P) {
+  return lm_build_kind<M0, kLmFull, TD>(T, terms, tstart, clean, P) ||
+         lm_build_kind<M0, kLmAdv1, TD>(T, terms, tstart, clean, P) ||
+         lm_build_kind<M0, kLmAdv1Es, TD>(T, terms, tstart, clean, P) ||
+         lm_build_kind<M0, kLmAdv2, TD>(T, terms, tstart, clean, P);
 }
 
-// The kernel's plan of a program; P->kind = -1 when no kernel matches it
-inline void lm_build(const HostTables& T, const std::vector<Term>& terms, const std::vector<int>& tstart, LmPlan* P) {
+// The kernel's plan of a program (terms, tstart, CLEAN_LOW_PC rows over all
+// planes); P->kind = -1 when no kernel matches it
+inline void lm_build(const HostTables& T, const std::vector<Term>& terms, const std::vector<int>& tstart,
+                     const std::vector<int>& clean, LmPlan* P) {
   bool ok = false;
   switch (T.m0 * 2 + (T.two_d ? 1 : 0)) {
-    case 2: ok = lm_build_m<1, false>(T, terms, tstart, P); break;
-    case 3: ok = lm_build_m<1, true>(T, terms, tstart, P); break;
-    case 4: ok = lm_build_m<2, false>(T, terms, tstart, P); break;
-    case 5: ok = lm_build_m<2, true>(T, terms, tstart, P); break;
-    case 6: ok = lm_build_m<3, false>(T, terms, tstart, P); break;
-    case 7: ok = lm_build_m<3, true>(T, terms, tstart, P); break;
-    case 8: ok = lm_build_m<4, false>(T, terms, tstart, P); break;
-    case 9: ok = lm_build_m<4, true>(T, terms, tstart, P); break;
+    case 2: ok = lm_build_m<1, false>(T, terms, tstart, clean, P); break;
+    case 3: ok = lm_build_m<1, true>(T, terms, tstart, clean, P); break;
+    case 4: ok = lm_build_m<2, false>(T, terms, tstart, clean, P); break;
+    case 5: ok = lm_build_m<2, true>(T, terms, tstart, clean, P); break;
+    case 6: ok = lm_build_m<3, false>(T, terms, tstart, clean, P); break;
+    case 7: ok = lm_build_m<3, true>(T, terms, tstart, clean, P); break;
+    case 8: ok = lm_build_m<4, false>(T, terms, tstart, clean, P); break;
+    case 9: ok = lm_build_m<4, true>(T, terms, tstart, clean, P); break;
     default: break;
   }
   if (!ok) {
@@ -310,13 +338,17 @@ struct LmRaw {
   T fm[NS], f0[NS], fp[NS], xm[NS], xp[NS], ym[NS], yp[NS];
 };
 
-// KIND: the program (LmKind); TD: it has y advection.  a.coef and a.tstart
-// are an LmPlan's coef and gstart.
+// KIND: the program (LmKind); TD: it has y advection.  a.coef, a.tstart and
+// a.lm_* are an LmPlan's coef, gstart and interior layout; a.mode is one of
+// the modes the kind runs (launch_stage: 0..4 with y0 the stage input in
+// mode 1 for the full program, 0, 5 with y0 the stage input and 6 for the
+// implicit ones).
 template <typename T, int M0, int KIND, bool TD>
 __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a) {
   using Raw = LmRaw<T, M0>;
   constexpr int NS = 2 * M0 + 1;
   constexpr int MASK = lm_mask(KIND);
+  constexpr bool RK4 = KIND == kLmFull;
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int x = blockIdx.x * kTileX + tx;
   const int p = blockIdx.y * kTileP + ty;
@@ -344,7 +376,6 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
     typedef const __attribute__((address_space(4))) T* TabPtr;
     const TabPtr hp0 = (TabPtr)a.hp0;
     const TabPtr cg = (TabPtr)a.cg;
-    const T* base = a.in + idx;
     // in-bounds offsets of every stencil load (a clamped one is not used:
     // p - 1 at p = 0, p + 1 at p = np - 1, the y pair of a 1D program); the
     // "last" points' subtrahend is zeroed after the load
@@ -352,36 +383,47 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
     const long long pp_off = p < np - 1 ? (long long)nx : 0;
     if (!TD) ym_off = yp_off = 0;
     const bool yzero = ylast || !TD;
-    // the stencil of layer l's planes (m <= min(l, M0))
-    auto load = [&](int l, Raw& r) {
-      const T* b = base + (long long)lm_layer_start(l, M0) * a.plane;
+    // Addresses: a wave-uniform plane base plus the lane's 32-bit byte offset
+    // within the plane (the host keeps a plane below 4 GiB), so that every
+    // load and store takes a scalar base and one offset register per stencil
+    // point instead of a 64-bit address per plane and point.
+    auto boff = [&](long long o) { return static_cast<unsigned>((idx + o) * (long long)sizeof(T)); };
+    const unsigned o0 = boff(0), opm = boff(pm_off), opp = boff(pp_off), oxm = boff(xm_off), oxp = boff(xp_off),
+                   oym = boff(ym_off), oyp = boff(yp_off);
+    const size_t pbytes = (size_t)a.plane * sizeof(T);
+    auto at = [](const void* b, unsigned o) -> const T& { return *reinterpret_cast<const T*>((const char*)b + o); };
+    auto atw = [](void* b, unsigned o) -> T& { return *reinterpret_cast<T*>((char*)b + o); };
+    // the stencil of layer l's planes (m <= min(l, M0)); IN: all 2 M0 + 1
+    auto load = [&](auto inc, int l, Raw& r) {
+      constexpr bool IN = decltype(inc)::value;
+      const char* lb = (const char*)a.in + (size_t)lm_layer_start(l, M0) * pbytes;
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
-        if (k > 2 * l) continue;
-        const T* s = b + (long long)k * a.plane;
-        r.f0[k] = s[0];
-        r.fm[k] = s[pm_off];
-        r.fp[k] = s[pp_off];
-        r.xm[k] = s[xm_off];
-        r.xp[k] = s[xp_off];
+        if (!IN && k > 2 * l) continue;
+        const char* s = lb + (size_t)k * pbytes;
+        r.f0[k] = at(s, o0);
+        r.fm[k] = at(s, opm);
+        r.fp[k] = at(s, opp);
+        r.xm[k] = at(s, oxm);
+        r.xp[k] = at(s, oxp);
         if (TD) {
-          r.ym[k] = s[ym_off];
-          r.yp[k] = s[yp_off];
+          r.ym[k] = at(s, oym);
+          r.yp[k] = at(s, oyp);
         } else {
           r.ym[k] = T(0);
           r.yp[k] = T(0);
         }
       }
     };
-    // G, H (MakeGH and G00, formed as op_value forms them), differences and
-    // value of layer l into layer slot J
-    auto form = [&](auto jc, int l, const Raw& r) {
+    // G, H (MakeGH and G00, formed as op_value forms them) and differences of
+    // layer l into layer slot J; IN: l >= 1 and all slots
+    auto form = [&](auto jc, auto inc, int l, const Raw& r) {
       constexpr int J = decltype(jc)::value;
+      constexpr bool IN = decltype(inc)::value;
 #pragma unroll
       for (int k = 0; k < NS; ++k) {
-        if (k > 2 * l) continue;
+        if (!IN && k > 2 * l) continue;
         const T f0 = r.f0[k];
-        e.f[J][k] = f0;
         e.Dx[J][k] = r.xm[k] - (xlast ? T(0) : r.xp[k]);
         e.Dy[J][k] = r.ym[k] - (yzero ? T(0) : r.yp[k]);
         // values first, then one unconditional store of each (stores under
@@ -389,7 +431,7 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
         T gv, hv = T(0);
         if (p == 0) {
           const T f1 = r.fp[k];
-          if (l == 0) {
+          if (!IN && l == 0) {
             const T fe = (f0 - f1 * a.p0p1_sq) * a.inv_mp0p1_sq;
             gv = (f1 - fe) * a.g_r;
             if (a.bgG) gv += bgg;
@@ -400,7 +442,7 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
         } else {
           const T fm = r.fm[k];
           const T dpf = (p == np - 1) ? (fm - f0) : (fm - r.fp[k]);
-          if (l == 0) {
+          if (!IN && l == 0) {
             gv = dpf;
             if (a.bgG) gv += bgg;
           } else {
@@ -414,42 +456,46 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
       }
     };
     {
-      Raw r0, r1;
-      load(0, r0);
-      if (L >= 1) load(1, r1);
-      form(std::integral_constant<int, 1>{}, 0, r0);
-      if (L >= 1) form(std::integral_constant<int, 2>{}, 1, r1);
+      Raw r0;
+      load(std::false_type{}, 0, r0);
+      form(std::integral_constant<int, 1>{}, std::false_type{}, 0, r0);
     }
     const LmCoefPtr coef = (LmCoefPtr)a.coef;
     const LmIntPtr gstart = (LmIntPtr)a.tstart;
-    const LmIntPtr clean = (LmIntPtr)a.clean;
-    const int mode = a.mode;
-    // RK operands read from memory: the accumulator (modes 2, 3, 4, 6) and
-    // y0 (modes 2, 3; modes 1 and 5 too unless y0 is the stage input, whose
-    // value the layer already holds)
-    const bool y0in = a.y0 == a.in;
-    const bool need_acc = mode == 2 || mode == 3 || mode == 4 || mode == 6;
-    const bool need_y0 = mode == 2 || mode == 3 || ((mode == 1 || mode == 5) && !y0in);
-    for (int l = 0; l <= L; ++l) {
-      const int c0 = lm_layer_start(l, M0);
-      T opa[NS], opy[NS];
-#pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        if (k > 2 * l) continue;
-        const long long o = (long long)(c0 + k) * a.plane + idx;
-        opa[k] = need_acc ? a.acc[o] : T(0);
-        opy[k] = need_y0 ? a.y0[o] : T(0);
-      }
-      Raw rn;
-      if (l + 2 <= L) load(l + 2, rn);  // one layer ahead
-      // the sums of layer l's targets
-      T sum[NS];
-      auto layer = [&](auto inc) {
+    // one RK mode, all layers
+    auto run = [&](auto modec) {
+      constexpr int MODE = decltype(modec)::value;
+      // read with the next layer's stencil at the top of each l: the
+      // accumulator, y0 (modes 2, 3; in modes 1 and 5 it is the stage input)
+      // and the layer's own values (the B rotation, and y0 = the input)
+      constexpr bool NEED_ACC = MODE == 2 || MODE == 3 || MODE == 4 || MODE == 6;
+      constexpr bool NEED_Y0 = MODE == 2 || MODE == 3;
+      constexpr bool NEED_F = (MASK & LM_B) != 0 || MODE == 1 || MODE == 5;
+      // layer l; IN: l is interior (every neighbour exists, every slot too)
+      auto body = [&](auto inc, int l) {
         constexpr bool IN = decltype(inc)::value;
+        const int c0 = lm_layer_start(l, M0);
+        const size_t lo = (size_t)c0 * pbytes;
+        T opa[NS], opy[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          if (!IN && k > 2 * l) continue;
+          if constexpr (NEED_ACC) opa[k] = at((const char*)a.acc + lo + k * pbytes, o0);
+          if constexpr (NEED_Y0) opy[k] = at((const char*)a.y0 + lo + k * pbytes, o0);
+          if constexpr (NEED_F) e.f[1][k] = at((const char*)a.in + lo + k * pbytes, o0);
+        }
+        if (IN || l + 1 <= L) {  // layer l+1, which the sums of layer l read
+          Raw rn;
+          load(inc, l + 1, rn);
+          form(std::integral_constant<int, 2>{}, inc, l + 1, rn);
+        }
+        // the sums of layer l's targets
+        T sum[NS];
+        const int ib = a.lm_istart + (l - M0 - 1) * a.lm_istride;
         auto target = [&](auto tmc) {
           constexpr int TM = decltype(tmc)::value;
           if (!IN && TM > l) return;
-          e.cf = coef + gstart[c0 + lm_slot(TM, 0)];
+          e.cf = IN ? coef + ib + a.lm_ioff[TM] : coef + gstart[c0 + lm_slot(TM, 0)];
           e.kc = 0;
           e.sa = T(0);
           e.sb = T(0);
@@ -462,64 +508,72 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
         if constexpr (M0 >= 2) target(std::integral_constant<int, 2>{});
         if constexpr (M0 >= 3) target(std::integral_constant<int, 3>{});
         if constexpr (M0 >= 4) target(std::integral_constant<int, 4>{});
-      };
-      if (l >= M0 + 1 && l <= L - 1) layer(std::true_type{});
-      else layer(std::false_type{});
-      // RK update of layer l's planes, def-RK.cpp:116-138 and :478-488, as vlasov_stage
+        // RK update of layer l's planes, def-RK.cpp:116-138 and :478-488, as vlasov_stage
+        const int rows = lm_clean_rows(KIND, l);
 #pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        if (k > 2 * l) continue;
-        const int tc = c0 + k;
-        T s = sum[k];
-        if (p < clean[tc]) s = T(0);  // CLEAN_LOW_PC, def-Actions.cpp:240-253
-        const long long o = (long long)tc * a.plane + idx;
-        const T y0 = y0in ? e.f[1][k] : opy[k];
-        switch (mode) {
-          case 0:
-            a.out[o] = s;
-            break;
-          case 1: {
+        for (int k = 0; k < NS; ++k) {
+          if (!IN && k > 2 * l) continue;
+          T s = sum[k];
+          if (p < rows) s = T(0);  // CLEAN_LOW_PC, def-Actions.cpp:240-253
+          T& out = atw((char*)a.out + lo + k * pbytes, o0);
+          T& acc = atw((char*)a.acc + lo + k * pbytes, o0);
+          if constexpr (MODE == 0) {
+            out = s;
+          } else if constexpr (MODE == 1) {
             const T kh = s * a.c05h;
-            a.out[o] = y0 + kh;
-            a.acc[o] = y0 + kh * a.third;
-          } break;
-          case 2: {
+            out = e.f[1][k] + kh;
+            acc = e.f[1][k] + kh * a.third;
+          } else if constexpr (MODE == 2) {
             const T kh = s * a.c05h;
-            a.out[o] = opy[k] + kh;
-            a.acc[o] = opa[k] + kh * a.twothird;
-          } break;
-          case 3: {
+            out = opy[k] + kh;
+            acc = opa[k] + kh * a.twothird;
+          } else if constexpr (MODE == 3) {
             const T kh = s * a.hh;
-            a.out[o] = opy[k] + kh;
-            a.acc[o] = opa[k] + kh * a.third;
-          } break;
-          case 5: {
+            out = opy[k] + kh;
+            acc = opa[k] + kh * a.third;
+          } else if constexpr (MODE == 5) {
             const T kh = s * a.hh;
-            a.out[o] = y0 + kh;
-            a.acc[o] = y0 + kh * T(0.5);
-          } break;
-          case 6: {
+            out = e.f[1][k] + kh;
+            acc = e.f[1][k] + kh * T(0.5);
+          } else if constexpr (MODE == 6) {
             const T kh = s * a.c05h;
-            a.out[o] = opa[k] + kh;
-          } break;
-          default: {
+            out = opa[k] + kh;
+          } else {
             const T kh = s * a.h6;
-            a.out[o] = opa[k] + kh;
-          } break;
+            out = opa[k] + kh;
+          }
         }
-      }
-      // l-1 <- l, l <- l+1, l+1 <- l+2
+        // l-1 <- l, l <- l+1
 #pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        e.G[0][k] = e.G[1][k];
-        e.Dx[0][k] = e.Dx[1][k];
-        e.Dy[0][k] = e.Dy[1][k];
-        e.G[1][k] = e.G[2][k];
-        e.Dx[1][k] = e.Dx[2][k];
-        e.Dy[1][k] = e.Dy[2][k];
-        e.f[1][k] = e.f[2][k];
+        for (int k = 0; k < NS; ++k) {
+          e.G[0][k] = e.G[1][k];
+          e.Dx[0][k] = e.Dx[1][k];
+          e.Dy[0][k] = e.Dy[1][k];
+          e.G[1][k] = e.G[2][k];
+          e.Dx[1][k] = e.Dx[2][k];
+          e.Dy[1][k] = e.Dy[2][k];
+        }
+      };
+      // layers 0..m0 and L at the edges, m0+1..L-1 interior
+      const int le = L < M0 ? L : M0;
+      for (int l = 0; l <= le; ++l) body(std::false_type{}, l);
+      for (int l = M0 + 1; l <= L - 1; ++l) body(std::true_type{}, l);
+      if (L >= M0 + 1) body(std::false_type{}, L);
+    };
+    if constexpr (RK4) {
+      switch (a.mode) {
+        case 0: run(std::integral_constant<int, 0>{}); break;
+        case 1: run(std::integral_constant<int, 1>{}); break;
+        case 2: run(std::integral_constant<int, 2>{}); break;
+        case 3: run(std::integral_constant<int, 3>{}); break;
+        default: run(std::integral_constant<int, 4>{}); break;
       }
-      if (l + 2 <= L) form(std::integral_constant<int, 2>{}, l + 2, rn);
+    } else {
+      switch (a.mode) {
+        case 0: run(std::integral_constant<int, 0>{}); break;
+        case 5: run(std::integral_constant<int, 5>{}); break;
+        default: run(std::integral_constant<int, 6>{}); break;
+      }
     }
     // trapezoid weights of Current::Jxyz: 1/2 at both ends of the p axis
     const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
