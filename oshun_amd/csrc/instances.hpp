@@ -69,7 +69,7 @@
   EXT template __global__ void oshun::vlasov_stage_lm<T, M0, KIND, TD>(oshun::StageArgs<T>);
 #define OSHUN_LM_TD(EXT, T, M0, KIND) OSHUN_I_LM(EXT, T, M0, KIND, false) OSHUN_I_LM(EXT, T, M0, KIND, true)
 #define OSHUN_LM_KINDS(EXT, T, M0) OSHUN_LM_TD(EXT, T, M0, 0) OSHUN_LM_TD(EXT, T, M0, 1) OSHUN_LM_TD(EXT, T, M0, 2) \
-  OSHUN_LM_TD(EXT, T, M0, 3)
+  OSHUN_LM_TD(EXT, T, M0, 3) OSHUN_LM_TD(EXT, T, M0, 4)
 #define OSHUN_FAMILY_LM_F64(EXT) OSHUN_LM_KINDS(EXT, double, 1) OSHUN_LM_KINDS(EXT, double, 2) \
   OSHUN_LM_KINDS(EXT, double, 3) OSHUN_LM_KINDS(EXT, double, 4)
 #define OSHUN_FAMILY_LM_F32(EXT) OSHUN_LM_TD(EXT, float, 1, 0) OSHUN_LM_TD(EXT, float, 2, 0) \

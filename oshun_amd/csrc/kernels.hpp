@@ -51,7 +51,7 @@ struct StageArgs {
   T c05h, hh, h6, third, twothird;
   int np, nx, ny, nc;
   // vlasov_stage_lm only: l0 and the interior coefficient layout (LmPlan)
-  int l0, lm_istart, lm_istride, lm_ioff[5];
+  int l0, lm_ltop, lm_istart, lm_istride, lm_ioff[5];
   long long plane;
   int mode;  // 0: slope, 1..4: RK4 stage, 5..6: RK2 stage (vlasov_stage only)
   int c10, c11r, c11i;

@@ -198,6 +198,7 @@ int launch_lm_kind(oshun_ctx* c, const oshun::StageArgs<T>& a, int kind, dim3 gr
       case oshun::kLmAdv1: launch_lm_td<T, M0, oshun::kLmAdv1>(c, a, grid, block); break;
       case oshun::kLmAdv1Es: launch_lm_td<T, M0, oshun::kLmAdv1Es>(c, a, grid, block); break;
       case oshun::kLmAdv2: launch_lm_td<T, M0, oshun::kLmAdv2>(c, a, grid, block); break;
+      case oshun::kLmProbe: launch_lm_td<T, M0, oshun::kLmProbe>(c, a, grid, block); break;
       default: return OSHUN_ERR_ARG;
     }
   }
@@ -357,10 +358,24 @@ int launch_stage(oshun_ctx* c, int mode, const void* in, const void* y0, void* o
   }
   dim3 block(oshun::kTileX, oshun::kTileP, 1);
   dim3 grid((c->nx + oshun::kTileX - 1) / oshun::kTileX, c->npb, c->ny);
-  // the l-streaming kernel: programs over all planes that it matches, the
-  // full program where no specialised kernel is built for the shape
-  const DevProg* lmp = prog ? (nc == c->nc ? prog : nullptr) : (c->spec < 0 ? &c->full : nullptr);
+  // the l-streaming kernel: programs that it matches (over all planes, or
+  // the probe over its plane prefix), the full program where no specialised
+  // kernel is built for the shape
+  const DevProg* lmp = prog ? prog : (c->spec < 0 ? &c->full : nullptr);
   int lm = (lmp && !c->force_general) ? lmp->lm : -1;
+  // the layers the kernel reads (0..a.l0) and writes (0..ltop): all, or for
+  // the probe sources up to l = 3 and the planes of l <= 2 or l <= 3
+  int ltop = c->T.l0;
+  if (lm == oshun::kLmProbe) {
+    a.l0 = std::min(c->T.l0, oshun::kLmProbeTop);
+    ltop = -1;
+    for (int t = 0; t <= a.l0; ++t)
+      if (nc == oshun::lm_layer_start(t + 1, c->T.m0)) ltop = t;
+    if (ltop < 0) lm = -1;
+  } else if (prog && nc != c->nc) {
+    lm = -1;
+  }
+  a.lm_ltop = ltop;
   // the kernels run the full program in modes 0..4 (1 with y0 the stage
   // input) and the implicit ones in modes 0, 6 and 5 (y0 the stage input);
   // lane byte offsets within a plane are 32-bit

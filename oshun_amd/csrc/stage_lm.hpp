@@ -38,13 +38,17 @@ namespace oshun {
 
 enum LmPhase { LM_E = 1, LM_X = 2, LM_Y = 4, LM_B = 8 };
 // The coupling programs the kernel is built for: Actions::Adv, and
-// Implicit_Actions::Adv1 with and without the B rotation and Adv2
-// (def-Actions.cpp:73-94, 118-160)
-enum LmKind { kLmFull = 0, kLmAdv1 = 1, kLmAdv1Es = 2, kLmAdv2 = 3, kLmKinds = 4 };
+// Implicit_Actions::Adv1 with and without the B rotation, Adv2, and the
+// field-solve probe (Implicit_Ex / Implicit_Eyz: the E-field couplings of
+// sources l <= 3 into targets l <= 2, Ex only from sources m <= 1;
+// def-Actions.cpp:73-94, 118-160, def-VlasovMax.cpp:157-196, 231-291)
+enum LmKind { kLmFull = 0, kLmAdv1 = 1, kLmAdv1Es = 2, kLmAdv2 = 3, kLmProbe = 4, kLmKinds = 5 };
 __host__ __device__ constexpr int lm_mask(int kind) {
   return kind == kLmFull ? (LM_E | LM_X | LM_Y | LM_B)
                          : (kind == kLmAdv1 ? (LM_X | LM_Y | LM_B) : (kind == kLmAdv1Es ? (LM_X | LM_Y) : LM_E));
 }
+// the probe's top source layer
+constexpr int kLmProbeTop = 3;
 // CLEAN_LOW_PC rows of a plane of harmonic l (def-Actions.cpp:240-253) in the
 // programs that have them (Actions::Adv and Adv2; tables.hpp build_tables)
 __host__ __device__ constexpr int lm_clean_rows(int kind, int l) {
@@ -68,9 +72,12 @@ __host__ __device__ constexpr int lm_layer_start(int l, int m0) {
 // IN: l is interior, M0 + 1 <= l <= L - 1, so that every source l-1, l, l+1
 // with m' <= M0 exists and l >= 2: the same calls, the existence tests
 // resolved at compile time.
-template <int M0, int TM, int MASK, bool TD, bool IN, class S>
+template <int M0, int TM, int KIND, bool TD, bool IN, class S>
 __host__ __device__ __forceinline__ void lm_groups(S& s, int l, int L) {
+  constexpr int MASK = lm_mask(KIND);
+  constexpr bool PROBE = KIND == kLmProbe;
   constexpr bool TB = TM > 0;
+  if (PROBE && l > kLmProbeTop - 1) return;
   // source (l + dl, sm) exists
   auto ok = [&](int dl, int sm) -> bool {
     if constexpr (IN) return true;
@@ -85,6 +92,7 @@ __host__ __device__ __forceinline__ void lm_groups(S& s, int l, int L) {
       if constexpr (SM >= 0 && SM <= M0) {
         constexpr int MR = SM == TM ? MUL_EX : MUL_EY;
         constexpr int MI = SM == TM ? kLmNone : MUL_EZ;
+        if constexpr (PROBE && MR == MUL_EX && SM > 1) return;  // the probe's Ex: sources m <= 1
         if (ok(-1, SM)) {
           if (!IN && l == 1) {  // G00 of f00 into (1,0) (Ex) and (1,1) (Em)
             if constexpr (SM == 0 && TM <= 1) s.template g<-1, 0, OP_G00, MR, MI, TB>();
@@ -165,13 +173,12 @@ struct LmRecord {
 
 template <int M0, int KIND, bool TD>
 void lm_record_target(LmRecord& r, int m, int L) {
-  constexpr int MK = lm_mask(KIND);
   switch (m) {
-    case 0: lm_groups<M0, 0, MK, TD, false>(r, r.l, L); break;
-    case 1: if constexpr (M0 >= 1) lm_groups<M0, 1, MK, TD, false>(r, r.l, L); break;
-    case 2: if constexpr (M0 >= 2) lm_groups<M0, 2, MK, TD, false>(r, r.l, L); break;
-    case 3: if constexpr (M0 >= 3) lm_groups<M0, 3, MK, TD, false>(r, r.l, L); break;
-    case 4: if constexpr (M0 >= 4) lm_groups<M0, 4, MK, TD, false>(r, r.l, L); break;
+    case 0: lm_groups<M0, 0, KIND, TD, false>(r, r.l, L); break;
+    case 1: if constexpr (M0 >= 1) lm_groups<M0, 1, KIND, TD, false>(r, r.l, L); break;
+    case 2: if constexpr (M0 >= 2) lm_groups<M0, 2, KIND, TD, false>(r, r.l, L); break;
+    case 3: if constexpr (M0 >= 3) lm_groups<M0, 3, KIND, TD, false>(r, r.l, L); break;
+    case 4: if constexpr (M0 >= 4) lm_groups<M0, 4, KIND, TD, false>(r, r.l, L); break;
     default: break;
   }
 }
@@ -234,13 +241,15 @@ bool lm_build_kind(const HostTables& T, const std::vector<Term>& terms, const st
       }
     }
   P->gstart[T.nc] = (int)P->coef.size();
-  // the interior layout (the same calls at every interior l)
+  // the interior layout (the same calls at every interior l of the kernel's
+  // layers: 0..l0, or 0..3 for the probe)
   const int l1 = M0 + 1;
-  if (l1 <= T.l0 - 1) {
+  const int lk = KIND == kLmProbe ? (T.l0 < kLmProbeTop ? T.l0 : kLmProbeTop) : T.l0;
+  if (l1 <= lk - 1) {
     P->istart = P->gstart[lm_layer_start(l1, M0)];
-    P->istride = l1 + 1 <= T.l0 - 1 ? P->gstart[lm_layer_start(l1 + 1, M0)] - P->istart : 0;
+    P->istride = l1 + 1 <= lk - 1 ? P->gstart[lm_layer_start(l1 + 1, M0)] - P->istart : 0;
     for (int m = 0; m <= M0; ++m) P->ioff[m] = P->gstart[lm_layer_start(l1, M0) + lm_slot(m, 0)] - P->istart;
-    for (int l = l1; l <= T.l0 - 1; ++l)
+    for (int l = l1; l <= lk - 1; ++l)
       for (int m = 0; m <= M0; ++m)
         if (P->gstart[lm_layer_start(l, M0) + lm_slot(m, 0)] != P->istart + (l - l1) * P->istride + P->ioff[m])
           return false;
@@ -255,7 +264,8 @@ bool lm_build_m(const HostTables& T, const std::vector<Term>& terms, const std::
   return lm_build_kind<M0, kLmFull, TD>(T, terms, tstart, clean, P) ||
          lm_build_kind<M0, kLmAdv1, TD>(T, terms, tstart, clean, P) ||
          lm_build_kind<M0, kLmAdv1Es, TD>(T, terms, tstart, clean, P) ||
-         lm_build_kind<M0, kLmAdv2, TD>(T, terms, tstart, clean, P);
+         lm_build_kind<M0, kLmAdv2, TD>(T, terms, tstart, clean, P) ||
+         lm_build_kind<M0, kLmProbe, TD>(T, terms, tstart, clean, P);
 }
 
 // The kernel's plan of a program (terms, tstart, CLEAN_LOW_PC rows over all
@@ -339,10 +349,11 @@ struct LmRaw {
 };
 
 // KIND: the program (LmKind); TD: it has y advection.  a.coef, a.tstart and
-// a.lm_* are an LmPlan's coef, gstart and interior layout; a.mode is one of
-// the modes the kind runs (launch_stage: 0..4 with y0 the stage input in
-// mode 1 for the full program, 0, 5 with y0 the stage input and 6 for the
-// implicit ones).
+// a.lm_* are an LmPlan's coef, gstart and interior layout; a.l0 is the top
+// source layer (l0, or 3 for the probe) and a.lm_ltop the top layer written;
+// a.mode is one of the modes the kind runs (launch_stage: 0..4 with y0 the
+// stage input in mode 1 for the full program, 0, 5 with y0 the stage input
+// and 6 for the others).
 template <typename T, int M0, int KIND, bool TD>
 __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a) {
   using Raw = LmRaw<T, M0>;
@@ -499,7 +510,7 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
           e.kc = 0;
           e.sa = T(0);
           e.sb = T(0);
-          lm_groups<M0, TM, MASK, TD, IN>(e, l, L);
+          lm_groups<M0, TM, KIND, TD, IN>(e, l, L);
           sum[lm_slot(TM, 0)] = e.sa;
           if constexpr (TM > 0) sum[lm_slot(TM, 1)] = e.sb;
         };
@@ -554,11 +565,13 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
           e.Dy[1][k] = e.Dy[2][k];
         }
       };
-      // layers 0..m0 and L at the edges, m0+1..L-1 interior
-      const int le = L < M0 ? L : M0;
+      // layers 0..m0 and L at the edges, m0+1..L-1 interior (up to lm_ltop)
+      const int lt = a.lm_ltop;
+      const int le = lt < M0 ? lt : M0;
+      const int li = L - 1 < lt ? L - 1 : lt;
       for (int l = 0; l <= le; ++l) body(std::false_type{}, l);
-      for (int l = M0 + 1; l <= L - 1; ++l) body(std::true_type{}, l);
-      if (L >= M0 + 1) body(std::false_type{}, L);
+      for (int l = M0 + 1; l <= li; ++l) body(std::true_type{}, l);
+      if (L >= M0 + 1 && L <= lt) body(std::false_type{}, L);
     };
     if constexpr (RK4) {
       switch (a.mode) {
