@@ -1228,9 +1228,24 @@ __device__ __forceinline__ void prev_row(const double (&v)[NPL], double (&pv)[NP
   pv[0] = last_of_prev;
 }
 
+// The p tables of the e-e slope in LDS, interleaved (row i of every lane
+// contiguous: conflict-free reads), shared by the workgroup's columns; zero
+// past np except U3 (one)
+enum { ET_U4 = 0, ET_U4M1, ET_U2, ET_U2M1, ET_U1S, ET_U1M1S, ET_U3, ET_QN, ET_PN, ET_COUNT };
 template <int NPL>
 struct EETab {
-  double U4[NPL], U4m1[NPL], U2[NPL], U2m1[NPL], U1s[NPL], U1m1s[NPL], U3[NPL], Qn[NPL], Pn[NPL];
+  const double* base;  // ET_COUNT x NPL x 64
+  int lane;
+  __device__ __forceinline__ double get(int t, int i) const { return base[(t * NPL + i) * 64 + lane]; }
+  __device__ __forceinline__ double U4(int i) const { return get(ET_U4, i); }
+  __device__ __forceinline__ double U4m1(int i) const { return get(ET_U4M1, i); }
+  __device__ __forceinline__ double U2(int i) const { return get(ET_U2, i); }
+  __device__ __forceinline__ double U2m1(int i) const { return get(ET_U2M1, i); }
+  __device__ __forceinline__ double U1s(int i) const { return get(ET_U1S, i); }
+  __device__ __forceinline__ double U1m1s(int i) const { return get(ET_U1M1S, i); }
+  __device__ __forceinline__ double U3(int i) const { return get(ET_U3, i); }
+  __device__ __forceinline__ double Qn(int i) const { return get(ET_QN, i); }
+  __device__ __forceinline__ double Pn(int i) const { return get(ET_PN, i); }
 };
 
 // EE_isotropic_conserva::Getslope, def-FokkerPlanck.cpp:192-267
@@ -1247,9 +1262,9 @@ __device__ __forceinline__ void ee_slope(const CollArgs<T>& a, const EETab<NPL>&
   for (int i = 0; i < NPL; ++i) {
     const int p = lane * NPL + i;
     const bool in1 = p >= 1 && p < np;
-    t4[i] = in1 ? tb.U4[i] * f[i] + tb.U4m1[i] * fp[i] : 0.0;
-    t2[i] = in1 ? tb.U2[i] * f[i] + tb.U2m1[i] * fp[i] : 0.0;
-    t1[i] = (p < np - 1) ? tb.U1s[i] * fn[i] + tb.U1m1s[i] * f[i] : 0.0;
+    t4[i] = in1 ? tb.U4(i) * f[i] + tb.U4m1(i) * fp[i] : 0.0;
+    t2[i] = in1 ? tb.U2(i) * f[i] + tb.U2m1(i) * fp[i] : 0.0;
+    t1[i] = (p < np - 1) ? tb.U1s(i) * fn[i] + tb.U1m1s(i) * f[i] : 0.0;
   }
   double I4[NPL], I2[NPL], J1[NPL], tot4, tot2;
   prefix_p<NPL>(t4, I4, tot4);
@@ -1266,7 +1281,7 @@ __device__ __forceinline__ void ee_slope(const CollArgs<T>& a, const EETab<NPL>&
     const int p = lane * NPL + i;
     double i2 = I2[i] * J1[i];
     i2 *= -3.0;
-    double g = I4[i] + tb.U3[i] * J1[i];
+    double g = I4[i] + tb.U3(i) * J1[i];
     g *= f[i];
     g += i2;
     if (p < a.nb) {  // EE_isotropic_conserva::G, :178-188
@@ -1291,7 +1306,7 @@ __device__ __forceinline__ void ee_slope(const CollArgs<T>& a, const EETab<NPL>&
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const int p = lane * NPL + i;
-    D2[i] = (p < np - 1) ? D1[i] * tb.Pn[i] : D1[i];
+    D2[i] = (p < np - 1) ? D1[i] * tb.Pn(i) : D1[i];
   }
   fh0 -= lane_read(D2[0], 0);
   double D2n[NPL], D3[NPL];
@@ -1308,7 +1323,7 @@ __device__ __forceinline__ void ee_slope(const CollArgs<T>& a, const EETab<NPL>&
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const int p = lane * NPL + i;
-    double v = (p == 0) ? fh0 : tb.Qn[i] * D3p[i];
+    double v = (p == 0) ? fh0 : tb.Qn(i) * D3p[i];
     fh[i] = (p < np) ? v * s : 0.0;
   }
 }
@@ -1467,33 +1482,75 @@ __device__ __forceinline__ void tri_solve(const double (&A)[NPL], const double (
 
 constexpr int kCollCols = 8;  // x columns (one wavefront each) per workgroup
 
+// One wavefront per (x,y) column (lane holds rows lane*NPL .. lane*NPL+NPL-1),
+// kCollCols adjacent columns per workgroup (a column past the window computes
+// on zeros and stores nothing).  Planes cross HBM in row order
+// (kCollCols adjacent x of one p row per 8 lanes: 64-byte runs) and are
+// turned to the lane layout through LDS (one pad per NPL rows keeps the
+// lane-layout reads free of bank conflicts); the next plane's row-order loads
+// are in flight while the current one is solved.
 template <typename T, int NPL>
 __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
+  constexpr int kRows = 64 * NPL;
+  // LDS column stride: kRows + kRows / NPL is a multiple of 32 doubles; the
+  // extra 8 put the kCollCols columns of a row-order pass on distinct banks
+  constexpr int kS = kRows + kRows / NPL + 8;
+  __shared__ double tr[kCollCols * kS];
   const int nx = a.nx, np = a.np;
   const int ntx = (a.cnx + kCollCols - 1) / kCollCols;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int y = tile / ntx + a.cy0;
-  const int xw = (tile % ntx) * kCollCols + (threadIdx.x >> 6);
-  if (xw >= a.cnx) return;  // whole wavefront leaves together
-  const int x = xw + a.cx0;
+  const int xw0 = (tile % ntx) * kCollCols;
+  const int wv = threadIdx.x >> 6;
   const int lane = lane_id();
-  const long long base = (long long)y * np * nx + x;
-  EETab<NPL> tb;
-  double f[NPL];
+  const int lrow = wv * kS + lane * (NPL + 1);  // LDS index of lane-layout row i: lrow + i
+  // row-order: thread t moves column cc = t % kCollCols, rows r0 + 64 k
+  const int cc = threadIdx.x % kCollCols, r0 = threadIdx.x / kCollCols;
+  const bool ccv = xw0 + cc < a.cnx;
+  const long long cb = (long long)y * np * nx + (a.cx0 + (ccv ? xw0 + cc : 0));
+  auto rok = [&](int k) { return ccv && r0 + 64 * k < np; };
+  auto gidx = [&](int k) { return cb + (long long)(r0 + 64 * k) * nx; };
+  auto lidx = [&](int k) { const int r = r0 + 64 * k; return cc * kS + r + r / NPL; };
+  auto load_rows = [&](const T* pl, T (&v)[NPL]) {
 #pragma unroll
-  for (int i = 0; i < NPL; ++i) {
-    const int p = lane * NPL + i;
-    const bool in = p < np;
-    tb.U4[i] = in ? a.t.U4[p] : 0.0;
-    tb.U4m1[i] = in ? a.t.U4m1[p] : 0.0;
-    tb.U2[i] = in ? a.t.U2[p] : 0.0;
-    tb.U2m1[i] = in ? a.t.U2m1[p] : 0.0;
-    tb.U1s[i] = in ? a.t.U1s[p] : 0.0;
-    tb.U1m1s[i] = in ? a.t.U1m1s[p] : 0.0;
-    tb.U3[i] = in ? a.t.U3[p] : 1.0;
-    tb.Qn[i] = in ? a.t.Qn[p] : 0.0;
-    tb.Pn[i] = in ? a.t.Pn[p] : 0.0;
-    f[i] = in ? static_cast<double>(a.f[base + (long long)p * nx]) + (a.bg ? a.bg[p] : 0.0) : 0.0;
+    for (int k = 0; k < NPL; ++k) v[k] = rok(k) ? pl[gidx(k)] : T(0);
+  };
+  // LDS round trips (callers keep the barriers)
+  auto rows_to_lds = [&](const T (&v)[NPL]) {
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) tr[lidx(k)] = static_cast<double>(v[k]);
+  };
+  auto lds_to_rows_store = [&](T* pl) {
+#pragma unroll
+    for (int k = 0; k < NPL; ++k)
+      if (rok(k)) pl[gidx(k)] = static_cast<T>(tr[lidx(k)]);
+  };
+  __shared__ double etab[ET_COUNT * kRows];
+  for (int q = threadIdx.x; q < kRows; q += 64 * kCollCols) {  // row p = q -> (i, lane) = (q % NPL, q / NPL)
+    const bool in = q < np;
+    const int o = (q % NPL) * 64 + q / NPL;
+    etab[ET_U4 * kRows + o] = in ? a.t.U4[q] : 0.0;
+    etab[ET_U4M1 * kRows + o] = in ? a.t.U4m1[q] : 0.0;
+    etab[ET_U2 * kRows + o] = in ? a.t.U2[q] : 0.0;
+    etab[ET_U2M1 * kRows + o] = in ? a.t.U2m1[q] : 0.0;
+    etab[ET_U1S * kRows + o] = in ? a.t.U1s[q] : 0.0;
+    etab[ET_U1M1S * kRows + o] = in ? a.t.U1m1s[q] : 0.0;
+    etab[ET_U3 * kRows + o] = in ? a.t.U3[q] : 1.0;
+    etab[ET_QN * kRows + o] = in ? a.t.Qn[q] : 0.0;
+    etab[ET_PN * kRows + o] = in ? a.t.Pn[q] : 0.0;
+  }
+  const EETab<NPL> tb{etab, lane};
+  double f[NPL];
+  {
+    T v[NPL];
+    load_rows(a.f, v);
+    rows_to_lds(v);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      const int p = lane * NPL + i;
+      f[i] = (p < np) ? tr[lrow + i] + (a.bg ? a.bg[p] : 0.0) : 0.0;
+    }
   }
   if (a.do_ee) {
     // Explicit_EC_FP::loop + RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553,590-625
@@ -1515,11 +1572,14 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
 #pragma unroll
       for (int i = 0; i < NPL; ++i) { fh[i] *= (h / 6.0); f[i] += fh[i]; }
     }
+    __syncthreads();  // every lane has read its rows of f00
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
       const int p = lane * NPL + i;
-      if (p < np) a.f[base + (long long)p * nx] = static_cast<T>(f[i] - (a.bg ? a.bg[p] : 0.0));
+      tr[lrow + i] = static_cast<double>(static_cast<T>(f[i] - ((a.bg && p < np) ? a.bg[p] : 0.0)));
     }
+    __syncthreads();
+    lds_to_rows_store(a.f);
     // the implicit step reads f00 as stored
     if (sizeof(T) < sizeof(double)) {
 #pragma unroll
@@ -1540,9 +1600,9 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
   for (int i = 0; i < NPL; ++i) {
     const int p = lane * NPL + i;
     const bool in1 = p >= 1 && p < np;
-    t4[i] = in1 ? tb.U4[i] * f[i] + tb.U4m1[i] * fp[i] : 0.0;
-    t2[i] = in1 ? tb.U2[i] * f[i] + tb.U2m1[i] * fp[i] : 0.0;
-    t1[i] = (p < np - 1) ? tb.U1s[i] * fn[i] + tb.U1m1s[i] * f[i] : 0.0;
+    t4[i] = in1 ? tb.U4(i) * f[i] + tb.U4m1(i) * fp[i] : 0.0;
+    t2[i] = in1 ? tb.U2(i) * f[i] + tb.U2m1(i) * fp[i] : 0.0;
+    t1[i] = (p < np - 1) ? tb.U1s(i) * fn[i] + tb.U1m1s(i) * f[i] : 0.0;
   }
   double I2[NPL], I0[NPL], J1m[NPL], tot2, tot0;
   prefix_p<NPL>(t4, I2, tot2);
@@ -1568,7 +1628,7 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
     const double T2 = (i2 + j1) / 3.0;
     double s = (p == 0) ? 0.0 : T1 * lnee;
     s += zi;
-    s /= tb.U3[i];
+    s /= tb.U3(i);
     s *= kd;
     Sct[i] = (p < np) ? s : 0.0;
     double dg = 0.0, lo = 0.0, up = 0.0;
@@ -1584,38 +1644,49 @@ __global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
     Bdia[i] = dg * sc;
     Csup[i] = up * sc;
   }
-  // Implicit_FLM_step::advance for every l >= 1 plane, def-FokkerPlanck.cpp:943-1035
+  // Implicit_FLM_step::advance for every l >= 1 plane, def-FokkerPlanck.cpp:943-1035.
+  // Planes come in ind(l,m) order (l non-decreasing): the pivots are formed
+  // when l changes (f1_loop handles m = 0, 1 only, :1130; flm_loop m <=
+  // min(m0, l)); every plane of each l in [lmin, lmax] is solved.
   const int lmin = a.do_f1 ? 1 : 2;
   const int lmax = a.do_flm ? a.l0 : 1;
-  for (int l = lmin; l <= lmax; ++l) {
-    double ll1 = static_cast<double>(l);
-    ll1 *= (-0.5) * (ll1 + 1.0);
-    double Bl[NPL], cp[NPL], w[NPL];
+  int c = 1;
+  while (c < a.nc && ldc(a.comp_l, c) < lmin) ++c;
+  int cend = c;
+  while (cend < a.nc && ldc(a.comp_l, cend) <= lmax) ++cend;
+  T vn[NPL];  // next plane, row order
+  if (c < cend) load_rows(a.f + (long long)c * a.plane, vn);
+  double cpv[NPL], w[NPL];
+  int lcur = -1;
+  for (; c < cend; ++c) {
+    __syncthreads();  // the previous plane (or f00) has left LDS
+    rows_to_lds(vn);
+    __syncthreads();
+    double d[NPL];
 #pragma unroll
-    for (int i = 0; i < NPL; ++i) {
-      const int p = lane * NPL + i;
-      double b = (p == 0 && l > 1) ? 0.0 : Bdia[i];
-      b += 1.0 - ll1 * Sct[i];
-      Bl[i] = (p < np) ? b : 1.0;
-    }
-    tri_pivots<NPL>(Asub, Bl, Csup, cp, w);
-    for (int c = 1; c < a.nc; ++c) {
-      if (a.comp_l[c] != l) continue;
-      // f1_loop handles m = 0, 1 only (def-FokkerPlanck.cpp:1130); flm_loop m <= min(m0,l)
-      T* fc = a.f + (long long)c * a.plane;
-      double d[NPL];
+    for (int i = 0; i < NPL; ++i) d[i] = (lane * NPL + i < np) ? tr[lrow + i] : 0.0;
+    if (c + 1 < cend) load_rows(a.f + (long long)(c + 1) * a.plane, vn);  // in flight during the solve
+    const int l = ldc(a.comp_l, c);
+    if (l != lcur) {
+      lcur = l;
+      double ll1 = static_cast<double>(l);
+      ll1 *= (-0.5) * (ll1 + 1.0);
+      double Bl[NPL];
 #pragma unroll
       for (int i = 0; i < NPL; ++i) {
         const int p = lane * NPL + i;
-        d[i] = (p < np) ? static_cast<double>(fc[base + (long long)p * nx]) : 0.0;
+        double b = (p == 0 && l > 1) ? 0.0 : Bdia[i];
+        b += 1.0 - ll1 * Sct[i];
+        Bl[i] = (p < np) ? b : 1.0;
       }
-      tri_solve<NPL>(Asub, cp, w, d);
-#pragma unroll
-      for (int i = 0; i < NPL; ++i) {
-        const int p = lane * NPL + i;
-        if (p < np) fc[base + (long long)p * nx] = static_cast<T>(d[i]);
-      }
+      tri_pivots<NPL>(Asub, Bl, Csup, cpv, w);
     }
+    tri_solve<NPL>(Asub, cpv, w, d);
+    __syncthreads();  // every lane has read its rows of this plane
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) tr[lrow + i] = d[i];
+    __syncthreads();
+    lds_to_rows_store(a.f + (long long)c * a.plane);
   }
 }
 
