@@ -30,7 +30,8 @@
   EXT template __global__ void oshun::collide_rows<T, NPL, SEGW, MODE>(oshun::CollArgs<T>);
 #define OSHUN_I_ROWSB(EXT, T, NPL, SEGW, MODE) \
   EXT template __global__ void oshun::collide_rows_batch<T, NPL, SEGW, MODE>(const oshun::CollArgs<T>*, const int*, int);
-#define OSHUN_I_GEN(EXT, T) EXT template __global__ void oshun::vlasov_stage<T>(oshun::StageArgs<T>);
+#define OSHUN_I_GEN(EXT, T) EXT template __global__ void oshun::vlasov_stage<T>(oshun::StageArgs<T>); \
+  EXT template __global__ void oshun::vlasov_current_tiles<T>(oshun::StageArgs<T>);
 #define OSHUN_I_SPEC(EXT, T, ID, L, M, TD) \
   EXT template __global__ void oshun::vlasov_stage_spec<T, L, M, TD, oshun::kSpecDepth>(oshun::StageArgs<T>);
 

@@ -180,12 +180,16 @@ int lm_prepare(oshun_ctx* c, const std::vector<Term>& terms, const std::vector<i
   return OSHUN_OK;
 }
 
+// the l-streaming kernel (one thread per plane element) and then the
+// stage input's current partials on the stage grid (grid, block)
 template <typename T, int M0, int KIND>
 void launch_lm_td(oshun_ctx* c, const oshun::StageArgs<T>& a, dim3 grid, dim3 block) {
+  const dim3 g1((unsigned)((c->plane + oshun::kLmBlock - 1) / oshun::kLmBlock));
   if (c->T.two_d)
-    hipLaunchKernelGGL((oshun::vlasov_stage_lm<T, M0, KIND, true>), grid, block, 0, c->stream, a);
+    hipLaunchKernelGGL((oshun::vlasov_stage_lm<T, M0, KIND, true>), g1, dim3(oshun::kLmBlock), 0, c->stream, a);
   else
-    hipLaunchKernelGGL((oshun::vlasov_stage_lm<T, M0, KIND, false>), grid, block, 0, c->stream, a);
+    hipLaunchKernelGGL((oshun::vlasov_stage_lm<T, M0, KIND, false>), g1, dim3(oshun::kLmBlock), 0, c->stream, a);
+  hipLaunchKernelGGL(oshun::vlasov_current_tiles<T>, grid, block, 0, c->stream, a);
 }
 template <typename T, int M0>
 int launch_lm_kind(oshun_ctx* c, const oshun::StageArgs<T>& a, int kind, dim3 grid, dim3 block) {

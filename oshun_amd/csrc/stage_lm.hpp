@@ -348,30 +348,33 @@ struct LmRaw {
   T fm[NS], f0[NS], fp[NS], xm[NS], xp[NS], ym[NS], yp[NS];
 };
 
+constexpr int kLmBlock = 256;
+
 // KIND: the program (LmKind); TD: it has y advection.  a.coef, a.tstart and
 // a.lm_* are an LmPlan's coef, gstart and interior layout; a.l0 is the top
 // source layer (l0, or 3 for the probe) and a.lm_ltop the top layer written;
 // a.mode is one of the modes the kind runs (launch_stage: 0..4 with y0 the
 // stage input in mode 1 for the full program, 0, 5 with y0 the stage input
-// and 6 for the others).
+// and 6 for the others).  One thread per element of a plane in memory order
+// (a workgroup writes kLmBlock consecutive elements of each plane: whole
+// cache lines whatever the row length); the current partials of the stage
+// input come from vlasov_current_tiles.
 template <typename T, int M0, int KIND, bool TD>
-__global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a) {
+__global__ __launch_bounds__(kLmBlock) void vlasov_stage_lm(StageArgs<T> a) {
   using Raw = LmRaw<T, M0>;
   constexpr int NS = 2 * M0 + 1;
   constexpr int MASK = lm_mask(KIND);
   constexpr bool RK4 = KIND == kLmFull;
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  const int x = blockIdx.x * kTileX + tx;
-  const int p = blockIdx.y * kTileP + ty;
-  const int y = blockIdx.z;
   const int nx = a.nx, np = a.np, ny = a.ny;
   const int nxy = nx * ny;
   const int L = a.l0;
-  __shared__ double jsh[3][kTileP][kTileX];
-  double j10 = 0.0, j11r = 0.0, j11i = 0.0;
-  if (x < nx && p < np) {
-    const long long idx = ((long long)y * np + p) * nx + x;
+  const long long idx = (long long)blockIdx.x * kLmBlock + threadIdx.x;
+  if (idx < a.plane) {
     const long long ystride = (long long)np * nx;
+    const int y = static_cast<int>(idx / ystride);
+    const int rem = static_cast<int>(idx - (long long)y * ystride);
+    const int p = rem / nx;
+    const int x = rem - p * nx;
     long long xm_off = (x == 0) ? (nx - 1) : -1;
     long long xp_off = (x == nx - 1) ? -(long long)(nx - 1) : 1;
     long long ym_off = (y == 0) ? (long long)(ny - 1) * ystride : -ystride;
@@ -588,6 +591,23 @@ __global__ __launch_bounds__(kTileX* kTileP) void vlasov_stage_lm(StageArgs<T> a
         default: run(std::integral_constant<int, 6>{}); break;
       }
     }
+  }
+}
+
+// The per-tile current partials of Current::Jxyz from the stage input, as
+// vlasov_stage forms them alongside its slopes (same tiles, same sums)
+template <typename T>
+__global__ __launch_bounds__(kTileX* kTileP) void vlasov_current_tiles(StageArgs<T> a) {
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int x = blockIdx.x * kTileX + tx;
+  const int p = blockIdx.y * kTileP + ty;
+  const int y = blockIdx.z;
+  const int nx = a.nx, np = a.np;
+  const int nxy = nx * a.ny;
+  __shared__ double jsh[3][kTileP][kTileX];
+  double j10 = 0.0, j11r = 0.0, j11i = 0.0;
+  if (x < nx && p < np) {
+    const long long idx = ((long long)y * np + p) * nx + x;
     // trapezoid weights of Current::Jxyz: 1/2 at both ends of the p axis
     const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
     const double g = a.p3og[p];
