@@ -59,32 +59,48 @@ __device__ __forceinline__ double src_row(const SrcArgs<T>& a, int k, int p) {
   return a.tab[k * a.np + p];
 }
 
-// Serial p sums of the operators below run in one lane, in the reference's
-// order (so every sum is the reference's to the last bit); the p-local work
-// runs across the wavefront's lanes.  LDS holds the cell's column.
+// The p sums of the operators below: the lanes form the terms (each as the
+// reference forms it), then one lane adds them in the reference's order, so
+// every sum is the reference's to the last bit; the p-local work runs across
+// the wavefront's lanes.  LDS holds the cell's column and the terms.
 constexpr int kSrcLanes = 64;
+
+// the ordered sum t[b] + t[b+1] + ... + t[e-1] onto s (one lane)
+__device__ __forceinline__ double src_tsum(double s, const double* t, int b, int e) {
+#pragma unroll 8
+  for (int n = b; n < e; ++n) s += t[n];
+  return s;
+}
 
 template <typename T>
 __global__ __launch_bounds__(kSrcLanes) void pls_kernel(SrcArgs<T> a) {
-  extern __shared__ double pls_sh[];  // the column F(p), then the unit Maxwellian gauss(p)
+  // the column F(p), the unit Maxwellian gauss(p), two term columns
+  extern __shared__ double pls_sh[];
   __shared__ double red[4];
   const int N = a.np, lane = threadIdx.x;
   const int i = blockIdx.x;
   const int ix = i % a.nx, iy = i / a.nx;
   double* Fc = pls_sh;
   double* gs = pls_sh + N;
-  T* col = a.f00 + (long long)iy * N * a.nx + ix;
-  for (int p = lane; p < N; p += kSrcLanes) Fc[p] = static_cast<double>(col[(long long)p * a.nx]) + (a.bg ? a.bg[p] : 0.0);
-  __syncthreads();
+  double* tA = pls_sh + 2 * N;
+  double* tB = pls_sh + 3 * N;
   const double* pr = a.tab + ST_P * N;
   const double* P4 = a.tab + ST_P4 * N;
   const double* G2 = a.tab + ST_G2 * N;
   const double* W4 = a.tab + ST_W4 * N;
+  T* col = a.f00 + (long long)iy * N * a.nx + ix;
+  for (int p = lane; p < N; p += kSrcLanes) {
+    const double F = static_cast<double>(col[(long long)p * a.nx]) + (a.bg ? a.bg[p] : 0.0);
+    Fc[p] = F;
+    tA[p] = pr[p] * pr[p] * F;  // the density sum's terms
+    tB[p] = W4[p] * F;          // the temperature sum's
+  }
+  __syncthreads();
   if (lane == 0) {  // density (:149-170)
     double n0 = Fc[0] * pr[0] * pr[0];
     n0 += Fc[N - 1] * pr[N - 1] * pr[N - 1];
     n0 *= 0.5;
-    for (int p = 1; p < N - 1; ++p) n0 += pr[p] * pr[p] * Fc[p];
+    n0 = src_tsum(n0, tA, 1, N - 1);
     double f00 = Fc[1] / 5.0 * a.p0p1_sq;
     f00 += (Fc[0] - Fc[1] * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 3.0 - 1.0 / 5.0 * a.p0p1_sq);
     f00 *= pr[0] * (pr[0] / a.dp) * pr[0];
@@ -95,7 +111,7 @@ __global__ __launch_bounds__(kSrcLanes) void pls_kernel(SrcArgs<T> a) {
     double vsq = Fc[0] * P4[0] / G2[0];
     vsq += Fc[N - 1] * P4[N - 1] / G2[N - 1];
     vsq *= 0.5;
-    for (int p = 1; p < N - 1; ++p) vsq += W4[p] * Fc[p];
+    vsq = src_tsum(vsq, tB, 1, N - 1);
     double f00 = Fc[1] / 7.0 * a.p0p1_sq;
     f00 += (Fc[0] - Fc[1] * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 5.0 - 1.0 / 7.0 * a.p0p1_sq);
     f00 *= P4[0] * (pr[0] / a.dp);
@@ -115,30 +131,40 @@ __global__ __launch_bounds__(kSrcLanes) void pls_kernel(SrcArgs<T> a) {
   const double g_c = 1.0 / pow(sqrt(2 * M_PI) * PTH, 3);
   const bool hot = PTH > pr[1];
   if (hot) {
-    for (int p = lane; p < N; p += kSrcLanes) gs[p] = exp((pr[p] * pr[p]) * e_c) * g_c;
+    __syncthreads();  // lanes 0 and 1 have read the term columns
+    for (int p = lane; p < N; p += kSrcLanes) {
+      const double g = exp((pr[p] * pr[p]) * e_c) * g_c;
+      gs[p] = g;
+      tA[p] = g * pr[p] * pr[p];  // its density sum's terms
+    }
     __syncthreads();
     if (lane == 0) {
       double g0 = gs[0] * pr[0] * pr[0];
       g0 += gs[N - 1] * pr[N - 1] * pr[N - 1];
       g0 *= 0.5;
-      for (int p = 1; p < N - 1; ++p) g0 += gs[p] * pr[p] * pr[p];
+      g0 = src_tsum(g0, tA, 1, N - 1);
       double gf = gs[1] / 5.0 * a.p0p1_sq;
       gf += (gs[0] - gs[1] * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 3.0 - 1.0 / 5.0 * a.p0p1_sq);
       gf *= pr[0] * (pr[0] / a.dp) * pr[0];
       g0 += gf;
       g0 *= 4.0 * M_PI * a.dp;
-      const double gnorm = 1.0 / g0;  // 1 / (its density before normalisation)
-      auto fg = [&](int p) { return gs[p] * gnorm; };
+      red[2] = 1.0 / g0;  // 1 / (its density before normalisation)
+    }
+    __syncthreads();
+    const double gnorm = red[2];
+    auto fg = [&](int p) { return gs[p] * gnorm; };
+    for (int p = lane; p < N; p += kSrcLanes) tB[p] = fg(p) * P4[p] / G2[p];  // its temperature sum's terms
+    __syncthreads();
+    if (lane == 0) {
       double vt = fg(0) * P4[0] / G2[0];
       vt += fg(N - 1) * P4[N - 1] / G2[N - 1];
       vt *= 0.5;
-      for (int p = 1; p < N - 1; ++p) vt += fg(p) * P4[p] / G2[p];
-      gf = fg(1) / 7.0 * a.p0p1_sq;
+      vt = src_tsum(vt, tB, 1, N - 1);
+      double gf = fg(1) / 7.0 * a.p0p1_sq;
       gf += (fg(0) - fg(1) * a.p0p1_sq) * a.inv_mp0p1_sq * (1.0 / 5.0 - 1.0 / 7.0 * a.p0p1_sq);
       gf *= P4[0] * (pr[0] / a.dp);
       vt += gf;
       vt *= 4.0 * M_PI / 3.0 * a.dp;
-      red[2] = gnorm;
       red[3] = vt * (3000.0 / 4.19) * (3000.0 / 4.19);
     }
     __syncthreads();
@@ -177,23 +203,21 @@ __device__ __forceinline__ double zlogei_v2(double ne, double Te, double density
   return 2.0 * zeta;
 }
 
-// sum over n = 1..N-1 of A[n] in[n] + B[n] in[n-1], in order
-__device__ __forceinline__ double ib_psum(const double* A, const double* B, const double* in, int N) {
-  double s = 0.0;
-  for (int n = 1; n < N; ++n) s += A[n] * in[n] + B[n] * in[n - 1];
-  return s;
+// the terms A[n] in[n] + B[n] in[n-1], n = 1..N-1, of the IB_Operator sums into t (all lanes)
+__device__ __forceinline__ void ib_terms(const double* A, const double* B, const double* in, double* t, int N) {
+  for (int n = 1 + (int)threadIdx.x; n < N; n += kSrcLanes) t[n] = A[n] * in[n] + B[n] * in[n - 1];
 }
 
 // IB_Operator::Getslope (:550-608) of the column `in` into fh (rows 0..np-2),
-// both in LDS; lanes 0 and 1 form the sums I4 and I2
+// both in LDS; lanes 0 and 1 add the sums I4 and I2 from the term columns t4, t2
 template <typename T>
-__device__ void ib_slope(const SrcArgs<T>& a, const double* in, double* fh, double* red, double vos) {
+__device__ void ib_slope(const SrcArgs<T>& a, const double* in, double* fh, double* red, double* t4, double* t2,
+                         double vos) {
   const int N = a.np, lane = threadIdx.x;
-  if (lane < 2) {
-    const double* A = a.tab + (lane == 0 ? ST_U4 : ST_U2) * N;
-    const double* B = a.tab + (lane == 0 ? ST_U4M1 : ST_U2M1) * N;
-    red[lane] = ib_psum(A, B, in, N);
-  }
+  ib_terms(a.tab + ST_U4 * N, a.tab + ST_U4M1 * N, in, t4, N);
+  ib_terms(a.tab + ST_U2 * N, a.tab + ST_U2M1 * N, in, t2, N);
+  __syncthreads();
+  if (lane < 2) red[lane] = src_tsum(0.0, lane == 0 ? t4 : t2, 1, N);
   __syncthreads();
   const double I4 = red[0], I2 = red[1];
   const double f00 = (in[0] - in[1] * a.p0overp1_sq) / (1.0 - a.p0overp1_sq);
@@ -221,7 +245,9 @@ __device__ void ib_slope(const SrcArgs<T>& a, const double* in, double* fh, doub
 // InverseBremsstrahlung::loop (:780-861) of one interior cell per workgroup
 template <typename T>
 __global__ __launch_bounds__(kSrcLanes) void ib_kernel(SrcArgs<T> a) {
-  extern __shared__ double ib_sh[];  // the integrated column fc, the stage inputs f0, f1 and the slope fh
+  // the integrated column fc, the stage inputs f0, f1, the slope fh and two
+  // term columns
+  extern __shared__ double ib_sh[];
   __shared__ double red[2];
   const int N = a.np, lane = threadIdx.x;
   const int w = blockIdx.x;
@@ -230,6 +256,8 @@ __global__ __launch_bounds__(kSrcLanes) void ib_kernel(SrcArgs<T> a) {
   double* f0 = ib_sh + N;
   double* f1 = ib_sh + 2 * N;
   double* fh = ib_sh + 3 * N;
+  double* t4 = ib_sh + 4 * N;
+  double* t2 = ib_sh + 5 * N;
   T* col = a.f00 + (long long)iy * N * a.nx + ix;
   const double vos = a.vos * sqrt(a.ILxe[ix] * a.ILye[iy]);
   for (int p = lane; p < N; p += kSrcLanes) {
@@ -237,14 +265,16 @@ __global__ __launch_bounds__(kSrcLanes) void ib_kernel(SrcArgs<T> a) {
     fh[p] = 0.0;  // row np-1 is never written by Getslope
   }
   __syncthreads();
-  if (lane == 0) red[0] = ib_psum(a.tab + ST_U2P * N, a.tab + ST_U2PM1 * N, fc, N);
+  ib_terms(a.tab + ST_U2P * N, a.tab + ST_U2PM1 * N, fc, t4, N);
+  __syncthreads();
+  if (lane == 0) red[0] = src_tsum(0.0, t4, 1, N);
   __syncthreads();
   const double I2_before = red[0];
   const double h = a.h;
   for (int s = 0; s < a.numh; ++s) {  // RungeKutta4_ib::advance (:713-757)
     for (int p = lane; p < N; p += kSrcLanes) { f0[p] = fc[p]; f1[p] = fc[p]; }
     __syncthreads();
-    ib_slope(a, f1, fh, red, vos);
+    ib_slope(a, f1, fh, red, t4, t2, vos);
     for (int p = lane; p < N; p += kSrcLanes) {
       double q = fh[p];
       q *= (0.5 * h); f1[p] += q;
@@ -252,7 +282,7 @@ __global__ __launch_bounds__(kSrcLanes) void ib_kernel(SrcArgs<T> a) {
       fh[p] = q;
     }
     __syncthreads();
-    ib_slope(a, f1, fh, red, vos);
+    ib_slope(a, f1, fh, red, t4, t2, vos);
     for (int p = lane; p < N; p += kSrcLanes) {
       double q = fh[p];
       f1[p] = f0[p];
@@ -261,7 +291,7 @@ __global__ __launch_bounds__(kSrcLanes) void ib_kernel(SrcArgs<T> a) {
       fh[p] = q;
     }
     __syncthreads();
-    ib_slope(a, f1, fh, red, vos);
+    ib_slope(a, f1, fh, red, t4, t2, vos);
     for (int p = lane; p < N; p += kSrcLanes) {
       double q = fh[p];
       q *= h; f0[p] += q;
@@ -269,7 +299,7 @@ __global__ __launch_bounds__(kSrcLanes) void ib_kernel(SrcArgs<T> a) {
       fh[p] = q;
     }
     __syncthreads();
-    ib_slope(a, f0, fh, red, vos);
+    ib_slope(a, f0, fh, red, t4, t2, vos);
     for (int p = lane; p < N; p += kSrcLanes) {
       double q = fh[p];
       q *= (h / 6.0); fc[p] += q;
@@ -277,7 +307,9 @@ __global__ __launch_bounds__(kSrcLanes) void ib_kernel(SrcArgs<T> a) {
     }
     __syncthreads();
   }
-  if (lane == 0) red[0] = ib_psum(a.tab + ST_U2P * N, a.tab + ST_U2PM1 * N, fc, N);
+  ib_terms(a.tab + ST_U2P * N, a.tab + ST_U2PM1 * N, fc, t4, N);
+  __syncthreads();
+  if (lane == 0) red[0] = src_tsum(0.0, t4, 1, N);
   __syncthreads();
   const double r = I2_before / red[0];
   for (int p = lane; p < N; p += kSrcLanes) {
@@ -356,7 +388,7 @@ int launch_pls(oshun_ctx* c, double tnew) {
   oshun::SrcArgs<T> a = src_args<T>(c);
   a.ILt = ILt;
   a.dt = dt;
-  hipLaunchKernelGGL(oshun::pls_kernel<T>, dim3(c->nxy), dim3(oshun::kSrcLanes), 2 * c->np * sizeof(double), c->stream,
+  hipLaunchKernelGGL(oshun::pls_kernel<T>, dim3(c->nxy), dim3(oshun::kSrcLanes), 4 * c->np * sizeof(double), c->stream,
                      a);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
@@ -369,8 +401,8 @@ int launch_ib(oshun_ctx* c, double tnew) {
   a.numh = numh;
   a.h = (tnew - c->src_t_ib) / static_cast<double>(static_cast<size_t>(numh));
   c->src_t_ib = tnew;
-  hipLaunchKernelGGL(oshun::ib_kernel<T>, dim3(c->nxi * c->nyi), dim3(oshun::kSrcLanes), 4 * c->np * sizeof(double),
-                     c->stream, a);
+  hipLaunchKernelGGL(oshun::ib_kernel<T>, dim3(c->nxi * c->nyi), dim3(oshun::kSrcLanes),
+                     6 * c->np * sizeof(double), c->stream, a);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
 }
