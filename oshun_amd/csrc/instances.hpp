@@ -25,7 +25,9 @@
   EXT template __global__ void oshun::vlasov_pair_1d<T, L0, R, P>(oshun::Stage1DArgs<T>);
 #define OSHUN_I_PAIRB(EXT, T, L0, R, P) \
   EXT template __global__ void oshun::vlasov_pair_1d_batch<T, L0, R, P>(const oshun::Stage1DArgs<T>*, const int*, int);
-#define OSHUN_I_COLL(EXT, T, NPL) EXT template __global__ void oshun::collide<T, NPL>(oshun::CollArgs<T>);
+#define OSHUN_I_COLL1(EXT, T, NPL, COLS) EXT template __global__ void oshun::collide<T, NPL, COLS>(oshun::CollArgs<T>);
+#define OSHUN_I_COLL(EXT, T, NPL) OSHUN_I_COLL1(EXT, T, NPL, 1) OSHUN_I_COLL1(EXT, T, NPL, 2) \
+  OSHUN_I_COLL1(EXT, T, NPL, 4) OSHUN_I_COLL1(EXT, T, NPL, 8)
 #define OSHUN_I_ROWS(EXT, T, NPL, SEGW, MODE) \
   EXT template __global__ void oshun::collide_rows<T, NPL, SEGW, MODE>(oshun::CollArgs<T>);
 #define OSHUN_I_ROWSB(EXT, T, NPL, SEGW, MODE) \

@@ -1480,17 +1480,17 @@ __device__ __forceinline__ void tri_solve(const double (&A)[NPL], const double (
   for (int i = NPL - 1; i >= 0; --i) { x = d[i] - cp[i] * x; d[i] = x; }
 }
 
-constexpr int kCollCols = 8;  // x columns (one wavefront each) per workgroup
-
 // One wavefront per (x,y) column (lane holds rows lane*NPL .. lane*NPL+NPL-1),
-// kCollCols adjacent columns per workgroup (a column past the window computes
-// on zeros and stores nothing).  Planes cross HBM in row order
-// (kCollCols adjacent x of one p row per 8 lanes: 64-byte runs) and are
-// turned to the lane layout through LDS (one pad per NPL rows keeps the
-// lane-layout reads free of bank conflicts); the next plane's row-order loads
-// are in flight while the current one is solved.
-template <typename T, int NPL>
-__global__ __launch_bounds__(64 * kCollCols) void collide(CollArgs<T> a) {
+// COLS adjacent columns per workgroup (a column past the window computes on
+// zeros and stores nothing; the host takes the widest COLS that still gives
+// a workgroup per CU).  Planes cross HBM in row order (COLS adjacent x of
+// one p row per COLS lanes) and are turned to the lane layout through LDS
+// (one pad per NPL rows keeps the lane-layout reads free of bank
+// conflicts); the next plane's row-order loads are in flight while the
+// current one is solved.
+template <typename T, int NPL, int COLS>
+__global__ __launch_bounds__(64 * COLS) void collide(CollArgs<T> a) {
+  constexpr int kCollCols = COLS;
   constexpr int kRows = 64 * NPL;
   // LDS column stride: kRows + kRows / NPL is a multiple of 32 doubles; the
   // extra 8 put the kCollCols columns of a row-order pass on distinct banks

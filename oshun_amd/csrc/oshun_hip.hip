@@ -621,8 +621,16 @@ template <typename T, int NPL>
 int launch_collide_npl(oshun_ctx* c, oshun::CollArgs<T>& a) {
   ProfRec r;
   prof_begin(c, 2, &r);
-  const int ntx = (c->nxi + oshun::kCollCols - 1) / oshun::kCollCols;
-  hipLaunchKernelGGL((oshun::collide<T, NPL>), dim3(ntx * c->nyi), dim3(64 * oshun::kCollCols), 0, c->stream, a);
+  // the widest column group that still gives a workgroup per CU
+  auto groups = [c](int cols) { return ((c->nxi + cols - 1) / cols) * c->nyi; };
+  const int cols = groups(8) >= 256 ? 8 : groups(4) >= 256 ? 4 : groups(2) >= 256 ? 2 : 1;
+  const dim3 grid(groups(cols)), block(64 * cols);
+  switch (cols) {
+    case 8: hipLaunchKernelGGL((oshun::collide<T, NPL, 8>), grid, block, 0, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((oshun::collide<T, NPL, 4>), grid, block, 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((oshun::collide<T, NPL, 2>), grid, block, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((oshun::collide<T, NPL, 1>), grid, block, 0, c->stream, a); break;
+  }
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
