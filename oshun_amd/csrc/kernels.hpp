@@ -1935,12 +1935,13 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
     double g = (I4[i] + ex4) + rt<SEGW>(tb, RT_U3, i, seg) * j1;
     g *= f[i];
     g += i2;
-    if (i < a.nb) {  // parabolic G for p < NB (EE_isotropic_conserva::G, :178-188); only lane 0 has such rows
+    {  // parabolic G for p < NB (EE_isotropic_conserva::G, :178-188): rows of lane 0 only,
+       // evaluated on every row and selected, so that the slope stays one basic block
       const double p3 = tb[RT_P3 * kRowP + i * SEGW];
       const double i2s = f00 * tb[RT_A2 * kRowP + i * SEGW] + (f1 - f00) * tb[RT_B2 * kRowP + i * SEGW];
       const double i4s = f00 * tb[RT_A4 * kRowP + i * SEGW] + (f1 - f00) * tb[RT_B4 * kRowP + i * SEGW];
       const double gl = f[i] * i4s + (p3 * f[i] - 3.0 * i2s) * j1;
-      g = (seg == 0) ? gl : g;
+      g = (seg == 0 && i < a.nb) ? gl : g;
     }
     G[i] = g;
   }
@@ -2063,9 +2064,16 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   constexpr int kCols = 256 / SEGW;  // columns per workgroup
   // Planes cross HBM in row order (a wavefront moves 64/kCols rows of kCols
   // adjacent x) and are turned to the lane layout (lane seg of column w holds
-  // rows seg*NPL .. seg*NPL+NPL-1) through LDS.  Column stride kS and one pad
-  // per NPL rows keep the lane-layout reads free of bank conflicts.
-  constexpr int kS = SEGW * NPL + SEGW + 8;
+  // rows seg*NPL .. seg*NPL+NPL-1) through LDS.  One pad per NPL rows keeps
+  // the lane-layout reads free of bank conflicts (16 lanes read 16 distinct
+  // 8-byte bank pairs); the column stride kS does the same for the row-order
+  // accesses, whose 16 lanes cover kCols columns x 16/kCols rows: kS = 16/kCols
+  // modulo 16 puts column cc at pair 2cc (kCols = 8) or cc (kCols = 16).
+  constexpr int kCols0 = 256 / SEGW;
+  constexpr int kSmin = SEGW * NPL + SEGW;
+  constexpr int kSres = (16 / kCols0) % 16;
+  constexpr int kS = kSmin + ((kSres - kSmin % 16) + 16) % 16;
+  static_assert(kCols0 == 8 || kCols0 == 16, "row-order lane mapping assumes 8 or 16 columns per workgroup");
   __shared__ double tb[RT_COUNT * kRowP];
   __shared__ double tr[kCols * kS];
   __shared__ double sbg[kRowP];  // f00 background per p (0 beyond np or without one), read in the lane layout
@@ -2155,6 +2163,23 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     __syncthreads();  // also covers the table fill
 #pragma unroll
     for (int i = 0; i < NPL; ++i) f[i] = tr[lrow + i] + sbg[p0 + i];
+  }
+  // the implicit step's planes (l in [lmin, lmax], ind(l,m) order); the first
+  // one's row-order loads are issued now, so that they are in flight during
+  // the e-e sub-steps (MODE 3) or the coefficients (MODE 2)
+  const int lmin = a.do_f1 ? 1 : 2;
+  const int lmax = a.do_flm ? a.l0 : 1;
+  int c = 1, cend = 1;
+  T vn[NPL];  // next plane, row order
+  if (MODE != 1) {
+    while (c < a.nc && ldc(a.comp_l, c) < lmin) ++c;
+    cend = c;
+    while (cend < a.nc && ldc(a.comp_l, cend) <= lmax) ++cend;
+    if (c < cend) {
+      const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)c * a.plane, pbytes);
+#pragma unroll
+      for (int k = 0; k < NPL; ++k) vn[k] = bload<T>(r, ld_ofs(k), 0);
+    }
   }
   if (MODE == 1 || MODE == 3) {
     // RungeKutta4_ee::advance, def-FokkerPlanck.cpp:518-553, numh times (:590-625)
@@ -2253,18 +2278,6 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
   // scans.  Planes come in ind(l,m) order (l non-decreasing): the pivots are
   // formed when l changes, and the next plane's row-order loads are in flight
   // while the current plane is solved.
-  const int lmin = a.do_f1 ? 1 : 2;
-  const int lmax = a.do_flm ? a.l0 : 1;
-  int c = 1;
-  while (c < a.nc && ldc(a.comp_l, c) < lmin) ++c;
-  int cend = c;
-  while (cend < a.nc && ldc(a.comp_l, cend) <= lmax) ++cend;
-  T vn[NPL];  // next plane, row order
-  if (c < cend) {
-    const __amdgpu_buffer_rsrc_t r = buf_rsrc(a.f + (long long)c * a.plane, pbytes);
-#pragma unroll
-    for (int k = 0; k < NPL; ++k) vn[k] = bload<T>(r, ld_ofs(k), 0);
-  }
   double w[NPL];
   int lcur = -1;
   for (; c < cend; ++c) {
@@ -2318,13 +2331,29 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
         q.c = bcast15_up(P.c, 0.0); q.d = bcast15_up(P.d, 1.0);
         P = mob_mul(P, q);
       }
-      // c' entering the lane from the exclusive prefix, then the reference's recurrence
-      double cprev = seg_prev0<SEGW>(P.b) * rcp_nr(seg_prev<SEGW>(P.d, 1.0));
+      // The state entering the lane is the exclusive prefix applied to (0, 1);
+      // then (P, Q)_i = M_i (P, Q)_{i-1} row by row, and the reference's pivot
+      // 1 / (b_i - a_i c'_{i-1}) is Q_{i-1} / Q_i: the recurrence carries two
+      // products per row and the divisions hang off it instead of lying on it.
+      double Pv = seg_prev0<SEGW>(P.b), Qv = seg_prev<SEGW>(P.d, 1.0);
+      double Qin[NPL];
 #pragma unroll
       for (int i = 0; i < NPL; ++i) {
-        w[i] = rcp_nr(bl[i] - cprev * Asub[i]);
-        cprev = Csup[i] * w[i];
+        Qin[i] = Qv;
+        const double Pn = Csup[i] * Qv;
+        const double Qn = bl[i] * Qv - Asub[i] * Pv;
+        w[i] = Qn;
+        Pv = Pn;
+        Qv = Qn;
+        if (i == NPL / 2 - 1) {  // exact power-of-two rescaling halfway (the ratios are unchanged)
+          int e;
+          frexp(Qv, &e);
+          Pv = ldexp(Pv, -e);
+          Qv = ldexp(Qv, -e);
+        }
       }
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) w[i] = Qin[i] * rcp_nr(w[i]);
     }
     // forward substitution d'_i = (d_i - a_i d'_{i-1}) w_i: affine map per lane, scanned
     double al = 1.0, be = 0.0;
