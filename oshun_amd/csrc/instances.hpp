@@ -23,6 +23,8 @@
 #define OSHUN_I_STAGE1D(EXT, T, L0) EXT template __global__ void oshun::vlasov_stage_1d<T, L0>(oshun::Stage1DArgs<T>);
 #define OSHUN_I_PAIR(EXT, T, L0, R, P) \
   EXT template __global__ void oshun::vlasov_pair_1d<T, L0, R, P>(oshun::Stage1DArgs<T>);
+#define OSHUN_I_MARCH(EXT, T, L0, P) \
+  EXT template __global__ void oshun::vlasov_march_1d<T, L0, P>(oshun::Stage1DArgs<T>);
 #define OSHUN_I_PAIRB(EXT, T, L0, R, P) \
   EXT template __global__ void oshun::vlasov_pair_1d_batch<T, L0, R, P>(const oshun::Stage1DArgs<T>*, const int*, int);
 #define OSHUN_I_COLL1(EXT, T, NPL, COLS) EXT template __global__ void oshun::collide<T, NPL, COLS>(oshun::CollArgs<T>);
@@ -47,6 +49,9 @@
   OSHUN_L0_1_8(OSHUN_P_ONE, T, EXT)               \
   OSHUN_L0_9_12(OSHUN_P_ONE, T, EXT)              \
   OSHUN_L0_1_8(OSHUN_P_R2, T, EXT)
+// single-run stage pairs marched per wavefront: every l0, both pairs
+#define OSHUN_P_MARCH(T, L0, EXT) OSHUN_I_MARCH(EXT, T, L0, 0) OSHUN_I_MARCH(EXT, T, L0, 1)
+#define OSHUN_FAMILY_MARCH(EXT, T) OSHUN_L0_1_8(OSHUN_P_MARCH, T, EXT) OSHUN_L0_9_12(OSHUN_P_MARCH, T, EXT)
 #define OSHUN_FAMILY_PAIR_IMP(EXT) OSHUN_L0_3_12(OSHUN_P_IMP, double, EXT) OSHUN_I_PAIR(EXT, double, 3, 3, 4)
 
 // batched stage pairs: l0 <= 8, R = 2, 3 (and 4 in FP32)
@@ -82,6 +87,7 @@
 
 #define OSHUN_EXTERN_KERNELS                                                                  \
   OSHUN_FAMILY_PAIR(extern, double) OSHUN_FAMILY_PAIR(extern, float) OSHUN_FAMILY_PAIR_IMP(extern) \
+  OSHUN_FAMILY_MARCH(extern, double) OSHUN_FAMILY_MARCH(extern, float)                        \
   OSHUN_FAMILY_PAIRB(extern, double) OSHUN_FAMILY_PAIRB(extern, float) OSHUN_FAMILY_PAIRB_F32_R4(extern) \
   OSHUN_FAMILY_COLL(extern, double) OSHUN_FAMILY_COLL(extern, float)                          \
   OSHUN_FAMILY_COLLB(extern, double) OSHUN_FAMILY_COLLB(extern, float)                        \

@@ -255,6 +255,8 @@ struct Stage1DArgs {
   T c05h, hh, h6, third, twothird;
   int np, nx, mode;
   long long plane;
+  int pw;            // marched stage pairs (vlasov_march_1d): p rows per wavefront
+  const T* rowc;     // their coefficients by row: [p][invpax l = 1..l0 | xv1 l = 0..l0-1 | xv2 l = 1..l0]
   // stage pairs (vlasov_pair_1d) only
   const double* fin2;  // Ex of the second stage's input
   double* jpA;         // [ptile][3][nx] current partials of the second stage's input
@@ -332,6 +334,60 @@ __device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, con
     k[l] = acc;
   }
   if (CLEAN && p < NL - 2) {  // CLEAN_LOW_PC, def-Actions.cpp:240-253 (rows p < min(l-1, 80) of l >= 1)
+#pragma unroll
+    for (int l = 2; l < NL; ++l)
+      if (p < min(l - 1, 80)) k[l] = T(0);
+  }
+}
+
+// The same right-hand side for the marched kernels: the x differences given
+// (dx[l] = f_l(x-1) - f_l(x+1)), the E multipliers formed where they are
+// used, and the coefficients of row p read from one contiguous block (a few
+// wide scalar loads): identical arithmetic.  GENERAL = false leaves out the
+// rows that need a special case (p = 0, CLEAN_LOW_PC rows; callers pass fn =
+// fc at pmax themselves).
+template <typename T, int NL, bool GENERAL>
+__device__ __forceinline__ void rhs_point_1d_dx(const Stage1DArgs<T>& a, int p, double Ex, T bgg, const T (&fm)[NL],
+                                                const T (&fc)[NL], const T (&fn)[NL], const T (&dx)[NL], T (&k)[NL]) {
+  constexpr int L0 = NL - 1;
+  const T* rc = a.rowc + (long long)p * (3 * L0);
+  T G[NL], Hh[NL];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    if (GENERAL && p == 0) {
+      if (l == 0) {
+        const T fe = (fc[0] - fn[0] * a.p0p1_sq) * a.inv_mp0p1_sq;   // G00, def-VlasovMax.cpp:531-532
+        G[0] = (fn[0] - fe) * a.g_r;
+        if (a.bgG) G[0] += bgg;
+        Hh[0] = T(0);
+      } else {
+        G[l] = T(0);
+        Hh[l] = fn[l] * ldc(a.hp0, l);                                    // H(p0), :513-514
+      }
+    } else {
+      const T dpf = fm[l] - fn[l];                                        // SHarmonic::Dp, def-State.cpp:359-380
+      if (l == 0) {
+        G[0] = dpf;
+        if (a.bgG) G[0] += bgg;
+        Hh[0] = T(0);
+      } else {
+        T hv = fc[l] * ldc(rc, l - 1);                                    // MakeGH, :504-509 (invpax)
+        hv = hv + dpf;
+        Hh[l] = hv;
+        G[l] = dpf * ldc(a.cg, l) + hv;
+      }
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    T acc = T(0);
+    if (l >= 1) acc += G[l - 1] * static_cast<T>(Ex * ldc(a.eA1, l - 1));
+    if (l + 1 < NL) acc += Hh[l + 1] * static_cast<T>(Ex * ldc(a.eA2, l + 1));
+    if (l >= 1) acc += dx[l - 1] * ldc(rc, L0 + l - 1);       // v * xA1 of source l-1
+    if (l + 1 < NL) acc += dx[l + 1] * ldc(rc, 2 * L0 + l);   // v * xA2 of source l+1
+    k[l] = acc;
+  }
+  if (GENERAL && p < NL - 2) {  // CLEAN_LOW_PC, def-Actions.cpp:240-253 (rows p < min(l-1, 80) of l >= 1)
 #pragma unroll
     for (int l = 2; l < NL; ++l)
       if (p < min(l - 1, 80)) k[l] = T(0);
@@ -687,6 +743,167 @@ __global__ __launch_bounds__(256) void vlasov_pair_1d_batch(const Stage1DArgs<T>
   if (g >= ts[nruns]) return;  // grid padding
   const int r = batch_run(ts, nruns, g);
   pair_body<T, L0, R, PAIR>(args[r], g - ts[r]);
+}
+
+// ---------------------------------------------------------------------------
+// Two RK4 stages in one pass with every wavefront marching along p on its
+// own (1D, m0 = 0).  A wavefront owns kMarchX x columns and PW p rows (one
+// "chunk"): lanes 2..61 are its columns, lanes 0, 1, 62 and 63 the halo
+// (stage one is exact on lanes 1..62, stage two on 2..61).  At step s it holds
+// the input rows q-1, q, q+1 with the next rows in flight, evaluates stage
+// one on row q = p0-1+s and stage two on row q-1 from the stage-one rows
+// q-2, q-1, q it keeps: no stage goes through LDS, no wavefront waits for
+// another, and loads are issued on every row rather than in a burst at the
+// start of a tile.  x neighbours come from the adjacent lanes (wave_shr:1,
+// wave_shl:1).  The point arithmetic is rhs_point_1d and the stage
+// epilogues are those of pair_body (def-RK.cpp:116-138): PAIR 0 = stages
+// 1+2, PAIR 1 = 3+4.  The current partials of the second stage's input (and,
+// for PAIR 0, of Y2) are summed per chunk in row order.
+constexpr int kMarchX = 60;
+
+template <typename T, int L0, int PAIR, bool GENERAL>
+__device__ __forceinline__ void march_body(const Stage1DArgs<T>& a, const int strip, const int chunk) {
+  constexpr int NL = L0 + 1;
+  constexpr int SZ = sizeof(T);
+  constexpr bool kDeep = sizeof(T) == 4;  // FP32: a second input row and the stage-3 rows one step ahead
+  constexpr int kOut = 0x7ffffff0;        // offset outside the buffer: loads return 0, stores drop
+  const int PW = a.pw;
+  const int NS = PW + 2;  // stage-one rows p0-1 .. p0+PW
+  const int lane = threadIdx.x & 63;
+  const int nx = a.nx, np = a.np;
+  const int x0 = strip * kMarchX;
+  const int xe = ((x0 - 2 + lane) % nx + nx) % nx;
+  const bool inner = lane >= 2 && lane < 2 + kMarchX && x0 + lane - 2 < nx;
+  const int p0 = chunk * PW;  // wave-uniform
+  const unsigned nbytes = static_cast<unsigned>(a.plane * NL * SZ);
+  const int pb = static_cast<int>(a.plane) * SZ;
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(a.in, nbytes);
+  const __amdgpu_buffer_rsrc_t rout = buf_rsrc(a.out, nbytes);
+  const __amdgpu_buffer_rsrc_t racc = buf_rsrc(a.acc, nbytes);
+  const __amdgpu_buffer_rsrc_t ry0 = buf_rsrc(PAIR == 1 ? a.y0 : a.in, nbytes);
+  // rows outside 0..np-1 read 0 (only chunks at the ends of p have them)
+  auto row_ofs = [&](int q) { return (!GENERAL || (q >= 0 && q < np)) ? (q * nx + xe) * SZ : kOut; };
+  auto ld_in = [&](int q, T (&v)[NL]) {
+    const int o = row_ofs(q);
+#pragma unroll
+    for (int l = 0; l < NL; ++l) v[l] = bload<T>(rin, o, l * pb);
+  };
+  // stage 3 reads Y0 at every stage-one row and the accumulator at the chunk's rows
+  auto ld_y0acc = [&](int s, T (&y0)[NL], T (&acv)[NL]) {
+    const int o = row_ofs(p0 - 1 + s);
+    const int oa = (s >= 1 && s <= PW) ? o : kOut;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      y0[l] = bload<T>(ry0, o, l * pb);
+      acv[l] = bload<T>(racc, oa, l * pb);
+    }
+  };
+  // input rows q-1, q, q+1 and the rows in flight; stage-one rows q-2, q-1;
+  // the accumulator of row q-1 (the windows move by one row per step)
+  T im[NL], ic[NL], inx[NL], in1[NL], in2[NL];
+  ld_in(p0 - 2, im);
+  ld_in(p0 - 1, ic);
+  ld_in(p0, inx);
+  if (kDeep) ld_in(p0 + 1, in1);
+  T cy0[NL], cac[NL], ny0[NL], nac[NL];
+  if (PAIR == 1 && kDeep) ld_y0acc(0, cy0, cac);
+  T ym[NL], yc[NL], acp[NL];
+  double jin = 0.0, jout = 0.0;
+#pragma unroll 1
+  for (int s = 0; s < NS; ++s) {
+    const int q = p0 - 1 + s;  // wave-uniform
+    if (kDeep) {
+      if (s + 2 < NS) ld_in(q + 3, in2);
+    } else if (s + 1 < NS) {
+      ld_in(q + 2, in1);
+    }
+    if (PAIR == 1) {
+      if (!kDeep) ld_y0acc(s, cy0, cac);
+      else if (s + 1 < NS) ld_y0acc(s + 1, ny0, nac);
+    }
+    // ---- stage one on row q (stage 1, or stage 3)
+    T yn[NL], acn[NL];
+    if (!GENERAL || (q >= 0 && q < np)) {
+      T fnv[NL], dx[NL], kv[NL];
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        fnv[l] = (!GENERAL || q < np - 1) ? inx[l] : ic[l];  // f(N-1) itself at pmax
+        dx[l] = wave_shift<kWaveShr1>(ic[l], ic[l]) - wave_shift<kWaveShl1>(ic[l], ic[l]);
+      }
+      rhs_point_1d_dx<T, NL, GENERAL>(a, q, a.fin[xe], a.bgG ? ldc(a.bgG, q) : T(0), im, ic, fnv, dx, kv);
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        if (PAIR == 0) {  // stage 1: Y1 = Y0 + h/2 k1, acc = Y0 + h/6 k1
+          const T kh = kv[l] * a.c05h;
+          yn[l] = ic[l] + kh;
+          acn[l] = ic[l] + kh * a.third;
+        } else {          // stage 3: Y3 = Y0 + h k3, acc += h/3 k3
+          const T kh = kv[l] * a.hh;
+          yn[l] = cy0[l] + kh;
+          acn[l] = cac[l] + kh * a.third;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < NL; ++l) { yn[l] = T(0); acn[l] = T(0); }
+    }
+    // ---- stage two on row r = q - 1 (stage 2, or stage 4)
+    const int r = q - 1;
+    if (s >= 2 && (!GENERAL || r < np)) {
+      T fnv[NL], dx[NL], kv[NL];
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        fnv[l] = (!GENERAL || r < np - 1) ? yn[l] : yc[l];
+        dx[l] = wave_shift<kWaveShr1>(yc[l], yc[l]) - wave_shift<kWaveShl1>(yc[l], yc[l]);
+      }
+      rhs_point_1d_dx<T, NL, GENERAL>(a, r, a.fin2[xe], a.bgG ? ldc(a.bgG, r) : T(0), ym, yc, fnv, dx, kv);
+      const int vc = inner ? (r * nx + xe) * SZ : kOut;
+      const double w = (GENERAL && (r == 0 || r == np - 1)) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
+#pragma unroll
+      for (int l = 0; l < NL; ++l) {
+        if (PAIR == 0) {  // stage 2: Y2 = Y0 + h/2 k2, acc += h/3 k2
+          const T kh = kv[l] * a.c05h;
+          const T y2 = im[l] + kh;
+          bstore<T>(y2, rout, vc, l * pb);
+          bstore<T>(acp[l] + kh * a.twothird, racc, vc, l * pb);
+          if (l == 1) jout += w * (static_cast<double>(y2) * ldc(a.p3og, r));
+        } else {          // stage 4: Y = acc + h/6 k4
+          bstore<T>(acp[l] + kv[l] * a.h6, rout, vc, l * pb);
+        }
+      }
+      if (NL > 1) {
+        jin += w * (static_cast<double>(yc[1]) * ldc(a.p3og, r));
+        if (GENERAL && r == 1 && inner) a.f1A[xe] = static_cast<double>(yc[1]);
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      im[l] = ic[l]; ic[l] = inx[l]; inx[l] = in1[l];
+      if (kDeep) in1[l] = in2[l];
+      ym[l] = yc[l]; yc[l] = yn[l]; acp[l] = acn[l];
+      if (PAIR == 1 && kDeep) { cy0[l] = ny0[l]; cac[l] = nac[l]; }
+    }
+  }
+  if (inner) {
+    const long long o = ((long long)chunk * 3) * nx + xe;
+    a.jpA[o] = jin;
+    if (PAIR == 0) a.jpB[o] = jout;
+  }
+}
+
+// one wavefront per (strip, chunk); the four wavefronts of a workgroup take
+// consecutive chunks of one strip, and each XCD a contiguous run of workgroups
+template <typename T, int L0, int PAIR>
+__global__ __launch_bounds__(256) void vlasov_march_1d(Stage1DArgs<T> a) {
+  const int nchunk = (a.np + a.pw - 1) / a.pw;
+  const int nstrip = (a.nx + kMarchX - 1) / kMarchX;
+  const int g = xcd_tile(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (g >= nstrip * nchunk) return;  // grid padding (whole wavefront)
+  const int chunk = g % nchunk, p0 = chunk * a.pw;
+  // interior chunks: every stage row has both p neighbours inside 1..np-2 and
+  // lies above the CLEAN_LOW_PC rows (no special rows, no range checks)
+  if (p0 - 1 >= max(1, L0 - 1) && p0 + a.pw <= a.np - 2) march_body<T, L0, PAIR, false>(a, g / nchunk, chunk);
+  else march_body<T, L0, PAIR, true>(a, g / nchunk, chunk);
 }
 
 template <typename T>
@@ -1979,7 +2196,8 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
 // recomputed from the stage-3 fields.  pre: the column's prefetched inputs
 // (stage-3 fields at x-2..x+2, accumulator at x-1..x+1); j3 and j0: the p sums
 // of the stage-4 input's and of the collided f10's current integrand.
-constexpr int kFuseTiles = 32;  // most stage-pair p tiles a fused collision launch takes (LDS slots per column)
+constexpr int kFuseTiles = 64;  // most stage-pair p tiles a fused collision launch takes (LDS slots per column)
+static_assert(kFuseTiles <= 4 * kFieldRows, "fused_fields_1d sums at most 4 partials per field row");
 struct FusePre {
   double cf[6][5], af[6][3];
 };

@@ -79,6 +79,7 @@ struct oshun_ctx {
   int* d_comp_m = nullptr;     // m of each plane (guard refresh)
   int* d_comp_isim = nullptr;  // 1 on imaginary planes (guard refresh)
   void* d_vt = nullptr;  // Vlasov tables in storage precision: invpax | vtab | cg | hp0
+  void* d_rowc = nullptr;  // the marched kernels' coefficients by row (storage precision)
   double* d_p3og = nullptr;
   double* d_e1d = nullptr;  // 1D m0 = 0 coefficient rows: eA1 | eA2 | xA1 | xA2
   double* d_mw = nullptr;   // moment weights w2 | w3 | w4 | w5
@@ -101,7 +102,9 @@ struct oshun_ctx {
   float* d_bgG = nullptr;
   // 1D m0 = 0: RK4 as two fused stage pairs (vlasov_pair_1d)
   bool use_pair = false;
-  int pair_r = 3;              // rows per thread of the RK4 pair kernels (tile height 4 pair_r - 2)
+  int pair_r = 3;              // rows per thread of the RK4 pair kernels (tile height 4 pair_r - 2);
+                               // 0: the marched pair kernels (march_pw rows per wavefront)
+  int march_pw = 8;
   int npt = 0;                 // p tiles of the pair kernels
   double* jp1 = nullptr;       // of the second stage input of a pair [npt][3][nx]
   double* jp2 = nullptr;       // of Y2 [npt][3][nx]
@@ -488,13 +491,38 @@ template <typename T>
 constexpr int pair_rows() { return 3; }
 constexpr int kPairTP = 4 * 3 - 2;
 constexpr int kPairSmallL0 = 8;  // R = 2 single-run kernels exist for l0 <= 8
-inline int pair_tiles_r(int np, int r) { return (np + 4 * r - 3) / (4 * r - 2); }
+// p tiles of the pair kernels (r = 0: chunks of pw rows of the marched kernels)
+inline int pair_tiles_r(int np, int r, int pw = 0) {
+  return r == 0 ? (np + pw - 1) / pw : (np + 4 * r - 3) / (4 * r - 2);
+}
 template <typename T>
-int pair_tiles(const oshun_ctx* c) { return pair_tiles_r(c->np, c->pair_r); }
+int pair_tiles(const oshun_ctx* c) { return pair_tiles_r(c->np, c->pair_r, c->march_pw); }
+inline int march_strips(const oshun_ctx* c) { return (c->nx + oshun::kMarchX - 1) / oshun::kMarchX; }
+inline int march_waves(const oshun_ctx* c) { return march_strips(c) * pair_tiles_r(c->np, 0, c->march_pw); }
+// Wavefronts per SIMD of the marched pair kernels (VGPRs per wavefront of the
+// compiled kernels: FP64 247-251 at l0 = 8, FP32 141-159; 512 per SIMD lane).
+inline int march_waves_per_simd(bool fp32, int l0) {
+  if (fp32) return l0 <= 8 ? 3 : 2;
+  return l0 <= 8 ? 2 : 1;
+}
+// rows per wavefront: the fewest that fit the whole grid in one round of
+// wavefronts on the chip's 1024 SIMDs (4 at least: shorter marches re-read
+// more halo rows than they save)
+inline int march_rows(int np, int nstrip, int wps) {
+  const int per_strip = std::max(1, (1024 * wps) / nstrip);
+  return std::max(4, (np + per_strip - 1) / per_strip);
+}
 constexpr int kPairTPMin = 4 * 2 - 2;  // smallest tile height the batched path can choose
 
 template <typename T, int L0>
 void launch_pair_l0(oshun_ctx* c, int pair, const oshun::Stage1DArgs<T>& a, dim3 grid) {
+  if (c->pair_r == 0) {
+    if (pair == 0)
+      hipLaunchKernelGGL((oshun::vlasov_march_1d<T, L0, 0>), grid, dim3(256), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((oshun::vlasov_march_1d<T, L0, 1>), grid, dim3(256), 0, c->stream, a);
+    return;
+  }
   if constexpr (L0 <= kPairSmallL0) {
     if (c->pair_r == 2) {
       if (pair == 0)
@@ -520,6 +548,8 @@ oshun::Stage1DArgs<T> pair_args(oshun_ctx* c, const void* in, const void* y0, vo
   a.jpA = jpA;
   a.jpB = jpB;
   a.f1A = c->f1y;
+  a.pw = c->march_pw;
+  a.rowc = static_cast<const T*>(c->d_rowc);
   return a;
 }
 
@@ -529,8 +559,10 @@ int launch_pair(oshun_ctx* c, int pair, const void* in, const void* y0, void* ou
   const oshun::Stage1DArgs<T> a = pair_args<T>(c, in, y0, out, acc, fin, fin2, jpA, jpB, h);
   const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
   // padded to a multiple of the 8 XCDs so that xcd_tile gives each XCD one
-  // contiguous run of tiles (p-tiles of a strip share their halo rows in L2)
-  dim3 grid((ntx * pair_tiles<T>(c) + 7) / 8 * 8);
+  // contiguous run of tiles (p-tiles of a strip share their halo rows in L2);
+  // the marched kernels take four (strip, chunk) wavefronts per workgroup
+  const int nwg = c->pair_r == 0 ? (march_waves(c) + 3) / 4 : ntx * pair_tiles<T>(c);
+  dim3 grid((nwg + 7) / 8 * 8);
   ProfRec r;
   prof_begin(c, 0, &r);
   switch (c->T.l0) {
@@ -907,7 +939,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_srcflag, c->d_imp, c->d_imp_zero, c->d_comp_m, c->d_comp_isim};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_rowc, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_srcflag, c->d_imp, c->d_imp_zero, c->d_comp_m, c->d_comp_isim};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (DevProg& q : c->iprog)
@@ -944,7 +976,21 @@ static void plan_kernels(oshun_ctx* c, std::vector<double>* spec_coef) {
         (!fusable || pair_tiles_r(c->np, 3) > oshun::kFuseTiles || pair_tiles_r(c->np, 2) <= oshun::kFuseTiles))
       c->pair_r = 2;  // small grid (keeping the fused field stages where R = 3 has them)
   }
-  c->npt = c->use_pair ? pair_tiles_r(c->np, c->pair_r) : 0;
+  if (c->use_pair && c->pair_r == 3) {
+    // grids that fill the chip with marches of at least 4 rows take the
+    // marched pair kernels (keeping the fused field stages where the tile
+    // kernels have them)
+    const bool fusable = (c->ph.collisions & OSHUN_COLL_EE) && (c->ph.collisions & OSHUN_COLL_FLM);
+    const int wps = march_waves_per_simd(c->fp32, g.l0);
+    const int pw = march_rows(c->np, march_strips(c), wps);
+    const int nchunk = pair_tiles_r(c->np, 0, pw);
+    if ((long long)march_strips(c) * nchunk >= 1024LL * wps * 3 / 4 && pw <= 32 &&
+        (!fusable || nchunk <= oshun::kFuseTiles || pair_tiles_r(c->np, 3) > oshun::kFuseTiles)) {
+      c->pair_r = 0;
+      c->march_pw = pw;
+    }
+  }
+  c->npt = c->use_pair ? pair_tiles_r(c->np, c->pair_r, c->march_pw) : 0;
   c->spec = -1;
   if (!c->use1d && !c->ext && (unsigned long long)c->plane * sizeof(double) < (1ull << 32)) {  // 32-bit offsets
     std::vector<double> coef;
@@ -1104,7 +1150,8 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
   if (hipMalloc((void**)&c->jpart, (size_t)c->npb * 3 * c->nxy * sizeof(double)) != hipSuccess)
     return bail(OSHUN_ERR_ALLOC);
   if (c->use_pair) {
-    const int npt_max = (c->np + kPairTPMin - 1) / kPairTPMin;  // partial buffers fit every tile height
+    constexpr int kTileMin = kPairTPMin < 4 ? kPairTPMin : 4;  // tile kernels' 6 rows, marched kernels' 4
+    const int npt_max = (c->np + kTileMin - 1) / kTileMin;      // partial buffers fit every tile height
     const size_t nx = c->nx;
     if (hipMalloc((void**)&c->jp1, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&c->jp2, (size_t)npt_max * 3 * nx * sizeof(double)) != hipSuccess ||
@@ -1157,20 +1204,41 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
           out[(L + l) * np + q] = v * static_cast<V>(c->T.x1d_A2[l]);
         }
     };
+    // the same invpax and x-advection multipliers by row for the marched pair
+    // kernels: [p][invpax l = 1..l0 | xv1 l = 0..l0-1 | xv2 l = 1..l0]
+    auto rowc = [&](const auto& x2, auto zero, std::vector<decltype(zero)>& out) {
+      using V = decltype(zero);
+      const size_t L0 = c->T.l0, L = L0 + 1, np = c->np;
+      out.assign(3 * L0 * np, V(0));
+      for (size_t q = 0; q < np; ++q)
+        for (size_t j = 0; j < L0; ++j) {
+          out[q * 3 * L0 + j] = static_cast<V>(c->T.invpax[(j + 1) * np + q]);
+          out[q * 3 * L0 + L0 + j] = x2[j * np + q];
+          out[q * 3 * L0 + 2 * L0 + j] = x2[(L + j + 1) * np + q];
+        }
+    };
     if (c->fp32) {
-      std::vector<float> vf(vt.begin(), vt.end()), x2;
+      std::vector<float> vf(vt.begin(), vt.end()), x2, rc;
       xv(0.0f, x2);
+      rowc(x2, 0.0f, rc);
       vf.insert(vf.end(), x2.begin(), x2.end());
       float* d = nullptr;
       if (upload_vec(c, &d, vf)) return bail(OSHUN_ERR_HIP);
       c->d_vt = d;
+      float* dr = nullptr;
+      if (upload_vec(c, &dr, rc)) return bail(OSHUN_ERR_HIP);
+      c->d_rowc = dr;
     } else {
-      std::vector<double> x2;
+      std::vector<double> x2, rc;
       xv(0.0, x2);
+      rowc(x2, 0.0, rc);
       vt.insert(vt.end(), x2.begin(), x2.end());
       double* d = nullptr;
       if (upload_vec(c, &d, vt)) return bail(OSHUN_ERR_HIP);
       c->d_vt = d;
+      double* dr = nullptr;
+      if (upload_vec(c, &dr, rc)) return bail(OSHUN_ERR_HIP);
+      c->d_rowc = dr;
     }
   }
   {
@@ -1575,11 +1643,12 @@ int oshun_pair_rows(oshun_ctx* c, int set, int* rows) {
     if (!c->use_pair) return fail(c, OSHUN_ERR_ARG, "no stage-pair kernels on this context (1D, m0 = 0 only)");
     if (set == 10) c->pair_r = 3;
     else if (set == 6 && c->T.l0 <= kPairSmallL0) c->pair_r = 2;
-    else return fail(c, OSHUN_ERR_ARG, "tile height must be 10, or 6 for l0 <= 8");
-    c->npt = pair_tiles_r(c->np, c->pair_r);
+    else if (set <= -4 && set >= -32) { c->pair_r = 0; c->march_pw = -set; }
+    else return fail(c, OSHUN_ERR_ARG, "tile height must be 10, or 6 for l0 <= 8; -4 .. -32 for the marched kernels");
+    c->npt = pair_tiles_r(c->np, c->pair_r, c->march_pw);
     c->next1_ready = false;
   }
-  if (rows) *rows = c->use_pair ? 4 * c->pair_r - 2 : 0;
+  if (rows) *rows = c->use_pair ? (c->pair_r == 0 ? -c->march_pw : 4 * c->pair_r - 2) : 0;
   return OSHUN_OK;
 }
 
