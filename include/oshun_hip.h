@@ -254,13 +254,16 @@ int oshun_diag_copy(oshun_ctx* ctx, long long nbytes);
  * 0 restores the default, -1 only queries.  The general kernels evaluate
  * Actions::Adv (def-Actions.cpp:73-94) with the same per-target sums. */
 int oshun_stage_kernel(oshun_ctx* ctx, int set, int* which);
-/* Tile height (p rows) of the 1D stage-pair kernels: 10 by default, 6 where
- * the grid at 10 would give fewer than two workgroups per CU (latency-bound
- * small runs, l0 <= 8).  set = 6 or 10 chooses it, 0 only queries (*rows = 0
- * on contexts without stage pairs).  The tile height groups the p sum of the
+/* p rows of the 1D stage-pair kernels.  Positive: the tile kernels with
+ * tiles of 10 rows (6 for l0 <= 8).  Negative: the marched kernels with -rows
+ * rows per wavefront (4 .. 32).  By default grids that fill the chip take the
+ * marched kernels with the fewest rows that fit one round of wavefronts, small
+ * grids the tile kernels (6 rows where 10 would give fewer than two
+ * workgroups per CU).  set chooses (6, 10, -4 .. -32), 0 only queries (*rows =
+ * 0 on contexts without stage pairs).  The choice groups the p sum of the
  * current differently (results agree to rounding, not bitwise); a batch
- * member equals its run alone bitwise at the same height.  No counterpart in
- * the reference. */
+ * member equals its run alone bitwise at the same tile height.  No
+ * counterpart in the reference. */
 int oshun_pair_rows(oshun_ctx* ctx, int set, int* rows);
 
 /* ---- Batched independent runs ------------------------------------------

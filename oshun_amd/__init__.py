@@ -398,7 +398,8 @@ class Context:
         return ("1d", "spec", "general", "lm")[w.value]
 
     def pair_rows(self, set=0):
-        """Tile height (p rows) of the 1D stage-pair kernels; set=6 or 10 chooses it, 0 only queries."""
+        """p rows of the 1D stage-pair kernels: 6 or 10 (tile kernels), -4 .. -32 (marched kernels, rows
+        per wavefront); set chooses, 0 only queries."""
         r = ctypes.c_int(0)
         self._chk(self._lib.oshun_pair_rows(self._h, int(set), ctypes.byref(r)))
         return r.value
