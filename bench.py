@@ -635,7 +635,11 @@ def main():
     ctx32, ms32, prof32 = run_path(oshun_amd.FP32_SPLIT, args.steps, args.warmup, f, fld, dx)
     S32 = ctx32.state_bytes()
 
-    def roof(prof, S, steps):
+    def pair_kernel(ctx):
+        # the stage-pair family the context runs (marched kernels: negative rows)
+        return "vlasov_march_1d" if ctx.pair_rows() < 0 else "vlasov_pair_1d"
+
+    def roof(prof, S, steps, ctx, prec_key):
         # the Vlasov launches of one RK4 step (two fused stage pairs on the 1D
         # path) share the 16 S_f algorithmic bytes of the step
         ms, n = prof["vlasov_stage"]
@@ -643,27 +647,32 @@ def main():
         per_step = n / steps
         bytes_per_launch = 16.0 * S / per_step
         ach = bytes_per_launch / avg_s
-        return {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": ach / HBM_PEAK, "traffic": None, "kernel": "vlasov_pair_1d",
-                "kernel_avg_us": avg_s * 1e6, "bytes_per_launch": bytes_per_launch,
-                "launches_per_step": per_step}
+        r = {"bound": "hbm", "achieved": ach / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+             "frac": ach / HBM_PEAK, "traffic": None, "kernel": pair_kernel(ctx),
+             "kernel_avg_us": avg_s * 1e6, "bytes_per_launch": bytes_per_launch,
+             "launches_per_step": per_step}
+        # HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE)
+        # on this workload, calibrated on a plain copy (profiles/traffic_*.json):
+        # the mean over the two pair launches of the family that ran
+        tpath = os.path.join(ROOT, "profiles", "traffic_l8_np256_nx4096.json")
+        if os.path.exists(tpath):
+            with open(tpath) as fh:
+                tr = json.load(fh).get(prec_key, {})
+            pairs = [v for name, v in tr.get("kernels", {}).items() if name.startswith("oshun::" + r["kernel"])]
+            if pairs:
+                r["traffic"] = sum(v["read_bytes"] + v["write_bytes"] for v in pairs) / len(pairs)
+                r["traffic_source"] = os.path.relpath(tpath, ROOT)
+                # the bytes the launches really move, at the same mean duration
+                r["traffic_gbs"] = r["traffic"] / avg_s / 1e9
+                r["traffic_frac"] = r["traffic"] / avg_s / HBM_PEAK
+        return r
 
     def breakdown(prof, steps):
         return {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps}
                 for k, v in prof.items() if v[1]}
 
-    r64 = roof(prof64, S64, args.steps)
-    # HBM bytes per launch measured by rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE) on
-    # this workload, calibrated on a plain copy (profiles/traffic_*.json)
-    tpath = os.path.join(ROOT, "profiles", "traffic_l8_np256_nx4096_fp64.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fh:
-            tr = json.load(fh)
-        pairs = [v for name, v in tr.get("kernels", {}).items() if name.startswith("oshun::vlasov_pair_1d")]
-        if pairs:  # mean HBM bytes per launch over the two pair kernels
-            r64["traffic"] = sum(v["read_bytes"] + v["write_bytes"] for v in pairs) / len(pairs)
-            r64["traffic_source"] = os.path.relpath(tpath, ROOT)
-    r32 = roof(prof32, S32, args.steps)
+    r64 = roof(prof64, S64, args.steps, ctx64, "fp64")
+    r32 = roof(prof32, S32, args.steps, ctx32, "fp32")
     (f64, fl64), (f32, fl32) = ctx64.final, ctx32.final
 
     def l2(a, b):
