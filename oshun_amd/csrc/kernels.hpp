@@ -257,6 +257,7 @@ struct Stage1DArgs {
   long long plane;
   int pw;            // marched stage pairs (vlasov_march_1d): p rows per wavefront
   const T* rowc;     // their coefficients by row: [p][invpax l = 1..l0 | xv1 l = 0..l0-1 | xv2 l = 1..l0]
+  const T* eAT;      // and eA1 | eA2 in the storage precision (the E multiplier is T(Ex) * eA)
   // stage pairs (vlasov_pair_1d) only
   const double* fin2;  // Ex of the second stage's input
   double* jpA;         // [ptile][3][nx] current partials of the second stage's input
@@ -343,7 +344,7 @@ __device__ __forceinline__ void rhs_point_1d(const Stage1DArgs<T>& a, int p, con
 // The same right-hand side for the marched kernels: the x differences given
 // (dx[l] = f_l(x-1) - f_l(x+1)), the E multipliers formed where they are
 // used, and the coefficients of row p read from one contiguous block (a few
-// wide scalar loads): identical arithmetic.  GENERAL = false leaves out the
+// wide scalar loads): identical arithmetic in FP64.  GENERAL = false leaves out the
 // rows that need a special case (p = 0, CLEAN_LOW_PC rows; callers pass fn =
 // fc at pmax themselves).
 template <typename T, int NL, bool GENERAL>
@@ -352,45 +353,48 @@ __device__ __forceinline__ void rhs_point_1d_dx(const Stage1DArgs<T>& a, int p, 
   constexpr int L0 = NL - 1;
   const T* rc = a.rowc + (long long)p * (3 * L0);
   T G[NL], Hh[NL];
+  // one branch for the p = 0 row, not one per harmonic
+  if (GENERAL && p == 0) {
+    const T fe = (fc[0] - fn[0] * a.p0p1_sq) * a.inv_mp0p1_sq;   // G00, def-VlasovMax.cpp:531-532
+    G[0] = (fn[0] - fe) * a.g_r;
+    if (a.bgG) G[0] += bgg;
+    Hh[0] = T(0);
 #pragma unroll
-  for (int l = 0; l < NL; ++l) {
-    if (GENERAL && p == 0) {
-      if (l == 0) {
-        const T fe = (fc[0] - fn[0] * a.p0p1_sq) * a.inv_mp0p1_sq;   // G00, def-VlasovMax.cpp:531-532
-        G[0] = (fn[0] - fe) * a.g_r;
-        if (a.bgG) G[0] += bgg;
-        Hh[0] = T(0);
-      } else {
-        G[l] = T(0);
-        Hh[l] = fn[l] * ldc(a.hp0, l);                                    // H(p0), :513-514
-      }
-    } else {
-      const T dpf = fm[l] - fn[l];                                        // SHarmonic::Dp, def-State.cpp:359-380
+    for (int l = 1; l < NL; ++l) {
+      G[l] = T(0);
+      Hh[l] = fn[l] * ldc(a.hp0, l);                                    // H(p0), :513-514
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      const T dpf = fm[l] - fn[l];                                      // SHarmonic::Dp, def-State.cpp:359-380
       if (l == 0) {
         G[0] = dpf;
         if (a.bgG) G[0] += bgg;
         Hh[0] = T(0);
       } else {
-        T hv = fc[l] * ldc(rc, l - 1);                                    // MakeGH, :504-509 (invpax)
+        T hv = fc[l] * ldc(rc, l - 1);                                  // MakeGH, :504-509 (invpax)
         hv = hv + dpf;
         Hh[l] = hv;
         G[l] = dpf * ldc(a.cg, l) + hv;
       }
     }
   }
+  // E multipliers in the storage precision: in FP64 the same products as
+  // rhs_point_1d; in FP32 a float product (the slope is float throughout)
+  const T ex = static_cast<T>(Ex);
 #pragma unroll
   for (int l = 0; l < NL; ++l) {
     T acc = T(0);
-    if (l >= 1) acc += G[l - 1] * static_cast<T>(Ex * ldc(a.eA1, l - 1));
-    if (l + 1 < NL) acc += Hh[l + 1] * static_cast<T>(Ex * ldc(a.eA2, l + 1));
+    if (l >= 1) acc += G[l - 1] * (ex * ldc(a.eAT, l - 1));
+    if (l + 1 < NL) acc += Hh[l + 1] * (ex * ldc(a.eAT, NL + l + 1));
     if (l >= 1) acc += dx[l - 1] * ldc(rc, L0 + l - 1);       // v * xA1 of source l-1
     if (l + 1 < NL) acc += dx[l + 1] * ldc(rc, 2 * L0 + l);   // v * xA2 of source l+1
     k[l] = acc;
   }
-  if (GENERAL && p < NL - 2) {  // CLEAN_LOW_PC, def-Actions.cpp:240-253 (rows p < min(l-1, 80) of l >= 1)
+  if (GENERAL) {  // CLEAN_LOW_PC, def-Actions.cpp:240-253 (rows p < min(l-1, 80) of l >= 1), by selection
 #pragma unroll
-    for (int l = 2; l < NL; ++l)
-      if (p < min(l - 1, 80)) k[l] = T(0);
+    for (int l = 2; l < NL; ++l) k[l] = (p < min(l - 1, 80)) ? T(0) : k[l];
   }
 }
 

@@ -80,6 +80,7 @@ struct oshun_ctx {
   int* d_comp_isim = nullptr;  // 1 on imaginary planes (guard refresh)
   void* d_vt = nullptr;  // Vlasov tables in storage precision: invpax | vtab | cg | hp0
   void* d_rowc = nullptr;  // the marched kernels' coefficients by row (storage precision)
+  void* d_eAT = nullptr;   // the 1D E coefficients eA1 | eA2 in the storage precision
   double* d_p3og = nullptr;
   double* d_e1d = nullptr;  // 1D m0 = 0 coefficient rows: eA1 | eA2 | xA1 | xA2
   double* d_mw = nullptr;   // moment weights w2 | w3 | w4 | w5
@@ -550,6 +551,7 @@ oshun::Stage1DArgs<T> pair_args(oshun_ctx* c, const void* in, const void* y0, vo
   a.f1A = c->f1y;
   a.pw = c->march_pw;
   a.rowc = static_cast<const T*>(c->d_rowc);
+  a.eAT = static_cast<const T*>(c->d_eAT);
   return a;
 }
 
@@ -939,7 +941,7 @@ int download_raw(oshun_ctx* c, const void* fsrc, const double* fldsrc, double* c
 
 void free_all(oshun_ctx* c) {
   void* ptrs[] = {c->S, c->A, c->B1, c->B2, c->Sf, c->Af, c->B1f, c->B2f, c->Cf, c->jpart, c->d_terms, c->d_tstart,
-                  c->d_clean, c->d_comp_l, c->d_vt, c->d_rowc, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_srcflag, c->d_imp, c->d_imp_zero, c->d_comp_m, c->d_comp_isim};
+                  c->d_clean, c->d_comp_l, c->d_vt, c->d_rowc, c->d_eAT, c->d_p3og, c->d_coll, c->d_e1d, c->d_mw, c->d_mom, c->d_mpart, c->d_rt, c->d_rt2, c->d_bg, c->d_bgG, c->jp1, c->jp2, c->f1y, c->Af2, c->d_coef, c->d_src, c->d_srcflag, c->d_imp, c->d_imp_zero, c->d_comp_m, c->d_comp_isim};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (DevProg& q : c->iprog)
@@ -1217,6 +1219,19 @@ int oshun_create(const oshun_grid* grid, const oshun_phys* phys, oshun_ctx** out
           out[q * 3 * L0 + 2 * L0 + j] = x2[(L + j + 1) * np + q];
         }
     };
+    {  // eA1 | eA2 of the 1D kernels in the storage precision (the marched kernels)
+      std::vector<double> ea(c->T.e1d_A1.begin(), c->T.e1d_A1.end());
+      ea.insert(ea.end(), c->T.e1d_A2.begin(), c->T.e1d_A2.end());
+      if (c->fp32) {
+        float* d = nullptr;
+        if (upload_vec(c, &d, std::vector<float>(ea.begin(), ea.end()))) return bail(OSHUN_ERR_HIP);
+        c->d_eAT = d;
+      } else {
+        double* d = nullptr;
+        if (upload_vec(c, &d, ea)) return bail(OSHUN_ERR_HIP);
+        c->d_eAT = d;
+      }
+    }
     if (c->fp32) {
       std::vector<float> vf(vt.begin(), vt.end()), x2, rc;
       xv(0.0f, x2);
