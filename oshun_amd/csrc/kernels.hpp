@@ -2166,24 +2166,29 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
   const double f0 = seg_first<SEGW>(f[0]);
   const double f1 = seg_first<SEGW>(f[1]);
   const double f00 = (f0 - f1 * a.r01) * a.inv_gden;
-  double G[NPL];
+  // G on every row first, in the basic block of the Coulomb logarithm so
+  // that the two chains interleave; then the parabolic rows (one branch each)
+  double G[NPL], J1x[NPL];
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const double j1 = J1[i] + ex1;
+    J1x[i] = j1;
     double i2 = (I2[i] + ex2) * j1;
     i2 *= -3.0;
     double g = (I4[i] + ex4) + rt<SEGW>(tb, RT_U3, i, seg) * j1;
     g *= f[i];
     g += i2;
-    {  // parabolic G for p < NB (EE_isotropic_conserva::G, :178-188): rows of lane 0 only,
-       // evaluated on every row and selected, so that the slope stays one basic block
+    G[i] = g;
+  }
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    if (i < a.nb) {  // parabolic G for p < NB (EE_isotropic_conserva::G, :178-188): rows of lane 0 only
       const double p3 = tb[RT_P3 * kRowP + i * SEGW];
       const double i2s = f00 * tb[RT_A2 * kRowP + i * SEGW] + (f1 - f00) * tb[RT_B2 * kRowP + i * SEGW];
       const double i4s = f00 * tb[RT_A4 * kRowP + i * SEGW] + (f1 - f00) * tb[RT_B4 * kRowP + i * SEGW];
-      const double gl = f[i] * i4s + (p3 * f[i] - 3.0 * i2s) * j1;
-      g = (seg == 0 && i < a.nb) ? gl : g;
+      const double gl = f[i] * i4s + (p3 * f[i] - 3.0 * i2s) * J1x[i];
+      G[i] = (seg == 0) ? gl : G[i];
     }
-    G[i] = g;
   }
   // -DG, -DG/(v Dv), DDG/(v Dv), DDG/(v^3 DDv)   (:247-260)
   const double Gnext = seg_next0<SEGW>(G[0]);
