@@ -500,11 +500,14 @@ template <typename T>
 int pair_tiles(const oshun_ctx* c) { return pair_tiles_r(c->np, c->pair_r, c->march_pw); }
 inline int march_strips(const oshun_ctx* c) { return (c->nx + oshun::kMarchX - 1) / oshun::kMarchX; }
 inline int march_waves(const oshun_ctx* c) { return march_strips(c) * pair_tiles_r(c->np, 0, c->march_pw); }
-// Wavefronts per SIMD of the marched pair kernels (VGPRs per wavefront of the
-// compiled kernels: FP64 247-251 at l0 = 8, FP32 141-159; 512 per SIMD lane).
+// Wavefronts per SIMD of the marched pair kernels by l0: 512 VGPRs per SIMD
+// lane over the larger VGPR count of the two pair kernels as compiled for
+// gfx950 (FP64 from 72 at l0 = 1 to 360 at l0 = 12; FP32 from 49 to 226).
 inline int march_waves_per_simd(bool fp32, int l0) {
-  if (fp32) return l0 <= 8 ? 3 : 2;
-  return l0 <= 8 ? 2 : 1;
+  static const int kF64[13] = {0, 7, 5, 4, 3, 2, 2, 2, 2, 1, 1, 1, 1};
+  static const int kF32[13] = {0, 9, 7, 5, 4, 4, 3, 3, 3, 2, 2, 2, 2};
+  const int i = std::min(std::max(l0, 1), 12);
+  return fp32 ? kF32[i] : kF64[i];
 }
 // rows per wavefront: the fewest that fit the whole grid in one round of
 // wavefronts on the chip's 1024 SIMDs (4 at least: shorter marches re-read
