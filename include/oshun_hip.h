@@ -284,10 +284,16 @@ int oshun_batch_size(const oshun_batch* batch);
 oshun_ctx* oshun_batch_run(oshun_batch* batch, int i);
 int oshun_batch_step(oshun_batch* batch, int nsteps);
 int oshun_batch_synchronize(oshun_batch* batch);
-/* Times `steps` steps with each candidate tile height of the stage-pair
- * kernels on the current state (restored afterwards) and keeps the fastest;
- * reports its p rows per tile and ms per step. */
+/* Times `steps` steps with each candidate stage-pair kernel (tiles of 6, 10
+ * and in FP32 14 p rows; marched kernels with 6, 8, 12 rows per wavefront) on
+ * the current state (restored afterwards) and keeps the fastest; reports its
+ * p rows as oshun_pair_rows does (negative: marched) and ms per step. */
 int oshun_batch_autotune(oshun_batch* batch, int steps, int* chosen_rows, double* ms_per_step);
+/* The batch's stage-pair kernels as oshun_pair_rows sets a run's (6, 10, 14
+ * in FP32, -4 .. -32 marched; 0 only queries); re-records the step graphs.
+ * A member equals its run alone bitwise at the same setting.  No counterpart
+ * in the reference. */
+int oshun_batch_pair_rows(oshun_batch* batch, int set, int* rows);
 
 /* ---- Adjoint of the step -------------------------------------------------
  * Reverse-mode derivative (vector-Jacobian product) of oshun_step: RK4 of the

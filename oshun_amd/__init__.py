@@ -152,6 +152,7 @@ def _load():
     lib.oshun_batch_step.argtypes = [P, I]
     lib.oshun_batch_synchronize.argtypes = [P]
     lib.oshun_batch_autotune.argtypes = [P, I, ctypes.POINTER(I), dp]
+    lib.oshun_batch_pair_rows.argtypes = [P, I, ctypes.POINTER(I)]
     lib.oshun_copy_device_state.argtypes = [P, P, P, I]
     lib.oshun_adjoint_step.argtypes = [P, D, P, P]
     lib.oshun_adjoint_forward.argtypes = [P, D, I, I, ctypes.POINTER(ctypes.c_longlong)]
@@ -509,11 +510,21 @@ class Batch:
         self._chk(self._lib.oshun_batch_synchronize(self._h))
 
     def autotune(self, steps=3):
-        """Times each stage-pair tile height on the batch (state restored); returns (p rows, ms/step)."""
+        """Times each stage-pair kernel choice on the batch (state restored); returns (p rows, ms/step).
+
+        p rows as pair_rows reports them: positive for the tile kernels, negative
+        (rows per wavefront) for the marched kernels."""
         rows = ctypes.c_int()
         ms = ctypes.c_double()
         self._chk(self._lib.oshun_batch_autotune(self._h, int(steps), ctypes.byref(rows), ctypes.byref(ms)))
         return rows.value, ms.value
+
+    def pair_rows(self, set=0):
+        """The batch's stage-pair kernels: 6, 10 (14 in FP32) tile rows, or -4 .. -32
+        marched rows per wavefront; 0 only queries.  Returns the setting."""
+        r = ctypes.c_int()
+        self._chk(self._lib.oshun_batch_pair_rows(self._h, int(set), ctypes.byref(r)))
+        return r.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -17,7 +17,8 @@ struct oshun_batch {
   std::vector<oshun_ctx*> runs;
   std::vector<double> h;  // RK4 step per run
   int fp32 = 0;
-  int pair_r = 3;         // rows per thread of the pair kernels (tile height 4R-2)
+  int pair_r = 3;         // rows per thread of the pair kernels (tile height 4R-2); 0: the marched kernels
+  int march_pw = 8;       // p rows per wavefront of the marched kernels
   hipStream_t stream = nullptr;
   double* d_t = nullptr;  // run clocks
   double* d_h = nullptr;
@@ -91,6 +92,21 @@ void launch_pair_batch(hipStream_t s, int l0, const BLaunch<oshun::Stage1DArgs<T
     default: hipLaunchKernelGGL((oshun::vlasov_pair_1d_batch<T, 8, R, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
   }
 }
+template <typename T, int PAIR>
+void launch_march_batch(hipStream_t s, int l0, const BLaunch<oshun::Stage1DArgs<T>>& L) {
+  const dim3 g(pad8((L.total() + 3) / 4)), blk(256);  // four wavefronts per workgroup
+  const int n = static_cast<int>(L.sel.size());
+  switch (l0) {
+    case 1: hipLaunchKernelGGL((oshun::vlasov_march_1d_batch<T, 1, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
+    case 2: hipLaunchKernelGGL((oshun::vlasov_march_1d_batch<T, 2, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
+    case 3: hipLaunchKernelGGL((oshun::vlasov_march_1d_batch<T, 3, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
+    case 4: hipLaunchKernelGGL((oshun::vlasov_march_1d_batch<T, 4, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
+    case 5: hipLaunchKernelGGL((oshun::vlasov_march_1d_batch<T, 5, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
+    case 6: hipLaunchKernelGGL((oshun::vlasov_march_1d_batch<T, 6, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
+    case 7: hipLaunchKernelGGL((oshun::vlasov_march_1d_batch<T, 7, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
+    default: hipLaunchKernelGGL((oshun::vlasov_march_1d_batch<T, 8, PAIR>), g, blk, 0, s, L.d_args, L.d_ts, n); break;
+  }
+}
 template <typename T, int MODE>
 void launch_coll_batch(hipStream_t s, int key, const BLaunch<oshun::CollArgs<T>>& L) {
   const dim3 g(pad8(L.total())), blk(256);
@@ -103,9 +119,10 @@ void launch_coll_batch(hipStream_t s, int key, const BLaunch<oshun::CollArgs<T>>
 }
 
 // Record one step (parity q: S/B1 of every run as after q steps) on b->stream.
+// R > 0: the tile kernels with tiles of 4R-2 rows; R = 0: the marched kernels
+// with b->march_pw rows per wavefront (the launch counts wavefronts).
 template <typename T, int R>
 int record_step(oshun_batch* b, int q) {
-  constexpr int TP = 4 * R - 2;
   const int nr = static_cast<int>(b->runs.size());
   BLaunch<oshun::FieldArgs<T>> f0, f2;
   BLaunch<oshun::Field23Args<T>> f23;
@@ -118,18 +135,22 @@ int record_step(oshun_batch* b, int q) {
     const int flip = (q != b->parity);
     void* S = flip ? c->B1 : c->S;
     void* B1 = flip ? c->S : c->B1;
-    const int npt = (c->np + TP - 1) / TP;
-    const int ntx = (c->nx + oshun::kPairX - 1) / oshun::kPairX;
+    const int npt = pair_tiles_r(c->np, R, b->march_pw);
+    const int units = R == 0 ? march_strips(c) * npt : ((c->nx + oshun::kPairX - 1) / oshun::kPairX) * npt;
     const int fblocks = (c->nxy + oshun::kFieldX - 1) / oshun::kFieldX;
     oshun::FieldArgs<T> fa0 = field_args<T>(c, 1, c->Sf, c->Sf, c->B1f, c->Af, S, h, nullptr, 0, nullptr);
     fa0.p3og = c->d_p3og;  // current of Y0 summed directly over p
     f0.add(i, fa0, fblocks);
-    p0[c->T.l0].add(i, pair_args<T>(c, S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h), ntx * npt);
+    oshun::Stage1DArgs<T> pa0 = pair_args<T>(c, S, nullptr, c->B2, c->A, c->Sf, c->B1f, c->jp1, c->jp2, h);
+    pa0.pw = b->march_pw;  // the batch's rows per wavefront (the run's own plan is left as it is)
+    p0[c->T.l0].add(i, pa0, units);
     oshun::Field23Args<T> fa23;
     fa23.a2 = field_args<T>(c, 2, c->B1f, c->Sf, c->B2f, c->Af, nullptr, h, c->jp1, npt, c->f1y);
     fa23.a3 = field_args<T>(c, 3, c->B2f, c->Sf, c->Cf, c->Af, c->B2, h, c->jp2, npt, nullptr);
     f23.add(i, fa23, fblocks);
-    p1[c->T.l0].add(i, pair_args<T>(c, c->B2, S, B1, c->A, c->B2f, c->Cf, c->jp1, nullptr, h), ntx * npt);
+    oshun::Stage1DArgs<T> pa1 = pair_args<T>(c, c->B2, S, B1, c->A, c->B2f, c->Cf, c->jp1, nullptr, h);
+    pa1.pw = b->march_pw;
+    p1[c->T.l0].add(i, pa1, units);
     f2.add(i, field_args<T>(c, 4, c->Cf, nullptr, c->Sf, c->Af, nullptr, h, c->jp1, npt, c->f1y), fblocks);
     const int coll = c->ph.collisions;
     if (coll) {
@@ -167,9 +188,15 @@ int record_step(oshun_batch* b, int q) {
   BCHK(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const dim3 fblk(oshun::kFieldX, oshun::kFieldRows);
   hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f0.total()), fblk, 0, s, f0.d_args, f0.d_ts, nr);
-  for (auto& kv : p0) launch_pair_batch<T, R, 0>(s, kv.first, kv.second);
+  for (auto& kv : p0) {
+    if (R == 0) launch_march_batch<T, 0>(s, kv.first, kv.second);
+    else launch_pair_batch<T, (R > 0 ? R : 3), 0>(s, kv.first, kv.second);
+  }
   hipLaunchKernelGGL(oshun::field_stage23_1d_batch<T>, dim3(f23.total()), fblk, 0, s, f23.d_args, f23.d_ts, nr);
-  for (auto& kv : p1) launch_pair_batch<T, R, 1>(s, kv.first, kv.second);
+  for (auto& kv : p1) {
+    if (R == 0) launch_march_batch<T, 1>(s, kv.first, kv.second);
+    else launch_pair_batch<T, (R > 0 ? R : 3), 1>(s, kv.first, kv.second);
+  }
   hipLaunchKernelGGL(oshun::field_stage_batch<T>, dim3(f2.total()), fblk, 0, s, f2.d_args, f2.d_ts, nr);
   for (auto& kv : c1) launch_coll_batch<T, 1>(s, kv.first, kv.second);
   for (auto& kv : c2) launch_coll_batch<T, 2>(s, kv.first, kv.second);
@@ -197,9 +224,11 @@ int build_graphs(oshun_batch* b) {
   for (int q = 0; q < 2; ++q) {
     int rc;
     if (b->fp32)
-      rc = b->pair_r == 2 ? record_step<float, 2>(b, q)
-                          : (b->pair_r == 3 ? record_step<float, 3>(b, q) : record_step<float, 4>(b, q));
-    else rc = b->pair_r == 2 ? record_step<double, 2>(b, q) : record_step<double, 3>(b, q);
+      rc = b->pair_r == 0   ? record_step<float, 0>(b, q)
+           : b->pair_r == 2 ? record_step<float, 2>(b, q)
+                            : (b->pair_r == 3 ? record_step<float, 3>(b, q) : record_step<float, 4>(b, q));
+    else rc = b->pair_r == 0   ? record_step<double, 0>(b, q)
+              : b->pair_r == 2 ? record_step<double, 2>(b, q) : record_step<double, 3>(b, q);
     if (rc) return rc;
   }
   return OSHUN_OK;
@@ -335,15 +364,22 @@ int oshun_batch_autotune(oshun_batch* b, int steps, int* chosen_rows, double* ms
     }
     hipStreamSynchronize(b->stream);
   };
-  std::vector<int> cand = {2, 3};  // rows per thread: 6 or 10 p rows per tile
+  // candidates: tile kernels with 6, 10 (and in FP32 14) p rows per tile;
+  // marched kernels with 6, 8 or 12 p rows per wavefront (negative)
+  std::vector<int> cand = {2, 3};
   if (b->fp32) cand.push_back(4);   // 14 rows (float only: double would exceed 256 VGPRs)
+  for (int pw : {6, 8, 12}) cand.push_back(-pw);
   double best = 1e300;
-  int best_r = b->pair_r;
+  int best_r = b->pair_r > 0 ? b->pair_r : -b->march_pw;
   hipEvent_t e0, e1;
   hipEventCreate(&e0);
   hipEventCreate(&e1);
+  auto choose = [b](int r) {
+    if (r > 0) b->pair_r = r;
+    else { b->pair_r = 0; b->march_pw = -r; }
+  };
   for (int r : cand) {
-    b->pair_r = r;
+    choose(r);
     int rc = build_graphs(b);
     if (!rc) rc = batch_steps(b, 1);  // warm-up
     if (!rc) {
@@ -361,11 +397,25 @@ int oshun_batch_autotune(oshun_batch* b, int steps, int* chosen_rows, double* ms
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   release();
-  b->pair_r = best_r;
+  choose(best_r);
   const int rc = build_graphs(b);
-  if (chosen_rows) *chosen_rows = 4 * best_r - 2;
+  if (chosen_rows) *chosen_rows = best_r > 0 ? 4 * best_r - 2 : best_r;
   if (ms_per_step) *ms_per_step = best;
   return rc;
+}
+
+int oshun_batch_pair_rows(oshun_batch* b, int set, int* rows) {
+  if (!b) return OSHUN_ERR_ARG;
+  if (set != 0) {
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (set == 6 || set == 10 || (set == 14 && b->fp32)) b->pair_r = (set + 2) / 4;
+    else if (set <= -4 && set >= -32) { b->pair_r = 0; b->march_pw = -set; }
+    else return bfail(b, OSHUN_ERR_ARG, "tile height must be 6, 10 (14 in FP32); -4 .. -32 for the marched kernels");
+    const int rc = build_graphs(b);
+    if (rc) return rc;
+  }
+  if (rows) *rows = b->pair_r > 0 ? 4 * b->pair_r - 2 : -b->march_pw;
+  return OSHUN_OK;
 }
 
 }  // extern "C"

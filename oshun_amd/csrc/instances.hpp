@@ -25,6 +25,8 @@
   EXT template __global__ void oshun::vlasov_pair_1d<T, L0, R, P>(oshun::Stage1DArgs<T>);
 #define OSHUN_I_MARCH(EXT, T, L0, P) \
   EXT template __global__ void oshun::vlasov_march_1d<T, L0, P>(oshun::Stage1DArgs<T>);
+#define OSHUN_I_MARCHB(EXT, T, L0, P) \
+  EXT template __global__ void oshun::vlasov_march_1d_batch<T, L0, P>(const oshun::Stage1DArgs<T>*, const int*, int);
 #define OSHUN_I_PAIRB(EXT, T, L0, R, P) \
   EXT template __global__ void oshun::vlasov_pair_1d_batch<T, L0, R, P>(const oshun::Stage1DArgs<T>*, const int*, int);
 #define OSHUN_I_COLL1(EXT, T, NPL, COLS) EXT template __global__ void oshun::collide<T, NPL, COLS>(oshun::CollArgs<T>);
@@ -58,6 +60,9 @@
 #define OSHUN_PB(T, L0, EXT, R) OSHUN_I_PAIRB(EXT, T, L0, R, 0) OSHUN_I_PAIRB(EXT, T, L0, R, 1)
 #define OSHUN_FAMILY_PAIRB(EXT, T) OSHUN_L0_1_8(OSHUN_PB, T, EXT, 2) OSHUN_L0_1_8(OSHUN_PB, T, EXT, 3)
 #define OSHUN_FAMILY_PAIRB_F32_R4(EXT) OSHUN_L0_1_8(OSHUN_PB, float, EXT, 4)
+// batched marched stage pairs: l0 <= 8
+#define OSHUN_MB(T, L0, EXT) OSHUN_I_MARCHB(EXT, T, L0, 0) OSHUN_I_MARCHB(EXT, T, L0, 1)
+#define OSHUN_FAMILY_MARCHB(EXT, T) OSHUN_L0_1_8(OSHUN_MB, T, EXT)
 
 // collisions: one wavefront per column (NPL 1, 2, 4, 8) and the lane layouts
 // (NPL, SEGW) = (4, 16), (8, 16), (8, 32) in modes 1 (e-e), 2 (l >= 1), 3 (both)
@@ -88,6 +93,7 @@
 #define OSHUN_EXTERN_KERNELS                                                                  \
   OSHUN_FAMILY_PAIR(extern, double) OSHUN_FAMILY_PAIR(extern, float) OSHUN_FAMILY_PAIR_IMP(extern) \
   OSHUN_FAMILY_MARCH(extern, double) OSHUN_FAMILY_MARCH(extern, float)                        \
+  OSHUN_FAMILY_MARCHB(extern, double) OSHUN_FAMILY_MARCHB(extern, float)                      \
   OSHUN_FAMILY_PAIRB(extern, double) OSHUN_FAMILY_PAIRB(extern, float) OSHUN_FAMILY_PAIRB_F32_R4(extern) \
   OSHUN_FAMILY_COLL(extern, double) OSHUN_FAMILY_COLL(extern, float)                          \
   OSHUN_FAMILY_COLLB(extern, double) OSHUN_FAMILY_COLLB(extern, float)                        \

@@ -902,14 +902,10 @@ __device__ __forceinline__ void march_body(const Stage1DArgs<T>& a, const int st
   }
 }
 
-// one wavefront per (strip, chunk); the four wavefronts of a workgroup take
-// consecutive chunks of one strip, and each XCD a contiguous run of workgroups
+// wavefront g of a run: strip g / nchunk, chunk g % nchunk
 template <typename T, int L0, int PAIR>
-__global__ __launch_bounds__(256) void vlasov_march_1d(Stage1DArgs<T> a) {
+__device__ __forceinline__ void march_wave(const Stage1DArgs<T>& a, const int g) {
   const int nchunk = (a.np + a.pw - 1) / a.pw;
-  const int nstrip = (a.nx + kMarchX - 1) / kMarchX;
-  const int g = xcd_tile(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (g >= nstrip * nchunk) return;  // grid padding (whole wavefront)
   const int chunk = g % nchunk, p0 = chunk * a.pw;
   // interior chunks: every stage row has both p neighbours inside 1..np-2 and
   // lies above the CLEAN_LOW_PC rows (no special rows, no range checks).
@@ -927,6 +923,27 @@ __global__ __launch_bounds__(256) void vlasov_march_1d(Stage1DArgs<T> a) {
     if (interior) march_body<T, L0, PAIR, false, -1>(a, strip, chunk);
     else march_body<T, L0, PAIR, true, -1>(a, strip, chunk);
   }
+}
+
+// one wavefront per (strip, chunk); the four wavefronts of a workgroup take
+// consecutive chunks of one strip, and each XCD a contiguous run of workgroups
+template <typename T, int L0, int PAIR>
+__global__ __launch_bounds__(256) void vlasov_march_1d(Stage1DArgs<T> a) {
+  const int nchunk = (a.np + a.pw - 1) / a.pw;
+  const int nstrip = (a.nx + kMarchX - 1) / kMarchX;
+  const int g = xcd_tile(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (g >= nstrip * nchunk) return;  // grid padding (whole wavefront)
+  march_wave<T, L0, PAIR>(a, g);
+}
+// Batched runs: ws[r] is the first global wavefront of run r (ws[nruns] =
+// total); a wavefront finds its run by binary search.
+template <typename T, int L0, int PAIR>
+__global__ __launch_bounds__(256) void vlasov_march_1d_batch(const Stage1DArgs<T>* __restrict__ args,
+                                                             const int* __restrict__ ws, int nruns) {
+  const int g = xcd_tile(blockIdx.x, gridDim.x) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (g >= ws[nruns]) return;  // grid padding (whole wavefront)
+  const int r = batch_run(ws, nruns, g);
+  march_wave<T, L0, PAIR>(args[r], g - ws[r]);
 }
 
 template <typename T>

@@ -30,11 +30,11 @@ def _setup(precision):
     return b, runs, states
 
 
-def _alone(r, f, fld, steps):
+def _alone(r, f, fld, steps, rows=10):
     import oshun_amd
     kw = {k: v for k, v in r.items() if k not in ("l0", "nump", "numx")}
     ctx = oshun_amd.Context(r["l0"], 0, r["nump"], r["numx"], 1, **kw)
-    ctx.pair_rows(10)  # the batch's default tile height (a small run alone would take 6)
+    ctx.pair_rows(rows)  # the batch's default tile height is 10 (a small run alone would take 6)
     ctx.upload(f, fld)
     ctx.step(H, steps)
     fo, fldo = ctx.download()
@@ -62,7 +62,8 @@ def test_batch_autotune_restores_state_and_agrees():
     before = [b.run(i).download() for i in range(len(runs))]
     rows, ms = b.autotune(2)
     print(f"autotune: {rows} p rows per tile, {ms:.3f} ms per batch step", flush=True)
-    assert rows in (6, 10) and ms > 0
+    assert rows in (6, 10, -6, -8, -12) and ms > 0
+    assert b.pair_rows() == rows
     for i in range(len(runs)):
         f, fld = b.run(i).download()
         assert np.array_equal(f, before[i][0]) and np.array_equal(fld, before[i][1])
@@ -73,3 +74,27 @@ def test_batch_autotune_restores_state_and_agrees():
         fs, flds, _ = _alone(r, *states[i], 3)
         assert np.abs(f - fs).max() <= 1e-12 * np.abs(fs).max(), i
         assert np.abs(fld - flds).max() <= 1e-12 * max(np.abs(flds).max(), 1e-300), i
+
+
+@pytest.mark.parametrize("precision,rows", [(0, -8), (1, -6), (0, 6)], ids=["fp64-march8", "fp32-march6", "fp64-tile6"])
+def test_batch_equals_runs_alone_at_set_rows(precision, rows):
+    print(f"case batch of {len(RUNS)} runs, precision {precision}, pair rows {rows}", flush=True)
+    b, runs, states = _setup(precision)
+    assert b.pair_rows(rows) == rows
+    b.step(3)
+    for i, r in enumerate(runs):
+        f, fld = b.run(i).download()
+        fs, flds, t = _alone(r, *states[i], 3, rows=rows)
+        assert np.array_equal(f, fs), i
+        assert np.array_equal(fld, flds), i
+        assert b.run(i).time() == t
+
+
+def test_batch_pair_rows_bounds():
+    import oshun_amd
+    print("case batch pair rows knob", flush=True)
+    b, _, _ = _setup(0)
+    for bad in (14, 7, -3, -33):
+        with pytest.raises(oshun_amd.OshunError):
+            b.pair_rows(bad)
+    assert b.pair_rows() == 10
