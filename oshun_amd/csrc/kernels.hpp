@@ -1047,7 +1047,7 @@ __device__ __forceinline__ void sum_direct(const T* plane, const double* p3og, i
   if (i < nxy) {
     const int x = i % nx, y = i / nx;
     const T* col = plane + (long long)y * np * nx + x;
-#pragma unroll 4
+#pragma unroll
     for (int p = row; p < np; p += kFieldRows) {
       const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
       s += w * (static_cast<double>(col[(long long)p * nx]) * p3og[p]);
@@ -2233,27 +2233,38 @@ __device__ __forceinline__ void ee_slope_rows(const CollArgs<T>& a, const double
   }
 }
 
+// the current of Y0 at x from its p sum j and f10(p1), as field_body forms it
+template <typename Z>
+__device__ __forceinline__ double fused_current(const Z& z, double j, double f1) {
+  j *= z.delta_p;
+  j += z.small * (f1 * z.p3og1);
+  j *= 4.0 * M_PI / 3.0;
+  return j;
+}
 // Field stage 4 of the step and field stage 1 of the next one for column x of
 // a 1D m0 = 0 run (FieldArgs modes 4 and 1, def-RK.cpp:116-138 with
 // Current::Jxyz and MaxwellEq of def-VlasovMax.cpp:680-745,979-1035), the
 // same arithmetic as field_body.  In 1D only Ex couples to f and its update is
-// local; the stage-1 curl at x needs the stage-4 fields at x +- 1, which are
-// recomputed from the stage-3 fields.  pre: the column's prefetched inputs
-// (stage-3 fields at x-2..x+2, accumulator at x-1..x+1); j3 and j0: the p sums
-// of the stage-4 input's and of the collided f10's current integrand.
+// local; the stage-1 curl at x needs the stage-4 fields at x +- 1, which the
+// launch forms once per position of its columns and the two beside them, from
+// the prefetched stage-3 fields and accumulator; the current sums are the
+// stage-4 input's (its pair kernels' partials) and the collided f10's.
 constexpr int kFuseTiles = 64;  // most stage-pair p tiles a fused collision launch takes (LDS slots per column)
-static_assert(kFuseTiles <= 4 * kFieldRows, "fused_fields_1d sums at most 4 partials per field row");
-struct FusePre {
-  double cf[6][5], af[6][3];
-};
+static_assert(kFuseTiles <= 4 * kFieldRows, "fused_stage4_1d sums at most 4 partials per field row");
 // row r of sum_direct (p = r, r + kFieldRows, ...) over a plane held in the
 // row-order LDS layout of collide_rows (row p at p + p / NPL), as stored in T;
 // jw = w p3og with w = 1 or 1/2, so f (w p3og) = w (f p3og) exactly
 template <typename T, int NPL, int SEGW>
 __device__ __forceinline__ double current_row(const double* trc, const double* jw, int r, int np) {
+  // all kRowP / kFieldRows terms are read (the LDS rows exist up to kRowP) and
+  // those of p < np added in order, so the loads do not wait on the sum
   double js = 0.0;
-  for (int p = r; p < np; p += kFieldRows)
-    js += static_cast<double>(static_cast<T>(trc[p + p / NPL])) * jw[(p % NPL) * SEGW + p / NPL];
+#pragma unroll
+  for (int k = 0; k < kRowP / kFieldRows; ++k) {
+    const int p = r + k * kFieldRows;
+    const double t = static_cast<double>(static_cast<T>(trc[p + p / NPL])) * jw[(p % NPL) * SEGW + p / NPL];
+    if (p < np) js += t;
+  }
   return js;
 }
 // the fixed tree of sum_parts / sum_direct over kFieldRows row sums
@@ -2264,51 +2275,55 @@ __device__ __forceinline__ double field_tree(double (&sh)[kFieldRows]) {
     for (int r = 0; r < w; ++r) sh[r] += sh[r + w];
   return sh[0];
 }
+// Stage 4 of the step at one position (its six fields into F): it needs only
+// the stage inputs, so it runs before the collisions.  cf: the stage-3 fields
+// at the position -1 .. +1 (component c at cf[c * SC]), af: the accumulator at
+// the position (af[c * SA]), both read from LDS where used.  jp3 (the
+// position's current partials) is null at the positions beside a column, whose
+// Ex the next stage does not read; at a column's own x the Ex slope takes the
+// current first and the fields are stored at x.
+template <typename T, int SC, int SA>
+__device__ void fused_stage4_1d(const CollArgs<T>& a, int x, const double* cf, const double* af, const double* jp3,
+                                double f1y3, double (&F)[6]) {
+  const auto& z = a.fz;
+  double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (jp3) {
+    double sh[kFieldRows];
+#pragma unroll
+    for (int r = 0; r < kFieldRows; ++r) {  // sum_parts with npb <= 4 kFieldRows partials
+      double v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = (r + kFieldRows * q < z.npt) ? jp3[r + kFieldRows * q] : 0.0;
+      double s = 0.0;
+      if (r < z.npt) s += (v[0] + v[1]) + (v[2] + v[3]);
+      sh[r] = s;
+    }
+    k[0] += fused_current(z, field_tree(sh), f1y3);
+  }
+  maxwell_terms([&](int c, int at) { return cf[c * SC + (at == 0 ? 0 : at == 1 ? 2 : 1)]; }, z.midx, z.midy, k);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double kh = k[c] * z.h6;
+    F[c] = af[c * SA] + kh;
+  }
+  if (jp3) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) z.sf[(long long)c * a.nx + x] = F[c];
+  }
+}
+// The next step's stage 1 at x from the stage-4 fields F4 at x-1, x, x+1 and
+// the current of the collided f10 (its kFieldRows row sums and f10(p1)).
 template <typename T>
-__device__ void fused_fields_1d(const CollArgs<T>& a, int x, const FusePre& q, const double* jp3, double f1y3,
-                                const double* jrows0, double f10p1) {
+__device__ void fused_stage1_1d(const CollArgs<T>& a, int x, const double (&F4)[3][6], const double* jrows0,
+                                double f10p1) {
   const auto& z = a.fz;
   const int nx = a.nx;
   double sh[kFieldRows];
 #pragma unroll
-  for (int r = 0; r < kFieldRows; ++r) {  // sum_parts with npb <= 4 kFieldRows partials
-    double v[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = (r + kFieldRows * k < z.npt) ? jp3[r + kFieldRows * k] : 0.0;
-    double s = 0.0;
-    if (r < z.npt) s += (v[0] + v[1]) + (v[2] + v[3]);
-    sh[r] = s;
-  }
-  const double j3 = field_tree(sh);
-  double j0 = 0.0;
-  if (z.next1) {
-#pragma unroll
-    for (int r = 0; r < kFieldRows; ++r) sh[r] = jrows0[r];
-    j0 = field_tree(sh);
-  }
-  auto current = [&](double j, double f1) {
-    j *= z.delta_p;
-    j += z.small * (f1 * z.p3og1);
-    j *= 4.0 * M_PI / 3.0;
-    return j;
-  };
-  double F4[3][6];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {  // stage 4 at x-1, x, x+1 (Ex only at x)
-    double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (d == 1) k[0] += current(j3, f1y3);
-    maxwell_terms([&](int c, int at) { return q.cf[c][at == 0 ? d : at == 1 ? d + 2 : d + 1]; }, z.midx, z.midy, k);
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      const double kh = k[c] * z.h6;
-      F4[d][c] = q.af[c][d] + kh;
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 6; ++c) z.sf[(long long)c * nx + x] = F4[1][c];
-  if (!z.next1) return;
+  for (int r = 0; r < kFieldRows; ++r) sh[r] = jrows0[r];
+  const double j0 = field_tree(sh);
   double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  k[0] += current(j0, f10p1);
+  k[0] += fused_current(z, j0, f10p1);
   maxwell_terms([&](int c, int at) { return F4[at == 0 ? 0 : at == 1 ? 2 : 1][c]; }, z.midx, z.midy, k);
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -2384,6 +2399,7 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
     }
   }
   __shared__ double zj[kCols][kFieldRows], zf1[kCols];  // the collided f10's current rows and f10(p1)
+  __shared__ double zf4[kCols + 2][6];                   // stage-4 fields at x0-1 .. x0+kCols
   const int seg = threadIdx.x & (SEGW - 1);
   const int p0 = seg * NPL;
   const int lrow = (threadIdx.x / SEGW) * kS + p0 + seg;  // LDS index of lane-layout row i: lrow + i
@@ -2420,6 +2436,20 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
 #pragma unroll
       for (int k = 0; k < kZper; ++k)
         if (zi[k] >= 0) zpre[zi[k]] = zv[k];
+      __syncthreads();  // the fused stages' inputs are in LDS
+      // stage 4 of the step while f00 is still in flight (it reads no state),
+      // once per position x0-1 .. x0+kCols: lane 0 of each column takes its
+      // own x, lane 1 of the first and last columns the positions beside them
+      const int col = threadIdx.x / SEGW;
+      const int j = seg == 0 ? col + 1 : (seg == 1 && col == 0) ? 0 : (seg == 1 && col == kCols - 1) ? kCols + 1 : -1;
+      if (j >= 0) {
+        const bool own = seg == 0 && x0 + col < nx;
+        double F[6];
+        fused_stage4_1d<T, kZcf, kZaf>(a, x0 + col, &zpre[j], &zpre[kZoAf + j],
+                                       own ? &zpre[kZoJp + col * kZpt] : nullptr, zpre[kZoF1 + col], F);
+#pragma unroll
+        for (int c6 = 0; c6 < 6; ++c6) zf4[j][c6] = F[c6];
+      }
     }
 #pragma unroll
     for (int k = 0; k < NPL; ++k) tr[crow + k * kStepL] = static_cast<double>(v[k]);  // rows >= np load 0
@@ -2689,17 +2719,14 @@ __device__ __forceinline__ void collide_body(const CollArgs<T>& a, const int til
       if (seg == 0) zf1[col] = static_cast<double>(static_cast<T>(tr[col * kS + 1]));
     }
   }
-  if (a.fuse && seg == 0 && x0 + threadIdx.x / SEGW < nx) {
+  if (a.fuse && a.fz.next1 && seg == 0 && x0 + static_cast<int>(threadIdx.x) / SEGW < nx) {
     const int col = threadIdx.x / SEGW;
-    FusePre q;
+    double F4[3][6];  // stage-4 fields at x-1, x, x+1 (positions col .. col+2 of zf4)
 #pragma unroll
-    for (int c6 = 0; c6 < 6; ++c6) {
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-      for (int d = 0; d < 5; ++d) q.cf[c6][d] = zpre[c6 * kZcf + col + d];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) q.af[c6][d] = zpre[kZoAf + c6 * kZaf + col + d];
-    }
-    fused_fields_1d<T>(a, x0 + col, q, &zpre[kZoJp + col * kZpt], zpre[kZoF1 + col], zj[col], zf1[col]);
+      for (int c6 = 0; c6 < 6; ++c6) F4[d][c6] = zf4[col + d][c6];
+    fused_stage1_1d<T>(a, x0 + col, F4, zj[col], zf1[col]);
   }
 }
 
