@@ -1169,7 +1169,10 @@ __device__ __forceinline__ void field23_body(const Field23Args<T>& A, const int 
   const int row = threadIdx.y;
   const int nx = a.nx, nxy = nx;  // ny = 1
   __shared__ double sh[2][kFieldRows][kFieldX];
-  auto wrap = [nx](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
+  auto wrap = [nx](int x) {  // x - 2 .. x + 2 into 0 .. nx-1 for any nx >= 1
+    x %= nx;
+    return x < 0 ? x + nx : x;
+  };
   // the field inputs of row 0 are loaded before the current sums, so that
   // their latency overlaps the partial-sum loads and the reduction barriers
   double fin[6][5], f0v[6][3], accv[6], f1a = 0.0, f2b = 0.0;
@@ -1246,31 +1249,78 @@ __global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage41_1d(Field41
   const int i = blockIdx.x * kFieldX + threadIdx.x;
   const int row = threadIdx.y;
   const int nx = a.nx, nxy = nx;  // ny = 1
-  __shared__ double sh[kFieldRows][kFieldX];
-  auto wrap = [nx](int x) { return x < 0 ? x + nx : (x >= nx ? x - nx : x); };
-  double j4 = 0.0, j1 = 0.0;
-  sum_parts(a.jpart, a.npb, nxy, i, 0, row, sh, &j4);
-  sum_direct<T>(b.fpl + (long long)b.c10 * b.plane, b.p3og, b.np, nx, nxy, i, row, sh, &j1);
+  __shared__ double sh[2][kFieldRows][kFieldX];
+  auto wrap = [nx](int x) {
+    x %= nx;
+    return x < 0 ? x + nx : x;
+  };
+  // the field inputs of row 0 are loaded before the current sums, so that
+  // their latency overlaps the sums' loads and barriers
+  double fin[6][5], accv[6][3], f1a = 0.0, f1b = 0.0;
+  if (i < nxy && row == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+#pragma unroll
+      for (int d = 0; d < 5; ++d) fin[c][d] = a.fin[(long long)c * nxy + wrap(i + d - 2)];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) accv[c][d] = a.facc[(long long)c * nxy + wrap(i + d - 1)];
+    }
+    f1a = a.f1p1[i];
+    f1b = static_cast<double>(b.fpl[(long long)b.c10 * b.plane + nx + i]);
+  }
+  // the stage-4 current from the pair partials (sum_parts) and the stage-1
+  // current summed directly over p (sum_direct): every load of both is issued
+  // before the trees, each sum keeps its own order, the tree levels share
+  // their barriers
+  double s4 = 0.0, s1 = 0.0;
+  if (i < nxy) {
+    for (int b0 = row; b0 < a.npb; b0 += 4 * kFieldRows) {
+      double v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int pb = b0 + kFieldRows * k;
+        v[k] = (pb < a.npb) ? a.jpart[((long long)pb * 3) * nxy + i] : 0.0;
+      }
+      s4 += (v[0] + v[1]) + (v[2] + v[3]);
+    }
+    const T* col = b.fpl + (long long)b.c10 * b.plane + i;
+#pragma unroll
+    for (int p = row; p < b.np; p += kFieldRows) {
+      const double w = (p == 0 || p == b.np - 1) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
+      s1 += w * (static_cast<double>(col[(long long)p * nx]) * b.p3og[p]);
+    }
+  }
+  const int lx = threadIdx.x;
+  sh[0][row][lx] = s4;
+  sh[1][row][lx] = s1;
+  __syncthreads();
+#pragma unroll
+  for (int w = kFieldRows / 2; w >= 1; w /= 2) {
+    if (row < w) {
+      sh[0][row][lx] += sh[0][row + w][lx];
+      sh[1][row][lx] += sh[1][row + w][lx];
+    }
+    __syncthreads();
+  }
   if (i >= nxy || row != 0) return;
+  const double j4 = sh[0][0][lx], j1 = sh[1][0][lx];
   // stage 4 at x-1, x, x+1: out = acc + (h/6) k
   double F4[3][6];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int xx = wrap(i + d - 1);
     double k[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (d == 1) {  // Current::Jxyz (def-VlasovMax.cpp:680-745), as field_body
       double j = j4;
       j *= a.delta_p;
-      j += a.small * (a.f1p1[i] * a.p3og1);
+      j += a.small * (f1a * a.p3og1);
       j *= 4.0 * M_PI / 3.0;
       k[0] += j;
     }
-    maxwell_terms([&](int c, int at) { return a.fin[(long long)c * nxy + (at == 0 ? wrap(xx - 1) : at == 1 ? wrap(xx + 1) : xx)]; },
-                  a.midx, a.midy, k);
+    maxwell_terms([&](int c, int at) { return fin[c][at == 0 ? d : at == 1 ? d + 2 : d + 1]; }, a.midx, a.midy, k);
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
       const double kh = k[c] * a.h6;
-      F4[d][c] = a.facc[(long long)c * nxy + xx] + kh;
+      F4[d][c] = accv[c][d] + kh;
     }
   }
 #pragma unroll
@@ -1280,7 +1330,7 @@ __global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage41_1d(Field41
   {
     double j = j1;
     j *= b.delta_p;
-    j += b.small * (static_cast<double>(b.fpl[(long long)b.c10 * b.plane + nx + i]) * b.p3og1);
+    j += b.small * (f1b * b.p3og1);
     j *= 4.0 * M_PI / 3.0;
     k[0] += j;
   }
