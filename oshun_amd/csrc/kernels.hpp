@@ -1960,6 +1960,15 @@ __global__ __launch_bounds__(64 * COLS) void collide(CollArgs<T> a) {
   while (c < a.nc && ldc(a.comp_l, c) < lmin) ++c;
   int cend = c;
   while (cend < a.nc && ldc(a.comp_l, cend) <= lmax) ++cend;
+  // a grid of gridDim.y > 1 splits the planes of each column group among
+  // that many workgroups, in contiguous runs (each forms the coefficients
+  // from f00 as stored, so the host launches it so only without the e-e
+  // step); a plane's solve is the same in any workgroup
+  if (gridDim.y > 1) {
+    const int n = cend - c, g = blockIdx.y, ng = gridDim.y;
+    cend = c + (n * (g + 1)) / ng;
+    c += (n * g) / ng;
+  }
   T vn[NPL];  // next plane, row order
   if (c < cend) load_rows(a.f + (long long)c * a.plane, vn);
   double cpv[NPL], w[NPL];

@@ -654,14 +654,50 @@ int do_rk4(oshun_ctx* c, double h) {
   return c->fp32 ? rk4<float>(c, h) : rk4<double>(c, h);
 }
 
+// workgroups of collide<T, NPL, COLS> resident per CU (registers and LDS)
+template <typename T, int NPL, int COLS>
+int collide_wgs_per_cu() {
+  static int n = 0;  // the same on every MI355X
+  if (n == 0) {
+    int b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, oshun::collide<T, NPL, COLS>, 64 * COLS, 0) != hipSuccess ||
+        b < 1)
+      b = 1;
+    n = b;
+  }
+  return n;
+}
+
 template <typename T, int NPL>
 int launch_collide_npl(oshun_ctx* c, oshun::CollArgs<T>& a) {
   ProfRec r;
   prof_begin(c, 2, &r);
-  // the widest column group that still gives a workgroup per CU
   auto groups = [c](int cols) { return ((c->nxi + cols - 1) / cols) * c->nyi; };
-  const int cols = groups(8) >= 256 ? 8 : groups(4) >= 256 ? 4 : groups(2) >= 256 ? 2 : 1;
-  const dim3 grid(groups(cols)), block(64 * cols);
+  // the widest column group that still gives a workgroup per CU
+  int cols = groups(8) >= 256 ? 8 : groups(4) >= 256 ? 4 : groups(2) >= 256 ? 2 : 1;
+  int split = 1;
+  if (!a.do_ee && (a.do_f1 || a.do_flm)) {
+    // Without the e-e step the planes of a column group can be split among
+    // workgroups (grid.y): a column's planes are one serial chain per
+    // wavefront, so the launch takes the (cols, split) with the least
+    // rounds of resident workgroups x (planes per chain + the set-up, about
+    // three planes' time).
+    const int lmin = a.do_f1 ? 1 : 2, lmax = a.do_flm ? a.l0 : 1;
+    int n = 0;
+    for (int i = 1; i < a.nc; ++i) n += (c->T.comp_l[i] >= lmin && c->T.comp_l[i] <= lmax) ? 1 : 0;
+    const int wpc[4] = {collide_wgs_per_cu<T, NPL, 8>(), collide_wgs_per_cu<T, NPL, 4>(),
+                        collide_wgs_per_cu<T, NPL, 2>(), collide_wgs_per_cu<T, NPL, 1>()};
+    const int colv[4] = {8, 4, 2, 1};
+    double best = 1e300;
+    for (int k = 0; k < 4; ++k) {
+      const long long tiles = groups(colv[k]), cap = 256LL * wpc[k];
+      for (int s = 1; s <= 16 && s * 4 <= n; ++s) {
+        const double cost = static_cast<double>((tiles * s + cap - 1) / cap) * ((n + s - 1) / s + 3);
+        if (cost < best * 0.999) { best = cost; cols = colv[k]; split = s; }
+      }
+    }
+  }
+  const dim3 grid(groups(cols), split), block(64 * cols);
   switch (cols) {
     case 8: hipLaunchKernelGGL((oshun::collide<T, NPL, 8>), grid, block, 0, c->stream, a); break;
     case 4: hipLaunchKernelGGL((oshun::collide<T, NPL, 4>), grid, block, 0, c->stream, a); break;
