@@ -1039,18 +1039,36 @@ __device__ __forceinline__ void sum_parts2(const double* ja, const double* jb, i
 
 // the current integrand of one plane summed over p by the rows of the block
 // (row r takes p = r, r + kFieldRows, ...), then the same fixed tree
+// row `row` of a column's current integrand (p = row, row + kFieldRows, ...,
+// Current::Jxyz trapezoid weights), added in p order; eight rows' loads are
+// issued before their adds
+template <typename T>
+__device__ __forceinline__ double direct_rows(const T* col, const double* p3og, int np, int nx, int row) {
+  double s = 0.0;
+  for (int p0 = row; p0 < np; p0 += 8 * kFieldRows) {
+    double v[8], q[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int p = p0 + k * kFieldRows;
+      v[k] = p < np ? static_cast<double>(col[(long long)p * nx]) : 0.0;
+      q[k] = p < np ? p3og[p] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int p = p0 + k * kFieldRows;
+      const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;
+      if (p < np) s += w * (v[k] * q[k]);
+    }
+  }
+  return s;
+}
 template <typename T>
 __device__ __forceinline__ void sum_direct(const T* plane, const double* p3og, int np, int nx, int nxy, int i,
                                            int row, double (&sh)[kFieldRows][kFieldX], double* out) {
   double s = 0.0;
   if (i < nxy) {
     const int x = i % nx, y = i / nx;
-    const T* col = plane + (long long)y * np * nx + x;
-#pragma unroll
-    for (int p = row; p < np; p += kFieldRows) {
-      const double w = (p == 0 || p == np - 1) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
-      s += w * (static_cast<double>(col[(long long)p * nx]) * p3og[p]);
-    }
+    s = direct_rows<T>(plane + (long long)y * np * nx + x, p3og, np, nx, row);
   }
   const int lx = threadIdx.x;
   sh[row][lx] = s;
@@ -1283,12 +1301,7 @@ __global__ __launch_bounds__(kFieldX * kFieldRows) void field_stage41_1d(Field41
       }
       s4 += (v[0] + v[1]) + (v[2] + v[3]);
     }
-    const T* col = b.fpl + (long long)b.c10 * b.plane + i;
-#pragma unroll
-    for (int p = row; p < b.np; p += kFieldRows) {
-      const double w = (p == 0 || p == b.np - 1) ? 0.5 : 1.0;  // Current::Jxyz trapezoid weights
-      s1 += w * (static_cast<double>(col[(long long)p * nx]) * b.p3og[p]);
-    }
+    s1 = direct_rows<T>(b.fpl + (long long)b.c10 * b.plane + i, b.p3og, b.np, nx, row);
   }
   const int lx = threadIdx.x;
   sh[0][row][lx] = s4;
