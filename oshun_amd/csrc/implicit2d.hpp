@@ -120,52 +120,51 @@ struct ImpCurArgs {
 
 // Current_xyz::calculate_J (def-ImplicitE.cpp:125-180): Jx from f10 with
 // -4pi/3; Jy = Re and Jz = Re(i J) of the -8pi/3 sum over f11; each sum in
-// the reference's order (p0 + pmax first, then p1..pmax-1), the block's 16
-// columns x all p rows staged in LDS so that every load is in flight at once.
+// the reference's order (p0 + pmax first, then p1..pmax-1).  The block's
+// kImpCols2D columns x all p rows of the three planes are staged in LDS
+// (3 np kImpCols2D doubles of dynamic shared memory) with every load in
+// flight at once, then three threads per column run the three serial sums
+// side by side.
+constexpr int kImpCols2D = 8;
 __global__ __launch_bounds__(256) void imp_current_2d(ImpCurArgs a) {
-  __shared__ double chunk[kImpRows][kImpCols];
-  const int tx = threadIdx.x % kImpCols, ty = threadIdx.x / kImpCols;
-  const int x = blockIdx.x * kImpCols + tx, y = blockIdx.y;
+  extern __shared__ double chunk_lds[];
+  const int tx = threadIdx.x % kImpCols2D, ty = threadIdx.x / kImpCols2D;
+  const int x = blockIdx.x * kImpCols2D + tx, y = blockIdx.y;
   const int np = a.np, nx = a.nx;
   const bool ok = x < nx;
   const long long col = (long long)y * np * nx + (ok ? x : 0);
   const int comp[3] = {a.c10, a.c11r, a.c11i};
-  double s[3];
-#pragma unroll 1
+#pragma unroll
   for (int q = 0; q < 3; ++q) {
     const double* f = a.f + (long long)comp[q] * a.plane + col;
-    double j = 0.0;
-    if (ok && ty == 0) {
-      j = f[0] * a.p3og[0];
-      j += f[(long long)(np - 1) * nx] * a.p3og[np - 1];
-    }
 #pragma unroll 8
-    for (int ip = 1 + ty; ip < np - 1; ip += 256 / kImpCols)
-      chunk[ip][tx] = ok ? f[(long long)ip * nx] * a.p3og[ip] : 0.0;
-    __syncthreads();
-    if (ok && ty == 0) {
-      if (a.ext && y == 0) {
-        // the guard row that jayx(ix) *= 0.5 addresses (an (ix, 0) subscript):
-        // halved once before its p sum and once more for every later row
-        j *= 0.5;
-        for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip][tx];
-        for (int r = 1; r < a.ny; ++r) j *= 0.5;
-      } else {
-        for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip][tx];
-      }
-      j *= a.delta_p;
-      j += a.small * (f[nx] * a.p3og[1]);
-    }
-    s[q] = j;
-    __syncthreads();
+    for (int ip = 1 + ty; ip < np - 1; ip += 256 / kImpCols2D)
+      chunk_lds[(q * np + ip) * kImpCols2D + tx] = ok ? f[(long long)ip * nx] * a.p3og[ip] : 0.0;
   }
-  if (!ok || ty != 0) return;
+  __syncthreads();
+  if (!ok || ty >= 3) return;
+  const int q = ty;  // this thread's sum
+  const double* f = a.f + (long long)comp[q] * a.plane + col;
+  const double* chunk = chunk_lds + (long long)q * np * kImpCols2D + tx;  // row ip at chunk[ip * kImpCols2D]
+  double j = f[0] * a.p3og[0];
+  j += f[(long long)(np - 1) * nx] * a.p3og[np - 1];
+  if (a.ext && y == 0) {
+    // the guard row that jayx(ix) *= 0.5 addresses (an (ix, 0) subscript):
+    // halved once before its p sum and once more for every later row
+    j *= 0.5;
+    for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip * kImpCols2D];
+    for (int r = 1; r < a.ny; ++r) j *= 0.5;
+  } else {
+    for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip * kImpCols2D];
+  }
+  j *= a.delta_p;
+  j += a.small * (f[nx] * a.p3og[1]);
   const long long nxy = (long long)nx * a.ny;
   const long long k = (long long)y * nx + x;
-  a.j[k] = s[0] * ((-4.0) * M_PI / 3.0);
   const double c11 = (-8.0) * M_PI / 3.0;
-  a.j[nxy + k] = s[1] * c11;
-  a.j[2 * nxy + k] = -(s[2] * c11);
+  if (q == 0) a.j[k] = j * ((-4.0) * M_PI / 3.0);
+  else if (q == 1) a.j[nxy + k] = j * c11;
+  else a.j[2 * nxy + k] = -(j * c11);
 }
 
 struct ImpSolveArgs {
@@ -305,8 +304,16 @@ int imp_current(oshun_ctx* c, const void* buf, int slot) {
   a.ext = c->ext ? 1 : 0;
   ProfRec r;
   prof_begin(c, 1, &r);
-  hipLaunchKernelGGL(oshun::imp_current_2d, dim3((c->nx + oshun::kImpCols - 1) / oshun::kImpCols, c->ny), dim3(256),
-                     0, c->stream, a);
+  const size_t lds = 3 * static_cast<size_t>(c->np) * oshun::kImpCols2D * sizeof(double);  // <= 96 KB (Np <= 512)
+  static bool lds_set = false;  // allow the largest staging area once
+  if (!lds_set) {
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void*>(oshun::imp_current_2d),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  3 * oshun::kImpRows * oshun::kImpCols2D * static_cast<int>(sizeof(double))));
+    lds_set = true;
+  }
+  hipLaunchKernelGGL(oshun::imp_current_2d, dim3((c->nx + oshun::kImpCols2D - 1) / oshun::kImpCols2D, c->ny), dim3(256),
+                     lds, c->stream, a);
   prof_end(c, &r);
   HIPCHK(c, hipGetLastError());
   return OSHUN_OK;
