@@ -87,8 +87,10 @@ __global__ __launch_bounds__(256) void imp_field_solve_1d(ImpFieldArgs a, int st
 #pragma unroll 8
   for (int ip = 1 + ty; ip < np - 1; ip += 256 / kImpCols) chunk[ip][tx] = ok ? f[(long long)ip * nx] * a.p3og[ip] : 0.0;
   __syncthreads();
-  if (ok && ty == 0)
-    for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip][tx];
+  if (ok && ty == 0) {
+#pragma unroll 8
+    for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip][tx];  // eight LDS reads ahead of their adds
+  }
   if (!ok || ty != 0) return;
   j *= a.delta_p;
   j += a.small * (f[nx] * a.p3og[1]);

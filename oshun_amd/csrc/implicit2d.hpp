@@ -152,10 +152,12 @@ __global__ __launch_bounds__(256) void imp_current_2d(ImpCurArgs a) {
     // the guard row that jayx(ix) *= 0.5 addresses (an (ix, 0) subscript):
     // halved once before its p sum and once more for every later row
     j *= 0.5;
+#pragma unroll 8
     for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip * kImpCols2D];
     for (int r = 1; r < a.ny; ++r) j *= 0.5;
   } else {
-    for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip * kImpCols2D];
+#pragma unroll 8
+    for (int ip = 1; ip < np - 1; ++ip) j += chunk[ip * kImpCols2D];  // eight LDS reads ahead of their adds
   }
   j *= a.delta_p;
   j += a.small * (f[nx] * a.p3og[1]);
